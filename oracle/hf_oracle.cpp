@@ -1,0 +1,1485 @@
+// CPU oracle for the Hartree-Fock energy-and-forces path.
+//
+// TEST INFRASTRUCTURE ONLY: this file is the checker that the GPU engine is
+// compared against (and the timed CPU baseline of bench.py). Nothing in the
+// product (ab-initio-md_amd/) links, loads or calls it.
+//
+// It restates, in plain C++ with the same loop structure and floating-point
+// operation order, the McMurchie-Davidson integral code of the reference:
+//   Boys function            aimd/qc/boys.py:49-140
+//   Hermite E / R            aimd/qc/hermite.py:55-178
+//   S, T, dipole, V          aimd/qc/integrals.py:146-222, 260-329
+//   one-electron gradients   aimd/qc/integrals.py:232-257, 332-501
+//   block-pair data          aimd/qc/integrals.py:504-599 (_build_pairs)
+//   ERI block                aimd/qc/integrals.py:646-861
+//   Schwarz bounds           aimd/qc/integrals.py:864-883
+//   ERI tensor + symmetrise  aimd/qc/integrals.py:886-967
+//   J/K from the tensor      aimd/qc/integrals.py:1009-1028
+//   ERI gradient             aimd/qc/integrals.py:1049-1369
+// plus one function the reference does not have: an integral-direct J/K over
+// the same screened unique block quartets (for systems whose tensor does not
+// fit in memory), digested with the permutational degeneracy factors.
+//
+// Build with -ffp-contract=off (numba does not fuse multiply-adds).
+// Work over block pairs is split into 64 interleaved chunks exactly as the
+// reference does (bit-reversed first index), so results do not depend on
+// the OpenMP thread count.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+typedef int64_t i64;
+
+namespace {
+
+// ------------------------------------------------------------------ Boys
+const int NMAX = 32;
+const int TAYLOR_TERMS = 10;
+const double STEP = 0.1;
+const double X_SWITCH = 40.0;
+const int NGRID = 402;          // int(round(X_SWITCH / STEP)) + 2
+const int NROWS = NMAX + TAYLOR_TERMS + 1;   // 43
+
+double TABLE[NGRID][NROWS];
+double INV_INT[TAYLOR_TERMS];
+double INV_ODD[NMAX + 1];
+const double INV_STEP = 1.0 / STEP;
+
+// ---------------------------------------------------------------- Hermite
+const int LMAX = 4;
+const int HMAX = 4 * LMAX + 1;  // 17
+int NHERM[HMAX + 2];
+int NHT;                        // number of Hermite indices up to HMAX
+int HERM_TUV[2000][3];
+int HERM_IDX[HMAX + 2][HMAX + 2][HMAX + 2];
+int HERM_DIR[2000], HERM_M1[2000], HERM_M2[2000];
+double HERM_RC[2000], HERM_SIGN[2000];
+int CART_OFF[LMAX + 3];
+int CART_LXYZ[64][3];
+double CART_FAC[64];
+
+const double PI = 3.141592653589793;
+double TWO_PI_2_5;
+const double PRIM_CUTOFF = 50.0;
+const int NCHUNK = 64;
+int CHUNK_FIRST[NCHUNK];
+
+int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+
+long dfact(int n) { long o = 1; while (n > 1) { o *= n; n -= 2; } return o; }
+
+struct Init {
+    Init() {
+        // Boys table: series for the top order, then downward recursion
+        const int top = NROWS - 1;
+        static double tab[NROWS][NGRID];
+        for (int k = 0; k < NGRID; ++k) {
+            double x = k * STEP;
+            double term = 1.0 / (2 * top + 1);
+            double total = term;
+            for (int m = 1; m < 400; ++m) {
+                term = term * (2.0 * x) / (double)(2 * top + 2 * m + 1);
+                total += term;
+            }
+            double ex = std::exp(-x);
+            tab[top][k] = ex * total;
+            for (int n = top - 1; n >= 0; --n)
+                tab[n][k] = (2.0 * x * tab[n + 1][k] + ex) / (double)(2 * n + 1);
+        }
+        for (int k = 0; k < NGRID; ++k)
+            for (int n = 0; n < NROWS; ++n) TABLE[k][n] = tab[n][k];
+        for (int m = 0; m < TAYLOR_TERMS; ++m) INV_INT[m] = 1.0 / (m + 1);
+        for (int n = 0; n <= NMAX; ++n) INV_ODD[n] = 1.0 / (2 * n + 1);
+        // Cartesian tables, l = 0..LMAX+1, PySCF order
+        int off = 0;
+        for (int l = 0; l <= LMAX + 1; ++l) {
+            CART_OFF[l] = off;
+            for (int lx = l; lx >= 0; --lx)
+                for (int ly = l - lx; ly >= 0; --ly) {
+                    int lz = l - lx - ly;
+                    CART_LXYZ[off][0] = lx; CART_LXYZ[off][1] = ly; CART_LXYZ[off][2] = lz;
+                    CART_FAC[off] = std::sqrt((double)dfact(2 * l - 1) /
+                        (double)(dfact(2 * lx - 1) * dfact(2 * ly - 1) * dfact(2 * lz - 1)));
+                    ++off;
+                }
+        }
+        CART_OFF[LMAX + 2] = off;
+        // Hermite index tables by increasing degree
+        for (int t = 0; t < HMAX + 2; ++t)
+            for (int u = 0; u < HMAX + 2; ++u)
+                for (int v = 0; v < HMAX + 2; ++v) HERM_IDX[t][u][v] = -1;
+        int n = 0;
+        for (int d = 0; d <= HMAX; ++d)
+            for (int t = d; t >= 0; --t)
+                for (int u = d - t; u >= 0; --u) {
+                    HERM_TUV[n][0] = t; HERM_TUV[n][1] = u; HERM_TUV[n][2] = d - t - u;
+                    HERM_IDX[t][u][d - t - u] = n;
+                    ++n;
+                }
+        NHT = n;
+        HERM_DIR[0] = HERM_M1[0] = HERM_M2[0] = 0; HERM_RC[0] = 0.0;
+        for (int h = 1; h < n; ++h) {
+            int c[3] = {HERM_TUV[h][0], HERM_TUV[h][1], HERM_TUV[h][2]};
+            int d = c[0] > 0 ? 0 : (c[1] > 0 ? 1 : 2);
+            HERM_DIR[h] = d;
+            int c1[3] = {c[0], c[1], c[2]}; c1[d] -= 1;
+            HERM_M1[h] = HERM_IDX[c1[0]][c1[1]][c1[2]];
+            HERM_M2[h] = 0; HERM_RC[h] = 0.0;
+            if (c[d] >= 2) {
+                int c2[3] = {c[0], c[1], c[2]}; c2[d] -= 2;
+                HERM_M2[h] = HERM_IDX[c2[0]][c2[1]][c2[2]];
+                HERM_RC[h] = c[d] - 1;
+            }
+        }
+        for (int h = 0; h < n; ++h)
+            HERM_SIGN[h] = ((HERM_TUV[h][0] + HERM_TUV[h][1] + HERM_TUV[h][2]) % 2 == 0) ? 1.0 : -1.0;
+        for (int L = 0; L < HMAX + 2; ++L) NHERM[L] = nherm(L);
+        TWO_PI_2_5 = 2.0 * std::pow(PI, 2.5);
+        for (int c = 0; c < NCHUNK; ++c) {
+            int r = 0;
+            for (int b = 0; b < 6; ++b) if (c & (1 << b)) r |= 1 << (5 - b);
+            CHUNK_FIRST[c] = r;
+        }
+    }
+} init_;
+
+inline double boys_top(int n, double x) {
+    if (x < X_SWITCH) {
+        int k = (int)(x * INV_STEP + 0.5);
+        double dx = k * STEP - x;
+        double acc = 0.0, fac = 1.0;
+        for (int m = 0; m < TAYLOR_TERMS; ++m) {
+            acc += TABLE[k][n + m] * fac;
+            fac *= dx * INV_INT[m];
+        }
+        return acc;
+    }
+    double f = 0.5 * std::sqrt(PI / x);
+    if (n > 0) {
+        double ex = std::exp(-x);
+        double inv2x = 0.5 / x;
+        for (int m = 0; m < n; ++m) f = ((2 * m + 1) * f - ex) * inv2x;
+    }
+    return f;
+}
+
+void boys_into(int nmax, double x, double* out) {
+    if (x >= X_SWITCH) {
+        double f = 0.5 * std::sqrt(PI / x);
+        out[0] = f;
+        double ex = std::exp(-x);
+        double inv2x = 0.5 / x;
+        for (int n = 0; n < nmax; ++n) { f = ((2 * n + 1) * f - ex) * inv2x; out[n + 1] = f; }
+        return;
+    }
+    double f = boys_top(nmax, x);
+    out[nmax] = f;
+    if (nmax > 0) {
+        double ex = std::exp(-x);
+        double x2 = 2.0 * x;
+        for (int n = nmax - 1; n >= 0; --n) { f = (x2 * f + ex) * INV_ODD[n]; out[n] = f; }
+    }
+}
+
+// E[i][j][t] with strides (sj, st): out[(i*sj + j)*st + t]
+struct E3 {
+    double* p; int sj, st;
+    double& operator()(int i, int j, int t) const { return p[(i * sj + j) * st + t]; }
+};
+
+void hermite_e(int imax, int jmax, double a, double b, double xab, double e00, E3 out) {
+    double p = a + b;
+    double xpa = -b * xab / p;
+    double xpb = a * xab / p;
+    double h = 0.5 / p;
+    int tmax = imax + jmax + 1;
+    for (int i = 0; i <= imax; ++i)
+        for (int j = 0; j <= jmax; ++j)
+            for (int t = 0; t <= tmax; ++t) out(i, j, t) = 0.0;
+    out(0, 0, 0) = e00;
+    for (int j = 1; j <= jmax; ++j)
+        for (int t = 0; t <= j; ++t) {
+            double v = xpb * out(0, j - 1, t) + (t + 1) * out(0, j - 1, t + 1);
+            if (t > 0) v += h * out(0, j - 1, t - 1);
+            out(0, j, t) = v;
+        }
+    for (int i = 1; i <= imax; ++i)
+        for (int j = 0; j <= jmax; ++j)
+            for (int t = 0; t <= i + j; ++t) {
+                double v = xpa * out(i - 1, j, t) + (t + 1) * out(i - 1, j, t + 1);
+                if (t > 0) v += h * out(i - 1, j, t - 1);
+                out(i, j, t) = v;
+            }
+}
+
+void hermite_r(int L, double alpha, double x, double y, double z, double* out, double* tmp) {
+    double T = alpha * (x * x + y * y + z * z);
+    double f = boys_top(L, T);
+    double ex = 0.0;
+    if (L > 0) ex = std::exp(-T);
+    double t2 = 2.0 * T;
+    double m2a = -2.0 * alpha;
+    double pw = 1.0;
+    for (int i = 0; i < L; ++i) pw *= m2a;
+    double xyz[3] = {x, y, z};
+    for (int n = L; n >= 0; --n) {
+        if (n < L) f = (t2 * f + ex) * INV_ODD[n];
+        double* cur = (n % 2 == 0) ? out : tmp;
+        double* prv = (n % 2 == 0) ? tmp : out;
+        cur[0] = pw * f;
+        for (int h = 1; h < NHERM[L - n]; ++h)
+            cur[h] = xyz[HERM_DIR[h]] * prv[HERM_M1[h]] + HERM_RC[h] * prv[HERM_M2[h]];
+        pw /= m2a;
+    }
+}
+
+// --------------------------------------------------------------- basis view
+struct Shells {
+    int ns;
+    const i64 *atom, *l, *prim, *ao;
+    const double *exp, *coef;
+    const double* pos;   // (natm, 3)
+};
+
+inline double kin1d(E3 E, int i, int j, double b) {
+    double v = -2.0 * b * (2 * j + 1) * E(i, j, 0) + 4.0 * b * b * E(i, j + 2, 0);
+    if (j >= 2) v += j * (j - 1) * E(i, j - 2, 0);
+    return -0.5 * v;
+}
+
+inline double herm3(const E3* E, int ax, int ay, int az, int bx, int by, int bz, int h) {
+    return E[0](ax, bx, HERM_TUV[h][0]) * E[1](ay, by, HERM_TUV[h][1]) * E[2](az, bz, HERM_TUV[h][2]);
+}
+
+inline double dherm3_a(const E3* E, int ax, int ay, int az, int bx, int by, int bz, int h, int d, double a) {
+    int t = HERM_TUV[h][0], u = HERM_TUV[h][1], v = HERM_TUV[h][2];
+    double ex = E[0](ax, bx, t), ey = E[1](ay, by, u), ez = E[2](az, bz, v);
+    if (d == 0) {
+        double dx = 2.0 * a * E[0](ax + 1, bx, t);
+        if (ax > 0) dx -= ax * E[0](ax - 1, bx, t);
+        return dx * ey * ez;
+    } else if (d == 1) {
+        double dy = 2.0 * a * E[1](ay + 1, by, u);
+        if (ay > 0) dy -= ay * E[1](ay - 1, by, u);
+        return ex * dy * ez;
+    }
+    double dz = 2.0 * a * E[2](az + 1, bz, v);
+    if (az > 0) dz -= az * E[2](az - 1, bz, v);
+    return ex * ey * dz;
+}
+
+int shells_lmax(const Shells& B) {
+    int lm = 0;
+    for (int s = 0; s < B.ns; ++s) lm = std::max(lm, (int)B.l[s]);
+    return lm;
+}
+
+// S, T, dipole (integrals.py:155-222); n = nao
+void overlap_type(const Shells& B, int n, const double* origin, double* S, double* T, double* M) {
+    int lmax = shells_lmax(B);
+    int d1 = lmax + 1, d2 = lmax + 3, d3 = 2 * lmax + 5;
+    std::vector<double> Ebuf(3 * d1 * d2 * d3, 0.0);
+    E3 E[3];
+    for (int d = 0; d < 3; ++d) E[d] = E3{Ebuf.data() + d * d1 * d2 * d3, d2, d3};
+    double sq_pi = std::sqrt(PI);
+    for (int si = 0; si < B.ns; ++si) {
+        int li = B.l[si], ai = B.atom[si], oi = B.ao[si];
+        int ci0 = CART_OFF[li], nci = CART_OFF[li + 1] - ci0;
+        for (int sj = 0; sj <= si; ++sj) {
+            int lj = B.l[sj], aj = B.atom[sj], oj = B.ao[sj];
+            int cj0 = CART_OFF[lj], ncj = CART_OFF[lj + 1] - cj0;
+            for (i64 ka = B.prim[si]; ka < B.prim[si + 1]; ++ka) {
+                double a = B.exp[ka];
+                for (i64 kb = B.prim[sj]; kb < B.prim[sj + 1]; ++kb) {
+                    double b = B.exp[kb];
+                    double p = a + b;
+                    double mu = a * b / p;
+                    double sq = sq_pi / std::sqrt(p);
+                    for (int d = 0; d < 3; ++d) {
+                        double xab = B.pos[ai * 3 + d] - B.pos[aj * 3 + d];
+                        double e00 = std::exp(-mu * xab * xab) * sq;
+                        if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
+                        hermite_e(li, lj + 2, a, b, xab, e00, E[d]);
+                    }
+                    double px = (a * B.pos[ai * 3 + 0] + b * B.pos[aj * 3 + 0]) / p - origin[0];
+                    double py = (a * B.pos[ai * 3 + 1] + b * B.pos[aj * 3 + 1]) / p - origin[1];
+                    double pz = (a * B.pos[ai * 3 + 2] + b * B.pos[aj * 3 + 2]) / p - origin[2];
+                    for (int ia = 0; ia < nci; ++ia) {
+                        int ix = CART_LXYZ[ci0 + ia][0], iy = CART_LXYZ[ci0 + ia][1], iz = CART_LXYZ[ci0 + ia][2];
+                        double fa = CART_FAC[ci0 + ia];
+                        for (int ib = 0; ib < ncj; ++ib) {
+                            int jx = CART_LXYZ[cj0 + ib][0], jy = CART_LXYZ[cj0 + ib][1], jz = CART_LXYZ[cj0 + ib][2];
+                            double f = fa * CART_FAC[cj0 + ib];
+                            double sx = E[0](ix, jx, 0), sy = E[1](iy, jy, 0), sz = E[2](iz, jz, 0);
+                            double kx = kin1d(E[0], ix, jx, b), ky = kin1d(E[1], iy, jy, b), kz = kin1d(E[2], iz, jz, b);
+                            int m = oi + ia, nn = oj + ib;
+                            S[m * n + nn] += f * sx * sy * sz;
+                            T[m * n + nn] += f * (kx * sy * sz + sx * ky * sz + sx * sy * kz);
+                            M[0 * n * n + m * n + nn] += f * (E[0](ix, jx, 1) + px * sx) * sy * sz;
+                            M[1 * n * n + m * n + nn] += f * sx * (E[1](iy, jy, 1) + py * sy) * sz;
+                            M[2 * n * n + m * n + nn] += f * sx * sy * (E[2](iz, jz, 1) + pz * sz);
+                        }
+                    }
+                }
+            }
+            if (si != sj)
+                for (int m = oi; m < oi + nci; ++m)
+                    for (int nn = oj; nn < oj + ncj; ++nn) {
+                        S[nn * n + m] = S[m * n + nn];
+                        T[nn * n + m] = T[m * n + nn];
+                        for (int d = 0; d < 3; ++d) M[d * n * n + nn * n + m] = M[d * n * n + m * n + nn];
+                    }
+        }
+    }
+}
+
+// V (integrals.py:260-329)
+void nuclear(const Shells& B, int n, int nc, const double* charges, const double* cpos, double* V) {
+    int lmax = shells_lmax(B);
+    int d1 = lmax + 1, d2 = lmax + 1, d3 = 2 * lmax + 2;
+    std::vector<double> Ebuf(3 * d1 * d2 * d3, 0.0);
+    E3 E[3];
+    for (int d = 0; d < 3; ++d) E[d] = E3{Ebuf.data() + d * d1 * d2 * d3, d2, d3};
+    int nhmax = NHERM[2 * lmax];
+    std::vector<double> R(nhmax), Rt(nhmax), Rsum(nhmax);
+    for (int si = 0; si < B.ns; ++si) {
+        int li = B.l[si], ai = B.atom[si], oi = B.ao[si];
+        int ci0 = CART_OFF[li], nci = CART_OFF[li + 1] - ci0;
+        for (int sj = 0; sj <= si; ++sj) {
+            int lj = B.l[sj], aj = B.atom[sj], oj = B.ao[sj];
+            int cj0 = CART_OFF[lj], ncj = CART_OFF[lj + 1] - cj0;
+            int L = li + lj, nh = NHERM[L];
+            for (i64 ka = B.prim[si]; ka < B.prim[si + 1]; ++ka) {
+                double a = B.exp[ka];
+                for (i64 kb = B.prim[sj]; kb < B.prim[sj + 1]; ++kb) {
+                    double b = B.exp[kb];
+                    double p = a + b;
+                    double mu = a * b / p;
+                    for (int d = 0; d < 3; ++d) {
+                        double xab = B.pos[ai * 3 + d] - B.pos[aj * 3 + d];
+                        double e00 = std::exp(-mu * xab * xab);
+                        if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
+                        hermite_e(li, lj, a, b, xab, e00, E[d]);
+                    }
+                    double bp = b / p;
+                    for (int h = 0; h < nh; ++h) Rsum[h] = 0.0;
+                    for (int c = 0; c < nc; ++c) {
+                        double X = (B.pos[ai * 3 + 0] - cpos[c * 3 + 0]) - bp * (B.pos[ai * 3 + 0] - B.pos[aj * 3 + 0]);
+                        double Y = (B.pos[ai * 3 + 1] - cpos[c * 3 + 1]) - bp * (B.pos[ai * 3 + 1] - B.pos[aj * 3 + 1]);
+                        double Z = (B.pos[ai * 3 + 2] - cpos[c * 3 + 2]) - bp * (B.pos[ai * 3 + 2] - B.pos[aj * 3 + 2]);
+                        hermite_r(L, p, X, Y, Z, R.data(), Rt.data());
+                        double w = -charges[c] * 2.0 * PI / p;
+                        for (int h = 0; h < nh; ++h) Rsum[h] += w * R[h];
+                    }
+                    for (int ia = 0; ia < nci; ++ia) {
+                        int ix = CART_LXYZ[ci0 + ia][0], iy = CART_LXYZ[ci0 + ia][1], iz = CART_LXYZ[ci0 + ia][2];
+                        double fa = CART_FAC[ci0 + ia];
+                        for (int ib = 0; ib < ncj; ++ib) {
+                            int jx = CART_LXYZ[cj0 + ib][0], jy = CART_LXYZ[cj0 + ib][1], jz = CART_LXYZ[cj0 + ib][2];
+                            double acc = 0.0;
+                            for (int h = 0; h < nh; ++h) acc += herm3(E, ix, iy, iz, jx, jy, jz, h) * Rsum[h];
+                            V[(oi + ia) * n + oj + ib] += fa * CART_FAC[cj0 + ib] * acc;
+                        }
+                    }
+                }
+            }
+            if (si != sj)
+                for (int m = oi; m < oi + nci; ++m)
+                    for (int nn = oj; nn < oj + ncj; ++nn) V[nn * n + m] = V[m * n + nn];
+        }
+    }
+}
+
+// grad += d/dR [sum W S + sum D T] (integrals.py:332-404)
+void overlap_kinetic_grad(const Shells& B, int n, const double* W, const double* D, double* grad) {
+    int lmax = shells_lmax(B);
+    int d1 = lmax + 2, d2 = lmax + 3, d3 = 2 * lmax + 6;
+    std::vector<double> Ebuf(3 * d1 * d2 * d3, 0.0);
+    E3 E[3];
+    for (int d = 0; d < 3; ++d) E[d] = E3{Ebuf.data() + d * d1 * d2 * d3, d2, d3};
+    double sq_pi = std::sqrt(PI);
+    double S1[3], K1[3], dS[3], dK[3];
+    for (int si = 0; si < B.ns; ++si) {
+        int li = B.l[si], ai = B.atom[si], oi = B.ao[si];
+        int ci0 = CART_OFF[li], nci = CART_OFF[li + 1] - ci0;
+        for (int sj = 0; sj < si; ++sj) {
+            int aj = B.atom[sj];
+            if (aj == ai) continue;
+            int lj = B.l[sj], oj = B.ao[sj];
+            int cj0 = CART_OFF[lj], ncj = CART_OFF[lj + 1] - cj0;
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+            for (i64 ka = B.prim[si]; ka < B.prim[si + 1]; ++ka) {
+                double a = B.exp[ka];
+                for (i64 kb = B.prim[sj]; kb < B.prim[sj + 1]; ++kb) {
+                    double b = B.exp[kb];
+                    double p = a + b;
+                    double mu = a * b / p;
+                    double sq = sq_pi / std::sqrt(p);
+                    for (int d = 0; d < 3; ++d) {
+                        double xab = B.pos[ai * 3 + d] - B.pos[aj * 3 + d];
+                        double e00 = std::exp(-mu * xab * xab) * sq;
+                        if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
+                        hermite_e(li + 1, lj + 2, a, b, xab, e00, E[d]);
+                    }
+                    for (int ia = 0; ia < nci; ++ia) {
+                        double fa = CART_FAC[ci0 + ia];
+                        for (int ib = 0; ib < ncj; ++ib) {
+                            double f = 2.0 * fa * CART_FAC[cj0 + ib];
+                            double wmn = f * W[(oi + ia) * n + oj + ib];
+                            double dmn = f * D[(oi + ia) * n + oj + ib];
+                            for (int d = 0; d < 3; ++d) {
+                                int i = CART_LXYZ[ci0 + ia][d], j = CART_LXYZ[cj0 + ib][d];
+                                S1[d] = E[d](i, j, 0);
+                                K1[d] = kin1d(E[d], i, j, b);
+                                dS[d] = 2.0 * a * E[d](i + 1, j, 0);
+                                dK[d] = 2.0 * a * kin1d(E[d], i + 1, j, b);
+                                if (i > 0) {
+                                    dS[d] -= i * E[d](i - 1, j, 0);
+                                    dK[d] -= i * kin1d(E[d], i - 1, j, b);
+                                }
+                            }
+                            g0 += wmn * dS[0] * S1[1] * S1[2] + dmn * (
+                                dK[0] * S1[1] * S1[2] + dS[0] * (K1[1] * S1[2] + S1[1] * K1[2]));
+                            g1 += wmn * S1[0] * dS[1] * S1[2] + dmn * (
+                                dK[1] * S1[0] * S1[2] + dS[1] * (K1[0] * S1[2] + S1[0] * K1[2]));
+                            g2 += wmn * S1[0] * S1[1] * dS[2] + dmn * (
+                                dK[2] * S1[0] * S1[1] + dS[2] * (K1[0] * S1[1] + S1[0] * K1[1]));
+                        }
+                    }
+                }
+            }
+            grad[ai * 3 + 0] += g0; grad[ai * 3 + 1] += g1; grad[ai * 3 + 2] += g2;
+            grad[aj * 3 + 0] -= g0; grad[aj * 3 + 1] -= g1; grad[aj * 3 + 2] -= g2;
+        }
+    }
+}
+
+// grad += d/dR sum D V (integrals.py:407-501); operator centres = the atoms
+void nuclear_grad(const Shells& B, int n, int natm, const double* charges, const double* D, double* grad) {
+    int lmax = shells_lmax(B);
+    int d1 = lmax + 2, d2 = lmax + 1, d3 = 2 * lmax + 3;
+    std::vector<double> Ebuf(3 * d1 * d2 * d3, 0.0);
+    E3 E[3];
+    for (int d = 0; d < 3; ++d) E[d] = E3{Ebuf.data() + d * d1 * d2 * d3, d2, d3};
+    int nh1max = NHERM[2 * lmax + 1];
+    std::vector<double> R(nh1max), Rt(nh1max), dE(nh1max), dA(3 * nh1max);
+    for (int si = 0; si < B.ns; ++si) {
+        int li = B.l[si], ai = B.atom[si], oi = B.ao[si];
+        int ci0 = CART_OFF[li], nci = CART_OFF[li + 1] - ci0;
+        for (int sj = 0; sj <= si; ++sj) {
+            int lj = B.l[sj], aj = B.atom[sj], oj = B.ao[sj];
+            int cj0 = CART_OFF[lj], ncj = CART_OFF[lj + 1] - cj0;
+            int L = li + lj, nh = NHERM[L], nh1 = NHERM[L + 1];
+            double sym = (si != sj) ? 2.0 : 1.0;
+            for (i64 ka = B.prim[si]; ka < B.prim[si + 1]; ++ka) {
+                double a = B.exp[ka];
+                for (i64 kb = B.prim[sj]; kb < B.prim[sj + 1]; ++kb) {
+                    double b = B.exp[kb];
+                    double p = a + b;
+                    double mu = a * b / p;
+                    for (int d = 0; d < 3; ++d) {
+                        double xab = B.pos[ai * 3 + d] - B.pos[aj * 3 + d];
+                        double e00 = std::exp(-mu * xab * xab);
+                        if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
+                        hermite_e(li + 1, lj, a, b, xab, e00, E[d]);
+                    }
+                    for (int h = 0; h < nh1; ++h) { dE[h] = 0.0; dA[h] = 0.0; dA[nh1max + h] = 0.0; dA[2 * nh1max + h] = 0.0; }
+                    for (int ia = 0; ia < nci; ++ia) {
+                        int ix = CART_LXYZ[ci0 + ia][0], iy = CART_LXYZ[ci0 + ia][1], iz = CART_LXYZ[ci0 + ia][2];
+                        double fa = CART_FAC[ci0 + ia];
+                        for (int ib = 0; ib < ncj; ++ib) {
+                            int jx = CART_LXYZ[cj0 + ib][0], jy = CART_LXYZ[cj0 + ib][1], jz = CART_LXYZ[cj0 + ib][2];
+                            double w = sym * fa * CART_FAC[cj0 + ib] * D[(oi + ia) * n + oj + ib];
+                            if (w == 0.0) continue;
+                            for (int h = 0; h < nh; ++h) dE[h] += w * herm3(E, ix, iy, iz, jx, jy, jz, h);
+                            for (int h = 0; h < nh1; ++h)
+                                for (int d = 0; d < 3; ++d)
+                                    dA[d * nh1max + h] += w * dherm3_a(E, ix, iy, iz, jx, jy, jz, h, d, a);
+                        }
+                    }
+                    double bp = b / p;
+                    for (int c = 0; c < natm; ++c) {
+                        double X = (B.pos[ai * 3 + 0] - B.pos[c * 3 + 0]) - bp * (B.pos[ai * 3 + 0] - B.pos[aj * 3 + 0]);
+                        double Y = (B.pos[ai * 3 + 1] - B.pos[c * 3 + 1]) - bp * (B.pos[ai * 3 + 1] - B.pos[aj * 3 + 1]);
+                        double Z = (B.pos[ai * 3 + 2] - B.pos[c * 3 + 2]) - bp * (B.pos[ai * 3 + 2] - B.pos[aj * 3 + 2]);
+                        hermite_r(L + 1, p, X, Y, Z, R.data(), Rt.data());
+                        double pref = -charges[c] * 2.0 * PI / p;
+                        for (int d = 0; d < 3; ++d) {
+                            double gA = 0.0;
+                            for (int h = 0; h < nh1; ++h) gA += dA[d * nh1max + h] * R[h];
+                            double gC = 0.0;
+                            for (int h = 0; h < nh; ++h) {
+                                int t = HERM_TUV[h][0], u = HERM_TUV[h][1], v = HERM_TUV[h][2];
+                                if (d == 0) t += 1; else if (d == 1) u += 1; else v += 1;
+                                gC -= dE[h] * R[HERM_IDX[t][u][v]];
+                            }
+                            gA *= pref;
+                            gC *= pref;
+                            grad[ai * 3 + d] += gA;
+                            grad[c * 3 + d] += gC;
+                            grad[aj * 3 + d] -= gA + gC;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ block pairs
+struct Blocks {
+    int nb;
+    const i64 *atom, *l, *prim, *ncon, *ao;
+    const double *exp, *coef;   // coef (nprim_blk, maxcon)
+    int maxcon;
+    const double* pos;
+};
+
+struct Pairs {
+    int nb, lmax, maxcon, npair;
+    i64 npp;
+    int LM2, TM;                 // pp_E dims: (npp, 3, LM2, LM2, TM)
+    std::vector<i64> pair_pp;    // npair + 1
+    std::vector<int> pair_blocks;  // npair * 2
+    std::vector<double> pp_ab, pp_P, pp_E, pp_C;
+    int maxpp, maxrow, maxpph;
+    E3 E(i64 k, int d) const {
+        return E3{const_cast<double*>(pp_E.data()) + (k * 3 + d) * LM2 * LM2 * TM, LM2, TM};
+    }
+};
+
+void build_pairs(const Blocks& B, Pairs& P) {
+    int nb = B.nb;
+    int npair = nb * (nb + 1) / 2;
+    int lmax = 0, maxcon = 1;
+    for (int s = 0; s < nb; ++s) { lmax = std::max(lmax, (int)B.l[s]); maxcon = std::max(maxcon, (int)B.ncon[s]); }
+    P.nb = nb; P.lmax = lmax; P.maxcon = maxcon; P.npair = npair;
+    P.pair_pp.assign(npair + 1, 0);
+    P.pair_blocks.assign(2 * npair, 0);
+    i64 npp = 0;
+    int ip = 0;
+    for (int i = 0; i < nb; ++i)
+        for (int j = 0; j <= i; ++j) {
+            P.pair_pp[ip] = npp;
+            P.pair_blocks[2 * ip] = i; P.pair_blocks[2 * ip + 1] = j;
+            double r2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                double dd = B.pos[B.atom[i] * 3 + d] - B.pos[B.atom[j] * 3 + d];
+                r2 += dd * dd;
+            }
+            for (i64 ka = B.prim[i]; ka < B.prim[i + 1]; ++ka)
+                for (i64 kb = B.prim[j]; kb < B.prim[j + 1]; ++kb) {
+                    double a = B.exp[ka], b = B.exp[kb];
+                    if (a * b / (a + b) * r2 <= PRIM_CUTOFF) npp += 1;
+                }
+            ++ip;
+        }
+    P.pair_pp[npair] = npp;
+    P.npp = npp;
+    P.LM2 = lmax + 2; P.TM = 2 * lmax + 4;
+    P.pp_ab.assign(npp * 2, 0.0);
+    P.pp_P.assign(npp * 3, 0.0);
+    P.pp_E.assign(npp * 3 * P.LM2 * P.LM2 * P.TM, 0.0);
+    P.pp_C.assign(npp * maxcon * maxcon, 0.0);
+    i64 k = 0;
+    for (int i = 0; i < nb; ++i) {
+        int ai = B.atom[i];
+        for (int j = 0; j <= i; ++j) {
+            int aj = B.atom[j];
+            int ncj = B.ncon[j];
+            double r2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                double dd = B.pos[ai * 3 + d] - B.pos[aj * 3 + d];
+                r2 += dd * dd;
+            }
+            for (i64 ka = B.prim[i]; ka < B.prim[i + 1]; ++ka) {
+                double a = B.exp[ka];
+                for (i64 kb = B.prim[j]; kb < B.prim[j + 1]; ++kb) {
+                    double b = B.exp[kb];
+                    double p = a + b;
+                    double mu = a * b / p;
+                    if (!(mu * r2 <= PRIM_CUTOFF)) continue;
+                    P.pp_ab[2 * k] = a; P.pp_ab[2 * k + 1] = b;
+                    for (int ci = 0; ci < B.ncon[i]; ++ci)
+                        for (int cj = 0; cj < ncj; ++cj)
+                            P.pp_C[k * maxcon * maxcon + ci * ncj + cj] = B.coef[ka * B.maxcon + ci] * B.coef[kb * B.maxcon + cj];
+                    for (int d = 0; d < 3; ++d) {
+                        double xab = B.pos[ai * 3 + d] - B.pos[aj * 3 + d];
+                        P.pp_P[3 * k + d] = B.pos[ai * 3 + d] - (b / p) * xab;
+                        double e00 = std::exp(-mu * xab * xab);
+                        hermite_e(B.l[i] + 1, B.l[j] + 1, a, b, xab, e00, P.E(k, d));
+                    }
+                    ++k;
+                }
+            }
+        }
+    }
+    int maxpp = 1, maxrow = 1, maxpph = 1;
+    for (int q = 0; q < npair; ++q) {
+        int i = P.pair_blocks[2 * q], j = P.pair_blocks[2 * q + 1];
+        int li = B.l[i], lj = B.l[j];
+        int nci = (li + 1) * (li + 2) / 2, ncj = (lj + 1) * (lj + 2) / 2;
+        int rows = B.ncon[i] * B.ncon[j] * nci * ncj;
+        int n = (int)(P.pair_pp[q + 1] - P.pair_pp[q]);
+        maxpp = std::max(maxpp, n);
+        maxrow = std::max(maxrow, rows);
+        maxpph = std::max(maxpph, n * NHERM[li + lj]);
+    }
+    P.maxpp = maxpp; P.maxrow = maxrow; P.maxpph = maxpph;
+}
+
+inline void pair_dims(const Pairs& P, const Blocks& B, int ip, int& ncart2, int& ncon2) {
+    int i = P.pair_blocks[2 * ip], j = P.pair_blocks[2 * ip + 1];
+    int li = B.l[i], lj = B.l[j];
+    ncart2 = (CART_OFF[li + 1] - CART_OFF[li]) * (CART_OFF[lj + 1] - CART_OFF[lj]);
+    ncon2 = B.ncon[i] * B.ncon[j];
+}
+
+struct Scratch {
+    int maxpp, nhk_max, nc2, maxrow, nhp, nR;
+    std::vector<double> block, Ek, X, T, W, RR;
+    std::vector<int> hidx;
+    int hidx_ld;
+    Scratch(int lmax, int maxpp_, int maxrow_, int deriv) {
+        int nc = (lmax + 1) * (lmax + 2) / 2;
+        maxpp = maxpp_; maxrow = maxrow_;
+        nhk_max = NHERM[2 * lmax];
+        nc2 = nc * nc;
+        nhp = NHERM[2 * lmax + deriv];
+        nR = NHERM[4 * lmax + deriv];
+        block.assign((size_t)maxrow * maxrow, 0.0);
+        Ek.assign((size_t)maxpp * nhk_max * nc2, 0.0);
+        X.assign((size_t)nhp * maxrow, 0.0);
+        T.assign(nc2, 0.0);
+        W.assign((size_t)nc2 * maxrow, 0.0);
+        RR.assign(2 * (size_t)nR, 0.0);
+        hidx.assign((size_t)nhp * nhk_max, 0);
+        hidx_ld = nhk_max;
+    }
+    double& ek(int kp, int k, int cd) { return Ek[((size_t)kp * nhk_max + k) * nc2 + cd]; }
+};
+
+void ket_hermite(const Pairs& P, const Blocks& B, int pk, Scratch& S) {
+    int sk = P.pair_blocks[2 * pk], sl = P.pair_blocks[2 * pk + 1];
+    int lk = B.l[sk], ll = B.l[sl];
+    int c0 = CART_OFF[lk], nck = CART_OFF[lk + 1] - c0;
+    int d0 = CART_OFF[ll], ncl = CART_OFF[ll + 1] - d0;
+    int nhk = NHERM[lk + ll];
+    i64 k0 = P.pair_pp[pk];
+    for (int kp = 0; kp < (int)(P.pair_pp[pk + 1] - k0); ++kp) {
+        E3 E[3] = {P.E(k0 + kp, 0), P.E(k0 + kp, 1), P.E(k0 + kp, 2)};
+        for (int k = 0; k < nhk; ++k)
+            for (int cd = 0; cd < nck * ncl; ++cd) S.ek(kp, k, cd) = 0.0;
+        for (int ic = 0; ic < nck; ++ic) {
+            int cx = CART_LXYZ[c0 + ic][0], cy = CART_LXYZ[c0 + ic][1], cz = CART_LXYZ[c0 + ic][2];
+            double fc = CART_FAC[c0 + ic];
+            for (int idd = 0; idd < ncl; ++idd) {
+                int dx = CART_LXYZ[d0 + idd][0], dy = CART_LXYZ[d0 + idd][1], dz = CART_LXYZ[d0 + idd][2];
+                double f = fc * CART_FAC[d0 + idd];
+                int cd = ic * ncl + idd;
+                for (int t = 0; t < cx + dx + 1; ++t) {
+                    double et = f * E[0](cx, dx, t);
+                    for (int u = 0; u < cy + dy + 1; ++u) {
+                        double etu = et * E[1](cy, dy, u);
+                        for (int v = 0; v < cz + dz + 1; ++v) {
+                            int k = HERM_IDX[t][u][v];
+                            S.ek(kp, k, cd) = HERM_SIGN[k] * etu * E[2](cz, dz, v);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+void hidx_fill(int nh, int nhk, Scratch& S) {
+    for (int h = 0; h < nh; ++h)
+        for (int k = 0; k < nhk; ++k)
+            S.hidx[h * S.hidx_ld + k] = HERM_IDX[HERM_TUV[h][0] + HERM_TUV[k][0]][HERM_TUV[h][1] + HERM_TUV[k][1]][HERM_TUV[h][2] + HERM_TUV[k][2]];
+}
+
+// R_tuv(alpha, (x,y,z)) for degrees <= L into RR[0:NH(L)] (integrals.py:723-746)
+inline void r_levels(int L, double alpha, double x, double y, double z, double* RR, int nR) {
+    double T2 = alpha * (x * x + y * y + z * z);
+    double f = boys_top(L, T2);
+    double ex = 0.0;
+    if (L > 0) ex = std::exp(-T2);
+    double m2a = -2.0 * alpha;
+    double pw = 1.0;
+    for (int i = 0; i < L; ++i) pw *= m2a;
+    for (int n = L; n >= 0; --n) {
+        if (n < L) f = (2.0 * T2 * f + ex) * INV_ODD[n];
+        int cur = n & 1, prv = 1 - cur;
+        RR[cur * nR] = pw * f;
+        for (int h = 1; h < NHERM[L - n]; ++h) {
+            int d = HERM_DIR[h];
+            double xd = d == 0 ? x : (d == 1 ? y : z);
+            RR[cur * nR + h] = xd * RR[prv * nR + HERM_M1[h]] + HERM_RC[h] * RR[prv * nR + HERM_M2[h]];
+        }
+        pw /= m2a;
+    }
+}
+
+// X[h, ckl ncd + cd] (integrals.py:705-774); X row stride = S.maxrow
+void ket_r_contract(const Pairs& P, i64 bp, int pk, int nh, int L, Scratch& S, int nhk, int ncd, int nkc) {
+    double p = P.pp_ab[2 * bp] + P.pp_ab[2 * bp + 1];
+    int ldx = S.maxrow;
+    for (int h = 0; h < nh; ++h)
+        for (int col = 0; col < nkc * ncd; ++col) S.X[h * ldx + col] = 0.0;
+    i64 k0 = P.pair_pp[pk];
+    int mc2 = P.maxcon * P.maxcon;
+    for (int kp = 0; kp < (int)(P.pair_pp[pk + 1] - k0); ++kp) {
+        double q = P.pp_ab[2 * (k0 + kp)] + P.pp_ab[2 * (k0 + kp) + 1];
+        double alpha = p * q / (p + q);
+        double pref = TWO_PI_2_5 / (p * q * std::sqrt(p + q));
+        double x = P.pp_P[3 * bp + 0] - P.pp_P[3 * (k0 + kp) + 0];
+        double y = P.pp_P[3 * bp + 1] - P.pp_P[3 * (k0 + kp) + 1];
+        double z = P.pp_P[3 * bp + 2] - P.pp_P[3 * (k0 + kp) + 2];
+        r_levels(L, alpha, x, y, z, S.RR.data(), S.nR);
+        const double* R0 = S.RR.data();
+        if (nkc == 1) {
+            double pc = pref * P.pp_C[(k0 + kp) * mc2 + 0];
+            for (int h = 0; h < nh; ++h)
+                for (int k = 0; k < nhk; ++k) {
+                    double r = pc * R0[S.hidx[h * S.hidx_ld + k]];
+                    if (r == 0.0) continue;
+                    for (int cd = 0; cd < ncd; ++cd) S.X[h * ldx + cd] += r * S.ek(kp, k, cd);
+                }
+        } else {
+            for (int h = 0; h < nh; ++h) {
+                for (int cd = 0; cd < ncd; ++cd) S.T[cd] = 0.0;
+                for (int k = 0; k < nhk; ++k) {
+                    double r = pref * R0[S.hidx[h * S.hidx_ld + k]];
+                    if (r == 0.0) continue;
+                    for (int cd = 0; cd < ncd; ++cd) S.T[cd] += r * S.ek(kp, k, cd);
+                }
+                for (int ckl = 0; ckl < nkc; ++ckl) {
+                    double c = P.pp_C[(k0 + kp) * mc2 + ckl];
+                    if (c == 0.0) continue;
+                    int o = ckl * ncd;
+                    for (int cd = 0; cd < ncd; ++cd) S.X[h * ldx + o + cd] += c * S.T[cd];
+                }
+            }
+        }
+    }
+}
+
+// block[r, s] (stride maxrow) = (IJ|KL) (integrals.py:777-842)
+void eri_block(const Pairs& P, const Blocks& B, int pb, int pk, Scratch& S) {
+    int si = P.pair_blocks[2 * pb], sj = P.pair_blocks[2 * pb + 1];
+    int sk = P.pair_blocks[2 * pk], sl = P.pair_blocks[2 * pk + 1];
+    int li = B.l[si], lj = B.l[sj];
+    int a0 = CART_OFF[li], nci = CART_OFF[li + 1] - a0;
+    int b0 = CART_OFF[lj], ncj = CART_OFF[lj + 1] - b0;
+    int nab, nbc, ncd, nkc;
+    pair_dims(P, B, pb, nab, nbc);
+    pair_dims(P, B, pk, ncd, nkc);
+    int ncol = nkc * ncd;
+    int Lab = li + lj;
+    int L = Lab + B.l[sk] + B.l[sl];
+    int nhb = NHERM[Lab];
+    int nhk = NHERM[B.l[sk] + B.l[sl]];
+    int ld = S.maxrow;
+    int mc2 = P.maxcon * P.maxcon;
+    hidx_fill(nhb, nhk, S);
+    ket_hermite(P, B, pk, S);
+    for (int r = 0; r < nbc * nab; ++r)
+        for (int col = 0; col < ncol; ++col) S.block[r * ld + col] = 0.0;
+    for (i64 bp = P.pair_pp[pb]; bp < P.pair_pp[pb + 1]; ++bp) {
+        ket_r_contract(P, bp, pk, nhb, L, S, nhk, ncd, nkc);
+        double cb = (nbc == 1) ? P.pp_C[bp * mc2 + 0] : 1.0;
+        E3 E[3] = {P.E(bp, 0), P.E(bp, 1), P.E(bp, 2)};
+        for (int ia = 0; ia < nci; ++ia) {
+            int ax = CART_LXYZ[a0 + ia][0], ay = CART_LXYZ[a0 + ia][1], az = CART_LXYZ[a0 + ia][2];
+            double fa = cb * CART_FAC[a0 + ia];
+            for (int ib = 0; ib < ncj; ++ib) {
+                int bx = CART_LXYZ[b0 + ib][0], by = CART_LXYZ[b0 + ib][1], bz = CART_LXYZ[b0 + ib][2];
+                double f = fa * CART_FAC[b0 + ib];
+                int ab = ia * ncj + ib;
+                double* out = (nbc == 1) ? &S.block[ab * ld] : &S.W[ab * ld];
+                if (nbc != 1)
+                    for (int col = 0; col < ncol; ++col) out[col] = 0.0;
+                for (int t = 0; t < ax + bx + 1; ++t) {
+                    double et = f * E[0](ax, bx, t);
+                    for (int u = 0; u < ay + by + 1; ++u) {
+                        double etu = et * E[1](ay, by, u);
+                        for (int v = 0; v < az + bz + 1; ++v) {
+                            double e = etu * E[2](az, bz, v);
+                            int h = HERM_IDX[t][u][v];
+                            for (int col = 0; col < ncol; ++col) out[col] += e * S.X[h * ld + col];
+                        }
+                    }
+                }
+            }
+        }
+        if (nbc != 1)
+            for (int cij = 0; cij < nbc; ++cij) {
+                double c = P.pp_C[bp * mc2 + cij];
+                if (c == 0.0) continue;
+                for (int ab = 0; ab < nab; ++ab) {
+                    int r = cij * nab + ab;
+                    for (int col = 0; col < ncol; ++col) S.block[r * ld + col] += c * S.W[ab * ld + col];
+                }
+            }
+    }
+}
+
+void schwarz(const Pairs& P, const Blocks& B, double* Q) {
+    int npair = P.npair;
+    for (int i = 0; i < npair; ++i) Q[i] = 0.0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < NCHUNK; ++c) {
+        if (CHUNK_FIRST[c] >= npair) continue;
+        Scratch S(P.lmax, P.maxpp, P.maxrow, 0);
+        for (int ip = CHUNK_FIRST[c]; ip < npair; ip += NCHUNK) {
+            eri_block(P, B, ip, ip, S);
+            int nab, nbc;
+            pair_dims(P, B, ip, nab, nbc);
+            double m = 0.0;
+            for (int r = 0; r < nab * nbc; ++r) m = std::max(m, std::fabs(S.block[r * S.maxrow + r]));
+            Q[ip] = std::sqrt(m);
+        }
+    }
+}
+
+// Iterate the unique block quartets (p1 >= p2, Q1 Q2 >= thr) of chunk c, calling fn(p1, p2, swapped, S)
+// after the block has been evaluated into S.block (orientation as integrals.py:900-914).
+template <class Fn>
+void for_quartets(const Pairs& P, const Blocks& B, const double* Q, double thr, int c, Scratch& S, Fn fn) {
+    int npair = P.npair;
+    for (int p1 = CHUNK_FIRST[c]; p1 < npair; p1 += NCHUNK) {
+        for (int p2 = 0; p2 <= p1; ++p2) {
+            if (Q[p1] * Q[p2] < thr) continue;
+            int n1, c1, n2, c2;
+            pair_dims(P, B, p1, n1, c1);
+            pair_dims(P, B, p2, n2, c2);
+            bool swapped = n1 * c1 < n2 * c2;
+            if (swapped) eri_block(P, B, p2, p1, S);
+            else eri_block(P, B, p1, p2, S);
+            fn(p1, p2, swapped, n1, n2);
+        }
+    }
+}
+
+// loop over every AO element of a block quartet (integrals.py:918-949)
+template <class Fn>
+void block_elements(const Pairs& P, const Blocks& B, int p1, int p2, int n1, int n2, bool swapped,
+                    const Scratch& S, Fn fn) {
+    int si = P.pair_blocks[2 * p1], sj = P.pair_blocks[2 * p1 + 1];
+    int sk = P.pair_blocks[2 * p2], sl = P.pair_blocks[2 * p2 + 1];
+    int nci = CART_OFF[B.l[si] + 1] - CART_OFF[B.l[si]];
+    int ncj = CART_OFF[B.l[sj] + 1] - CART_OFF[B.l[sj]];
+    int nck = CART_OFF[B.l[sk] + 1] - CART_OFF[B.l[sk]];
+    int ncl = CART_OFF[B.l[sl] + 1] - CART_OFF[B.l[sl]];
+    int nqj = B.ncon[sj], nql = B.ncon[sl];
+    int ld = S.maxrow;
+    for (int qi = 0; qi < B.ncon[si]; ++qi)
+        for (int a = 0; a < nci; ++a) {
+            int m = B.ao[si] + qi * nci + a;
+            for (int qj = 0; qj < nqj; ++qj)
+                for (int b = 0; b < ncj; ++b) {
+                    int n = B.ao[sj] + qj * ncj + b;
+                    int r1 = (qi * nqj + qj) * n1 + a * ncj + b;
+                    for (int qk = 0; qk < B.ncon[sk]; ++qk)
+                        for (int cc = 0; cc < nck; ++cc) {
+                            int lam = B.ao[sk] + qk * nck + cc;
+                            for (int ql = 0; ql < nql; ++ql)
+                                for (int d = 0; d < ncl; ++d) {
+                                    int sig = B.ao[sl] + ql * ncl + d;
+                                    int r2 = (qk * nql + ql) * n2 + cc * ncl + d;
+                                    double v = swapped ? S.block[r2 * ld + r1] : S.block[r1 * ld + r2];
+                                    fn(m, n, lam, sig, v);
+                                }
+                        }
+                }
+        }
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int oracle_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+
+void oracle_boys(int nmax, double x, double* out) { boys_into(nmax, x, out); }
+
+void oracle_boys_table(double* out) {
+    for (int k = 0; k < NGRID; ++k)
+        for (int n = 0; n < NROWS; ++n) out[k * NROWS + n] = TABLE[k][n];
+}
+
+void oracle_hermite_e(int imax, int jmax, double a, double b, double xab, double e00, double* out) {
+    // out dims (imax+1, jmax+1, imax+jmax+2)
+    hermite_e(imax, jmax, a, b, xab, e00, E3{out, jmax + 1, imax + jmax + 2});
+}
+
+void oracle_hermite_r(int L, double alpha, double x, double y, double z, double* out) {
+    std::vector<double> tmp(NHERM[L] + 1);
+    hermite_r(L, alpha, x, y, z, out, tmp.data());
+}
+
+void oracle_one_electron(int ns, const i64* atom, const i64* l, const i64* prim, const i64* ao,
+                         const double* ex, const double* coef, const double* pos, int nao,
+                         int ncharge, const double* charges, const double* cpos, const double* origin,
+                         double* S, double* T, double* M, double* V) {
+    Shells B{ns, atom, l, prim, ao, ex, coef, pos};
+    std::memset(S, 0, sizeof(double) * nao * nao);
+    std::memset(T, 0, sizeof(double) * nao * nao);
+    std::memset(M, 0, sizeof(double) * 3 * nao * nao);
+    std::memset(V, 0, sizeof(double) * nao * nao);
+    overlap_type(B, nao, origin, S, T, M);
+    nuclear(B, nao, ncharge, charges, cpos, V);
+}
+
+// grad (natm,3) = d/dR [tr(D (T+V)) - tr(W S)] (integrals.py:1483-1495); D, W symmetric
+void oracle_one_electron_grad(int ns, const i64* atom, const i64* l, const i64* prim, const i64* ao,
+                              const double* ex, const double* coef, const double* pos, int nao,
+                              int natm, const double* charges, const double* D, const double* W,
+                              double* grad) {
+    Shells B{ns, atom, l, prim, ao, ex, coef, pos};
+    std::vector<double> mW(W, W + (size_t)nao * nao);
+    for (auto& v : mW) v = -v;
+    std::memset(grad, 0, sizeof(double) * natm * 3);
+    overlap_kinetic_grad(B, nao, mW.data(), D, grad);
+    nuclear_grad(B, nao, natm, charges, D, grad);
+}
+
+// Separate terms: kinetic (D), overlap (W; d/dR tr(W S)), nuclear attraction (D)
+void oracle_one_electron_grad_terms(int ns, const i64* atom, const i64* l, const i64* prim, const i64* ao,
+                                    const double* ex, const double* coef, const double* pos, int nao,
+                                    int natm, const double* charges, const double* D, const double* W,
+                                    double* g_kin, double* g_ovl, double* g_nuc) {
+    Shells B{ns, atom, l, prim, ao, ex, coef, pos};
+    std::vector<double> zero((size_t)nao * nao, 0.0);
+    std::memset(g_kin, 0, sizeof(double) * natm * 3);
+    std::memset(g_ovl, 0, sizeof(double) * natm * 3);
+    std::memset(g_nuc, 0, sizeof(double) * natm * 3);
+    overlap_kinetic_grad(B, nao, zero.data(), D, g_kin);
+    overlap_kinetic_grad(B, nao, W, zero.data(), g_ovl);
+    nuclear_grad(B, nao, natm, charges, D, g_nuc);
+}
+
+struct OraclePairs {
+    Blocks B;
+    Pairs P;
+    std::vector<double> Q;
+    std::vector<double> pos;
+    std::vector<i64> atom, lv, prim, ncon, ao;
+    std::vector<double> ex, coef;
+};
+
+void* oracle_pairs_create(int nb, const i64* atom, const i64* l, const i64* prim, const i64* ncon,
+                          const i64* ao, const double* ex, const double* coef, int maxcon,
+                          int natm, const double* pos) {
+    OraclePairs* h = new OraclePairs();
+    int nprim = (int)prim[nb];
+    h->atom.assign(atom, atom + nb); h->lv.assign(l, l + nb); h->prim.assign(prim, prim + nb + 1);
+    h->ncon.assign(ncon, ncon + nb); h->ao.assign(ao, ao + nb);
+    h->ex.assign(ex, ex + nprim); h->coef.assign(coef, coef + (size_t)nprim * maxcon);
+    h->pos.assign(pos, pos + natm * 3);
+    h->B = Blocks{nb, h->atom.data(), h->lv.data(), h->prim.data(), h->ncon.data(), h->ao.data(),
+                  h->ex.data(), h->coef.data(), maxcon, h->pos.data()};
+    build_pairs(h->B, h->P);
+    h->Q.assign(h->P.npair, 0.0);
+    schwarz(h->P, h->B, h->Q.data());
+    return h;
+}
+
+void oracle_pairs_destroy(void* hp) { delete (OraclePairs*)hp; }
+
+int oracle_pairs_npair(void* hp) { return ((OraclePairs*)hp)->P.npair; }
+long oracle_pairs_npp(void* hp) { return (long)((OraclePairs*)hp)->P.npp; }
+
+void oracle_pairs_get(void* hp, i64* pair_pp, i64* pair_blocks, double* pp_ab, double* pp_P, double* Q) {
+    OraclePairs* h = (OraclePairs*)hp;
+    const Pairs& P = h->P;
+    for (int i = 0; i <= P.npair; ++i) pair_pp[i] = P.pair_pp[i];
+    for (int i = 0; i < 2 * P.npair; ++i) pair_blocks[i] = P.pair_blocks[i];
+    for (i64 i = 0; i < 2 * P.npp; ++i) pp_ab[i] = P.pp_ab[i];
+    for (i64 i = 0; i < 3 * P.npp; ++i) pp_P[i] = P.pp_P[i];
+    for (int i = 0; i < P.npair; ++i) Q[i] = h->Q[i];
+}
+
+// Surviving unique block quartets (p1 >= p2, Q1 Q2 >= thr) in (p1, p2) order; returns the count,
+// writes at most cap pairs.
+long oracle_quartets(void* hp, double thr, i64* out, long cap) {
+    OraclePairs* h = (OraclePairs*)hp;
+    long n = 0;
+    for (int p1 = 0; p1 < h->P.npair; ++p1)
+        for (int p2 = 0; p2 <= p1; ++p2) {
+            if (h->Q[p1] * h->Q[p2] < thr) continue;
+            if (n < cap) { out[2 * n] = p1; out[2 * n + 1] = p2; }
+            ++n;
+        }
+    return n;
+}
+
+// Full tensor (nao^4), as eri_tensor (integrals.py:886-1006)
+void oracle_eri_tensor(void* hp, double thr, int nao, double* eri) {
+    OraclePairs* h = (OraclePairs*)hp;
+    const Pairs& P = h->P;
+    const Blocks& B = h->B;
+    size_t n = nao;
+    std::memset(eri, 0, sizeof(double) * n * n * n * n);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < NCHUNK; ++c) {
+        if (CHUNK_FIRST[c] >= P.npair) continue;
+        Scratch S(P.lmax, P.maxpp, P.maxrow, 0);
+        for_quartets(P, B, h->Q.data(), thr, c, S, [&](int p1, int p2, bool sw, int n1, int n2) {
+            double scale = (p1 == p2) ? 0.5 : 1.0;
+            block_elements(P, B, p1, p2, n1, n2, sw, S, [&](int m, int nn, int lam, int sig, double v0) {
+                double v = scale * v0;
+                eri[((m * n + nn) * n + lam) * n + sig] = v;
+                eri[((nn * n + m) * n + lam) * n + sig] = v;
+                eri[((m * n + nn) * n + sig) * n + lam] = v;
+                eri[((nn * n + m) * n + sig) * n + lam] = v;
+            });
+        });
+    }
+    // M <- M + M^T on the (n^2, n^2) view, 32 x 32 tiles (integrals.py:952-967)
+    size_t N = n * n;
+    const size_t Bt = 32;
+    size_t nbk = (N + Bt - 1) / Bt;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long I = 0; I < (long)nbk; ++I) {
+        size_t r1 = std::min(N, (size_t)(I + 1) * Bt);
+        for (size_t J = 0; J <= (size_t)I; ++J) {
+            size_t c0 = J * Bt;
+            for (size_t r = I * Bt; r < r1; ++r) {
+                size_t c1 = (J < (size_t)I) ? std::min(N, (J + 1) * Bt) : r + 1;
+                for (size_t cc = c0; cc < c1; ++cc) {
+                    double v = eri[r * N + cc] + eri[cc * N + r];
+                    eri[r * N + cc] = v;
+                    eri[cc * N + r] = v;
+                }
+            }
+        }
+    }
+}
+
+// J[s] = sum (mn|ls) D[s]_ls, K[s] = sum (ml|ns) D[s]_ls (integrals.py:1009-1028)
+void oracle_jk_from_eri(int nao, int nset, const double* eri, const double* D, double* J, double* K) {
+    size_t n = nao;
+    std::memset(J, 0, sizeof(double) * nset * n * n);
+    std::memset(K, 0, sizeof(double) * nset * n * n);
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < (long)n; ++m) {
+        for (size_t nn = 0; nn < n; ++nn) {
+            const double* blk = eri + (m * n + nn) * n * n;
+            for (int s = 0; s < nset; ++s) {
+                const double* Ds = D + s * n * n;
+                double acc = 0.0;
+                for (size_t l = 0; l < n; ++l)
+                    for (size_t q = 0; q < n; ++q) acc += blk[l * n + q] * Ds[l * n + q];
+                J[s * n * n + m * n + nn] = acc;
+                for (size_t l = 0; l < n; ++l) {
+                    double a = 0.0;
+                    for (size_t q = 0; q < n; ++q) a += blk[l * n + q] * Ds[nn * n + q];
+                    K[s * n * n + m * n + l] += a;
+                }
+            }
+        }
+    }
+}
+
+// Integral-direct J/K over the same unique screened block quartets, every
+// element weighted by 1/(its permutational multiplicity) and applied to all 8
+// index permutations; per-chunk partial matrices summed in chunk order.
+void oracle_jk_direct(void* hp, double thr, int nao, int nset, const double* D, double* J, double* K) {
+    OraclePairs* h = (OraclePairs*)hp;
+    const Pairs& P = h->P;
+    const Blocks& B = h->B;
+    size_t n = nao, nn2 = n * n;
+    std::vector<double> partJ((size_t)NCHUNK * nset * nn2, 0.0), partK((size_t)NCHUNK * nset * nn2, 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < NCHUNK; ++c) {
+        if (CHUNK_FIRST[c] >= P.npair) continue;
+        Scratch S(P.lmax, P.maxpp, P.maxrow, 0);
+        double* Jc = partJ.data() + (size_t)c * nset * nn2;
+        double* Kc = partK.data() + (size_t)c * nset * nn2;
+        for_quartets(P, B, h->Q.data(), thr, c, S, [&](int p1, int p2, bool sw, int n1, int n2) {
+            int si = P.pair_blocks[2 * p1], sj = P.pair_blocks[2 * p1 + 1];
+            int sk = P.pair_blocks[2 * p2], sl = P.pair_blocks[2 * p2 + 1];
+            double w = 1.0;
+            if (si == sj) w *= 0.5;
+            if (sk == sl) w *= 0.5;
+            if (p1 == p2) w *= 0.5;
+            block_elements(P, B, p1, p2, n1, n2, sw, S, [&](int m, int nq, int l, int s, double v0) {
+                double v = w * v0;
+                for (int x = 0; x < nset; ++x) {
+                    const double* Dx = D + x * nn2;
+                    double* Jx = Jc + x * nn2;
+                    double* Kx = Kc + x * nn2;
+                    double jls = v * (Dx[l * n + s] + Dx[s * n + l]);
+                    double jmn = v * (Dx[m * n + nq] + Dx[nq * n + m]);
+                    Jx[m * n + nq] += jls; Jx[nq * n + m] += jls;
+                    Jx[l * n + s] += jmn; Jx[s * n + l] += jmn;
+                    Kx[m * n + l] += v * Dx[nq * n + s];
+                    Kx[nq * n + l] += v * Dx[m * n + s];
+                    Kx[m * n + s] += v * Dx[nq * n + l];
+                    Kx[nq * n + s] += v * Dx[m * n + l];
+                    Kx[l * n + m] += v * Dx[s * n + nq];
+                    Kx[s * n + m] += v * Dx[l * n + nq];
+                    Kx[l * n + nq] += v * Dx[s * n + m];
+                    Kx[s * n + nq] += v * Dx[l * n + m];
+                }
+            });
+        });
+    }
+    std::memset(J, 0, sizeof(double) * nset * nn2);
+    std::memset(K, 0, sizeof(double) * nset * nn2);
+    for (int c = 0; c < NCHUNK; ++c)
+        for (size_t i = 0; i < (size_t)nset * nn2; ++i) {
+            J[i] += partJ[(size_t)c * nset * nn2 + i];
+            K[i] += partK[(size_t)c * nset * nn2 + i];
+        }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ ERI gradient
+namespace {
+
+struct GradScratch {
+    Scratch S;
+    std::vector<double> X, Gk, Gkb, Gb, Y, G, Gt;
+    int ldX, nh1max, ncc;
+    GradScratch(const Pairs& P) : S(P.lmax, P.maxpp, 1, 1) {
+        int lmax = P.lmax;
+        int nc = (lmax + 1) * (lmax + 2) / 2;
+        ncc = nc * nc;
+        nh1max = NHERM[2 * lmax + 1];
+        ldX = P.maxrow;
+        X.assign((size_t)nh1max * P.maxrow, 0.0);
+        Gk.assign((size_t)P.maxpph * P.maxrow, 0.0);
+        Gkb.assign((size_t)P.maxpp * ncc * NHERM[2 * lmax], 0.0);
+        Gb.assign((size_t)ncc * P.maxrow, 0.0);
+        Y.assign((size_t)ncc * nh1max, 0.0);
+        G.assign((size_t)P.maxrow * P.maxrow, 0.0);
+        Gt.assign((size_t)P.maxrow * P.maxrow, 0.0);
+    }
+};
+
+// ket_r_contract writing into an explicit X buffer (row stride ldX), as used by the gradient pass
+void ket_r_contract_x(const Pairs& P, i64 bp, int pk, int nh, int L, Scratch& S, double* X, int ldX,
+                      int nhk, int ncd, int nkc) {
+    double p = P.pp_ab[2 * bp] + P.pp_ab[2 * bp + 1];
+    for (int h = 0; h < nh; ++h)
+        for (int col = 0; col < nkc * ncd; ++col) X[h * ldX + col] = 0.0;
+    i64 k0 = P.pair_pp[pk];
+    int mc2 = P.maxcon * P.maxcon;
+    for (int kp = 0; kp < (int)(P.pair_pp[pk + 1] - k0); ++kp) {
+        double q = P.pp_ab[2 * (k0 + kp)] + P.pp_ab[2 * (k0 + kp) + 1];
+        double alpha = p * q / (p + q);
+        double pref = TWO_PI_2_5 / (p * q * std::sqrt(p + q));
+        double x = P.pp_P[3 * bp + 0] - P.pp_P[3 * (k0 + kp) + 0];
+        double y = P.pp_P[3 * bp + 1] - P.pp_P[3 * (k0 + kp) + 1];
+        double z = P.pp_P[3 * bp + 2] - P.pp_P[3 * (k0 + kp) + 2];
+        r_levels(L, alpha, x, y, z, S.RR.data(), S.nR);
+        const double* R0 = S.RR.data();
+        if (nkc == 1) {
+            double pc = pref * P.pp_C[(k0 + kp) * mc2 + 0];
+            for (int h = 0; h < nh; ++h)
+                for (int k = 0; k < nhk; ++k) {
+                    double r = pc * R0[S.hidx[h * S.hidx_ld + k]];
+                    if (r == 0.0) continue;
+                    for (int cd = 0; cd < ncd; ++cd) X[h * ldX + cd] += r * S.ek(kp, k, cd);
+                }
+        } else {
+            for (int h = 0; h < nh; ++h) {
+                for (int cd = 0; cd < ncd; ++cd) S.T[cd] = 0.0;
+                for (int k = 0; k < nhk; ++k) {
+                    double r = pref * R0[S.hidx[h * S.hidx_ld + k]];
+                    if (r == 0.0) continue;
+                    for (int cd = 0; cd < ncd; ++cd) S.T[cd] += r * S.ek(kp, k, cd);
+                }
+                for (int ckl = 0; ckl < nkc; ++ckl) {
+                    double c = P.pp_C[(k0 + kp) * mc2 + ckl];
+                    if (c == 0.0) continue;
+                    int o = ckl * ncd;
+                    for (int cd = 0; cd < ncd; ++cd) X[h * ldX + o + cd] += c * S.T[cd];
+                }
+            }
+        }
+    }
+}
+
+// integrals.py:1049-1244; G has row stride ldG
+void grad_pass(const Pairs& P, const Blocks& B, int pb, int pk, const double* G, int ldG, GradScratch& GS,
+               double out[2][3]) {
+    Scratch& S = GS.S;
+    int si = P.pair_blocks[2 * pb], sj = P.pair_blocks[2 * pb + 1];
+    int sk = P.pair_blocks[2 * pk], sl = P.pair_blocks[2 * pk + 1];
+    int li = B.l[si], lj = B.l[sj], lk = B.l[sk], ll = B.l[sl];
+    int a0 = CART_OFF[li], nci = CART_OFF[li + 1] - a0;
+    int b0 = CART_OFF[lj], ncj = CART_OFF[lj + 1] - b0;
+    int nab, nbc, ncd, nkc;
+    pair_dims(P, B, pb, nab, nbc);
+    pair_dims(P, B, pk, ncd, nkc);
+    int nrow = nbc * nab, ncol = nkc * ncd;
+    int Lab = li + lj;
+    int L = Lab + lk + ll + 1;
+    int nh1 = NHERM[Lab + 1];
+    int nhk = NHERM[lk + ll];
+    bool use_x = ncol < nab;
+    int mc2 = P.maxcon * P.maxcon;
+    hidx_fill(nh1, nhk, S);
+    ket_hermite(P, B, pk, S);
+    i64 k0 = P.pair_pp[pk];
+    int nkp = (int)(P.pair_pp[pk + 1] - k0);
+    double* Gk3 = GS.Gk.data();     // (nkp, nrow, nhk)
+    double* Gkb3 = GS.Gkb.data();   // (nkp, nab, nhk)
+    if (!use_x) {
+        for (int kp = 0; kp < nkp; ++kp)
+            for (int r = 0; r < nrow; ++r) {
+                for (int cd = 0; cd < ncd; ++cd) {
+                    double s = 0.0;
+                    for (int ckl = 0; ckl < nkc; ++ckl) s += P.pp_C[(k0 + kp) * mc2 + ckl] * G[r * ldG + ckl * ncd + cd];
+                    S.T[cd] = s;
+                }
+                for (int k = 0; k < nhk; ++k) {
+                    double s = 0.0;
+                    for (int cd = 0; cd < ncd; ++cd) s += S.T[cd] * S.ek(kp, k, cd);
+                    Gk3[((size_t)kp * nrow + r) * nhk + k] = s;
+                }
+            }
+    }
+    for (int d = 0; d < 3; ++d) { out[0][d] = 0.0; out[1][d] = 0.0; }
+    int ldY = GS.nh1max;
+    int ldGb = P.maxrow;
+    for (i64 bp = P.pair_pp[pb]; bp < P.pair_pp[pb + 1]; ++bp) {
+        double a = P.pp_ab[2 * bp], b = P.pp_ab[2 * bp + 1];
+        double p = a + b;
+        double cb = (nbc == 1) ? P.pp_C[bp * mc2 + 0] : 1.0;
+        const double* Gsrc = nullptr;
+        if (use_x) {
+            ket_r_contract_x(P, bp, pk, nh1, L, S, GS.X.data(), GS.ldX, nhk, ncd, nkc);
+            if (nbc != 1) {
+                for (int ab = 0; ab < nab; ++ab)
+                    for (int col = 0; col < ncol; ++col) GS.Gb[ab * ldGb + col] = 0.0;
+                for (int cij = 0; cij < nbc; ++cij) {
+                    double c = P.pp_C[bp * mc2 + cij];
+                    if (c == 0.0) continue;
+                    for (int ab = 0; ab < nab; ++ab)
+                        for (int col = 0; col < ncol; ++col) GS.Gb[ab * ldGb + col] += c * G[(cij * nab + ab) * ldG + col];
+                }
+            }
+        } else {
+            if (nbc != 1) {
+                for (int kp = 0; kp < nkp; ++kp) {
+                    for (int ab = 0; ab < nab; ++ab)
+                        for (int k = 0; k < nhk; ++k) Gkb3[((size_t)kp * nab + ab) * nhk + k] = 0.0;
+                    for (int cij = 0; cij < nbc; ++cij) {
+                        double c = P.pp_C[bp * mc2 + cij];
+                        if (c == 0.0) continue;
+                        for (int ab = 0; ab < nab; ++ab)
+                            for (int k = 0; k < nhk; ++k)
+                                Gkb3[((size_t)kp * nab + ab) * nhk + k] += c * Gk3[((size_t)kp * nrow + cij * nab + ab) * nhk + k];
+                    }
+                }
+                Gsrc = Gkb3;
+            } else {
+                Gsrc = Gk3;
+            }
+            int srow = (nbc != 1) ? nab : nrow;   // row count of the Gsrc view per kp
+            for (int ab = 0; ab < nab; ++ab)
+                for (int h = 0; h < nh1; ++h) GS.Y[ab * ldY + h] = 0.0;
+            for (int kp = 0; kp < nkp; ++kp) {
+                double q = P.pp_ab[2 * (k0 + kp)] + P.pp_ab[2 * (k0 + kp) + 1];
+                double alpha = p * q / (p + q);
+                double pref = TWO_PI_2_5 / (p * q * std::sqrt(p + q));
+                double x = P.pp_P[3 * bp + 0] - P.pp_P[3 * (k0 + kp) + 0];
+                double y = P.pp_P[3 * bp + 1] - P.pp_P[3 * (k0 + kp) + 1];
+                double z = P.pp_P[3 * bp + 2] - P.pp_P[3 * (k0 + kp) + 2];
+                r_levels(L, alpha, x, y, z, S.RR.data(), S.nR);
+                const double* R0 = S.RR.data();
+                for (int h = 0; h < nh1; ++h)
+                    for (int k = 0; k < nhk; ++k) {
+                        double r = pref * R0[S.hidx[h * S.hidx_ld + k]];
+                        if (r == 0.0) continue;
+                        for (int ab = 0; ab < nab; ++ab)
+                            GS.Y[ab * ldY + h] += r * Gsrc[((size_t)kp * srow + ab) * nhk + k];
+                    }
+            }
+        }
+        E3 E[3] = {P.E(bp, 0), P.E(bp, 1), P.E(bp, 2)};
+        for (int ia = 0; ia < nci; ++ia) {
+            int ax = CART_LXYZ[a0 + ia][0], ay = CART_LXYZ[a0 + ia][1], az = CART_LXYZ[a0 + ia][2];
+            double fa = cb * CART_FAC[a0 + ia];
+            for (int ib = 0; ib < ncj; ++ib) {
+                int bx = CART_LXYZ[b0 + ib][0], by = CART_LXYZ[b0 + ib][1], bz = CART_LXYZ[b0 + ib][2];
+                double f = fa * CART_FAC[b0 + ib];
+                int ab = ia * ncj + ib;
+                const double* grow = (nbc == 1) ? &G[ab * ldG] : &GS.Gb[ab * ldGb];
+                int tx = ax + bx, ty = ay + by, tz = az + bz;
+                for (int t = 0; t < tx + 2; ++t) {
+                    double ex = E[0](ax, bx, t);
+                    double dax = 2.0 * a * E[0](ax + 1, bx, t);
+                    double dbx = 2.0 * b * E[0](ax, bx + 1, t);
+                    if (ax > 0) dax -= ax * E[0](ax - 1, bx, t);
+                    if (bx > 0) dbx -= bx * E[0](ax, bx - 1, t);
+                    for (int u = 0; u < ty + 2; ++u) {
+                        if (t > tx && u > ty) break;
+                        double ey = E[1](ay, by, u);
+                        double day = 2.0 * a * E[1](ay + 1, by, u);
+                        double dby = 2.0 * b * E[1](ay, by + 1, u);
+                        if (ay > 0) day -= ay * E[1](ay - 1, by, u);
+                        if (by > 0) dby -= by * E[1](ay, by - 1, u);
+                        int vmax = (t <= tx && u <= ty) ? tz + 2 : tz + 1;
+                        for (int v = 0; v < vmax; ++v) {
+                            int h = HERM_IDX[t][u][v];
+                            double yv;
+                            if (use_x) {
+                                yv = 0.0;
+                                for (int col = 0; col < ncol; ++col) yv += grow[col] * GS.X[h * GS.ldX + col];
+                            } else {
+                                yv = GS.Y[ab * ldY + h];
+                            }
+                            yv *= f;
+                            if (yv == 0.0) continue;
+                            double ez = E[2](az, bz, v);
+                            double daz = 2.0 * a * E[2](az + 1, bz, v);
+                            double dbz = 2.0 * b * E[2](az, bz + 1, v);
+                            if (az > 0) daz -= az * E[2](az - 1, bz, v);
+                            if (bz > 0) dbz -= bz * E[2](az, bz - 1, v);
+                            out[0][0] += yv * dax * ey * ez;
+                            out[0][1] += yv * ex * day * ez;
+                            out[0][2] += yv * ex * ey * daz;
+                            out[1][0] += yv * dbx * ey * ez;
+                            out[1][1] += yv * ex * dby * ez;
+                            out[1][2] += yv * ex * ey * dbz;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// d/dR of E2 (integrals.py:1247-1369, 1498-1520); Dc (nao,nao), Dx (nx,nao,nao), all symmetric
+void oracle_eri_grad(void* hp, double thr, int nao, int natm, const double* Dc, int nx, const double* Dx,
+                     double kfac, double* grad) {
+    OraclePairs* h = (OraclePairs*)hp;
+    const Pairs& P = h->P;
+    const Blocks& B = h->B;
+    const double* Q = h->Q.data();
+    size_t n = nao;
+    int npair = P.npair;
+    std::vector<double> parts((size_t)NCHUNK * natm * 3, 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int chunk = 0; chunk < NCHUNK; ++chunk) {
+        if (CHUNK_FIRST[chunk] >= npair) continue;
+        GradScratch GS(P);
+        int ldG = P.maxrow;
+        double gab[2][3], gcd[2][3];
+        double* grad_c = parts.data() + (size_t)chunk * natm * 3;
+        for (int pb = CHUNK_FIRST[chunk]; pb < npair; pb += NCHUNK) {
+            int si = P.pair_blocks[2 * pb], sj = P.pair_blocks[2 * pb + 1];
+            int nci = CART_OFF[B.l[si] + 1] - CART_OFF[B.l[si]];
+            int ncj = CART_OFF[B.l[sj] + 1] - CART_OFF[B.l[sj]];
+            int nqi = B.ncon[si], nqj = B.ncon[sj];
+            int nab = nci * ncj;
+            int aA = B.atom[si], aB = B.atom[sj];
+            for (int pk = 0; pk <= pb; ++pk) {
+                int sk = P.pair_blocks[2 * pk], sl = P.pair_blocks[2 * pk + 1];
+                int aC = B.atom[sk], aD = B.atom[sl];
+                if (aA == aB && aB == aC && aC == aD) continue;
+                if (Q[pb] * Q[pk] == 0.0) continue;
+                int nck = CART_OFF[B.l[sk] + 1] - CART_OFF[B.l[sk]];
+                int ncl = CART_OFF[B.l[sl] + 1] - CART_OFF[B.l[sl]];
+                int nqk = B.ncon[sk], nql = B.ncon[sl];
+                int ncd = nck * ncl;
+                int nrow = nqi * nqj * nab;
+                int ncol = nqk * nql * ncd;
+                double deg = 0.5;
+                if (si != sj) deg *= 2.0;
+                if (sk != sl) deg *= 2.0;
+                if (pb != pk) deg *= 2.0;
+                double gmax = 0.0;
+                for (int qi = 0; qi < nqi; ++qi)
+                    for (int a = 0; a < nci; ++a) {
+                        size_t m = B.ao[si] + qi * nci + a;
+                        for (int qj = 0; qj < nqj; ++qj)
+                            for (int b = 0; b < ncj; ++b) {
+                                size_t nn = B.ao[sj] + qj * ncj + b;
+                                int r = (qi * nqj + qj) * nab + a * ncj + b;
+                                for (int qk = 0; qk < nqk; ++qk)
+                                    for (int c = 0; c < nck; ++c) {
+                                        size_t lam = B.ao[sk] + qk * nck + c;
+                                        for (int ql = 0; ql < nql; ++ql)
+                                            for (int d = 0; d < ncl; ++d) {
+                                                size_t sig = B.ao[sl] + ql * ncl + d;
+                                                int s = (qk * nql + ql) * ncd + c * ncl + d;
+                                                double v = Dc[m * n + nn] * Dc[lam * n + sig];
+                                                for (int x = 0; x < nx; ++x) {
+                                                    const double* X = Dx + x * n * n;
+                                                    v -= 0.5 * kfac * (X[m * n + lam] * X[nn * n + sig] + X[m * n + sig] * X[nn * n + lam]);
+                                                }
+                                                v *= deg;
+                                                GS.G[r * ldG + s] = v;
+                                                GS.Gt[s * ldG + r] = v;
+                                                gmax = std::max(gmax, std::fabs(v));
+                                            }
+                                    }
+                            }
+                    }
+                if (Q[pb] * Q[pk] * gmax < thr) continue;
+                bool need_bra = aA != aB;
+                bool need_ket = aC != aD;
+                if (!need_bra && !need_ket) {
+                    if (nrow >= ncol) {
+                        grad_pass(P, B, pb, pk, GS.G.data(), ldG, GS, gab);
+                        for (int d = 0; d < 3; ++d) {
+                            double g = gab[0][d] + gab[1][d];
+                            grad_c[aA * 3 + d] += g;
+                            grad_c[aC * 3 + d] -= g;
+                        }
+                    } else {
+                        grad_pass(P, B, pk, pb, GS.Gt.data(), ldG, GS, gcd);
+                        for (int d = 0; d < 3; ++d) {
+                            double g = gcd[0][d] + gcd[1][d];
+                            grad_c[aC * 3 + d] += g;
+                            grad_c[aA * 3 + d] -= g;
+                        }
+                    }
+                    continue;
+                }
+                if (need_bra) grad_pass(P, B, pb, pk, GS.G.data(), ldG, GS, gab);
+                if (need_ket) grad_pass(P, B, pk, pb, GS.Gt.data(), ldG, GS, gcd);
+                for (int d = 0; d < 3; ++d) {
+                    if (need_bra && need_ket) {
+                        grad_c[aA * 3 + d] += gab[0][d];
+                        grad_c[aB * 3 + d] += gab[1][d];
+                        grad_c[aC * 3 + d] += gcd[0][d];
+                        grad_c[aD * 3 + d] += gcd[1][d];
+                    } else if (need_bra) {
+                        grad_c[aA * 3 + d] += gab[0][d];
+                        grad_c[aB * 3 + d] += gab[1][d];
+                        grad_c[aC * 3 + d] -= gab[0][d] + gab[1][d];
+                    } else {
+                        grad_c[aC * 3 + d] += gcd[0][d];
+                        grad_c[aD * 3 + d] += gcd[1][d];
+                        grad_c[aA * 3 + d] -= gcd[0][d] + gcd[1][d];
+                    }
+                }
+            }
+        }
+    }
+    for (int i = 0; i < natm * 3; ++i) grad[i] = 0.0;
+    for (int c = 0; c < NCHUNK; ++c)
+        for (int i = 0; i < natm * 3; ++i) grad[i] += parts[(size_t)c * natm * 3 + i];
+}
+
+}  // extern "C"
