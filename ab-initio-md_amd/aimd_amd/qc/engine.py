@@ -1,0 +1,106 @@
+"""
+Device-side integral engine: one plan per basis, one geometry at a time.
+
+Wraps the C ABI (include/aimdhf.h) for the Python host layer. Matrices live
+in PyTorch tensors on the current HIP device (float64, row-major); every
+call is enqueued on torch's current stream.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from aimd_amd import _lib
+from aimd_amd.qc.basis import BasisSet
+
+DEFAULT_SCHWARZ = 1e-12          # aimd/qc/integrals.py:114
+DEFAULT_SCHWARZ_GRAD = 1e-14     # aimd/qc/integrals.py:119
+
+
+def _ptr(a) -> ctypes.c_void_p:
+    if isinstance(a, torch.Tensor):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def require_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("the aimd_amd engine needs a ROCm GPU (torch.cuda.is_available() is False); "
+                           "there is no CPU fallback")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class Engine:
+    """Integral plan of one basis on the current device."""
+
+    def __init__(self, basis: BasisSet) -> None:
+        self.device = require_device()
+        L = _lib.lib()
+        self.basis = basis
+        self.nao = basis.nao
+        self.natm = basis.natm
+        self._keep = [np.ascontiguousarray(a) for a in (
+            basis.shell_atom.astype(np.int64), basis.shell_l.astype(np.int64),
+            basis.shell_prim.astype(np.int64), basis.shell_block.astype(np.int64),
+            basis.prim_exp.astype(np.float64), basis.prim_coef.astype(np.float64),
+            basis.nuclear_charges.astype(np.float64))]
+        h = ctypes.c_void_p()
+        _lib.check(L.aimdhf_plan_create(self.natm, basis.nshell, *[_ptr(a) for a in self._keep],
+                                        ctypes.byref(h)))
+        self._h = h
+        self.positions: np.ndarray | None = None
+        self.threshold: float | None = None
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().aimdhf_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ geometry
+    def set_geometry(self, positions, schwarz_threshold: float = DEFAULT_SCHWARZ) -> None:
+        x = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(self.natm, 3))
+        _lib.check(_lib.lib().aimdhf_set_geometry(self._h, _ptr(x), float(schwarz_threshold), _stream()))
+        self.positions = x.copy()
+        self.threshold = float(schwarz_threshold)
+
+    def schwarz_bounds(self) -> np.ndarray:
+        nb = self.basis.nblock
+        out = np.zeros(nb * (nb + 1) // 2)
+        _lib.check(_lib.lib().aimdhf_schwarz_bounds(self._h, _ptr(out), out.size))
+        return out
+
+    def quartet_count(self) -> int:
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().aimdhf_quartet_count(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def quartets(self) -> np.ndarray:
+        """Surviving unique shell quartets as (P, Q) shell-pair indices, P >= Q."""
+        n = self.quartet_count()
+        out = np.zeros((max(n, 1), 2), np.int64)
+        cnt = ctypes.c_int64()
+        _lib.check(_lib.lib().aimdhf_quartets(self._h, _ptr(out), n, ctypes.byref(cnt)))
+        return out[:n]
+
+    # ---------------------------------------------------------------- J / K
+    def jk(self, dens: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+        """J, K of (nset, nao, nao) (or (nao, nao)) densities; symmetric part of each D is used."""
+        single = dens.dim() == 2
+        D = dens.reshape(-1, self.nao, self.nao).to(device=self.device, dtype=torch.float64).contiguous()
+        J = torch.empty_like(D)
+        K = torch.empty_like(D)
+        _lib.check(_lib.lib().aimdhf_jk(self._h, D.shape[0], _ptr(D), _ptr(J), _ptr(K), _stream()))
+        return (J[0], K[0]) if single else (J, K)

@@ -1,0 +1,416 @@
+// Host side of the C ABI (include/aimdhf.h): plan state, device buffers,
+// launch sequencing. Compiled with hipcc together with the kernels.
+
+#include "../../include/aimdhf.h"
+#include "aimdhf_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+struct HipError {
+    int code;
+};
+
+#define HIPCHECK(x)                                                                  \
+    do {                                                                             \
+        hipError_t e_ = (x);                                                         \
+        if (e_ != hipSuccess) {                                                      \
+            g_err = std::string(#x) + ": " + hipGetErrorString(e_);                  \
+            throw HipError{e_ == hipErrorOutOfMemory ? AIMDHF_E_NOMEM : AIMDHF_E_HIP}; \
+        }                                                                            \
+    } while (0)
+
+// Device buffer that grows on demand and is freed with the plan.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        if (p) HIPCHECK(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        HIPCHECK(hipMalloc(&p, std::max<size_t>(n, 256)));
+        bytes = std::max<size_t>(n, 256);
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& v) {
+    b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (!v.empty()) HIPCHECK(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+}
+
+int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+constexpr int NQCLASS = 21;
+constexpr int REC_HDR = 8;
+
+// Boys grid table F_n(k/10), n < 43 (aimd/qc/boys.py:57-75): series for the
+// highest order, then the downward recursion.
+std::vector<double> boys_table() {
+    const int NG = 402, NR = 43, top = NR - 1;
+    std::vector<double> tab((size_t)NG * NR);
+    std::vector<double> col(NR);
+    for (int k = 0; k < NG; ++k) {
+        double x = k * 0.1;
+        double term = 1.0 / (2 * top + 1), total = term;
+        for (int m = 1; m < 400; ++m) {
+            term = term * (2.0 * x) / (double)(2 * top + 2 * m + 1);
+            total += term;
+        }
+        double ex = std::exp(-x);
+        col[top] = ex * total;
+        for (int n = top - 1; n >= 0; --n) col[n] = (2.0 * x * col[n + 1] + ex) / (double)(2 * n + 1);
+        for (int n = 0; n < NR; ++n) tab[(size_t)k * NR + n] = col[n];
+    }
+    return tab;
+}
+
+}  // namespace
+
+struct aimdhf_plan {
+    int natm = 0, nshell = 0, nao = 0, npair = 0, nblock = 0, nbpair = 0, lmax = 0;
+    std::vector<int> shell_atom, shell_l, shell_prim, shell_ao, shell_block;
+    std::vector<double> prim_exp, prim_coef, charges, pos;
+    std::vector<int> pair_a, pair_b, pair_cls, rec_size, cls_pairs, cls_pair_off, ncls;
+    std::vector<long long> rec_off;
+    std::vector<int> bp_off, bp_list;
+    long long rec_total = 0;
+    bool have_geometry = false;
+    double threshold = 0.0;
+    std::vector<long long> qcount;   // per class
+    long long qtotal = 0;
+
+    DevBuf d_shell_atom, d_shell_l, d_shell_prim, d_shell_ao, d_prim_exp, d_prim_coef, d_pos;
+    DevBuf d_pair_a, d_pair_b, d_pair_cls, d_rec_size, d_rec_off, d_cls_pairs, d_cls_pair_off, d_ncls;
+    DevBuf d_bp_off, d_bp_list, d_boys;
+    DevBuf d_recs, d_npp, d_qshell, d_qblock, d_qscr, d_counts, d_offsets, d_list;
+    DevBuf d_fx, d_dsym;
+
+    ~aimdhf_plan() {
+        for (DevBuf* b : {&d_shell_atom, &d_shell_l, &d_shell_prim, &d_shell_ao, &d_prim_exp, &d_prim_coef,
+                          &d_pos, &d_pair_a, &d_pair_b, &d_pair_cls, &d_rec_size, &d_rec_off, &d_cls_pairs,
+                          &d_cls_pair_off, &d_ncls, &d_bp_off, &d_bp_list, &d_boys, &d_recs, &d_npp,
+                          &d_qshell, &d_qblock, &d_qscr, &d_counts, &d_offsets, &d_list, &d_fx, &d_dsym})
+            b->release();
+    }
+
+    aimdhf::PairArgs pair_args() const {
+        aimdhf::PairArgs g;
+        g.npair = npair;
+        g.nbpair = nbpair;
+        g.pair_a = d_pair_a.as<int>();
+        g.pair_b = d_pair_b.as<int>();
+        g.pair_cls = d_pair_cls.as<int>();
+        g.shell_l = d_shell_l.as<int>();
+        g.shell_atom = d_shell_atom.as<int>();
+        g.shell_prim = d_shell_prim.as<int>();
+        g.prim_exp = d_prim_exp.as<double>();
+        g.prim_coef = d_prim_coef.as<double>();
+        g.pos = d_pos.as<double>();
+        g.rec_off = d_rec_off.as<long long>();
+        g.rec_size = d_rec_size.as<int>();
+        g.recs = d_recs.as<double>();
+        g.npp = d_npp.as<int>();
+        g.cls_pairs = d_cls_pairs.as<int>();
+        g.cls_pair_off = d_cls_pair_off.as<int>();
+        g.ncls_pairs = d_ncls.as<int>();
+        g.qshell = d_qshell.as<double>();
+        g.qblock = d_qblock.as<double>();
+        g.qscr = d_qscr.as<double>();
+        g.bp_off = d_bp_off.as<int>();
+        g.bp_list = d_bp_list.as<int>();
+        g.boys = d_boys.as<double>();
+        return g;
+    }
+
+    const int2* list_of_class(int k) const {
+        long long off = 0;
+        for (int i = 0; i < k; ++i) off += qcount[i];
+        return d_list.as<int2>() + off;
+    }
+};
+
+static int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
+
+extern "C" {
+
+int aimdhf_last_error(char* buf, size_t len) {
+    if (buf && len) {
+        std::strncpy(buf, g_err.c_str(), len - 1);
+        buf[len - 1] = 0;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_lmax(void) { return 2; }
+
+int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const int64_t* shell_l,
+                       const int64_t* shell_prim, const int64_t* shell_block, const double* prim_exp,
+                       const double* prim_coef, const double* charges, aimdhf_plan** out) {
+    if (!out) return fail(AIMDHF_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (natm <= 0 || nshell <= 0) return fail(AIMDHF_E_ARG, "natm and nshell must be positive");
+    if (!shell_atom || !shell_l || !shell_prim || !shell_block || !prim_exp || !prim_coef || !charges)
+        return fail(AIMDHF_E_ARG, "NULL basis array");
+    aimdhf_plan* p = new (std::nothrow) aimdhf_plan();
+    if (!p) return fail(AIMDHF_E_NOMEM, "host allocation failed");
+    try {
+        p->natm = natm;
+        p->nshell = nshell;
+        int nprim = (int)shell_prim[nshell];
+        p->shell_atom.resize(nshell);
+        p->shell_l.resize(nshell);
+        p->shell_prim.resize(nshell + 1);
+        p->shell_block.resize(nshell);
+        p->shell_ao.resize(nshell + 1);
+        p->shell_ao[0] = 0;
+        for (int s = 0; s < nshell; ++s) {
+            if (shell_l[s] < 0 || shell_l[s] > 2) {
+                delete p;
+                return fail(AIMDHF_E_UNSUPPORTED, "angular momentum l = " + std::to_string(shell_l[s]) +
+                                                      " > 2 is not supported by the gfx950 kernels of this build");
+            }
+            if (shell_atom[s] < 0 || shell_atom[s] >= natm || shell_prim[s + 1] <= shell_prim[s]) {
+                delete p;
+                return fail(AIMDHF_E_ARG, "inconsistent shell arrays");
+            }
+            p->shell_atom[s] = (int)shell_atom[s];
+            p->shell_l[s] = (int)shell_l[s];
+            p->shell_prim[s] = (int)shell_prim[s];
+            p->shell_block[s] = (int)shell_block[s];
+            p->shell_ao[s + 1] = p->shell_ao[s] + ncart(p->shell_l[s]);
+            p->lmax = std::max(p->lmax, p->shell_l[s]);
+        }
+        p->shell_prim[nshell] = nprim;
+        p->nao = p->shell_ao[nshell];
+        p->nblock = p->shell_block[nshell - 1] + 1;
+        p->prim_exp.assign(prim_exp, prim_exp + nprim);
+        p->prim_coef.assign(prim_coef, prim_coef + nprim);
+        p->charges.assign(charges, charges + natm);
+        // shell pairs (s1 >= s2) in the reference's pair order, oriented l_A >= l_B
+        p->npair = nshell * (nshell + 1) / 2;
+        p->pair_a.resize(p->npair);
+        p->pair_b.resize(p->npair);
+        p->pair_cls.resize(p->npair);
+        p->rec_size.resize(p->npair);
+        p->rec_off.resize(p->npair + 1);
+        long long off = 0;
+        int P = 0;
+        std::vector<std::vector<int>> bylcls(6);
+        p->nbpair = p->nblock * (p->nblock + 1) / 2;
+        std::vector<std::vector<int>> bpl(p->nbpair);
+        for (int s1 = 0; s1 < nshell; ++s1)
+            for (int s2 = 0; s2 <= s1; ++s2, ++P) {
+                int a = s1, b = s2;
+                if (p->shell_l[s2] > p->shell_l[s1]) std::swap(a, b);
+                int la = p->shell_l[a], lb = p->shell_l[b];
+                p->pair_a[P] = a;
+                p->pair_b[P] = b;
+                p->pair_cls[P] = pair_class(la, lb);
+                int rs = REC_HDR + nherm(la + lb) * ncart(la) * ncart(lb);
+                p->rec_size[P] = rs;
+                p->rec_off[P] = off;
+                off += (long long)rs * (shell_prim[s1 + 1] - shell_prim[s1]) * (shell_prim[s2 + 1] - shell_prim[s2]);
+                bylcls[p->pair_cls[P]].push_back(P);
+                int b1 = p->shell_block[s1], b2 = p->shell_block[s2];
+                bpl[b1 * (b1 + 1) / 2 + b2].push_back(P);
+            }
+        p->rec_off[p->npair] = off;
+        p->rec_total = off;
+        p->cls_pair_off.assign(7, 0);
+        p->ncls.assign(6, 0);
+        for (int c = 0; c < 6; ++c) {
+            p->cls_pair_off[c + 1] = p->cls_pair_off[c] + (int)bylcls[c].size();
+            p->ncls[c] = (int)bylcls[c].size();
+            p->cls_pairs.insert(p->cls_pairs.end(), bylcls[c].begin(), bylcls[c].end());
+        }
+        p->bp_off.assign(p->nbpair + 1, 0);
+        for (int i = 0; i < p->nbpair; ++i) {
+            p->bp_off[i + 1] = p->bp_off[i] + (int)bpl[i].size();
+            p->bp_list.insert(p->bp_list.end(), bpl[i].begin(), bpl[i].end());
+        }
+        // device copies
+        upload(p->d_shell_atom, p->shell_atom);
+        upload(p->d_shell_l, p->shell_l);
+        upload(p->d_shell_prim, p->shell_prim);
+        upload(p->d_shell_ao, p->shell_ao);
+        upload(p->d_prim_exp, p->prim_exp);
+        upload(p->d_prim_coef, p->prim_coef);
+        upload(p->d_pair_a, p->pair_a);
+        upload(p->d_pair_b, p->pair_b);
+        upload(p->d_pair_cls, p->pair_cls);
+        upload(p->d_rec_size, p->rec_size);
+        upload(p->d_rec_off, p->rec_off);
+        upload(p->d_cls_pairs, p->cls_pairs);
+        upload(p->d_cls_pair_off, p->cls_pair_off);
+        upload(p->d_ncls, p->ncls);
+        upload(p->d_bp_off, p->bp_off);
+        upload(p->d_bp_list, p->bp_list);
+        upload(p->d_boys, boys_table());
+        p->d_pos.ensure(sizeof(double) * 3 * natm);
+        p->d_recs.ensure(sizeof(double) * std::max<long long>(off, 1));
+        p->d_npp.ensure(sizeof(int) * p->npair);
+        p->d_qshell.ensure(sizeof(double) * p->npair);
+        p->d_qscr.ensure(sizeof(double) * p->npair);
+        p->d_qblock.ensure(sizeof(double) * p->nbpair);
+        p->d_counts.ensure(sizeof(int) * (size_t)p->npair * NQCLASS);
+        p->d_offsets.ensure(sizeof(long long) * (size_t)p->npair * NQCLASS);
+        p->qcount.assign(NQCLASS, 0);
+    } catch (const HipError& e) {
+        delete p;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        delete p;
+        return fail(AIMDHF_E_NOMEM, "host allocation failed");
+    }
+    *out = p;
+    return AIMDHF_OK;
+}
+
+int aimdhf_plan_destroy(aimdhf_plan* plan) {
+    delete plan;
+    return AIMDHF_OK;
+}
+
+int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, void* stream) {
+    if (!p || !positions) return fail(AIMDHF_E_ARG, "NULL plan or positions");
+    for (int i = 0; i < 3 * p->natm; ++i)
+        if (!std::isfinite(positions[i])) return fail(AIMDHF_E_ARG, "atomic positions must be finite (got NaN or inf)");
+    if (!(thr >= 0.0)) return fail(AIMDHF_E_ARG, "schwarz threshold must be >= 0");
+    hipStream_t st = (hipStream_t)stream;
+    try {
+        p->pos.assign(positions, positions + 3 * p->natm);
+        HIPCHECK(hipMemcpyAsync(p->d_pos.p, positions, sizeof(double) * 3 * p->natm, hipMemcpyHostToDevice, st));
+        aimdhf::PairArgs g = p->pair_args();
+        aimdhf::launch_build_pairs(g, st);
+        aimdhf::launch_schwarz(g, p->ncls.data(), st);
+        aimdhf::launch_count_quartets(g, thr, p->d_counts.as<int>(), st);
+        HIPCHECK(hipGetLastError());
+        std::vector<int> counts((size_t)p->npair * NQCLASS);
+        HIPCHECK(hipMemcpyAsync(counts.data(), p->d_counts.p, sizeof(int) * counts.size(), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        std::vector<long long> tot(NQCLASS, 0);
+        for (int P = 0; P < p->npair; ++P)
+            for (int k = 0; k < NQCLASS; ++k) tot[k] += counts[(size_t)P * NQCLASS + k];
+        std::vector<long long> base(NQCLASS + 1, 0);
+        for (int k = 0; k < NQCLASS; ++k) base[k + 1] = base[k] + tot[k];
+        std::vector<long long> offs((size_t)p->npair * NQCLASS);
+        std::vector<long long> run(base.begin(), base.end() - 1);
+        for (int P = 0; P < p->npair; ++P)
+            for (int k = 0; k < NQCLASS; ++k) {
+                offs[(size_t)P * NQCLASS + k] = run[k];
+                run[k] += counts[(size_t)P * NQCLASS + k];
+            }
+        p->qcount = tot;
+        p->qtotal = base[NQCLASS];
+        p->d_list.ensure(sizeof(int2) * std::max<long long>(p->qtotal, 1));
+        HIPCHECK(hipMemcpyAsync(p->d_offsets.p, offs.data(), sizeof(long long) * offs.size(), hipMemcpyHostToDevice, st));
+        g = p->pair_args();
+        aimdhf::launch_fill_quartets(g, thr, p->d_offsets.as<long long>(), p->d_list.as<int2>(), st);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(st));
+        p->threshold = thr;
+        p->have_geometry = true;
+    } catch (const HipError& e) {
+        p->have_geometry = false;
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_schwarz_bounds(aimdhf_plan* p, double* out, int64_t cap) {
+    if (!p || !out) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    if (cap < p->nbpair) return fail(AIMDHF_E_ARG, "output too small");
+    try {
+        HIPCHECK(hipMemcpy(out, p->d_qblock.p, sizeof(double) * p->nbpair, hipMemcpyDeviceToHost));
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_quartet_count(aimdhf_plan* p, int64_t* count) {
+    if (!p || !count) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    *count = p->qtotal;
+    return AIMDHF_OK;
+}
+
+int aimdhf_quartets(aimdhf_plan* p, int64_t* out, int64_t cap, int64_t* count) {
+    if (!p || !count) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    *count = p->qtotal;
+    if (!out || cap <= 0) return AIMDHF_OK;
+    try {
+        long long n = std::min<long long>(cap, p->qtotal);
+        std::vector<int2> tmp(n);
+        if (n) HIPCHECK(hipMemcpy(tmp.data(), p->d_list.p, sizeof(int2) * n, hipMemcpyDeviceToHost));
+        for (long long i = 0; i < n; ++i) {
+            int a = tmp[i].x, b = tmp[i].y;
+            out[2 * i] = std::max(a, b);
+            out[2 * i + 1] = std::min(a, b);
+        }
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K, void* stream) {
+    if (!p || !dens || !J || !K) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (nset < 1 || nset > 8) return fail(AIMDHF_E_ARG, "nset must be in 1..8");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    hipStream_t st = (hipStream_t)stream;
+    try {
+        size_t nn = (size_t)p->nao * p->nao;
+        p->d_fx.ensure(sizeof(long long) * 4 * nset * nn);
+        p->d_dsym.ensure(sizeof(double) * nset * nn);
+        aimdhf::launch_zero64(p->d_fx.as<long long>(), 4 * nset * nn, st);
+        aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st);
+        aimdhf::JKArgs a;
+        a.nao = p->nao;
+        a.nset = nset;
+        a.pair_a = p->d_pair_a.as<int>();
+        a.pair_b = p->d_pair_b.as<int>();
+        a.shell_ao = p->d_shell_ao.as<int>();
+        a.npp = p->d_npp.as<int>();
+        a.rec_off = p->d_rec_off.as<long long>();
+        a.recs = p->d_recs.as<double>();
+        a.boys = p->d_boys.as<double>();
+        a.dens = p->d_dsym.as<double>();
+        a.fx = p->d_fx.as<long long>();
+        const int2* lists[NQCLASS];
+        for (int k = 0; k < NQCLASS; ++k) lists[k] = p->list_of_class(k);
+        aimdhf::launch_jk(a, lists, p->qcount.data(), st);
+        aimdhf::launch_jk_finalize(p->d_fx.as<long long>(), nset, p->nao, J, K, st);
+        HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+}  // extern "C"

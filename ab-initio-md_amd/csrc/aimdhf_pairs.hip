@@ -1,0 +1,202 @@
+// gfx950 kernels of the Hartree-Fock force engine: primitive-pair build,
+// Schwarz bounds, screened quartet lists, and the integral-direct J/K build.
+//
+// One thread evaluates one contracted shell quartet (class-templated, l <= 2)
+// and digests it straight from registers into J and K, accumulated as exact
+// fixed-point integers (md_core.h: fx_split), so the Fock build is bitwise
+// reproducible from run to run whatever order the waves execute in.
+
+#include "aimdhf_internal.h"
+#include "md_eri.h"
+#include "md_pairs.h"
+
+namespace aimdhf {
+using namespace md;
+
+// ------------------------------------------------------------- pair build
+// Primitive pairs kept iff mu |A-B|^2 <= 50 (aimd/qc/integrals.py:504-599).
+// The test is evaluated without contractions so it is bit-identical to the
+// CPU restatement.
+__device__ __noinline__ bool keep_prim_pair(double a, double b, double r2) {
+#pragma clang fp contract(off)
+    return a * b / (a + b) * r2 <= 50.0;
+}
+
+__device__ __noinline__ double dist2(const double* A, const double* B) {
+#pragma clang fp contract(off)
+    double r2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        double x = A[d] - B[d];
+        r2 += x * x;
+    }
+    return r2;
+}
+
+// thread per shell pair: npp[P] and the records of the kept primitive pairs
+__global__ void k_build_pairs(PairArgs g) {
+    int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= g.npair) return;
+    int sA = g.pair_a[P], sB = g.pair_b[P];
+    int la = g.shell_l[sA], lb = g.shell_l[sB];
+    const double* A = g.pos + 3 * g.shell_atom[sA];
+    const double* B = g.pos + 3 * g.shell_atom[sB];
+    double r2 = dist2(A, B);
+    double* rec = g.recs + g.rec_off[P];
+    int rs = g.rec_size[P];
+    int n = 0;
+    for (int ka = g.shell_prim[sA]; ka < g.shell_prim[sA + 1]; ++ka) {
+        double a = g.prim_exp[ka];
+        for (int kb = g.shell_prim[sB]; kb < g.shell_prim[sB + 1]; ++kb) {
+            double b = g.prim_exp[kb];
+            if (!keep_prim_pair(a, b, r2)) continue;
+            double ca = g.prim_coef[ka], cb = g.prim_coef[kb];
+            double* r = rec + n * rs;
+            write_record_any(la, lb, a, b, ca, cb, A, B, r);
+            ++n;
+        }
+    }
+    g.npp[P] = n;
+}
+
+// ---------------------------------------------------------------- Schwarz
+template <int LA, int LB>
+__device__ double schwarz_diag(const double* rec, int n, const double* tab) {
+    using Qt = Quartet<LA, LB, LA, LB>;
+    double m = 0.0;
+    if constexpr (split_by_cd<LA, LB, LA, LB>()) {
+        for (int cd = 0; cd < Qt::NCD; ++cd) {
+            double acc[Qt::NAB];
+            eri_quartet_cd<LA, LB, LA, LB>(rec, n, rec, n, tab, cd, acc);
+            m = fmax(m, fabs(acc[cd]));
+        }
+    } else {
+        double acc[Qt::NAB * Qt::NCD];
+        eri_quartet<LA, LB, LA, LB>(rec, n, rec, n, tab, acc);
+        for (int ab = 0; ab < Qt::NAB; ++ab) m = fmax(m, fabs(acc[ab * Qt::NCD + ab]));
+    }
+    return sqrt(m);
+}
+
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_schwarz(PairArgs g) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.ncls_pairs[pair_class(LA, LB)]) return;
+    int P = g.cls_pairs[g.cls_pair_off[pair_class(LA, LB)] + i];
+    g.qshell[P] = schwarz_diag<LA, LB>(g.recs + g.rec_off[P], g.npp[P], g.boys);
+}
+
+// block-pair bound = max over its shell pairs; every shell pair screens with it
+__global__ void k_block_q(PairArgs g) {
+    int bp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bp >= g.nbpair) return;
+    double m = 0.0;
+    for (int i = g.bp_off[bp]; i < g.bp_off[bp + 1]; ++i) m = fmax(m, g.qshell[g.bp_list[i]]);
+    g.qblock[bp] = m;
+    for (int i = g.bp_off[bp]; i < g.bp_off[bp + 1]; ++i) g.qscr[g.bp_list[i]] = m;
+}
+
+// ------------------------------------------------------------ quartet lists
+// Unique shell quartets (P >= Q) kept iff not (q_P q_Q < thr), the test of
+// aimd/qc/integrals.py:902; binned by class with the heavier pair as bra.
+__device__ __forceinline__ int qclass(int cp, int cq) {
+    int hi = cp > cq ? cp : cq, lo = cp > cq ? cq : cp;
+    return hi * (hi + 1) / 2 + lo;
+}
+
+__global__ void k_count_quartets(PairArgs g, double thr, int* counts) {
+    int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= g.npair) return;
+    int c[NQCLASS];
+    for (int i = 0; i < NQCLASS; ++i) c[i] = 0;
+    double qp = g.qscr[P];
+    int cp = g.pair_cls[P];
+    for (int Q = 0; Q <= P; ++Q) {
+        if (qp * g.qscr[Q] < thr) continue;
+        c[qclass(cp, g.pair_cls[Q])]++;
+    }
+    for (int i = 0; i < NQCLASS; ++i) counts[(size_t)P * NQCLASS + i] = c[i];
+}
+
+__global__ void k_fill_quartets(PairArgs g, double thr, const long long* offsets, int2* list) {
+    int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= g.npair) return;
+    long long o[NQCLASS];
+    for (int i = 0; i < NQCLASS; ++i) o[i] = offsets[(size_t)P * NQCLASS + i];
+    double qp = g.qscr[P];
+    int cp = g.pair_cls[P];
+    for (int Q = 0; Q <= P; ++Q) {
+        if (qp * g.qscr[Q] < thr) continue;
+        int cq = g.pair_cls[Q];
+        int k = qclass(cp, cq);
+        list[o[k]++] = cp >= cq ? make_int2(P, Q) : make_int2(Q, P);
+    }
+}
+
+// J = J' + J'^T, K = K' + K'^T from the fixed-point accumulators
+__global__ void k_jk_finalize(const long long* fx, int nset, int n, double* J, double* K) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t nn = (size_t)n * n;
+    if (i >= (size_t)nset * nn) return;
+    size_t s = i / nn, r = (i % nn) / n, c = i % n;
+    size_t ij = r * n + c, ji = c * n + r;
+    const long long* Jh = fx + (4 * s + 0) * nn;
+    const long long* Jl = fx + (4 * s + 1) * nn;
+    const long long* Kh = fx + (4 * s + 2) * nn;
+    const long long* Kl = fx + (4 * s + 3) * nn;
+    J[i] = fx_value(Jh[ij] + Jh[ji], Jl[ij] + Jl[ji]);
+    K[i] = fx_value(Kh[ij] + Kh[ji], Kl[ij] + Kl[ji]);
+}
+
+__global__ void k_symmetrize(const double* D, int nset, int n, double* out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t nn = (size_t)n * n;
+    if (i >= (size_t)nset * nn) return;
+    size_t s = i / nn, r = (i % nn) / n, c = i % n;
+    out[i] = 0.5 * (D[s * nn + r * n + c] + D[s * nn + c * n + r]);
+}
+
+__global__ void k_zero64(long long* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------- launchers
+#define GRID(n, b) dim3((unsigned)(((n) + (b) - 1) / (b))), dim3(b)
+
+void launch_build_pairs(const PairArgs& g, hipStream_t st) {
+    if (g.npair) k_build_pairs<<<GRID(g.npair, 64), 0, st>>>(g);
+}
+
+void launch_schwarz(const PairArgs& g, const int* ncls_host, hipStream_t st) {
+    if (ncls_host[0]) k_schwarz<0, 0><<<GRID(ncls_host[0], 64), 0, st>>>(g);
+    if (ncls_host[1]) k_schwarz<1, 0><<<GRID(ncls_host[1], 64), 0, st>>>(g);
+    if (ncls_host[2]) k_schwarz<1, 1><<<GRID(ncls_host[2], 64), 0, st>>>(g);
+    if (ncls_host[3]) k_schwarz<2, 0><<<GRID(ncls_host[3], 64), 0, st>>>(g);
+    if (ncls_host[4]) k_schwarz<2, 1><<<GRID(ncls_host[4], 64), 0, st>>>(g);
+    if (ncls_host[5]) k_schwarz<2, 2><<<GRID(ncls_host[5], 64), 0, st>>>(g);
+    if (g.nbpair) k_block_q<<<GRID(g.nbpair, 64), 0, st>>>(g);
+}
+
+void launch_count_quartets(const PairArgs& g, double thr, int* counts, hipStream_t st) {
+    if (g.npair) k_count_quartets<<<GRID(g.npair, 64), 0, st>>>(g, thr, counts);
+}
+
+void launch_fill_quartets(const PairArgs& g, double thr, const long long* offsets, int2* list, hipStream_t st) {
+    if (g.npair) k_fill_quartets<<<GRID(g.npair, 64), 0, st>>>(g, thr, offsets, list);
+}
+
+void launch_zero64(long long* p, size_t n, hipStream_t st) {
+    if (n) k_zero64<<<GRID(n, 256), 0, st>>>(p, n);
+}
+
+void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st) {
+    size_t m = (size_t)nset * n * n;
+    if (m) k_symmetrize<<<GRID(m, 256), 0, st>>>(D, nset, n, out);
+}
+
+void launch_jk_finalize(const long long* fx, int nset, int n, double* J, double* K, hipStream_t st) {
+    size_t m = (size_t)nset * n * n;
+    if (m) k_jk_finalize<<<GRID(m, 256), 0, st>>>(fx, nset, n, J, K);
+}
+
+}  // namespace aimdhf

@@ -1,0 +1,22 @@
+// Launches the 21 per-class J/K kernels (jk_class.hip, one object per class).
+#include "aimdhf_internal.h"
+
+namespace aimdhf {
+
+#define DECL(k) void launch_jk_class_##k(const JKArgs&, const int2*, long long, hipStream_t);
+DECL(0) DECL(1) DECL(2) DECL(3) DECL(4) DECL(5) DECL(6) DECL(7) DECL(8) DECL(9) DECL(10)
+DECL(11) DECL(12) DECL(13) DECL(14) DECL(15) DECL(16) DECL(17) DECL(18) DECL(19) DECL(20)
+#undef DECL
+
+void launch_jk(const JKArgs& a, const int2* const* lists, const long long* counts, hipStream_t st) {
+    typedef void (*Fn)(const JKArgs&, const int2*, long long, hipStream_t);
+    static const Fn fns[21] = {
+        launch_jk_class_0, launch_jk_class_1, launch_jk_class_2, launch_jk_class_3, launch_jk_class_4,
+        launch_jk_class_5, launch_jk_class_6, launch_jk_class_7, launch_jk_class_8, launch_jk_class_9,
+        launch_jk_class_10, launch_jk_class_11, launch_jk_class_12, launch_jk_class_13, launch_jk_class_14,
+        launch_jk_class_15, launch_jk_class_16, launch_jk_class_17, launch_jk_class_18, launch_jk_class_19,
+        launch_jk_class_20};
+    for (int k = 0; k < 21; ++k) fns[k](a, lists[k], counts[k], st);
+}
+
+}  // namespace aimdhf

@@ -1,0 +1,236 @@
+// McMurchie-Davidson building blocks shared by the gfx950 kernels.
+//
+// Everything here is __host__ __device__ so that the host-side check harness
+// (host_check.cpp, test-only) evaluates exactly the arithmetic the kernels do.
+//
+//   Boys function      F_n(T): 402 x 43 grid table, 10-term Taylor expansion
+//                      about the nearest grid point for T < 40, asymptotic
+//                      F_0 and upward recursion above (the scheme of
+//                      aimd/qc/boys.py:49-140); lower orders by the downward
+//                      recursion F_n = (2T F_{n+1} + e^{-T}) / (2n+1).
+//   Hermite R          R^n_{tuv}(alpha, PQ) levels n = L..0 updated in place,
+//                      highest degree first (aimd/qc/hermite.py:144-178).
+//   Hermite E          1-D E^{ij}_t recursion (aimd/qc/hermite.py:109-141).
+//
+// Hermite indices are linearised by degree exactly as the reference does
+// (aimd/qc/hermite.py:73-105): within degree d, t from d down to 0, then u
+// from d-t down to 0. hidx() computes that position arithmetically so that
+// fully unrolled loops index registers with compile-time constants.
+
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+#define MD_HD __host__ __device__ __forceinline__
+
+namespace md {
+
+constexpr int NH(int L) { return L < 0 ? 0 : (L + 1) * (L + 2) * (L + 3) / 6; }
+constexpr int NC(int l) { return (l + 1) * (l + 2) / 2; }
+
+// linear Hermite index of (t, u, v)
+constexpr int hidx(int t, int u, int v) {
+    return NH(t + u + v - 1) + (u + v) * (u + v + 1) / 2 + v;
+}
+// (d-t)(d-t+1)/2 + (d-t-u) with d = t+u+v equals (u+v)(u+v+1)/2 + v
+
+// Cartesian component i of angular momentum l in PySCF order (xx, xy, xz, yy, yz, zz)
+constexpr int cart_x(int l, int i) {
+    int n = 0;
+    for (int lx = l; lx >= 0; --lx)
+        for (int ly = l - lx; ly >= 0; --ly) {
+            if (n == i) return lx;
+            ++n;
+        }
+    return 0;
+}
+constexpr int cart_y(int l, int i) {
+    int n = 0;
+    for (int lx = l; lx >= 0; --lx)
+        for (int ly = l - lx; ly >= 0; --ly) {
+            if (n == i) return ly;
+            ++n;
+        }
+    return 0;
+}
+constexpr int cart_z(int l, int i) { return l - cart_x(l, i) - cart_y(l, i); }
+
+constexpr int BOYS_NGRID = 402;
+constexpr int BOYS_NCOL = 43;
+constexpr double BOYS_XSWITCH = 40.0;
+constexpr double MD_PI = 3.141592653589793;
+constexpr double TWO_PI_2_5 = 34.986836655249725;  // 2 pi^(5/2)
+
+// F_L(T) (single top order), as aimd/qc/boys.py:87-114
+MD_HD double boys_top(int L, double T, const double* __restrict__ tab) {
+    if (T < BOYS_XSWITCH) {
+        int k = (int)(T * 10.0 + 0.5);
+        double dx = k * 0.1 - T;
+        const double* row = tab + k * BOYS_NCOL + L;
+        double acc = 0.0, fac = 1.0;
+        acc += row[0] * fac; fac *= dx * 1.0;
+        acc += row[1] * fac; fac *= dx * 0.5;
+        acc += row[2] * fac; fac *= dx * (1.0 / 3.0);
+        acc += row[3] * fac; fac *= dx * 0.25;
+        acc += row[4] * fac; fac *= dx * 0.2;
+        acc += row[5] * fac; fac *= dx * (1.0 / 6.0);
+        acc += row[6] * fac; fac *= dx * (1.0 / 7.0);
+        acc += row[7] * fac; fac *= dx * 0.125;
+        acc += row[8] * fac; fac *= dx * (1.0 / 9.0);
+        acc += row[9] * fac;
+        return acc;
+    }
+    double f = 0.5 * sqrt(MD_PI / T);
+    if (L > 0) {
+        double ex = exp(-T);
+        double inv2x = 0.5 / T;
+        for (int m = 0; m < L; ++m) f = ((2 * m + 1) * f - ex) * inv2x;
+    }
+    return f;
+}
+
+// F_0..F_L into F (downward recursion from F_L, as the reference's R code does)
+template <int L>
+MD_HD void boys_all(double T, const double* __restrict__ tab, double* F) {
+    F[L] = boys_top(L, T, tab);
+    if (L > 0) {
+        double ex = exp(-T);
+        double t2 = 2.0 * T;
+#pragma unroll
+        for (int n = L - 1; n >= 0; --n) F[n] = (t2 * F[n + 1] + ex) * (1.0 / (2 * n + 1));
+    }
+}
+
+// compile-time loop: f(std::integral_constant<int, i>) for i in [B, E)
+template <int B, int E, class F>
+MD_HD void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// (t, u, v) of every linear Hermite index up to degree HDEG_MAX
+constexpr int HDEG_MAX = 10;
+struct HermTable {
+    int t[NH(HDEG_MAX)], u[NH(HDEG_MAX)], v[NH(HDEG_MAX)];
+};
+constexpr HermTable make_herm_table() {
+    HermTable h{};
+    int n = 0;
+    for (int d = 0; d <= HDEG_MAX; ++d)
+        for (int t = d; t >= 0; --t)
+            for (int u = d - t; u >= 0; --u) {
+                h.t[n] = t; h.u[n] = u; h.v[n] = d - t - u;
+                ++n;
+            }
+    return h;
+}
+inline constexpr HermTable HT = make_herm_table();
+
+// R[hidx(t,u,v)] = scale * R_{tuv}(alpha, (x,y,z)) for t+u+v <= L
+template <int L>
+MD_HD void hermite_r(double alpha, double x, double y, double z, double scale,
+                     const double* __restrict__ tab, double* R) {
+    double T = alpha * (x * x + y * y + z * z);
+    double F[L + 1];
+    boys_all<L>(T, tab, F);
+    double m2a = -2.0 * alpha;
+    double pw[L + 1];
+    pw[0] = scale;
+#pragma unroll
+    for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * m2a;
+    R[0] = pw[L] * F[L];
+    static_for<0, L>([&](auto nn) {
+        constexpr int n = L - 1 - decltype(nn)::value;
+        constexpr int nhn = NH(L - n);
+        // highest degree first, so the level-(n+1) values read below are not yet overwritten
+        static_for<1, nhn>([&](auto hh) {
+            constexpr int h = nhn - decltype(hh)::value;
+            constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
+            double r;
+            if constexpr (t > 0) {
+                r = x * R[hidx(t - 1, u, v)];
+                if constexpr (t > 1) r += (t - 1) * R[hidx(t - 2, u, v)];
+            } else if constexpr (u > 0) {
+                r = y * R[hidx(0, u - 1, v)];
+                if constexpr (u > 1) r += (u - 1) * R[hidx(0, u - 2, v)];
+            } else {
+                r = z * R[hidx(0, 0, v - 1)];
+                if constexpr (v > 1) r += (v - 1) * R[hidx(0, 0, v - 2)];
+            }
+            R[h] = r;
+        });
+        R[0] = pw[n] * F[n];
+    });
+}
+
+// 1-D Hermite expansion E^{ij}_t, i <= IM, j <= JM, t <= i+j; e[(i*(JM+1)+j)*(IM+JM+1)+t]
+template <int IM, int JM>
+MD_HD void hermite_e(double a, double b, double xab, double e00, double* e) {
+    constexpr int TS = IM + JM + 1;
+    constexpr int JS = JM + 1;
+    double p = a + b;
+    double xpa = -b * xab / p;
+    double xpb = a * xab / p;
+    double h = 0.5 / p;
+#pragma unroll
+    for (int i = 0; i < (IM + 1) * JS * TS; ++i) e[i] = 0.0;
+    e[0] = e00;
+#pragma unroll
+    for (int j = 1; j <= JM; ++j)
+#pragma unroll
+        for (int t = 0; t <= j; ++t) {
+            double v = xpb * e[(j - 1) * TS + t];
+            if (t + 1 <= j - 1) v += (t + 1) * e[(j - 1) * TS + t + 1];
+            if (t > 0) v += h * e[(j - 1) * TS + t - 1];
+            e[j * TS + t] = v;
+        }
+#pragma unroll
+    for (int i = 1; i <= IM; ++i)
+#pragma unroll
+        for (int j = 0; j <= JM; ++j)
+#pragma unroll
+            for (int t = 0; t <= i + j; ++t) {
+                double v = xpa * e[((i - 1) * JS + j) * TS + t];
+                if (t + 1 <= i - 1 + j) v += (t + 1) * e[((i - 1) * JS + j) * TS + t + 1];
+                if (t > 0) v += h * e[((i - 1) * JS + j) * TS + t - 1];
+                e[(i * JS + j) * TS + t] = v;
+            }
+}
+
+// ---------------------------------------------------------------- pair records
+// A primitive pair of shells (A, B), l_A >= l_B, is stored as a record of
+// REC_HDR header doubles followed by the 3-D Hermite table
+//   E[h][ab] = f_a f_b c_a c_b E^{a_x b_x}_t E^{a_y b_y}_u E^{a_z b_z}_v,  h = hidx(t,u,v)
+// (NH(l_A+l_B) x NC(l_A) NC(l_B), row-major), i.e. the contraction
+// coefficients and Cartesian normalisation factors are folded in.
+enum { R_P = 0, R_PX = 1, R_PY = 2, R_PZ = 3, R_A = 4, R_B = 5, R_C = 6, REC_HDR = 8 };
+
+constexpr int rec_size(int la, int lb) { return REC_HDR + NH(la + lb) * NC(la) * NC(lb); }
+
+// pair class index of (la >= lb): ss 0, ps 1, pp 2, ds 3, dp 4, dd 5
+constexpr int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
+constexpr int NPAIRCLASS = 6;
+constexpr int NQCLASS = NPAIRCLASS * (NPAIRCLASS + 1) / 2;   // 21
+
+// ---------------------------------------------------------------- fixed point
+// Order-independent accumulation: x is split into hi = round(x 2^32) and the
+// remainder lo = round((x 2^32 - hi) 2^42); hi and lo are summed in separate
+// int64 words. Integer addition is associative, so the sums (and the double
+// they are converted to) do not depend on the order in which contributions
+// arrive. Resolution 2^-74 per contribution, range |sum| < 2^31.
+MD_HD void fx_split(double x, long long& hi, long long& lo) {
+    double xs = x * 4294967296.0;          // 2^32, exact
+    double h = rint(xs);
+    double r = xs - h;                     // exact
+    hi = (long long)h;
+    lo = (long long)rint(r * 4398046511104.0);   // 2^42
+}
+
+MD_HD double fx_value(long long hi, long long lo) {
+    return (double)hi * (1.0 / 4294967296.0) + (double)lo * (1.0 / 18889465931478580854784.0);  // 2^-32, 2^-74
+}
+
+}  // namespace md

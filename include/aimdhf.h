@@ -1,0 +1,99 @@
+/*
+ * aimdhf -- C ABI of the MI355X (gfx950) Hartree-Fock energy-and-forces engine.
+ *
+ * The reference (Nathan-W123/Ab-Initio-MD, package aimd) has no native ABI:
+ * its hot path is numba-jitted Python. Each entry point below replaces the
+ * reference function cited next to it; the Python host layer
+ * (aimd_amd.qc.engine) binds these through ctypes and mirrors the
+ * reference's Python API on top of them.
+ *
+ * Conventions
+ *   - Return value: 0 on success, otherwise one of the AIMDHF_E_* codes; the
+ *     message is available from aimdhf_last_error(). Nothing throws across
+ *     the ABI.
+ *   - Host pointers carry basis data and positions; "device" pointers are
+ *     HBM buffers owned by the caller (PyTorch tensors), row-major doubles.
+ *   - Every computation is enqueued on the caller's hipStream_t (passed as
+ *     void*; NULL = the default stream) and is asynchronous unless noted.
+ *   - One plan is used by one host thread at a time; plans are independent.
+ *   - Units: bohr, hartree. AOs: unit-normalised Cartesian functions in
+ *     PySCF order, shells of an atom sorted by l (the reference's layout,
+ *     aimd/qc/basis.py:17-68).
+ */
+#ifndef AIMDHF_H
+#define AIMDHF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AIMDHF_OK 0
+#define AIMDHF_E_ARG 1          /* invalid argument (Python: ValueError)        */
+#define AIMDHF_E_NOMEM 2        /* allocation failed (Python: MemoryError)      */
+#define AIMDHF_E_HIP 3          /* HIP runtime error (Python: RuntimeError)     */
+#define AIMDHF_E_UNSUPPORTED 4  /* e.g. l > 2 in this build (Python: ValueError) */
+#define AIMDHF_E_STATE 5        /* call order (e.g. no geometry set yet)        */
+
+typedef struct aimdhf_plan aimdhf_plan;
+
+/* Copy the message of the last failed call on this thread into buf. */
+int aimdhf_last_error(char* buf, size_t len);
+
+/* Highest angular momentum this build's kernels are instantiated for. */
+int aimdhf_lmax(void);
+
+/*
+ * Stage a basis (flat shell arrays of aimd/qc/basis.py:140-183 plus the
+ * contraction-block index of every shell, aimd/qc/basis.py:185-228) and the
+ * Boys table (aimd/qc/boys.py:57-75) on the current device.
+ * Replaces: BasisSet / build_basis (aimd/qc/basis.py:140-312) and the
+ * geometry-independent part of _Pairs (aimd/qc/integrals.py:602-621).
+ * shell_prim has nshell+1 entries; charges are the nuclear charges Z_A.
+ */
+int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const int64_t* shell_l,
+                       const int64_t* shell_prim, const int64_t* shell_block, const double* prim_exp,
+                       const double* prim_coef, const double* charges, aimdhf_plan** out);
+
+int aimdhf_plan_destroy(aimdhf_plan* plan);
+
+/*
+ * Move the basis to positions (host, (natm,3) bohr) and build everything that
+ * depends on the geometry: primitive-pair records (mu |A-B|^2 <= 50 kept),
+ * Schwarz bounds Q = max |(ab|ab)|^(1/2), and the class-binned list of unique
+ * shell quartets whose contraction-block pairs satisfy Q_P Q_Q >= threshold.
+ * Synchronises the stream (the list sizes come back to the host).
+ * Replaces: BasisSet.with_positions (aimd/qc/basis.py:242-246), _build_pairs
+ * (aimd/qc/integrals.py:525-599), schwarz_bounds (aimd/qc/integrals.py:970-983)
+ * and the screening loop of _eri_tensor_kernel (aimd/qc/integrals.py:900-903).
+ */
+int aimdhf_set_geometry(aimdhf_plan* plan, const double* positions, double schwarz_threshold, void* stream);
+
+/* Schwarz bound of every contraction-block pair, index i(i+1)/2+j (i >= j),
+ * copied to host memory (npair_block doubles). Synchronous.
+ * Replaces: schwarz_bounds (aimd/qc/integrals.py:970-983). */
+int aimdhf_schwarz_bounds(aimdhf_plan* plan, double* q_block_host, int64_t cap);
+
+/* Number of surviving unique shell quartets of the current geometry. */
+int aimdhf_quartet_count(aimdhf_plan* plan, int64_t* count);
+
+/* The surviving quartets as (P, Q) shell-pair indices with P >= Q, copied to
+ * host memory (2*cap int64), grouped by class. Synchronous. */
+int aimdhf_quartets(aimdhf_plan* plan, int64_t* out_host, int64_t cap, int64_t* count);
+
+/*
+ * Coulomb and exchange matrices of nset densities (device (nset,nao,nao)):
+ *   J_mn = sum_ls (mn|ls) D_ls,   K_mn = sum_ls (ml|ns) D_ls
+ * of the symmetric part of each D, integral-direct over the screened
+ * quartets, accumulated in exact fixed point (bitwise reproducible).
+ * Replaces: eri_tensor + jk_from_eri (aimd/qc/integrals.py:986-1044).
+ */
+int aimdhf_jk(aimdhf_plan* plan, int nset, const double* dens, double* J, double* K, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AIMDHF_H */

@@ -1,0 +1,225 @@
+// Test harness (CPU only): runs the engine's own kernel arithmetic -- the
+// __host__ __device__ templates of md_core.h / md_pairs.h / md_eri.h /
+// md_digest.h -- serially on the host, with the same pair orientation,
+// record layout, screening and fixed-point J/K accumulation as the gfx950
+// path. The CPU test suite compares it with the oracle, which checks the
+// kernels' mathematics without a GPU. Not part of the product.
+
+#include "md_digest.h"
+#include "md_pairs.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace md;
+
+namespace {
+
+struct Basis {
+    int nshell, natm;
+    const long long *atom, *l, *prim, *block;
+    const double *ex, *coef, *pos;
+};
+
+struct State {
+    int npair, nao;
+    std::vector<int> pa, pb, cls, ao, npp;
+    std::vector<long long> off;
+    std::vector<int> rs;
+    std::vector<double> recs, qshell;
+};
+
+std::vector<double> boys_tab() {
+    const int NG = 402, NR = 43, top = NR - 1;
+    std::vector<double> tab((size_t)NG * NR), col(NR);
+    for (int k = 0; k < NG; ++k) {
+        double x = k * 0.1, term = 1.0 / (2 * top + 1), total = term;
+        for (int m = 1; m < 400; ++m) {
+            term = term * (2.0 * x) / (double)(2 * top + 2 * m + 1);
+            total += term;
+        }
+        double e = std::exp(-x);
+        col[top] = e * total;
+        for (int n = top - 1; n >= 0; --n) col[n] = (2.0 * x * col[n + 1] + e) / (double)(2 * n + 1);
+        for (int n = 0; n < NR; ++n) tab[(size_t)k * NR + n] = col[n];
+    }
+    return tab;
+}
+
+const std::vector<double>& TAB() {
+    static std::vector<double> t = boys_tab();
+    return t;
+}
+
+void build(const Basis& B, State& S) {
+    S.ao.assign(B.nshell + 1, 0);
+    for (int s = 0; s < B.nshell; ++s) S.ao[s + 1] = S.ao[s] + NC((int)B.l[s]);
+    S.nao = S.ao[B.nshell];
+    S.npair = B.nshell * (B.nshell + 1) / 2;
+    S.pa.resize(S.npair); S.pb.resize(S.npair); S.cls.resize(S.npair); S.npp.resize(S.npair);
+    S.off.resize(S.npair); S.rs.resize(S.npair); S.qshell.resize(S.npair);
+    long long off = 0;
+    int P = 0;
+    for (int s1 = 0; s1 < B.nshell; ++s1)
+        for (int s2 = 0; s2 <= s1; ++s2, ++P) {
+            int a = s1, b = s2;
+            if (B.l[s2] > B.l[s1]) std::swap(a, b);
+            S.pa[P] = a; S.pb[P] = b;
+            int la = (int)B.l[a], lb = (int)B.l[b];
+            S.cls[P] = pair_class(la, lb);
+            S.rs[P] = REC_HDR + NH(la + lb) * NC(la) * NC(lb);
+            S.off[P] = off;
+            off += (long long)S.rs[P] * (B.prim[a + 1] - B.prim[a]) * (B.prim[b + 1] - B.prim[b]);
+        }
+    S.recs.assign(off, 0.0);
+    for (P = 0; P < S.npair; ++P) {
+        int a = S.pa[P], b = S.pb[P];
+        const double* A = B.pos + 3 * B.atom[a];
+        const double* C = B.pos + 3 * B.atom[b];
+        double r2 = 0.0;
+        for (int d = 0; d < 3; ++d) { double x = A[d] - C[d]; r2 += x * x; }
+        int n = 0;
+        for (long long ka = B.prim[a]; ka < B.prim[a + 1]; ++ka)
+            for (long long kb = B.prim[b]; kb < B.prim[b + 1]; ++kb) {
+                double ea = B.ex[ka], eb = B.ex[kb];
+                if (!(ea * eb / (ea + eb) * r2 <= 50.0)) continue;
+                write_record_any((int)B.l[a], (int)B.l[b], ea, eb, B.coef[ka], B.coef[kb], A, C,
+                                 S.recs.data() + S.off[P] + (long long)n * S.rs[P]);
+                ++n;
+            }
+        S.npp[P] = n;
+    }
+}
+
+template <int LA, int LB>
+double schwarz(const State& S, int P) {
+    using Qt = Quartet<LA, LB, LA, LB>;
+    const double* r = S.recs.data() + S.off[P];
+    double m = 0.0;
+    if constexpr (split_by_cd<LA, LB, LA, LB>()) {
+        for (int cd = 0; cd < Qt::NCD; ++cd) {
+            double acc[Qt::NAB];
+            eri_quartet_cd<LA, LB, LA, LB>(r, S.npp[P], r, S.npp[P], TAB().data(), cd, acc);
+            m = std::max(m, std::fabs(acc[cd]));
+        }
+    } else {
+        double acc[Qt::NAB * Qt::NCD];
+        eri_quartet<LA, LB, LA, LB>(r, S.npp[P], r, S.npp[P], TAB().data(), acc);
+        for (int ab = 0; ab < Qt::NAB; ++ab) m = std::max(m, std::fabs(acc[ab * Qt::NCD + ab]));
+    }
+    return std::sqrt(m);
+}
+
+constexpr int la_of(int pc) { return pc == 0 ? 0 : pc <= 2 ? 1 : 2; }
+constexpr int lb_of(int pc) { return pc - la_of(pc) * (la_of(pc) + 1) / 2; }
+
+struct HostSink {
+    long long *Jh, *Jl, *Kh, *Kl;
+    void j(size_t i, double x) const { long long h, l; fx_split(x, h, l); Jh[i] += h; Jl[i] += l; }
+    void k(size_t i, double x) const { long long h, l; fx_split(x, h, l); Kh[i] += h; Kl[i] += l; }
+};
+
+template <int QC>
+void jk_quartet(const State& S, int P, int Q, const double* D, int nset, std::vector<long long>& fx) {
+    constexpr int hi = [] { int h = 0; while ((h + 1) * (h + 2) / 2 <= QC) ++h; return h; }();
+    constexpr int lo = QC - hi * (hi + 1) / 2;
+    constexpr int LA = la_of(hi), LB = lb_of(hi), LC = la_of(lo), LD = lb_of(lo);
+    using Qt = Quartet<LA, LB, LC, LD>;
+    int sA = S.pa[P], sB = S.pb[P], sC = S.pa[Q], sD = S.pb[Q];
+    double w = 1.0;
+    if (sA == sB) w *= 0.5;
+    if (sC == sD) w *= 0.5;
+    if (P == Q) w *= 0.5;
+    size_t nn = (size_t)S.nao * S.nao;
+    auto sink_of = [&](int s) {
+        return HostSink{fx.data() + (4 * s + 0) * nn, fx.data() + (4 * s + 1) * nn, fx.data() + (4 * s + 2) * nn,
+                        fx.data() + (4 * s + 3) * nn};
+    };
+    const double* rb = S.recs.data() + S.off[P];
+    const double* rk = S.recs.data() + S.off[Q];
+    if constexpr (split_by_cd<LA, LB, LC, LD>()) {
+        for (int cd = 0; cd < Qt::NCD; ++cd) {
+            double acc[Qt::NAB];
+            eri_quartet_cd<LA, LB, LC, LD>(rb, S.npp[P], rk, S.npp[Q], TAB().data(), cd, acc);
+            digest_cd<LA, LB, LC, LD>(acc, D, S.nao, nset, sink_of, S.ao[sA], S.ao[sB], S.ao[sC], S.ao[sD], cd, w);
+        }
+    } else {
+        double acc[Qt::NAB * Qt::NCD];
+        eri_quartet<LA, LB, LC, LD>(rb, S.npp[P], rk, S.npp[Q], TAB().data(), acc);
+        digest_full<LA, LB, LC, LD>(acc, D, S.nao, nset, sink_of, S.ao[sA], S.ao[sB], S.ao[sC], S.ao[sD], w);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Schwarz bounds per shell pair and per block pair, the screened quartet
+// count, and J/K of nset symmetric densities, all through the kernel code.
+long hc_run(int nshell, int natm, const long long* atom, const long long* l, const long long* prim,
+            const long long* block, const double* ex, const double* coef, const double* pos, double thr,
+            int nset, const double* D, double* J, double* K, double* qshell, double* qblock, long long* quartets,
+            long cap) {
+    Basis B{nshell, natm, atom, l, prim, block, ex, coef, pos};
+    State S;
+    build(B, S);
+    for (int P = 0; P < S.npair; ++P) {
+        double q = 0.0;
+        switch (S.cls[P]) {
+            case 0: q = schwarz<0, 0>(S, P); break;
+            case 1: q = schwarz<1, 0>(S, P); break;
+            case 2: q = schwarz<1, 1>(S, P); break;
+            case 3: q = schwarz<2, 0>(S, P); break;
+            case 4: q = schwarz<2, 1>(S, P); break;
+            default: q = schwarz<2, 2>(S, P); break;
+        }
+        S.qshell[P] = q;
+        if (qshell) qshell[P] = q;
+    }
+    int nblock = (int)block[nshell - 1] + 1;
+    int nbp = nblock * (nblock + 1) / 2;
+    std::vector<double> qb(nbp, 0.0);
+    std::vector<int> bpof(S.npair);
+    int P = 0;
+    for (int s1 = 0; s1 < nshell; ++s1)
+        for (int s2 = 0; s2 <= s1; ++s2, ++P) {
+            int b1 = (int)block[s1], b2 = (int)block[s2];
+            bpof[P] = b1 * (b1 + 1) / 2 + b2;
+            qb[bpof[P]] = std::max(qb[bpof[P]], S.qshell[P]);
+        }
+    if (qblock) for (int i = 0; i < nbp; ++i) qblock[i] = qb[i];
+    size_t nn = (size_t)S.nao * S.nao;
+    std::vector<long long> fx(4 * nset * nn, 0);
+    std::vector<double> Ds(nset * nn);
+    for (int s = 0; s < nset; ++s)
+        for (int r = 0; r < S.nao; ++r)
+            for (int c = 0; c < S.nao; ++c)
+                Ds[s * nn + r * S.nao + c] = 0.5 * (D[s * nn + r * S.nao + c] + D[s * nn + c * S.nao + r]);
+    long nq = 0;
+    for (P = 0; P < S.npair; ++P)
+        for (int Q = 0; Q <= P; ++Q) {
+            if (qb[bpof[P]] * qb[bpof[Q]] < thr) continue;
+            if (quartets && nq < cap) { quartets[2 * nq] = P; quartets[2 * nq + 1] = Q; }
+            ++nq;
+            int cp = S.cls[P], cq = S.cls[Q];
+            int hi = std::max(cp, cq), lo = std::min(cp, cq);
+            int bra = cp >= cq ? P : Q, ket = cp >= cq ? Q : P;
+            int qc = hi * (hi + 1) / 2 + lo;
+            static_for<0, NQCLASS>([&](auto k) {
+                if (decltype(k)::value == qc) jk_quartet<decltype(k)::value>(S, bra, ket, Ds.data(), nset, fx);
+            });
+        }
+    for (int s = 0; s < nset; ++s)
+        for (int r = 0; r < S.nao; ++r)
+            for (int c = 0; c < S.nao; ++c) {
+                size_t ij = r * S.nao + c, ji = c * S.nao + r;
+                const long long* f = fx.data() + 4 * s * nn;
+                J[s * nn + ij] = fx_value(f[ij] + f[ji], f[nn + ij] + f[nn + ji]);
+                K[s * nn + ij] = fx_value(f[2 * nn + ij] + f[2 * nn + ji], f[3 * nn + ij] + f[3 * nn + ji]);
+            }
+    return nq;
+}
+
+}  // extern "C"
