@@ -104,3 +104,31 @@ class Engine:
         K = torch.empty_like(D)
         _lib.check(_lib.lib().aimdhf_jk(self._h, D.shape[0], _ptr(D), _ptr(J), _ptr(K), _stream()))
         return (J[0], K[0]) if single else (J, K)
+
+    # ------------------------------------------------------- one-electron
+    def one_electron(self, S: torch.Tensor, T: torch.Tensor, V: torch.Tensor, M: torch.Tensor) -> None:
+        """Fill S, T, V (nao, nao) and the dipole integrals M (3, nao, nao) of the current geometry."""
+        for t, shp in ((S, (self.nao, self.nao)), (T, (self.nao, self.nao)), (V, (self.nao, self.nao)),
+                       (M, (3, self.nao, self.nao))):
+            if tuple(t.shape) != shp or t.dtype != torch.float64 or not t.is_contiguous() or t.device != self.device:
+                raise ValueError(f"output tensor must be a contiguous float64 {shp} tensor on {self.device}")
+        _lib.check(_lib.lib().aimdhf_one_electron(self._h, _ptr(S), _ptr(T), _ptr(V), _ptr(M), _stream()))
+
+    # ----------------------------------------------------------- gradients
+    def grad_one(self, D: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+        """d/dR [tr(D h) - tr(W S)] as a (natm, 3) device tensor."""
+        Dd = D.to(device=self.device, dtype=torch.float64).contiguous()
+        Wd = W.to(device=self.device, dtype=torch.float64).contiguous()
+        g = torch.empty((self.natm, 3), dtype=torch.float64, device=self.device)
+        _lib.check(_lib.lib().aimdhf_grad_one(self._h, _ptr(Dd), _ptr(Wd), _ptr(g), _stream()))
+        return g
+
+    def grad_two(self, Dc: torch.Tensor, Dx: torch.Tensor, kfac: float,
+                 threshold: float = DEFAULT_SCHWARZ_GRAD) -> torch.Tensor:
+        """d/dR of the two-electron energy at fixed densities, (natm, 3) device tensor."""
+        Dc = Dc.to(device=self.device, dtype=torch.float64).contiguous()
+        Dx = Dx.reshape(-1, self.nao, self.nao).to(device=self.device, dtype=torch.float64).contiguous()
+        g = torch.empty((self.natm, 3), dtype=torch.float64, device=self.device)
+        _lib.check(_lib.lib().aimdhf_grad_two(self._h, _ptr(Dc), Dx.shape[0], _ptr(Dx), float(kfac),
+                                              float(threshold), _ptr(g), _stream()))
+        return g

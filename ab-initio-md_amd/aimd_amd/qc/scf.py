@@ -1,0 +1,783 @@
+"""
+Restricted (RHF) and unrestricted (UHF) Hartree-Fock SCF on the GPU.
+
+Mirrors the reference's solver (aimd/qc/scf.py) -- the same options and
+defaults, guesses, DIIS, convergence test and result fields -- with every
+matrix held in float64 PyTorch tensors on the HIP device and the two-electron
+part computed by the gfx950 engine (aimd_amd.qc.engine), integral-direct:
+no ERI tensor is stored, so there is no nao limit of the reference's kind.
+
+Equations (aimd/qc/scf.py:9-23, 80-104): spin densities D_s, g = 2/nspin,
+    F_s = h + g sum_s' J[D_s'] - K[D_s],   E_el = 1/2 g sum_s <D_s, h + F_s>,
+    e_s = X^T (F_s D_s S - S D_s F_s) X,   converged iff |dE| < conv_energy
+                                           and max|e| < conv_commutator.
+The returned state is D_k, F[D_k], E[D_k] of the last iteration; the MOs are
+the canonical orbitals of F_k.
+
+One host synchronisation per iteration reads the two convergence scalars;
+DIIS coefficients come from the (m x m) Gram matrix of the error
+differences, formed on the device and solved on the host (m <= diis_space).
+"""
+
+from __future__ import annotations
+
+import time
+import warnings
+from dataclasses import dataclass, field, fields, replace
+from typing import Any, Sequence
+
+import numpy as np
+import torch
+
+from aimd_amd.elements import normalize_symbol
+from aimd_amd.qc import engine as qengine
+from aimd_amd.qc.basis import BasisSet, build_basis
+
+LINDEP_THRESHOLD = 1e-7
+GWH_K = 1.75
+MIN_NUCLEAR_SEPARATION = 1e-5
+_GUESSES = ("sad", "core", "gwh")
+_ORTHO = ("auto", "symmetric", "canonical")
+_REFERENCES = {"rhf": "rhf", "restricted": "rhf", "uhf": "uhf", "unrestricted": "uhf"}
+
+
+class SCFConvergenceWarning(RuntimeWarning):
+    """Emitted when an SCF stops at ``max_iter`` without converging."""
+
+
+@dataclass(frozen=True)
+class SCFOptions:
+    """
+    SCF settings; names, defaults and validation as aimd/qc/scf.py:185-247.
+    ``max_nao`` and ``blas_threads`` are accepted for compatibility: the
+    engine is integral-direct (no tensor to cap) and runs no BLAS threads.
+    """
+    conv_energy: float = 1e-10
+    conv_commutator: float = 1e-7
+    max_iter: int = 100
+    guess: str = "sad"
+    diis: bool = True
+    diis_space: int = 8
+    diis_restart: int = 12
+    level_shift: float = 0.0
+    damping: float = 0.0
+    stabilize_until: float = 1e-3
+    orthogonalization: str = "auto"
+    lindep_threshold: float = LINDEP_THRESHOLD
+    schwarz_threshold: float = qengine.DEFAULT_SCHWARZ
+    max_nao: int = 120
+    warn_unconverged: bool = True
+    blas_threads: int | None = 1
+
+    def __post_init__(self) -> None:
+        if not (self.conv_energy > 0.0 and self.conv_commutator > 0.0):
+            raise ValueError("SCF convergence thresholds must be positive")
+        if self.max_iter < 1:
+            raise ValueError("max_iter must be >= 1")
+        if self.guess not in _GUESSES:
+            raise ValueError(f"unknown guess {self.guess!r}; use one of {_GUESSES}")
+        if self.diis_space < 2:
+            raise ValueError("diis_space must be >= 2")
+        if self.diis_restart < 0:
+            raise ValueError("diis_restart must be >= 0")
+        if self.level_shift < 0.0:
+            raise ValueError("level_shift must be >= 0")
+        if not 0.0 <= self.damping < 1.0:
+            raise ValueError("damping must be in [0, 1)")
+        if self.orthogonalization not in _ORTHO:
+            raise ValueError(f"unknown orthogonalization {self.orthogonalization!r}; use one of {_ORTHO}")
+        if not self.lindep_threshold > 0.0:
+            raise ValueError("lindep_threshold must be positive")
+        if not self.schwarz_threshold >= 0.0:
+            raise ValueError("schwarz_threshold must be >= 0")
+        if self.blas_threads is not None and self.blas_threads < 1:
+            raise ValueError("blas_threads must be >= 1 or None")
+
+
+def electron_counts(nuclear_charge: int, charge: int, multiplicity: int,
+                    reference: str = "uhf") -> tuple[int, int]:
+    """(n_alpha, n_beta); ValueError for impossible combinations (aimd/qc/scf.py:250-279)."""
+    n = int(nuclear_charge) - int(charge)
+    mult = int(multiplicity)
+    if n <= 0:
+        raise ValueError(f"charge {charge} leaves {n} electrons; need at least one")
+    if mult < 1:
+        raise ValueError(f"multiplicity must be >= 1, got {mult}")
+    if (n + mult - 1) % 2:
+        raise ValueError(
+            f"multiplicity {mult} is impossible with {n} electrons "
+            f"({'even' if n % 2 == 0 else 'odd'} electron counts need "
+            f"{'odd' if n % 2 == 0 else 'even'} multiplicities)")
+    if mult - 1 > n:
+        raise ValueError(f"multiplicity {mult} needs at least {mult - 1} electrons, have {n}")
+    na, nb = (n + mult - 1) // 2, (n - mult + 1) // 2
+    ref = str(reference).strip().lower()
+    if ref not in _REFERENCES:
+        raise ValueError(f"unknown reference {reference!r}; use 'rhf' or 'uhf'")
+    if _REFERENCES[ref] == "rhf" and na != nb:
+        raise ValueError(f"RHF needs a closed-shell singlet; {n} electrons with multiplicity "
+                         f"{mult} is open-shell (use UHF)")
+    return na, nb
+
+
+def check_nuclear_separation(positions: np.ndarray, atomic_numbers: np.ndarray) -> None:
+    """ValueError if two nuclei are closer than 1e-5 bohr (aimd/qc/scf.py:666-684)."""
+    x = np.asarray(positions, dtype=float).reshape(-1, 3)
+    z = np.asarray(atomic_numbers)
+    if len(x) < 2 or not np.all(np.isfinite(x)):
+        return
+    r = np.linalg.norm(x[:, None, :] - x[None, :, :], axis=-1)
+    i, j = np.triu_indices(len(x), k=1)
+    bad = (r[i, j] < MIN_NUCLEAR_SEPARATION) & (z[i] > 0) & (z[j] > 0)
+    if np.any(bad):
+        k = int(np.argmax(bad))
+        raise ValueError(
+            f"atoms {i[k]} and {j[k]} are {r[i[k], j[k]]:.3g} bohr apart (coincident "
+            f"nuclei, < {MIN_NUCLEAR_SEPARATION:g} bohr): the nuclear repulsion is infinite")
+
+
+def orthogonalizer(S: torch.Tensor, method: str = "auto",
+                   threshold: float = LINDEP_THRESHOLD) -> tuple[torch.Tensor, int, float]:
+    """X with X^T S X = 1, removed combinations, smallest eigenvalue (aimd/qc/scf.py:284-303)."""
+    if method not in _ORTHO:
+        raise ValueError(f"unknown orthogonalization {method!r}; use one of {_ORTHO}")
+    s, U = torch.linalg.eigh(S)
+    s_h = s.cpu().numpy()
+    smin = float(s_h[0])
+    keep = s_h >= threshold
+    if method == "symmetric" or (method == "auto" and keep.all()):
+        if not keep.all():
+            raise ValueError(
+                f"overlap eigenvalue {smin:.3e} < lindep_threshold {threshold:.1e}: "
+                "the basis is nearly linearly dependent; use canonical orthogonalization")
+        return (U * s.rsqrt()) @ U.T, 0, smin
+    if not keep.any():
+        raise ValueError("every overlap eigenvalue is below lindep_threshold")
+    idx = torch.as_tensor(np.nonzero(keep)[0], device=S.device)
+    return U[:, idx] * s[idx].rsqrt(), int((~keep).sum()), smin
+
+
+def nuclear_repulsion_energy(charges: np.ndarray, positions: np.ndarray) -> float:
+    q = np.asarray(charges, dtype=float).reshape(-1)
+    x = np.asarray(positions, dtype=float).reshape(-1, 3)
+    i, j = np.triu_indices(len(q), k=1)
+    return float(np.sum(q[i] * q[j] / np.linalg.norm(x[i] - x[j], axis=1)))
+
+
+def nuclear_repulsion_gradient(charges: np.ndarray, positions: np.ndarray) -> np.ndarray:
+    q = np.asarray(charges, dtype=float).reshape(-1)
+    x = np.asarray(positions, dtype=float).reshape(-1, 3)
+    i, j = np.triu_indices(len(q), k=1)
+    d = x[i] - x[j]
+    r = np.linalg.norm(d, axis=1)
+    pair = -(q[i] * q[j] / r ** 3)[:, None] * d
+    g = np.zeros_like(x)
+    np.add.at(g, i, pair)
+    np.add.at(g, j, -pair)
+    return g
+
+
+def nuclear_dipole(charges, positions, origin=None) -> np.ndarray:
+    q = np.asarray(charges, dtype=float).reshape(-1)
+    x = np.asarray(positions, dtype=float).reshape(-1, 3)
+    o = np.zeros(3) if origin is None else np.asarray(origin, dtype=float).reshape(3)
+    return q @ (x - o)
+
+
+# ================================================================ integrals
+@dataclass(eq=False)
+class SCFIntegrals:
+    """One-electron matrices and the engine state of one geometry (device tensors)."""
+    basis: BasisSet
+    engine: qengine.Engine
+    S: torch.Tensor
+    H: torch.Tensor
+    M: torch.Tensor          # dipole integrals (3, nao, nao) about the origin
+    X: torch.Tensor
+    n_removed: int
+    min_overlap_eigenvalue: float
+    nuclear_repulsion: float
+
+    @property
+    def nao(self) -> int:
+        return self.basis.nao
+
+    @property
+    def nmo(self) -> int:
+        return int(self.X.shape[1])
+
+
+def build_integrals(basis: BasisSet, engine: qengine.Engine, options: SCFOptions) -> SCFIntegrals:
+    """Geometry set-up, S, h = T + V, dipole integrals and X (aimd/qc/scf.py:329-336)."""
+    engine.set_geometry(basis.positions, options.schwarz_threshold)
+    n = basis.nao
+    dev = engine.device
+    S = torch.empty((n, n), dtype=torch.float64, device=dev)
+    T = torch.empty_like(S)
+    V = torch.empty_like(S)
+    M = torch.empty((3, n, n), dtype=torch.float64, device=dev)
+    engine.one_electron(S, T, V, M)
+    X, n_removed, smin = orthogonalizer(S, options.orthogonalization, options.lindep_threshold)
+    enuc = nuclear_repulsion_energy(basis.nuclear_charges, basis.positions)
+    return SCFIntegrals(basis, engine, S, T + V, M, X, n_removed, smin, enuc)
+
+
+# ================================================================ guesses
+_MADELUNG = sorted(((n, l) for n in range(1, 8) for l in range(n)), key=lambda t: (t[0] + t[1], t[0]))
+
+
+def ground_state_l_counts(Z: int) -> list[int]:
+    counts = [0, 0, 0, 0]
+    left = int(Z)
+    for _n, l in _MADELUNG:
+        if left <= 0:
+            break
+        k = min(left, 2 * (2 * l + 1))
+        counts[l] += k
+        left -= k
+    return counts
+
+
+def _degenerate_groups(eps: np.ndarray, tol: float) -> list[tuple[int, int]]:
+    groups, i = [], 0
+    while i < len(eps):
+        j = i + 1
+        while j < len(eps) and eps[j] - eps[j - 1] < tol:
+            j += 1
+        groups.append((i, j))
+        i = j
+    return groups
+
+
+def _atomic_occupations(eps: np.ndarray, l_counts: Sequence[int], tol: float = 1e-6) -> np.ndarray:
+    """Spin-summed shell occupations of a spherical atom (aimd/qc/scf.py:394-424)."""
+    groups = _degenerate_groups(eps, tol)
+    occ = np.zeros(len(eps))
+    left = list(l_counts) + [0] * (5 - len(l_counts))
+    ok = True
+    for i, j in groups:
+        g = j - i
+        if g % 2 == 0 or g > 9:
+            ok = False
+            break
+        l = (g - 1) // 2
+        k = min(left[l], 2 * g)
+        occ[i:j] = k / g
+        left[l] -= k
+    if ok and not any(left):
+        return occ
+    occ[:] = 0.0
+    n = float(sum(l_counts))
+    for i, j in groups:
+        k = min(n, 2.0 * (j - i))
+        occ[i:j] = k / (j - i)
+        n -= k
+    return occ
+
+
+def atom_basis(basis: BasisSet, atom: int) -> BasisSet:
+    """Shells of one atom as a one-atom basis at the origin (aimd/qc/scf.py:492-501)."""
+    sh = np.nonzero(basis.shell_atom == atom)[0]
+    k0 = [basis.shell_prim[s] for s in sh]
+    k1 = [basis.shell_prim[s + 1] for s in sh]
+    prim = np.concatenate([np.arange(a, b) for a, b in zip(k0, k1)]) if len(sh) else np.zeros(0, int)
+    shell_prim = np.concatenate([[0], np.cumsum([b - a for a, b in zip(k0, k1)])])
+    return BasisSet(basis.name, [basis.symbols[atom]], np.zeros((1, 3)),
+                    np.zeros(len(sh), dtype=np.int64), basis.shell_l[sh], shell_prim,
+                    basis.prim_exp[prim], basis.prim_coef[prim])
+
+
+_ATOMIC_CACHE: dict[tuple, tuple[np.ndarray, float, bool]] = {}
+
+
+def atomic_hf(basis_atom: BasisSet, max_iter: int = 200) -> tuple[np.ndarray, float, bool]:
+    """
+    Spherically averaged fractional-occupation RHF of a neutral atom in its
+    molecular basis (aimd/qc/scf.py:430-484), on the device; cached per
+    element and shell data.
+    """
+    if basis_atom.natm != 1:
+        raise ValueError("atomic_hf needs a one-atom basis")
+    key = (int(basis_atom.atomic_numbers[0]), basis_atom.shell_l.tobytes(), basis_atom.shell_prim.tobytes(),
+           basis_atom.prim_exp.tobytes(), basis_atom.prim_coef.tobytes())
+    hit = _ATOMIC_CACHE.get(key)
+    if hit is not None:
+        return hit[0].copy(), hit[1], hit[2]
+    Z = int(basis_atom.atomic_numbers[0])
+    lc = ground_state_l_counts(Z)
+    eng = qengine.Engine(basis_atom)
+    try:
+        I = build_integrals(basis_atom, eng, SCFOptions(orthogonalization="canonical"))
+        S, H = I.S, I.H
+        X, _, _ = orthogonalizer(S, "canonical", LINDEP_THRESHOLD)
+        if Z == 1:
+            eps, Cp = torch.linalg.eigh(X.T @ H @ X)
+            c = X @ Cp[:, 0]
+            D = torch.outer(c, c).cpu().numpy()
+            res = (D, float(eps[0]), True)
+            _ATOMIC_CACHE[key] = res
+            return D.copy(), res[1], True
+
+        def density(Fo: torch.Tensor) -> torch.Tensor:
+            eps, Cp = torch.linalg.eigh(Fo)
+            C = X @ Cp
+            occ = torch.as_tensor(_atomic_occupations(eps.cpu().numpy(), lc), device=C.device)
+            return (C * occ) @ C.T
+
+        D = density(X.T @ H @ X)
+        diis = DIIS(8)
+        e_old = np.inf
+        converged = False
+        e = 0.0
+        for _ in range(max_iter):
+            J, K = eng.jk(D)
+            F = H + J - 0.5 * K
+            F = 0.5 * (F + F.T)
+            e = 0.5 * float(torch.vdot(D.reshape(-1), (H + F).reshape(-1)))
+            err = X.T @ (F @ D @ S - S @ D @ F) @ X
+            if abs(e - e_old) < 1e-11 and float(err.abs().max()) < 1e-8:
+                converged = True
+                break
+            e_old = e
+            diis.push((X.T @ F @ X)[None], err[None])
+            D = density(diis.extrapolate()[0])
+        Dn = (0.5 * (D + D.T)).cpu().numpy()
+    finally:
+        eng.close()
+    _ATOMIC_CACHE[key] = (Dn, e, converged)
+    return Dn.copy(), e, converged
+
+
+def sad_density(basis: BasisSet) -> np.ndarray:
+    """Superposition of atomic densities (total density, (nao, nao)) (aimd/qc/scf.py:504-511)."""
+    P = np.zeros((basis.nao, basis.nao))
+    for a in range(basis.natm):
+        i, j = basis.atom_ao[a]
+        if j > i:
+            P[i:j, i:j] = atomic_hf(atom_basis(basis, a))[0]
+    return P
+
+
+# =================================================================== DIIS
+class DIIS:
+    """
+    Pulay DIIS anchored on the newest vector (aimd/qc/scf.py:516-568):
+    minimise |e_m + sum_i c_i (e_i - e_m)|; the minimum-norm solution is taken
+    from the eigen-decomposition of the small Gram matrix of the differences
+    (singular values below eps max(M, N) of the largest are dropped, numpy's
+    lstsq rule), and the oldest vectors are dropped while |c| > 1e4.
+    """
+
+    def __init__(self, space: int = 8, max_coefficient: float = 1e4) -> None:
+        self.space = int(space)
+        self.max_coefficient = float(max_coefficient)
+        self.focks: list[torch.Tensor] = []
+        self.errors: list[torch.Tensor] = []
+
+    def __len__(self) -> int:
+        return len(self.focks)
+
+    def reset(self) -> None:
+        self.focks.clear()
+        self.errors.clear()
+
+    def push(self, fock: torch.Tensor, error: torch.Tensor) -> None:
+        self.focks.append(fock.clone())
+        self.errors.append(error.clone())
+        while len(self.focks) > self.space:
+            self.focks.pop(0)
+            self.errors.pop(0)
+
+    def coefficients(self) -> np.ndarray:
+        m = len(self.errors)
+        if m == 1:
+            return np.ones(1)
+        last = self.errors[-1].reshape(-1)
+        delta = torch.stack([e.reshape(-1) - last for e in self.errors[:-1]], dim=1)
+        G = (delta.T @ delta).cpu().numpy()
+        r = (delta.T @ (-last)).cpu().numpy()
+        lam, V = np.linalg.eigh(G)
+        sig = np.sqrt(np.maximum(lam, 0.0))
+        cut = np.finfo(float).eps * max(delta.shape) * sig.max()
+        keep = sig > cut
+        x = V[:, keep] @ ((V[:, keep].T @ r) / lam[keep]) if keep.any() else np.zeros(m - 1)
+        return np.append(x, 1.0 - x.sum())
+
+    def extrapolate(self) -> torch.Tensor:
+        while len(self.focks) > 1:
+            c = self.coefficients()
+            if np.all(np.isfinite(c)) and np.abs(c).max() <= self.max_coefficient:
+                out = self.focks[0] * float(c[0])
+                for ci, Fi in zip(c[1:], self.focks[1:]):
+                    out = out + float(ci) * Fi
+                return out
+            self.focks.pop(0)
+            self.errors.pop(0)
+        return self.focks[-1].clone()
+
+
+# ================================================================ results
+@dataclass(eq=False)
+class SCFResult:
+    """Converged (or flagged) SCF state; numpy arrays as aimd/qc/scf.py:573-656."""
+    reference: str
+    converged: bool
+    iterations: int
+    energy: float
+    electronic_energy: float
+    nuclear_repulsion: float
+    mo_energy: np.ndarray
+    mo_coeff: np.ndarray
+    mo_occ: np.ndarray
+    spin_densities: np.ndarray
+    fock: np.ndarray
+    energy_weighted_density: np.ndarray
+    mulliken_charges: np.ndarray
+    dipole: np.ndarray
+    s2: float
+    n_alpha: int
+    n_beta: int
+    energy_change: float
+    commutator_norm: float
+    basis: BasisSet
+    n_removed: int = 0
+    guess: str = ""
+    diis_restarts: int = 0
+    history: list[dict[str, float]] = field(default_factory=list)
+    timings: dict[str, float] = field(default_factory=dict)
+    # device state of the last iteration (for the gradient)
+    integrals: SCFIntegrals | None = None
+    device_spin_densities: torch.Tensor | None = None   # (nspin, nao, nao)
+    device_fock: torch.Tensor | None = None             # (nspin, nao, nao)
+
+    @property
+    def unrestricted(self) -> bool:
+        return self.reference == "uhf"
+
+    @property
+    def density(self) -> np.ndarray:
+        return self.spin_densities[0] + self.spin_densities[1]
+
+    @property
+    def density_alpha(self) -> np.ndarray:
+        return self.spin_densities[0]
+
+    @property
+    def density_beta(self) -> np.ndarray:
+        return self.spin_densities[1]
+
+    @property
+    def spin_density(self) -> np.ndarray:
+        return self.spin_densities[0] - self.spin_densities[1]
+
+    @property
+    def guess_density(self) -> np.ndarray:
+        return self.spin_densities.copy() if self.unrestricted else self.density
+
+    @property
+    def nmo(self) -> int:
+        return int(self.mo_coeff.shape[-1])
+
+    @property
+    def s2_exact(self) -> float:
+        sz = 0.5 * (self.n_alpha - self.n_beta)
+        return sz * (sz + 1.0)
+
+    def coulomb_exchange_densities(self) -> tuple[np.ndarray, list[np.ndarray], float]:
+        if self.unrestricted:
+            return self.density, [self.density_alpha, self.density_beta], 1.0
+        P = self.density
+        return P, [P], 0.5
+
+    def __repr__(self) -> str:
+        flag = "converged" if self.converged else "NOT CONVERGED"
+        return (f"SCFResult({self.reference.upper()}/{self.basis.name}, E = {self.energy:.10f} Eh, "
+                f"{self.iterations} iterations, {flag})")
+
+
+# ================================================================= solver
+class SCFSolver:
+    """
+    Reusable RHF/UHF solver for one molecule (aimd/qc/scf.py:687-965): the
+    basis is parsed and staged on the device once; every run moves it to new
+    positions and iterates from the requested guess.
+    """
+
+    def __init__(self, symbols: Sequence[str], basis: str | BasisSet = "sto-3g", charge: int = 0,
+                 multiplicity: int = 1, reference: str | None = None, options: SCFOptions | None = None,
+                 **overrides: Any) -> None:
+        opt = options or SCFOptions()
+        if overrides:
+            known = {f.name for f in fields(SCFOptions)}
+            bad = sorted(set(overrides) - known)
+            if bad:
+                raise TypeError(f"unknown SCF option(s): {', '.join(bad)}")
+            opt = replace(opt, **overrides)
+        self.options = opt
+        n = len(symbols)
+        if isinstance(basis, BasisSet):
+            if [normalize_symbol(s) for s in symbols] != basis.symbols:
+                raise ValueError("symbols do not match the atoms of the given BasisSet")
+            self.basis = basis
+        else:
+            self.basis = build_basis(symbols, np.zeros((n, 3)), basis)
+        self.symbols = list(self.basis.symbols)
+        self.charge = int(charge)
+        self.multiplicity = int(multiplicity)
+        if reference is None:
+            ref = "rhf" if self.multiplicity == 1 else "uhf"
+        else:
+            key = str(reference).strip().lower()
+            if key not in _REFERENCES:
+                raise ValueError(f"unknown reference {reference!r}; use 'rhf' or 'uhf'")
+            ref = _REFERENCES[key]
+        self.reference = ref
+        self.n_alpha, self.n_beta = electron_counts(
+            int(self.basis.atomic_numbers.sum()), self.charge, self.multiplicity, ref)
+        if self.n_alpha > self.basis.nao:
+            raise ValueError(f"{self.n_alpha} alpha electrons do not fit in {self.basis.nao} AOs")
+        self.engine = qengine.Engine(self.basis)
+        self._ints: SCFIntegrals | None = None
+        self._ints_positions: np.ndarray | None = None
+        self._ints_options: SCFOptions | None = None
+        self._sad: np.ndarray | None = None
+
+    @property
+    def unrestricted(self) -> bool:
+        return self.reference == "uhf"
+
+    @property
+    def nao(self) -> int:
+        return self.basis.nao
+
+    @property
+    def nelectron(self) -> int:
+        return self.n_alpha + self.n_beta
+
+    @property
+    def density_shape(self) -> tuple[int, ...]:
+        n = self.nao
+        return (2, n, n) if self.unrestricted else (n, n)
+
+    @property
+    def device(self) -> torch.device:
+        return self.engine.device
+
+    def integrals(self, positions: np.ndarray) -> SCFIntegrals:
+        """Integrals at ``positions`` (bohr); cached for the last geometry (aimd/qc/scf.py:770-782)."""
+        x = np.array(positions, dtype=float).reshape(-1, 3)
+        if x.shape[0] != self.basis.natm:
+            raise ValueError(f"expected {self.basis.natm} positions, got {x.shape[0]}")
+        check_nuclear_separation(x, self.basis.atomic_numbers)
+        if (self._ints is None or self._ints_options is not self.options
+                or not np.array_equal(x, self._ints_positions)):
+            self._ints = None
+            self._ints = build_integrals(self.basis.with_positions(x), self.engine, self.options)
+            self._ints_positions = x
+            self._ints_options = self.options
+        return self._ints
+
+    def clear_cache(self) -> None:
+        self._ints = None
+        self._ints_positions = None
+
+    def close(self) -> None:
+        self.clear_cache()
+        self.engine.close()
+
+    # ---------------------------------------------------------------- guesses
+    def _nocc(self) -> tuple[int, ...]:
+        return (self.n_alpha,) if not self.unrestricted else (self.n_alpha, self.n_beta)
+
+    def _occupied_density(self, F: torch.Tensor, X: torch.Tensor, nocc: int) -> torch.Tensor:
+        _, Cp = torch.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp[:, :nocc]
+        return C @ C.T
+
+    def initial_spin_densities(self, I: SCFIntegrals, guess: Any) -> tuple[torch.Tensor, str]:
+        """(nspin, nao, nao) starting spin densities (aimd/qc/scf.py:793-827)."""
+        nspin = 2 if self.unrestricted else 1
+        n = self.nao
+        dev = self.device
+        if isinstance(guess, SCFResult):
+            if guess.basis.nao != n:
+                raise ValueError("guess SCFResult has a different basis")
+            D = guess.spin_densities
+            Ds = D[:nspin].copy() if self.unrestricted else 0.5 * (D[0] + D[1])[None]
+            return torch.as_tensor(Ds, dtype=torch.float64, device=dev), "result"
+        if guess is None:
+            guess = self.options.guess
+        if isinstance(guess, str):
+            kind = guess.strip().lower()
+            if kind == "sad":
+                if self._sad is None:
+                    self._sad = sad_density(self.basis)
+                Ds = np.repeat(0.5 * self._sad[None], nspin, axis=0)
+                return torch.as_tensor(Ds, dtype=torch.float64, device=dev), kind
+            if kind in ("core", "gwh"):
+                if kind == "core":
+                    F = I.H
+                else:
+                    h = torch.diagonal(I.H)
+                    F = 0.5 * GWH_K * I.S * (h[:, None] + h[None, :])
+                    F = F - torch.diag(torch.diagonal(F)) + torch.diag(h)
+                return torch.stack([self._occupied_density(F, I.X, k) for k in self._nocc()]), kind
+            raise ValueError(f"unknown guess {guess!r}; use one of {_GUESSES} or a density")
+        D = np.array(guess, dtype=float)
+        if D.shape == (n, n):
+            Ds = np.repeat(0.5 * D[None], nspin, axis=0)
+        elif D.shape == (2, n, n):
+            Ds = D.copy() if self.unrestricted else 0.5 * (D[0] + D[1])[None]
+        else:
+            raise ValueError(
+                f"density guess has shape {D.shape}; expected {self.density_shape} "
+                f"({self.reference.upper()}) or {(n, n) if self.unrestricted else (2, n, n)}")
+        if not np.all(np.isfinite(Ds)):
+            raise ValueError("density guess contains non-finite values")
+        Ds = 0.5 * (Ds + Ds.transpose(0, 2, 1))
+        return torch.as_tensor(Ds, dtype=torch.float64, device=dev), "density"
+
+    # ------------------------------------------------------------------- SCF
+    def run(self, positions: np.ndarray, guess: Any = None) -> SCFResult:
+        return self._run(positions, guess, stacklevel=3)
+
+    def _run(self, positions: np.ndarray, guess: Any, stacklevel: int) -> SCFResult:
+        t0 = time.perf_counter()
+        I = self.integrals(positions)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        Ds, label = self.initial_spin_densities(I, guess)
+        t2 = time.perf_counter()
+        if I.nmo < self.n_alpha:
+            raise ValueError(f"{self.n_alpha} alpha electrons do not fit in {I.nmo} "
+                             "linearly independent orbitals")
+        state = self._iterate(I, Ds)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        res = self._result(I, state, label)
+        res.timings = {"integrals": t1 - t0, "guess": t2 - t1, "iterations": t3 - t2,
+                       "total": time.perf_counter() - t0}
+        if not res.converged and self.options.warn_unconverged:
+            warnings.warn(
+                f"SCF not converged after {res.iterations} iterations "
+                f"(dE = {res.energy_change:.2e} Eh, commutator {res.commutator_norm:.2e}); "
+                "returning the last iterate flagged converged=False",
+                SCFConvergenceWarning, stacklevel=stacklevel)
+        return res
+
+    def _fock(self, I: SCFIntegrals, Ds: torch.Tensor) -> torch.Tensor:
+        g = 2.0 / Ds.shape[0]
+        J, K = I.engine.jk(Ds)
+        F = I.H + g * J.sum(dim=0) - K
+        return 0.5 * (F + F.transpose(1, 2))
+
+    def _iterate(self, I: SCFIntegrals, Ds: torch.Tensor) -> dict[str, Any]:
+        opt = self.options
+        nspin = Ds.shape[0]
+        g = 2.0 / nspin
+        nocc = self._nocc()
+        S, H, X = I.S, I.H, I.X
+        XtS = X.T @ S
+        diis = DIIS(opt.diis_space) if opt.diis else None
+        e_prev = np.nan
+        best = np.inf
+        since_best = 0
+        restarts = 0
+        history: list[dict[str, float]] = []
+        converged = False
+        eye = torch.eye(I.nmo, dtype=torch.float64, device=S.device)
+        for it in range(1, opt.max_iter + 1):
+            F = self._fock(I, Ds)
+            e_dev = 0.5 * g * torch.sum(Ds * (H + F))
+            FDS = F @ Ds @ S
+            err = X.T @ (FDS - FDS.transpose(1, 2)) @ X
+            scal = torch.stack([e_dev, err.abs().max()]).cpu().numpy()
+            e_el, err_norm = float(scal[0]), float(scal[1])
+            de = e_el - e_prev
+            history.append({"energy": e_el + I.nuclear_repulsion, "energy_change": de,
+                            "commutator": err_norm, "diis": len(diis) if diis else 0})
+            if abs(de) < opt.conv_energy and err_norm < opt.conv_commutator:
+                converged = True
+                break
+            if it == opt.max_iter:
+                break
+            Fo = X.T @ F @ X
+            if diis is not None:
+                if err_norm < best:
+                    best, since_best = err_norm, 0
+                else:
+                    since_best += 1
+                if opt.diis_restart and since_best >= opt.diis_restart:
+                    diis.reset()
+                    restarts += 1
+                    best, since_best = err_norm, 0
+                diis.push(Fo, err)
+                Fo = diis.extrapolate()
+            stabilize = err_norm > opt.stabilize_until
+            if stabilize and opt.level_shift > 0.0:
+                Do = XtS @ Ds @ XtS.T
+                Fo = Fo + opt.level_shift * (eye - Do)
+            _, Cp = torch.linalg.eigh(Fo)
+            Dn = torch.empty_like(Ds)
+            for s in range(nspin):
+                C = X @ Cp[s][:, :nocc[s]]
+                Dn[s] = C @ C.T
+            if stabilize and opt.damping > 0.0:
+                Dn = (1.0 - opt.damping) * Dn + opt.damping * Ds
+            Ds = Dn
+            e_prev = e_el
+        return {"Ds": Ds, "F": F, "e_el": e_el, "de": de, "err": err_norm, "it": it,
+                "converged": converged, "history": history, "restarts": restarts}
+
+    def _result(self, I: SCFIntegrals, st: dict[str, Any], label: str) -> SCFResult:
+        basis = I.basis
+        Ds, F = st["Ds"], st["F"]
+        nspin = Ds.shape[0]
+        g = 2.0 / nspin
+        nocc = self._nocc()
+        X, S = I.X, I.S
+        eps_t, Cp = torch.linalg.eigh(X.T @ F @ X)
+        C_t = X @ Cp
+        W = torch.zeros((I.nao, I.nao), dtype=torch.float64, device=S.device)
+        for s in range(nspin):
+            Co = C_t[s][:, :nocc[s]]
+            W = W + g * (Co * eps_t[s, :nocc[s]]) @ Co.T
+        spin_t = torch.stack([Ds[0], Ds[-1]])
+        P = spin_t[0] + spin_t[1]
+        PS_diag = torch.diagonal(P @ S)
+        dip_el = -torch.einsum("xmn,mn->x", I.M, P)
+        s2_t = torch.sum((spin_t[0] @ S) * (spin_t[1] @ S).T) if nspin == 2 else None
+        host = [eps_t, C_t, spin_t, F, 0.5 * (W + W.T), PS_diag, dip_el]
+        if s2_t is not None:
+            host.append(s2_t)
+        host = [t.cpu().numpy() for t in host]
+        eps, C, spin, Fh, Wh, psd, dipel = host[:7]
+        q = basis.nuclear_charges.copy()
+        np.subtract.at(q, basis.ao_atom, psd)
+        dip = dipel + nuclear_dipole(basis.nuclear_charges, basis.positions)
+        sz = 0.5 * (self.n_alpha - self.n_beta)
+        s2 = sz * (sz + 1.0) + self.n_beta - float(host[7]) if nspin == 2 else 0.0
+        occ = np.zeros((nspin, I.nmo))
+        for s in range(nspin):
+            occ[s, :nocc[s]] = g
+        if nspin == 1:
+            eps, C, occ, Fh = eps[0], C[0], occ[0], Fh[0]
+        e_el = st["e_el"]
+        return SCFResult(
+            reference=self.reference, converged=st["converged"], iterations=st["it"],
+            energy=e_el + I.nuclear_repulsion, electronic_energy=e_el,
+            nuclear_repulsion=I.nuclear_repulsion, mo_energy=eps, mo_coeff=C, mo_occ=occ,
+            spin_densities=spin, fock=Fh, energy_weighted_density=Wh,
+            mulliken_charges=q, dipole=dip, s2=float(s2), n_alpha=self.n_alpha,
+            n_beta=self.n_beta, energy_change=float(st["de"]), commutator_norm=st["err"],
+            basis=basis, n_removed=I.n_removed, guess=label, diis_restarts=st["restarts"],
+            history=st["history"], integrals=I, device_spin_densities=Ds, device_fock=F)
+
+
+def run_scf(symbols: Sequence[str], positions: np.ndarray, basis: str | BasisSet = "sto-3g", charge: int = 0,
+            multiplicity: int = 1, reference: str | None = None, guess: Any = None,
+            options: SCFOptions | None = None, **overrides: Any) -> SCFResult:
+    """One-shot SCF: ``SCFSolver(...).run(positions, guess)`` (aimd/qc/scf.py:968-982)."""
+    solver = SCFSolver(symbols, basis, charge=charge, multiplicity=multiplicity,
+                       reference=reference, options=options, **overrides)
+    return solver._run(positions, guess, stacklevel=3)

@@ -106,13 +106,15 @@ struct aimdhf_plan {
     DevBuf d_pair_a, d_pair_b, d_pair_cls, d_rec_size, d_rec_off, d_cls_pairs, d_cls_pair_off, d_ncls;
     DevBuf d_bp_off, d_bp_list, d_boys;
     DevBuf d_recs, d_npp, d_qshell, d_qblock, d_qscr, d_counts, d_offsets, d_list;
-    DevBuf d_fx, d_dsym;
+    DevBuf d_fx, d_dsym, d_charges, d_glist, d_gfx, d_gden, d_scratch;
+    std::vector<long long> gcount;
 
     ~aimdhf_plan() {
         for (DevBuf* b : {&d_shell_atom, &d_shell_l, &d_shell_prim, &d_shell_ao, &d_prim_exp, &d_prim_coef,
                           &d_pos, &d_pair_a, &d_pair_b, &d_pair_cls, &d_rec_size, &d_rec_off, &d_cls_pairs,
                           &d_cls_pair_off, &d_ncls, &d_bp_off, &d_bp_list, &d_boys, &d_recs, &d_npp,
-                          &d_qshell, &d_qblock, &d_qscr, &d_counts, &d_offsets, &d_list, &d_fx, &d_dsym})
+                          &d_qshell, &d_qblock, &d_qscr, &d_counts, &d_offsets, &d_list, &d_fx, &d_dsym,
+                          &d_charges, &d_glist, &d_gfx, &d_gden, &d_scratch})
             b->release();
     }
 
@@ -149,6 +151,52 @@ struct aimdhf_plan {
         long long off = 0;
         for (int i = 0; i < k; ++i) off += qcount[i];
         return d_list.as<int2>() + off;
+    }
+
+    // Class-binned list of unique shell quartets with q_P q_Q >= thr into buf;
+    // counts[k] receives the per-class sizes. Synchronises the stream.
+    void build_list(double thr, DevBuf& buf, std::vector<long long>& counts, hipStream_t st) {
+        aimdhf::PairArgs g = pair_args();
+        aimdhf::launch_count_quartets(g, thr, d_counts.as<int>(), st);
+        HIPCHECK(hipGetLastError());
+        std::vector<int> cnt((size_t)npair * NQCLASS);
+        HIPCHECK(hipMemcpyAsync(cnt.data(), d_counts.p, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        std::vector<long long> tot(NQCLASS, 0);
+        for (int P = 0; P < npair; ++P)
+            for (int k = 0; k < NQCLASS; ++k) tot[k] += cnt[(size_t)P * NQCLASS + k];
+        std::vector<long long> run(NQCLASS, 0);
+        for (int k = 1; k < NQCLASS; ++k) run[k] = run[k - 1] + tot[k - 1];
+        long long total = run[NQCLASS - 1] + tot[NQCLASS - 1];
+        std::vector<long long> offs((size_t)npair * NQCLASS);
+        for (int P = 0; P < npair; ++P)
+            for (int k = 0; k < NQCLASS; ++k) {
+                offs[(size_t)P * NQCLASS + k] = run[k];
+                run[k] += cnt[(size_t)P * NQCLASS + k];
+            }
+        counts = tot;
+        buf.ensure(sizeof(int2) * std::max<long long>(total, 1));
+        HIPCHECK(hipMemcpyAsync(d_offsets.p, offs.data(), sizeof(long long) * offs.size(), hipMemcpyHostToDevice, st));
+        aimdhf::launch_fill_quartets(g, thr, d_offsets.as<long long>(), buf.as<int2>(), st);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    aimdhf::OneElArgs oneel_args() const {
+        aimdhf::OneElArgs g;
+        g.npair = npair;
+        g.nao = nao;
+        g.natm = natm;
+        g.shell_atom = d_shell_atom.as<int>();
+        g.shell_l = d_shell_l.as<int>();
+        g.shell_prim = d_shell_prim.as<int>();
+        g.shell_ao = d_shell_ao.as<int>();
+        g.prim_exp = d_prim_exp.as<double>();
+        g.prim_coef = d_prim_coef.as<double>();
+        g.pos = d_pos.as<double>();
+        g.charges = d_charges.as<double>();
+        g.boys = d_boys.as<double>();
+        return g;
     }
 };
 
@@ -269,6 +317,8 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
         upload(p->d_bp_off, p->bp_off);
         upload(p->d_bp_list, p->bp_list);
         upload(p->d_boys, boys_table());
+        upload(p->d_charges, p->charges);
+        p->d_scratch.ensure(sizeof(double) * 1024);
         p->d_pos.ensure(sizeof(double) * 3 * natm);
         p->d_recs.ensure(sizeof(double) * std::max<long long>(off, 1));
         p->d_npp.ensure(sizeof(int) * p->npair);
@@ -306,31 +356,10 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
         aimdhf::PairArgs g = p->pair_args();
         aimdhf::launch_build_pairs(g, st);
         aimdhf::launch_schwarz(g, p->ncls.data(), st);
-        aimdhf::launch_count_quartets(g, thr, p->d_counts.as<int>(), st);
         HIPCHECK(hipGetLastError());
-        std::vector<int> counts((size_t)p->npair * NQCLASS);
-        HIPCHECK(hipMemcpyAsync(counts.data(), p->d_counts.p, sizeof(int) * counts.size(), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        std::vector<long long> tot(NQCLASS, 0);
-        for (int P = 0; P < p->npair; ++P)
-            for (int k = 0; k < NQCLASS; ++k) tot[k] += counts[(size_t)P * NQCLASS + k];
-        std::vector<long long> base(NQCLASS + 1, 0);
-        for (int k = 0; k < NQCLASS; ++k) base[k + 1] = base[k] + tot[k];
-        std::vector<long long> offs((size_t)p->npair * NQCLASS);
-        std::vector<long long> run(base.begin(), base.end() - 1);
-        for (int P = 0; P < p->npair; ++P)
-            for (int k = 0; k < NQCLASS; ++k) {
-                offs[(size_t)P * NQCLASS + k] = run[k];
-                run[k] += counts[(size_t)P * NQCLASS + k];
-            }
-        p->qcount = tot;
-        p->qtotal = base[NQCLASS];
-        p->d_list.ensure(sizeof(int2) * std::max<long long>(p->qtotal, 1));
-        HIPCHECK(hipMemcpyAsync(p->d_offsets.p, offs.data(), sizeof(long long) * offs.size(), hipMemcpyHostToDevice, st));
-        g = p->pair_args();
-        aimdhf::launch_fill_quartets(g, thr, p->d_offsets.as<long long>(), p->d_list.as<int2>(), st);
-        HIPCHECK(hipGetLastError());
-        HIPCHECK(hipStreamSynchronize(st));
+        p->build_list(thr, p->d_list, p->qcount, st);
+        p->qtotal = 0;
+        for (long long c : p->qcount) p->qtotal += c;
         p->threshold = thr;
         p->have_geometry = true;
     } catch (const HipError& e) {
@@ -406,6 +435,99 @@ int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K
         for (int k = 0; k < NQCLASS; ++k) lists[k] = p->list_of_class(k);
         aimdhf::launch_jk(a, lists, p->qcount.data(), st);
         aimdhf::launch_jk_finalize(p->d_fx.as<long long>(), nset, p->nao, J, K, st);
+        HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_one_electron(aimdhf_plan* p, double* S, double* T, double* V, double* M, void* stream) {
+    if (!p || !S || !T || !V || !M) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    hipStream_t st = (hipStream_t)stream;
+    try {
+        aimdhf::launch_oneel(p->oneel_args(), S, T, M, V, st);
+        HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_grad_one(aimdhf_plan* p, const double* D, const double* W, double* grad, void* stream) {
+    if (!p || !D || !W || !grad) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    hipStream_t st = (hipStream_t)stream;
+    try {
+        size_t nn = (size_t)p->nao * p->nao;
+        p->d_gden.ensure(sizeof(double) * 2 * nn);
+        double* Ds = p->d_gden.as<double>();
+        double* Wn = Ds + nn;
+        aimdhf::launch_symmetrize(D, 1, p->nao, Ds, st);
+        aimdhf::launch_symmetrize(W, 1, p->nao, Wn, st);
+        aimdhf::launch_negate(Wn, nn, Wn, st);
+        p->d_gfx.ensure(sizeof(long long) * 6 * p->natm);
+        aimdhf::launch_zero64(p->d_gfx.as<long long>(), 6 * p->natm, st);
+        aimdhf::launch_grad_one(p->oneel_args(), Ds, Wn, p->d_gfx.as<long long>(), st);
+        aimdhf::launch_fx_to_double(p->d_gfx.as<long long>(), 3 * p->natm, grad, st);
+        HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, double kfac, double thr,
+                    double* grad, void* stream) {
+    if (!p || !Dc || !Dx || !grad) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (nx < 1 || nx > 2) return fail(AIMDHF_E_ARG, "nx must be 1 (RHF) or 2 (UHF)");
+    if (!(thr >= 0.0)) return fail(AIMDHF_E_ARG, "gradient screening threshold must be >= 0");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    hipStream_t st = (hipStream_t)stream;
+    try {
+        size_t nn = (size_t)p->nao * p->nao;
+        p->d_gden.ensure(sizeof(double) * (1 + nx) * nn);
+        double* dc = p->d_gden.as<double>();
+        double* dx = dc + nn;
+        aimdhf::launch_symmetrize(Dc, 1, p->nao, dc, st);
+        aimdhf::launch_symmetrize(Dx, nx, p->nao, dx, st);
+        // candidate quartets: q_P q_Q >= thr / max|Gamma| over every possible element
+        double mc = aimdhf::device_absmax(dc, nn, p->d_scratch.as<double>(), st);
+        double mx = aimdhf::device_absmax(dx, nx * nn, p->d_scratch.as<double>(), st);
+        if (mc < 0.0 || mx < 0.0) return fail(AIMDHF_E_HIP, "max reduction failed");
+        double gbound = 4.0 * (mc * mc + std::fabs(kfac) * nx * mx * mx);
+        double lthr = gbound > 0.0 ? thr / gbound : 0.0;
+        p->build_list(lthr, p->d_glist, p->gcount, st);
+        p->d_gfx.ensure(sizeof(long long) * 6 * p->natm);
+        aimdhf::launch_zero64(p->d_gfx.as<long long>(), 6 * p->natm, st);
+        aimdhf::GradArgs a;
+        a.nao = p->nao;
+        a.natm = p->natm;
+        a.nx = nx;
+        a.kfac = kfac;
+        a.thr = thr;
+        a.pair_a = p->d_pair_a.as<int>();
+        a.pair_b = p->d_pair_b.as<int>();
+        a.shell_ao = p->d_shell_ao.as<int>();
+        a.shell_atom = p->d_shell_atom.as<int>();
+        a.npp = p->d_npp.as<int>();
+        a.rec_off = p->d_rec_off.as<long long>();
+        a.recs = p->d_recs.as<double>();
+        a.boys = p->d_boys.as<double>();
+        a.pos = p->d_pos.as<double>();
+        a.qscr = p->d_qscr.as<double>();
+        a.dc = dc;
+        a.dx = dx;
+        a.out = p->d_gfx.as<long long>();
+        const int2* lists[NQCLASS];
+        long long off = 0;
+        for (int k = 0; k < NQCLASS; ++k) {
+            lists[k] = p->d_glist.as<int2>() + off;
+            off += p->gcount[k];
+        }
+        aimdhf::launch_grad(a, lists, p->gcount.data(), st);
+        aimdhf::launch_fx_to_double(p->d_gfx.as<long long>(), 3 * p->natm, grad, st);
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
         return e.code;

@@ -46,6 +46,42 @@ struct JKArgs {
     long long* fx;              // (nset, 4, nao, nao): J hi, J lo, K hi, K lo
 };
 
+struct OneElArgs {
+    int npair;                  // shell pairs s1 >= s2
+    int nao;
+    int natm;
+    const int* shell_atom;
+    const int* shell_l;
+    const int* shell_prim;
+    const int* shell_ao;
+    const double* prim_exp;
+    const double* prim_coef;
+    const double* pos;
+    const double* charges;
+    const double* boys;
+};
+
+struct GradArgs {
+    int nao;
+    int natm;
+    int nx;
+    double kfac;
+    double thr;
+    const int* pair_a;
+    const int* pair_b;
+    const int* shell_ao;
+    const int* shell_atom;
+    const int* npp;
+    const long long* rec_off;
+    const double* recs;
+    const double* boys;
+    const double* pos;
+    const double* qscr;
+    const double* dc;           // (nao, nao), symmetric
+    const double* dx;           // (nx, nao, nao), symmetric
+    long long* out;             // (natm, 3) fixed point, hi/lo interleaved
+};
+
 void launch_build_pairs(const PairArgs& g, hipStream_t st);
 void launch_schwarz(const PairArgs& g, const int* ncls_host, hipStream_t st);
 void launch_count_quartets(const PairArgs& g, double thr, int* counts, hipStream_t st);
@@ -53,6 +89,12 @@ void launch_fill_quartets(const PairArgs& g, double thr, const long long* offset
 void launch_zero64(long long* p, size_t n, hipStream_t st);
 void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st);
 void launch_jk(const JKArgs& a, const int2* const* lists, const long long* counts, hipStream_t st);
+void launch_oneel(const OneElArgs& g, double* S, double* T, double* M, double* V, hipStream_t st);
+void launch_grad_one(const OneElArgs& g, const double* D, const double* Wneg, long long* out, hipStream_t st);
+void launch_negate(const double* W, size_t n, double* out, hipStream_t st);
+void launch_fx_to_double(const long long* fx, int n, double* out, hipStream_t st);
+void launch_grad(const GradArgs& a, const int2* const* lists, const long long* counts, hipStream_t st);
+double device_absmax(const double* x, size_t n, double* scratch, hipStream_t st);
 void launch_jk_finalize(const long long* fx, int nset, int n, double* J, double* K, hipStream_t st);
 
 }  // namespace aimdhf

@@ -155,6 +155,31 @@ __global__ void k_symmetrize(const double* D, int nset, int n, double* out) {
     out[i] = 0.5 * (D[s * nn + r * n + c] + D[s * nn + c * n + r]);
 }
 
+// max |x| over n doubles: per-block maxima, then one block reduces them
+__global__ void k_absmax_part(const double* x, size_t n, double* part) {
+    __shared__ double sm[256];
+    double m = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(x[i]));
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+double device_absmax(const double* x, size_t n, double* scratch, hipStream_t st) {
+    const int nb = 256;
+    k_absmax_part<<<dim3(nb), dim3(256), 0, st>>>(x, n, scratch);
+    k_absmax_part<<<dim3(1), dim3(256), 0, st>>>(scratch, nb, scratch + nb);
+    double h = 0.0;
+    if (hipMemcpyAsync(&h, scratch + nb, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) return -1.0;
+    if (hipStreamSynchronize(st) != hipSuccess) return -1.0;
+    return h;
+}
+
 __global__ void k_zero64(long long* p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;

@@ -1,0 +1,134 @@
+// Derivative-ERI gradient kernel of one quartet class (compiled once per
+// class with -DAIMDHF_QCLASS=k, like jk_class.hip).
+//
+// Work item = (quartet, j): j < NCD is the bra pass for ket component j,
+// NCD <= j < NCD + NAB the ket pass for bra component j - NCD. Quartets are
+// skipped exactly as aimd/qc/integrals.py:1279-1349 does: all four centres on
+// one atom, or Q_P Q_Q max|Gamma| below the threshold; a pass is replaced by
+// translational invariance when its pair sits on one atom. Contributions are
+// accumulated per workgroup in LDS and flushed once, as exact fixed-point
+// integers, so the gradient is bitwise reproducible.
+
+#include "aimdhf_internal.h"
+#include "md_grad.h"
+
+#ifndef AIMDHF_QCLASS
+#error "compile with -DAIMDHF_QCLASS=<0..20>"
+#endif
+
+namespace aimdhf {
+using namespace md;
+
+namespace {
+constexpr int QC = AIMDHF_QCLASS;
+constexpr int pc_hi() { int h = 0; while ((h + 1) * (h + 2) / 2 <= QC) ++h; return h; }
+constexpr int pc_lo() { return QC - pc_hi() * (pc_hi() + 1) / 2; }
+constexpr int la_of(int pc) { return pc == 0 ? 0 : pc <= 2 ? 1 : 2; }
+constexpr int lb_of(int pc) { return pc - la_of(pc) * (la_of(pc) + 1) / 2; }
+constexpr int LA = la_of(pc_hi()), LB = lb_of(pc_hi()), LC = la_of(pc_lo()), LD = lb_of(pc_lo());
+using Qt = Quartet<LA, LB, LC, LD>;
+constexpr int NAB = Qt::NAB, NCD = Qt::NCD, NB = Qt::NB, ND = Qt::ND;
+constexpr int NT = NCD + NAB;
+
+__device__ __forceinline__ void lds_add(long long* lds, int atom, int d, double x) {
+    long long h, l;
+    fx_split(x, h, l);
+    if (h) atomicAdd((unsigned long long*)&lds[2 * (3 * atom + d)], (unsigned long long)h);
+    if (l) atomicAdd((unsigned long long*)&lds[2 * (3 * atom + d) + 1], (unsigned long long)l);
+}
+
+__device__ __forceinline__ double gamma_el(const GradArgs& a, int m, int n, int l, int s, double deg) {
+    const size_t N = a.nao;
+    double v = a.dc[m * N + n] * a.dc[l * N + s];
+    for (int x = 0; x < a.nx; ++x) {
+        const double* X = a.dx + x * N * N;
+        v -= 0.5 * a.kfac * (X[m * N + l] * X[n * N + s] + X[m * N + s] * X[n * N + l]);
+    }
+    return v * deg;
+}
+
+__global__ __launch_bounds__(128) void k_grad(GradArgs a, const int2* __restrict__ list, long long nq) {
+    extern __shared__ long long lds[];
+    for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
+    __syncthreads();
+    const long long nitems = nq * NT;
+    for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
+         it += (long long)gridDim.x * blockDim.x) {
+        const long long q = it / NT;
+        const int j = (int)(it % NT);
+        const int2 pq = list[q];
+        const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
+        const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
+        if (aA == aB && aB == aC && aC == aD) continue;
+        const bool need_bra = aA != aB, need_ket = aC != aD;
+        const bool bra = j < NCD;
+        if (bra && !(need_bra || (!need_ket && NAB >= NCD))) continue;
+        if (!bra && !(need_ket || (!need_bra && NAB < NCD))) continue;
+        double deg = 0.5;
+        if (sA != sB) deg *= 2.0;
+        if (sC != sD) deg *= 2.0;
+        if (pq.x != pq.y) deg *= 2.0;
+        const int oA = a.shell_ao[sA], oB = a.shell_ao[sB], oC = a.shell_ao[sC], oD = a.shell_ao[sD];
+        double gmax = 0.0;
+        for (int ab = 0; ab < NAB; ++ab)
+            for (int cd = 0; cd < NCD; ++cd)
+                gmax = fmax(gmax, fabs(gamma_el(a, oA + ab / NB, oB + ab % NB, oC + cd / ND, oD + cd % ND, deg)));
+        if (a.qscr[pq.x] * a.qscr[pq.y] * gmax < a.thr) continue;
+        const double* rb = a.recs + a.rec_off[pq.x];
+        const double* rk = a.recs + a.rec_off[pq.y];
+        double g1[3], g2[3];
+        if (bra) {
+            const int cd = j;
+            double gam[NAB];
+            for (int ab = 0; ab < NAB; ++ab)
+                gam[ab] = gamma_el(a, oA + ab / NB, oB + ab % NB, oC + cd / ND, oD + cd % ND, deg);
+            grad_pass_kc<LA, LB, LC, LD>(rb, a.npp[pq.x], rk, a.npp[pq.y], a.boys, a.pos + 3 * aA, a.pos + 3 * aB,
+                                         cd, gam, g1, g2);
+            for (int d = 0; d < 3; ++d) {
+                if (need_bra) {
+                    lds_add(lds, aA, d, g1[d]);
+                    lds_add(lds, aB, d, g2[d]);
+                    if (!need_ket) lds_add(lds, aC, d, -(g1[d] + g2[d]));
+                } else {
+                    lds_add(lds, aA, d, g1[d] + g2[d]);
+                    lds_add(lds, aC, d, -(g1[d] + g2[d]));
+                }
+            }
+        } else {
+            const int ab = j - NCD;
+            double gam[NCD];
+            for (int cd = 0; cd < NCD; ++cd)
+                gam[cd] = gamma_el(a, oA + ab / NB, oB + ab % NB, oC + cd / ND, oD + cd % ND, deg);
+            grad_pass_kc<LC, LD, LA, LB>(rk, a.npp[pq.y], rb, a.npp[pq.x], a.boys, a.pos + 3 * aC, a.pos + 3 * aD,
+                                         ab, gam, g1, g2);
+            for (int d = 0; d < 3; ++d) {
+                if (need_ket) {
+                    lds_add(lds, aC, d, g1[d]);
+                    lds_add(lds, aD, d, g2[d]);
+                    if (!need_bra) lds_add(lds, aA, d, -(g1[d] + g2[d]));
+                } else {
+                    lds_add(lds, aC, d, g1[d] + g2[d]);
+                    lds_add(lds, aA, d, -(g1[d] + g2[d]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x)
+        if (lds[i]) atomicAdd((unsigned long long*)&a.out[i], (unsigned long long)lds[i]);
+}
+
+}  // namespace
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+
+void CAT(launch_grad_class_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list, long long nq, hipStream_t st) {
+    long long items = nq * NT;
+    if (items <= 0) return;
+    long long blocks = (items + 127) / 128;
+    if (blocks > 4096) blocks = 4096;
+    k_grad<<<dim3((unsigned)blocks), dim3(128), sizeof(long long) * 6 * a.natm, st>>>(a, list, nq);
+}
+
+}  // namespace aimdhf

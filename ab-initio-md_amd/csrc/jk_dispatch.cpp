@@ -19,4 +19,21 @@ void launch_jk(const JKArgs& a, const int2* const* lists, const long long* count
     for (int k = 0; k < 21; ++k) fns[k](a, lists[k], counts[k], st);
 }
 
+#define DECLG(k) void launch_grad_class_##k(const GradArgs&, const int2*, long long, hipStream_t);
+DECLG(0) DECLG(1) DECLG(2) DECLG(3) DECLG(4) DECLG(5) DECLG(6) DECLG(7) DECLG(8) DECLG(9) DECLG(10)
+DECLG(11) DECLG(12) DECLG(13) DECLG(14) DECLG(15) DECLG(16) DECLG(17) DECLG(18) DECLG(19) DECLG(20)
+#undef DECLG
+
+void launch_grad(const GradArgs& a, const int2* const* lists, const long long* counts, hipStream_t st) {
+    typedef void (*Fn)(const GradArgs&, const int2*, long long, hipStream_t);
+    static const Fn fns[21] = {
+        launch_grad_class_0, launch_grad_class_1, launch_grad_class_2, launch_grad_class_3,
+        launch_grad_class_4, launch_grad_class_5, launch_grad_class_6, launch_grad_class_7,
+        launch_grad_class_8, launch_grad_class_9, launch_grad_class_10, launch_grad_class_11,
+        launch_grad_class_12, launch_grad_class_13, launch_grad_class_14, launch_grad_class_15,
+        launch_grad_class_16, launch_grad_class_17, launch_grad_class_18, launch_grad_class_19,
+        launch_grad_class_20};
+    for (int k = 0; k < 21; ++k) fns[k](a, lists[k], counts[k], st);
+}
+
 }  // namespace aimdhf

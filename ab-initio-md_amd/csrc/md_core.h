@@ -167,37 +167,41 @@ MD_HD void hermite_r(double alpha, double x, double y, double z, double scale,
 }
 
 // 1-D Hermite expansion E^{ij}_t, i <= IM, j <= JM, t <= i+j; e[(i*(JM+1)+j)*(IM+JM+1)+t]
+// (entries with t > i+j are zero)
 template <int IM, int JM>
 MD_HD void hermite_e(double a, double b, double xab, double e00, double* e) {
     constexpr int TS = IM + JM + 1;
     constexpr int JS = JM + 1;
-    double p = a + b;
-    double xpa = -b * xab / p;
-    double xpb = a * xab / p;
-    double h = 0.5 / p;
+    const double p = a + b;
+    const double xpa = -b * xab / p;
+    const double xpb = a * xab / p;
+    const double h = 0.5 / p;
 #pragma unroll
     for (int i = 0; i < (IM + 1) * JS * TS; ++i) e[i] = 0.0;
     e[0] = e00;
-#pragma unroll
-    for (int j = 1; j <= JM; ++j)
-#pragma unroll
-        for (int t = 0; t <= j; ++t) {
+    static_for<1, JM + 1>([&](auto jj) {
+        constexpr int j = decltype(jj)::value;
+        static_for<0, j + 1>([&](auto tt) {
+            constexpr int t = decltype(tt)::value;
             double v = xpb * e[(j - 1) * TS + t];
-            if (t + 1 <= j - 1) v += (t + 1) * e[(j - 1) * TS + t + 1];
-            if (t > 0) v += h * e[(j - 1) * TS + t - 1];
+            if constexpr (t + 1 <= j - 1) v += (t + 1) * e[(j - 1) * TS + t + 1];
+            if constexpr (t > 0) v += h * e[(j - 1) * TS + t - 1];
             e[j * TS + t] = v;
-        }
-#pragma unroll
-    for (int i = 1; i <= IM; ++i)
-#pragma unroll
-        for (int j = 0; j <= JM; ++j)
-#pragma unroll
-            for (int t = 0; t <= i + j; ++t) {
+        });
+    });
+    static_for<1, IM + 1>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        static_for<0, JM + 1>([&](auto jj) {
+            constexpr int j = decltype(jj)::value;
+            static_for<0, i + j + 1>([&](auto tt) {
+                constexpr int t = decltype(tt)::value;
                 double v = xpa * e[((i - 1) * JS + j) * TS + t];
-                if (t + 1 <= i - 1 + j) v += (t + 1) * e[((i - 1) * JS + j) * TS + t + 1];
-                if (t > 0) v += h * e[((i - 1) * JS + j) * TS + t - 1];
+                if constexpr (t + 1 <= i - 1 + j) v += (t + 1) * e[((i - 1) * JS + j) * TS + t + 1];
+                if constexpr (t > 0) v += h * e[((i - 1) * JS + j) * TS + t - 1];
                 e[(i * JS + j) * TS + t] = v;
-            }
+            });
+        });
+    });
 }
 
 // ---------------------------------------------------------------- pair records

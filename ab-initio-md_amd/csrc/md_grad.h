@@ -1,0 +1,108 @@
+// Derivative-ERI contraction of one shell quartet with its two-particle
+// density Gamma (aimd/qc/integrals.py:1049-1369), recomputing the derivative
+// integrals on the fly.
+//
+// A "pass" differentiates with respect to the two centres of the bra pair:
+//   d(ab|cd)/dA_x = sum_prim [2a (a+1_x b|cd) - a_x (a-1_x b|cd)],
+// i.e. the bra Hermite expansion is replaced by the shifted expansion
+// 2a E^{a_x+1,b_x}_t - a_x E^{a_x-1,b_x}_t (t <= a_x+b_x+1), contracted with
+// the ket-summed Hermite integrals X[h] = sum_k (-1)^|k| E^{cd}_k R_{h+k} of
+// degree up to l_A+l_B+1. The ket centres come from a second pass with the
+// pair roles swapped, or from translational invariance when a pair sits on
+// one atom. Each pass handles one Cartesian component pair kc of its ket
+// (one thread per (quartet, kc)), with gam[ab] = Gamma[ab][kc].
+
+#pragma once
+#include "md_eri.h"
+#include "md_pairs.h"
+
+namespace md {
+
+template <int LA, int LB, int LC, int LD>
+MD_HD void grad_pass_kc(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
+                        const double* __restrict__ tab, const double* Apos, const double* Bpos, int kc,
+                        const double* gam, double* gA, double* gB) {
+    using Q = Quartet<LA, LB, LC, LD>;
+    constexpr int NAB = Q::NAB, NKC = Q::NCD, NHK = Q::NHK;
+    constexpr int NHB1 = NH(Q::LAB + 1), L1 = Q::L + 1;
+    constexpr int NA = Q::NA, NB = Q::NB;
+    constexpr int IS = LA + 2, JS = LB + 2, TS = LA + LB + 3;
+    double xab[3];
+    for (int d = 0; d < 3; ++d) { gA[d] = 0.0; gB[d] = 0.0; xab[d] = Apos[d] - Bpos[d]; }
+    double fcart[NAB];
+    static_for<0, NAB>([&](auto aa) {
+        constexpr int ab = decltype(aa)::value;
+        constexpr int ia = ab / NB, ib = ab % NB;
+        fcart[ab] = gam[ab] * cart_factor(cart_x(LA, ia), cart_y(LA, ia), cart_z(LA, ia)) *
+                    cart_factor(cart_x(LB, ib), cart_y(LB, ib), cart_z(LB, ib));
+    });
+    for (int ibp = 0; ibp < nb; ++ibp) {
+        const double* Bp = rb + ibp * Q::RSB;
+        const double p = Bp[R_P], a = Bp[R_A], b = Bp[R_B], cab = Bp[R_C];
+        const double px = Bp[R_PX], py = Bp[R_PY], pz = Bp[R_PZ];
+        double X[NHB1];
+#pragma unroll
+        for (int i = 0; i < NHB1; ++i) X[i] = 0.0;
+        for (int ik = 0; ik < nk; ++ik) {
+            const double* K = rk + ik * Q::RSK;
+            const double q = K[R_P];
+            const double s = p + q;
+            const double alpha = p * q / s;
+            const double pref = TWO_PI_2_5 / (p * q * sqrt(s));
+            double R[NH(L1)];
+            hermite_r<L1>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
+            static_for<0, NHK>([&](auto kk) {
+                constexpr int k = decltype(kk)::value;
+                constexpr int tk = HT.t[k], uk = HT.u[k], vk = HT.v[k];
+                constexpr bool odd = (tk + uk + vk) & 1;
+                const double e = odd ? -K[REC_HDR + k * NKC + kc] : K[REC_HDR + k * NKC + kc];
+                static_for<0, NHB1>([&](auto hh) {
+                    constexpr int h = decltype(hh)::value;
+                    X[h] += R[hidx(HT.t[h] + tk, HT.u[h] + uk, HT.v[h] + vk)] * e;
+                });
+            });
+        }
+        const double mu = a * b / p;
+        double ex[IS * JS * TS], ey[IS * JS * TS], ez[IS * JS * TS];
+        hermite_e<LA + 1, LB + 1>(a, b, xab[0], exp(-mu * xab[0] * xab[0]), ex);
+        hermite_e<LA + 1, LB + 1>(a, b, xab[1], exp(-mu * xab[1] * xab[1]), ey);
+        hermite_e<LA + 1, LB + 1>(a, b, xab[2], exp(-mu * xab[2] * xab[2]), ez);
+        const double ta = 2.0 * a, tb = 2.0 * b;
+        double sA[3] = {0.0, 0.0, 0.0}, sB[3] = {0.0, 0.0, 0.0};
+        static_for<0, NAB>([&](auto aa) {
+            constexpr int ab = decltype(aa)::value;
+            constexpr int ia = ab / NB, ib = ab % NB;
+            constexpr int ax = cart_x(LA, ia), ay = cart_y(LA, ia), az = cart_z(LA, ia);
+            constexpr int bx = cart_x(LB, ib), by = cart_y(LB, ib), bz = cart_z(LB, ib);
+            const double f = fcart[ab] * cab;
+            static_for<0, NHB1>([&](auto hh) {
+                constexpr int h = decltype(hh)::value;
+                constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
+                constexpr int ox = t > ax + bx, oy = u > ay + by, oz = v > az + bz;
+                if constexpr (ox + oy + oz <= 1 && t <= ax + bx + 1 && u <= ay + by + 1 && v <= az + bz + 1) {
+                    auto E = [](const double* e, int i, int j, int tt) { return e[(i * JS + j) * TS + tt]; };
+                    const double y = f * X[h];
+                    const double e0x = E(ex, ax, bx, t), e0y = E(ey, ay, by, u), e0z = E(ez, az, bz, v);
+                    double dax = ta * E(ex, ax + 1, bx, t), dbx = tb * E(ex, ax, bx + 1, t);
+                    double day = ta * E(ey, ay + 1, by, u), dby = tb * E(ey, ay, by + 1, u);
+                    double daz = ta * E(ez, az + 1, bz, v), dbz = tb * E(ez, az, bz + 1, v);
+                    if constexpr (ax > 0) dax -= ax * E(ex, ax - 1, bx, t);
+                    if constexpr (bx > 0) dbx -= bx * E(ex, ax, bx - 1, t);
+                    if constexpr (ay > 0) day -= ay * E(ey, ay - 1, by, u);
+                    if constexpr (by > 0) dby -= by * E(ey, ay, by - 1, u);
+                    if constexpr (az > 0) daz -= az * E(ez, az - 1, bz, v);
+                    if constexpr (bz > 0) dbz -= bz * E(ez, az, bz - 1, v);
+                    sA[0] += y * dax * e0y * e0z;
+                    sA[1] += y * e0x * day * e0z;
+                    sA[2] += y * e0x * e0y * daz;
+                    sB[0] += y * dbx * e0y * e0z;
+                    sB[1] += y * e0x * dby * e0z;
+                    sB[2] += y * e0x * e0y * dbz;
+                }
+            });
+        });
+        for (int d = 0; d < 3; ++d) { gA[d] += sA[d]; gB[d] += sB[d]; }
+    }
+}
+
+}  // namespace md

@@ -92,6 +92,33 @@ int aimdhf_quartets(aimdhf_plan* plan, int64_t* out_host, int64_t cap, int64_t* 
  */
 int aimdhf_jk(aimdhf_plan* plan, int nset, const double* dens, double* J, double* K, void* stream);
 
+/*
+ * One-electron matrices of the current geometry into device buffers:
+ * S, T, V (nao,nao) and the dipole integrals M (3,nao,nao) about the origin,
+ * V over all nuclei as point charges Z_A.
+ * Replaces: one_electron_matrices and dipole_integrals
+ * (aimd/qc/integrals.py:1379-1418).
+ */
+int aimdhf_one_electron(aimdhf_plan* plan, double* S, double* T, double* V, double* M, void* stream);
+
+/*
+ * grad (device (natm,3)) = d/dR [sum D (T+V)] - d/dR [sum W S] at fixed D, W
+ * (symmetric parts used), including the Hellmann-Feynman term.
+ * Replaces: one_electron_gradient (aimd/qc/integrals.py:1483-1495).
+ */
+int aimdhf_grad_one(aimdhf_plan* plan, const double* D, const double* W, double* grad, void* stream);
+
+/*
+ * grad (device (natm,3)) = d/dR of
+ *   E2 = 1/2 sum Dc_mn Dc_ls (mn|ls) - k/2 sum_x sum Dx_ml Dx_ns (mn|ls)
+ * at fixed densities (RHF: Dc = P, Dx = [P], k = 0.5; UHF: Dc = Pa+Pb,
+ * Dx = [Pa, Pb], k = 1), derivative integrals recomputed on the fly; quartets
+ * with Q_P Q_Q max|Gamma| < threshold are skipped. Synchronises the stream.
+ * Replaces: two_electron_gradient (aimd/qc/integrals.py:1498-1520).
+ */
+int aimdhf_grad_two(aimdhf_plan* plan, const double* Dc, int nx, const double* Dx, double kfac,
+                    double threshold, double* grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
