@@ -36,6 +36,10 @@ SIGNATURES = {
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_grad_one": (_I, [_P, _P, _P, _P, _P]),
     "aimdhf_grad_two": (_I, [_P, _P, _I, _P, _D, _D, _P, _P]),
+    "aimdhf_profile_enable": (_I, [_P, _I]),
+    "aimdhf_profile_read": (_I, [_P, _P, _P, _I, _I]),
+    "aimdhf_class_counts": (_I, [_P, _P, _P]),
+    "aimdhf_pair_npp": (_I, [_P, _P, _L]),
 }
 
 _lib = None

@@ -132,3 +132,31 @@ class Engine:
         _lib.check(_lib.lib().aimdhf_grad_two(self._h, _ptr(Dc), Dx.shape[0], _ptr(Dx), float(kfac),
                                               float(threshold), _ptr(g), _stream()))
         return g
+
+    # ------------------------------------------------------------ profiling
+    KIND_NAMES = ([f"jk_class_{k}" for k in range(21)] + [f"grad_class_{k}" for k in range(21)]
+                  + ["geometry", "one_electron", "grad_one", "jk_prologue_epilogue"])
+
+    def profile(self, on: bool = True) -> None:
+        """Record HIP events around every engine launch (on torch's current stream)."""
+        _lib.check(_lib.lib().aimdhf_profile_enable(self._h, int(bool(on))))
+
+    def profile_read(self, reset: bool = False) -> tuple[np.ndarray, np.ndarray]:
+        """(milliseconds, launch counts) per kind, 46 kinds (see KIND_NAMES)."""
+        ms = np.zeros(64)
+        n = np.zeros(64, np.int64)
+        _lib.check(_lib.lib().aimdhf_profile_read(self._h, _ptr(ms), _ptr(n), 64, int(bool(reset))))
+        return ms[:46], n[:46]
+
+    def class_counts(self) -> tuple[np.ndarray, np.ndarray]:
+        """Per-class sizes of the energy and (last) gradient quartet lists."""
+        e = np.zeros(21, np.int64)
+        g = np.zeros(21, np.int64)
+        _lib.check(_lib.lib().aimdhf_class_counts(self._h, _ptr(e), _ptr(g)))
+        return e, g
+
+    def pair_npp(self) -> np.ndarray:
+        npair = self.basis.nshell * (self.basis.nshell + 1) // 2
+        out = np.zeros(npair, np.int32)
+        _lib.check(_lib.lib().aimdhf_pair_npp(self._h, _ptr(out), npair))
+        return out

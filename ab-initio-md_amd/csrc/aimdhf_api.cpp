@@ -87,9 +87,69 @@ std::vector<double> boys_table() {
     return tab;
 }
 
+// Optional per-launch timing with HIP events on the caller's stream.
+// Kinds: 0..20 J/K class kernels, 21..41 gradient class kernels, 42 geometry
+// set-up, 43 one-electron integrals, 44 one-electron gradient, 45 J/K prologue
+// and epilogue (zeroing, symmetrisation, fixed-point conversion).
+struct Profiler {
+    static constexpr int NKIND = 64;
+    bool on = false;
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> pool;
+    std::vector<Rec> recs;
+    double ms[NKIND] = {0};
+    long long n[NKIND] = {0};
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        return e;
+    }
+    template <class F>
+    void time(int kind, hipStream_t st, F&& launch) {
+        if (!on) {
+            launch();
+            return;
+        }
+        Rec r{kind, get(), get()};
+        HIPCHECK(hipEventRecord(r.a, st));
+        launch();
+        HIPCHECK(hipEventRecord(r.b, st));
+        recs.push_back(r);
+    }
+    void collect() {
+        for (Rec& r : recs) {
+            HIPCHECK(hipEventSynchronize(r.b));
+            float t = 0.0f;
+            HIPCHECK(hipEventElapsedTime(&t, r.a, r.b));
+            ms[r.kind] += t;
+            n[r.kind] += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        recs.clear();
+    }
+    void reset() {
+        collect();
+        for (int i = 0; i < NKIND; ++i) { ms[i] = 0.0; n[i] = 0; }
+    }
+    ~Profiler() {
+        for (Rec& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
 }  // namespace
 
 struct aimdhf_plan {
+    Profiler prof;
     int natm = 0, nshell = 0, nao = 0, npair = 0, nblock = 0, nbpair = 0, lmax = 0;
     std::vector<int> shell_atom, shell_l, shell_prim, shell_ao, shell_block;
     std::vector<double> prim_exp, prim_coef, charges, pos;
@@ -354,8 +414,10 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
         p->pos.assign(positions, positions + 3 * p->natm);
         HIPCHECK(hipMemcpyAsync(p->d_pos.p, positions, sizeof(double) * 3 * p->natm, hipMemcpyHostToDevice, st));
         aimdhf::PairArgs g = p->pair_args();
-        aimdhf::launch_build_pairs(g, st);
-        aimdhf::launch_schwarz(g, p->ncls.data(), st);
+        p->prof.time(42, st, [&] {
+            aimdhf::launch_build_pairs(g, st);
+            aimdhf::launch_schwarz(g, p->ncls.data(), st);
+        });
         HIPCHECK(hipGetLastError());
         p->build_list(thr, p->d_list, p->qcount, st);
         p->qtotal = 0;
@@ -417,8 +479,10 @@ int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K
         size_t nn = (size_t)p->nao * p->nao;
         p->d_fx.ensure(sizeof(long long) * 4 * nset * nn);
         p->d_dsym.ensure(sizeof(double) * nset * nn);
-        aimdhf::launch_zero64(p->d_fx.as<long long>(), 4 * nset * nn, st);
-        aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st);
+        p->prof.time(45, st, [&] {
+            aimdhf::launch_zero64(p->d_fx.as<long long>(), 4 * nset * nn, st);
+            aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st);
+        });
         aimdhf::JKArgs a;
         a.nao = p->nao;
         a.nset = nset;
@@ -431,10 +495,10 @@ int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K
         a.boys = p->d_boys.as<double>();
         a.dens = p->d_dsym.as<double>();
         a.fx = p->d_fx.as<long long>();
-        const int2* lists[NQCLASS];
-        for (int k = 0; k < NQCLASS; ++k) lists[k] = p->list_of_class(k);
-        aimdhf::launch_jk(a, lists, p->qcount.data(), st);
-        aimdhf::launch_jk_finalize(p->d_fx.as<long long>(), nset, p->nao, J, K, st);
+        for (int k = 0; k < NQCLASS; ++k)
+            if (p->qcount[k] > 0)
+                p->prof.time(k, st, [&] { aimdhf::launch_jk_one(k, a, p->list_of_class(k), p->qcount[k], st); });
+        p->prof.time(45, st, [&] { aimdhf::launch_jk_finalize(p->d_fx.as<long long>(), nset, p->nao, J, K, st); });
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
         return e.code;
@@ -447,7 +511,7 @@ int aimdhf_one_electron(aimdhf_plan* p, double* S, double* T, double* V, double*
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
     try {
-        aimdhf::launch_oneel(p->oneel_args(), S, T, M, V, st);
+        p->prof.time(43, st, [&] { aimdhf::launch_oneel(p->oneel_args(), S, T, M, V, st); });
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
         return e.code;
@@ -469,7 +533,7 @@ int aimdhf_grad_one(aimdhf_plan* p, const double* D, const double* W, double* gr
         aimdhf::launch_negate(Wn, nn, Wn, st);
         p->d_gfx.ensure(sizeof(long long) * 6 * p->natm);
         aimdhf::launch_zero64(p->d_gfx.as<long long>(), 6 * p->natm, st);
-        aimdhf::launch_grad_one(p->oneel_args(), Ds, Wn, p->d_gfx.as<long long>(), st);
+        p->prof.time(44, st, [&] { aimdhf::launch_grad_one(p->oneel_args(), Ds, Wn, p->d_gfx.as<long long>(), st); });
         aimdhf::launch_fx_to_double(p->d_gfx.as<long long>(), 3 * p->natm, grad, st);
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
@@ -520,15 +584,62 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
         a.dc = dc;
         a.dx = dx;
         a.out = p->d_gfx.as<long long>();
-        const int2* lists[NQCLASS];
         long long off = 0;
         for (int k = 0; k < NQCLASS; ++k) {
-            lists[k] = p->d_glist.as<int2>() + off;
+            const int2* lst = p->d_glist.as<int2>() + off;
+            if (p->gcount[k] > 0)
+                p->prof.time(21 + k, st, [&] { aimdhf::launch_grad_one_class(k, a, lst, p->gcount[k], st); });
             off += p->gcount[k];
         }
-        aimdhf::launch_grad(a, lists, p->gcount.data(), st);
         aimdhf::launch_fx_to_double(p->d_gfx.as<long long>(), 3 * p->natm, grad, st);
         HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_profile_enable(aimdhf_plan* p, int on) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    try {
+        p->prof.collect();
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    p->prof.on = on != 0;
+    return AIMDHF_OK;
+}
+
+int aimdhf_profile_read(aimdhf_plan* p, double* ms, int64_t* launches, int nkind, int reset) {
+    if (!p || !ms || !launches) return fail(AIMDHF_E_ARG, "NULL argument");
+    try {
+        p->prof.collect();
+        for (int i = 0; i < nkind && i < Profiler::NKIND; ++i) {
+            ms[i] = p->prof.ms[i];
+            launches[i] = p->prof.n[i];
+        }
+        if (reset) p->prof.reset();
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_class_counts(aimdhf_plan* p, int64_t* energy_counts, int64_t* gradient_counts) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    for (int k = 0; k < NQCLASS; ++k) {
+        if (energy_counts) energy_counts[k] = k < (int)p->qcount.size() ? p->qcount[k] : 0;
+        if (gradient_counts) gradient_counts[k] = k < (int)p->gcount.size() ? p->gcount[k] : 0;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_pair_npp(aimdhf_plan* p, int32_t* npp_host, int64_t cap) {
+    if (!p || !npp_host) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    if (cap < p->npair) return fail(AIMDHF_E_ARG, "output too small");
+    try {
+        HIPCHECK(hipMemcpy(npp_host, p->d_npp.p, sizeof(int) * p->npair, hipMemcpyDeviceToHost));
     } catch (const HipError& e) {
         return e.code;
     }

@@ -88,6 +88,8 @@ void launch_count_quartets(const PairArgs& g, double thr, int* counts, hipStream
 void launch_fill_quartets(const PairArgs& g, double thr, const long long* offsets, int2* list, hipStream_t st);
 void launch_zero64(long long* p, size_t n, hipStream_t st);
 void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st);
+void launch_jk_one(int k, const JKArgs& a, const int2* list, long long nq, hipStream_t st);
+void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
 void launch_jk(const JKArgs& a, const int2* const* lists, const long long* counts, hipStream_t st);
 void launch_oneel(const OneElArgs& g, double* S, double* T, double* M, double* V, hipStream_t st);
 void launch_grad_one(const OneElArgs& g, const double* D, const double* Wneg, long long* out, hipStream_t st);

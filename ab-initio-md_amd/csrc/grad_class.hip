@@ -47,7 +47,8 @@ __device__ __forceinline__ double gamma_el(const GradArgs& a, int m, int n, int 
     return v * deg;
 }
 
-__global__ __launch_bounds__(128) void k_grad(GradArgs a, const int2* __restrict__ list, long long nq) {
+template <int Q>
+__global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __restrict__ list, long long nq) {
     extern __shared__ long long lds[];
     for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
     __syncthreads();
@@ -128,7 +129,7 @@ void CAT(launch_grad_class_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list,
     if (items <= 0) return;
     long long blocks = (items + 127) / 128;
     if (blocks > 4096) blocks = 4096;
-    k_grad<<<dim3((unsigned)blocks), dim3(128), sizeof(long long) * 6 * a.natm, st>>>(a, list, nq);
+    k_grad_class<QC><<<dim3((unsigned)blocks), dim3(128), sizeof(long long) * 6 * a.natm, st>>>(a, list, nq);
 }
 
 }  // namespace aimdhf

@@ -40,7 +40,8 @@ __device__ __forceinline__ Sink sink_of(const JKArgs& a, int s) {
     return Sink{a.fx + (4 * s + 0) * nn, a.fx + (4 * s + 1) * nn, a.fx + (4 * s + 2) * nn, a.fx + (4 * s + 3) * nn};
 }
 
-__global__ __launch_bounds__(64) void k_jk(JKArgs a, const int2* __restrict__ list, long long nq) {
+template <int Q>
+__global__ __launch_bounds__(64) void k_jk_class(JKArgs a, const int2* __restrict__ list, long long nq) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     constexpr int NSPLIT = SPLIT ? Qt::NCD : 1;
     if (t >= nq * NSPLIT) return;
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(64) void k_jk(JKArgs a, const int2* __restrict__ li
 void CAT(launch_jk_class_, AIMDHF_QCLASS)(const JKArgs& a, const int2* list, long long nq, hipStream_t st) {
     constexpr int NSPLIT = SPLIT ? Qt::NCD : 1;
     long long nt = nq * NSPLIT;
-    if (nt > 0) k_jk<<<dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, st>>>(a, list, nq);
+    if (nt > 0) k_jk_class<QC><<<dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, st>>>(a, list, nq);
 }
 
 }  // namespace aimdhf

@@ -119,6 +119,26 @@ int aimdhf_grad_one(aimdhf_plan* plan, const double* D, const double* W, double*
 int aimdhf_grad_two(aimdhf_plan* plan, const double* Dc, int nx, const double* Dx, double kfac,
                     double threshold, double* grad, void* stream);
 
+/*
+ * Kernel timing with HIP events recorded on the caller's stream around every
+ * launch while enabled (no counterpart in the reference, whose timings are
+ * wall-clock dicts, aimd/backends/hf.py:347-354). Kinds: 0..20 J/K kernels of
+ * the 21 quartet classes, 21..41 derivative-ERI kernels, 42 geometry set-up,
+ * 43 one-electron integrals, 44 one-electron gradient, 45 J/K prologue and
+ * epilogue. aimdhf_profile_read synchronises, returns the summed
+ * milliseconds and launch counts per kind (nkind <= 64) and optionally resets.
+ */
+int aimdhf_profile_enable(aimdhf_plan* plan, int on);
+int aimdhf_profile_read(aimdhf_plan* plan, double* ms, int64_t* launches, int nkind, int reset);
+
+/* Per-class sizes (21 each) of the energy quartet list and of the last
+ * gradient quartet list (either pointer may be NULL). */
+int aimdhf_class_counts(aimdhf_plan* plan, int64_t* energy_counts, int64_t* gradient_counts);
+
+/* Kept primitive pairs of every shell pair (npair = nshell (nshell+1)/2),
+ * copied to host memory. Synchronous. */
+int aimdhf_pair_npp(aimdhf_plan* plan, int32_t* npp_host, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

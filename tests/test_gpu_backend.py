@@ -1,0 +1,139 @@
+"""
+The hf backend's contract on the GPU (aimd/backends/hf.py:181-363 and its
+tests, tests/test_hf_backend.py): info keys, density-guess protocol, error
+behaviour, non-convergence flagging, forces vs finite differences, the
+autograd path, NVE energy conservation.
+"""
+
+import numpy as np
+import pytest
+
+from tests.conftest import WATER, WATER_STO3G_MIN, bohr
+
+pytestmark = pytest.mark.gpu
+
+INFO_KEYS = {"method", "basis", "reference", "scf_converged", "scf_iterations", "scf_energy",
+             "electronic_energy", "nuclear_repulsion", "energy_change", "commutator_norm", "s2", "s2_exact",
+             "mulliken_charges", "guess", "init_guess", "n_removed", "warnings", "timings"}
+TIMING_KEYS = {"integrals", "guess", "scf_iterations", "scf", "gradient", "total"}
+
+
+def _hf(s, **kw):
+    from aimd_amd.backends import get_backend
+    return get_backend("hf")(s, **kw)
+
+
+def test_result_fields_and_info_keys():
+    s, x = bohr(WATER)
+    be = _hf(s, basis="6-31g*")
+    r = be.compute(x)
+    assert set(r.info) == INFO_KEYS
+    assert set(r.info["timings"]) == TIMING_KEYS
+    assert r.info["method"] == "RHF" and r.info["basis"] == "6-31g*" and r.info["guess"] == "init"
+    assert r.density.shape == be.density_shape == (19, 19)
+    assert r.gradient.shape == (3, 3) and r.dipole.shape == (3,)
+    np.testing.assert_array_equal(r.forces, -r.gradient)
+    assert be.name == "hf" and be.supports_density_guess and be.label == "RHF" and be.nao == 19
+    r2 = be.compute(x + 0.01)
+    assert r2.info["guess"] == "previous"
+    assert r2.info["scf_iterations"] < r.info["scf_iterations"]
+
+
+def test_density_guess_protocol():
+    s, x = bohr(WATER)
+    be = _hf(s, basis="sto-3g")
+    r = be.compute(x)
+    d = r.density
+    d[0, 0] = 123.0                                   # returned density is a fresh copy
+    be.set_density_guess(be.last_scf.density)
+    r2 = be.compute(x)
+    assert r2.info["guess"] == "external"
+    assert be.compute(x).info["guess"] == "previous"   # external guess is used once
+    with pytest.raises(ValueError, match="shape"):
+        be.set_density_guess(np.zeros((2, 7, 7)))
+    with pytest.raises(ValueError, match="non-finite"):
+        be.set_density_guess(np.full((7, 7), np.nan))
+    be.reset_guess()
+    assert be.compute(x).info["guess"] == "init"
+    off = _hf(s, basis="sto-3g", reuse_density=False)
+    off.compute(x)
+    assert off.compute(x).info["guess"] == "init"
+
+
+def test_uhf_density_shape_and_label():
+    s, x = bohr(WATER)
+    be = _hf(s, basis="sto-3g", charge=1, multiplicity=2)
+    r = be.compute(x)
+    assert be.label == "UHF" and r.density.shape == (2, 7, 7) and be.density_shape == (2, 7, 7)
+    assert r.info["s2"] > 0.75 - 1e-6
+
+
+def test_errors():
+    s, x = bohr(WATER)
+    be = _hf(s, basis="sto-3g")
+    with pytest.raises(ValueError, match="shape"):
+        be.compute(x[:2])
+    bad = x.copy()
+    bad[1] = bad[0] + 1e-7
+    with pytest.raises(ValueError, match="coincident"):
+        be.compute(bad)
+    bad = x.copy()
+    bad[0, 0] = np.nan
+    with pytest.raises(ValueError, match="finite"):
+        be.compute(bad)
+
+
+def test_unconverged_is_flagged_not_raised():
+    s, x = bohr(WATER, 0.2, 3)
+    be = _hf(s, basis="6-31g*", max_cycles=2)
+    r = be.compute(x)
+    assert not r.converged and r.info["scf_converged"] is False
+    assert r.info["scf_iterations"] == 2
+    assert any("not converged" in w for w in r.info["warnings"])
+    assert np.all(np.isfinite(r.gradient))
+
+
+def test_forces_match_central_differences():
+    s, x = bohr(WATER, 0.05, 5)
+    be = _hf(s, basis="6-31g*", conv_tol=1e-12, conv_tol_grad=1e-10)
+    g = be.compute(x).gradient
+    h = 1e-3
+    for a, d in [(0, 0), (1, 2), (2, 1)]:
+        y1, y2 = x.copy(), x.copy()
+        y1[a, d] -= h
+        y2[a, d] += h
+        fd = (be.compute(y2).energy - be.compute(y1).energy) / (2 * h)
+        assert g[a, d] == pytest.approx(fd, abs=2e-6)
+
+
+def test_autograd_backward_is_the_analytic_gradient():
+    import torch
+    from aimd_amd.qc.autograd import HFEnergy
+    s, x = bohr(WATER, 0.05, 6)
+    be = _hf(s, basis="6-31g*", conv_tol=1e-11, conv_tol_grad=1e-9)
+    r = be.compute(x)
+    pos = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+    E = HFEnergy.apply(pos, be.solver, None, None, 1e-14)
+    (2.0 * E).backward()
+    np.testing.assert_allclose(pos.grad.numpy(), 2.0 * r.gradient, rtol=0, atol=1e-9)
+    assert float(E) == pytest.approx(r.energy, abs=1e-9)
+
+
+def test_nve_energy_conservation_scales_as_dt_squared():
+    """Reference tests/test_hf_backend.py:451-479 (water/STO-3G): drift < 1.5e-4 Eh, dt^2 ratio 3.5-4.5."""
+    from aimd_amd.md import VelocityVerletNVE, masses_au, maxwell_boltzmann
+    s, x = bohr(WATER_STO3G_MIN)
+    m = masses_au(s)
+    drifts = []
+    for dt, steps in ((0.5, 20), (0.25, 40)):
+        be = _hf(s, basis="sto-3g", conv_tol=1e-11, conv_tol_grad=1e-9)
+        md = VelocityVerletNVE(be, m, x, maxwell_boltzmann(m, 300.0, seed=7), dt)
+        md.initialize()
+        e0 = md.total_energy()
+        dev = 0.0
+        for _ in range(steps):
+            md.step()
+            dev = max(dev, abs(md.total_energy() - e0))
+        drifts.append(dev)
+    assert drifts[0] < 1.5e-4
+    assert 3.5 < drifts[0] / drifts[1] < 4.5

@@ -1,0 +1,79 @@
+"""GPU SCF and analytic gradients against the oracle (energy 1e-9 Eh, forces 1e-7 Eh/bohr) and known answers."""
+
+import numpy as np
+import pytest
+
+from tests.conftest import AMMONIA, METHANOL, METHYL, WATER, WATER_STO3G_MIN, bohr, cluster
+
+pytestmark = pytest.mark.gpu
+TIGHT = dict(conv_tol=1e-11, conv_tol_grad=1e-9)
+
+
+def _hf(s, basis, **kw):
+    from aimd_amd.backends import get_backend
+    return get_backend("hf")(s, basis=basis, **{**TIGHT, **kw})
+
+
+@pytest.mark.parametrize("mol,basis,mult", [
+    (WATER, "6-31g*", 1), (WATER, "cc-pvdz", 1), (WATER, "6-31g**", 1), (AMMONIA, "6-31g*", 1),
+    (METHYL, "6-31g*", 2), (METHANOL, "6-31g*", 1), (METHANOL, "sto-3g", 1)])
+def test_energy_and_forces_match_oracle(oracle_lib, mol, basis, mult):
+    s, x = bohr(mol, 0.05, 11)
+    print(f"case {s} {basis} {mult}", flush=True)
+    r = _hf(s, basis, multiplicity=mult).compute(x)
+    st, g = oracle_lib.energy_and_gradient(s, x, basis, mult=mult, conv_energy=1e-11, conv_commutator=1e-9)
+    assert r.converged
+    assert abs(r.energy - st.energy) < 1e-9
+    assert np.abs(r.gradient - g).max() < 1e-7
+
+
+def test_known_answers():
+    s, x = bohr(WATER_STO3G_MIN)
+    r = _hf(s, "sto-3g").compute(x)
+    assert r.energy == pytest.approx(-74.965901192299, abs=1e-10)
+    r = _hf(["H", "H"], "sto-3g").compute(np.array([[0, 0, 0], [0, 0, 1.4]]))
+    assert abs(r.energy - (-1.1167)) < 5e-5
+    s, x = bohr(METHYL)
+    be = _hf(s, "6-31g*", multiplicity=2)
+    r = be.compute(x)
+    assert r.energy == pytest.approx(-39.558992, abs=1e-6)
+    assert r.info["s2"] == pytest.approx(0.7615, abs=1e-4)
+    s, x = bohr(METHANOL)
+    assert _hf(s, "6-31g*").compute(x).energy == pytest.approx(-115.035418, abs=1e-6)
+
+
+def test_broken_symmetry_uhf():
+    """Reference tests/test_hf_backend.py:258-296 (stretched H2)."""
+    x = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 4.0]])
+    rhf = _hf(["H", "H"], "sto-3g").compute(x)
+    uhf = _hf(["H", "H"], "sto-3g", reference="uhf", reuse_density=False)
+    on_a, on_b = np.diag([1.0, 0.0]), np.diag([0.0, 1.0])
+    uhf.set_density_guess(np.stack([rhf.density / 2, rhf.density / 2]))
+    sym = uhf.compute(x)
+    uhf.set_density_guess(np.stack([on_a, on_b]))
+    ab = uhf.compute(x)
+    uhf.set_density_guess(np.stack([on_b, on_a]))
+    ba = uhf.compute(x)
+    assert rhf.energy == pytest.approx(-0.76108, abs=1e-5)
+    assert sym.energy == pytest.approx(rhf.energy, abs=1e-10)
+    assert ab.energy == pytest.approx(-0.93584, abs=1e-5)
+    assert ba.energy == pytest.approx(ab.energy, abs=1e-10)
+    assert ab.info["s2"] > 0.5 and sym.info["s2"] == pytest.approx(0.0, abs=1e-10)
+    spin_ab, spin_ba = ab.density[0] - ab.density[1], ba.density[0] - ba.density[1]
+    np.testing.assert_allclose(spin_ba, -spin_ab, atol=1e-6)
+
+
+@pytest.mark.parametrize("guess", ["core", "gwh", "sad"])
+def test_initial_guesses_reach_the_same_state(guess):
+    s, x = bohr(WATER, 0.05, 2)
+    e = _hf(s, "6-31g*", guess=guess).compute(x).energy
+    e0 = _hf(s, "6-31g*").compute(x).energy
+    assert e == pytest.approx(e0, abs=1e-9)
+
+
+def test_water8_cluster_against_oracle(oracle_lib):
+    s, x = cluster("water8.xyz")
+    r = _hf(s, "6-31g*").compute(x)
+    st, g = oracle_lib.energy_and_gradient(s, x, "6-31g*", conv_energy=1e-11, conv_commutator=1e-9, stored=False)
+    assert abs(r.energy - st.energy) < 1e-9
+    assert np.abs(r.gradient - g).max() < 1e-7
