@@ -33,6 +33,8 @@ SIGNATURES = {
     "aimdhf_quartet_count": (_I, [_P, ctypes.POINTER(_L)]),
     "aimdhf_quartets": (_I, [_P, _P, _L, ctypes.POINTER(_L)]),
     "aimdhf_jk": (_I, [_P, _I, _P, _P, _P, _P]),
+    "aimdhf_set_jk_mode": (_I, [_P, _I, _L]),
+    "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P]),
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_grad_one": (_I, [_P, _P, _P, _P, _P]),
     "aimdhf_grad_two": (_I, [_P, _P, _I, _P, _D, _D, _P, _P]),

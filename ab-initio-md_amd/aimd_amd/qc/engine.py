@@ -96,6 +96,26 @@ class Engine:
         return out[:n]
 
     # ---------------------------------------------------------------- J / K
+    JK_MODES = {"auto": 0, "direct": 1, "store": 2}
+
+    def set_jk_mode(self, mode: str = "auto", max_store_bytes: int = 0) -> None:
+        """J/K mode from the next set_geometry on: "direct" recomputes the integrals every
+        build, "store" evaluates them once per geometry into device memory, "auto" stores
+        when the estimated size is within max_store_bytes (0: half the free device memory)."""
+        if mode not in self.JK_MODES:
+            raise ValueError(f"jk mode must be one of {sorted(self.JK_MODES)}, got {mode!r}")
+        _lib.check(_lib.lib().aimdhf_set_jk_mode(self._h, self.JK_MODES[mode], int(max_store_bytes)))
+        self.jk_mode = mode
+
+    def jk_info(self) -> dict:
+        """Store state of the current geometry: stored, bytes, K and J gather-list entries."""
+        st = ctypes.c_int()
+        b, ke, je = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.lib().aimdhf_jk_info(self._h, ctypes.byref(st), ctypes.byref(b), ctypes.byref(ke),
+                                             ctypes.byref(je)))
+        return {"stored": bool(st.value), "store_bytes": int(b.value), "k_entries": int(ke.value),
+                "j_entries": int(je.value)}
+
     def jk(self, dens: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
         """J, K of (nset, nao, nao) (or (nao, nao)) densities; symmetric part of each D is used."""
         single = dens.dim() == 2
@@ -134,19 +154,29 @@ class Engine:
         return g
 
     # ------------------------------------------------------------ profiling
-    KIND_NAMES = ([f"jk_class_{k}" for k in range(21)] + [f"grad_class_{k}" for k in range(21)]
-                  + ["geometry", "one_electron", "grad_one", "jk_prologue_epilogue"])
+    PAIR_CLASS_NAMES = ["ss", "ps", "pp", "ds", "dp", "dd"]
+    KIND_NAMES = ([f"jengine_bra_{c}" for c in PAIR_CLASS_NAMES]
+                  + [f"exchange_{'spd'[t // 3]}{'spd'[t % 3]}" for t in range(9)]
+                  + [f"unused_{k}" for k in range(15, 21)]
+                  + [f"grad_class_{k}" for k in range(21)]
+                  + ["geometry", "one_electron", "grad_one", "jk_prologue_epilogue", "jk_hermite_density",
+                     "eri_store", "gather_lists"]
+                  + [f"jstore_{c}" for c in PAIR_CLASS_NAMES]
+                  + [f"kstore_{'spd'[t // 3]}{'spd'[t % 3]}" for t in range(9)])
+    NKIND = 64
+    FOCK_KINDS = list(range(0, 15)) + [45, 46] + list(range(49, 64))    # per-iteration J/K work, either mode
+    STORE_KINDS = [47, 48]                                              # per-geometry store build
 
     def profile(self, on: bool = True) -> None:
         """Record HIP events around every engine launch (on torch's current stream)."""
         _lib.check(_lib.lib().aimdhf_profile_enable(self._h, int(bool(on))))
 
     def profile_read(self, reset: bool = False) -> tuple[np.ndarray, np.ndarray]:
-        """(milliseconds, launch counts) per kind, 46 kinds (see KIND_NAMES)."""
+        """(milliseconds, launch counts) per kind, 64 kinds (see KIND_NAMES)."""
         ms = np.zeros(64)
         n = np.zeros(64, np.int64)
         _lib.check(_lib.lib().aimdhf_profile_read(self._h, _ptr(ms), _ptr(n), 64, int(bool(reset))))
-        return ms[:46], n[:46]
+        return ms, n
 
     def class_counts(self) -> tuple[np.ndarray, np.ndarray]:
         """Per-class sizes of the energy and (last) gradient quartet lists."""

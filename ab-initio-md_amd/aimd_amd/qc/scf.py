@@ -50,7 +50,12 @@ class SCFOptions:
     """
     SCF settings; names, defaults and validation as aimd/qc/scf.py:185-247.
     ``max_nao`` and ``blas_threads`` are accepted for compatibility: the
-    engine is integral-direct (no tensor to cap) and runs no BLAS threads.
+    engine never forms the nao^4 tensor (nothing to cap) and runs no BLAS
+    threads. ``jk_mode`` ("auto", "direct", "store") and ``max_store_bytes``
+    have no counterpart in the reference: they choose between recomputing the
+    screened integrals every J/K build and evaluating them once per geometry
+    into device memory (auto: store when it fits in max_store_bytes, 0 = half
+    the free device memory).
     """
     conv_energy: float = 1e-10
     conv_commutator: float = 1e-7
@@ -68,6 +73,8 @@ class SCFOptions:
     max_nao: int = 120
     warn_unconverged: bool = True
     blas_threads: int | None = 1
+    jk_mode: str = "auto"
+    max_store_bytes: int = 0
 
     def __post_init__(self) -> None:
         if not (self.conv_energy > 0.0 and self.conv_commutator > 0.0):
@@ -92,6 +99,10 @@ class SCFOptions:
             raise ValueError("schwarz_threshold must be >= 0")
         if self.blas_threads is not None and self.blas_threads < 1:
             raise ValueError("blas_threads must be >= 1 or None")
+        if self.jk_mode not in qengine.Engine.JK_MODES:
+            raise ValueError(f"unknown jk_mode {self.jk_mode!r}; use one of {sorted(qengine.Engine.JK_MODES)}")
+        if self.max_store_bytes < 0:
+            raise ValueError("max_store_bytes must be >= 0")
 
 
 def electron_counts(nuclear_charge: int, charge: int, multiplicity: int,
@@ -537,6 +548,7 @@ class SCFSolver:
         if self.n_alpha > self.basis.nao:
             raise ValueError(f"{self.n_alpha} alpha electrons do not fit in {self.basis.nao} AOs")
         self.engine = qengine.Engine(self.basis)
+        self.engine.set_jk_mode(self.options.jk_mode, self.options.max_store_bytes)
         self._ints: SCFIntegrals | None = None
         self._ints_positions: np.ndarray | None = None
         self._ints_options: SCFOptions | None = None

@@ -88,9 +88,7 @@ std::vector<double> boys_table() {
 }
 
 // Optional per-launch timing with HIP events on the caller's stream.
-// Kinds: 0..20 J/K class kernels, 21..41 gradient class kernels, 42 geometry
-// set-up, 43 one-electron integrals, 44 one-electron gradient, 45 J/K prologue
-// and epilogue (zeroing, symmetrisation, fixed-point conversion).
+// Kinds as listed at aimdhf_profile_enable in aimdhf.h (0..63).
 struct Profiler {
     static constexpr int NKIND = 64;
     bool on = false;
@@ -168,13 +166,36 @@ struct aimdhf_plan {
     DevBuf d_recs, d_npp, d_qshell, d_qblock, d_qscr, d_counts, d_offsets, d_list;
     DevBuf d_fx, d_dsym, d_charges, d_glist, d_gfx, d_gden, d_scratch;
     std::vector<long long> gcount;
+    // owner-computes J/K: static output lists and per-geometry streams / partner lists
+    std::vector<int> out_x, out_y, ao_shell;
+    int out_off[10] = {0};
+    int has_l[3] = {0, 0, 0};
+    DevBuf d_out_x, d_out_y, d_ao_shell;
+    std::vector<int> ket_pair, ket_off, part_off, part_shell, part_pair, kpair;
+    std::vector<double> ket_q, part_q;
+    std::vector<long long> kpre, krec;
+    long long pbase[7] = {0}, dh_base2[7] = {0};
+    int nket[6] = {0};
+    DevBuf d_ket_q, d_kpre, d_krec, d_kpair, d_part_off, d_part_shell, d_part_pair, d_part_q;
+    DevBuf d_dh, d_jpart, d_kpart, d_ket_pair;
+    // integral store: mode (0 auto, 1 direct, 2 store), byte limit (0: half the free device memory)
+    int jk_mode = 0;
+    long long store_limit = 0;
+    bool stored = false;
+    long long store_base[NQCLASS + 1] = {0};
+    long long store_bytes = 0, k_entries = 0, j_entries = 0;
+    std::vector<long long> kseg, jseg;
+    DevBuf d_store, d_qmap, d_kcnt, d_jcnt, d_kseg, d_jseg, d_klist, d_jlist, d_lerr;
 
     ~aimdhf_plan() {
         for (DevBuf* b : {&d_shell_atom, &d_shell_l, &d_shell_prim, &d_shell_ao, &d_prim_exp, &d_prim_coef,
                           &d_pos, &d_pair_a, &d_pair_b, &d_pair_cls, &d_rec_size, &d_rec_off, &d_cls_pairs,
                           &d_cls_pair_off, &d_ncls, &d_bp_off, &d_bp_list, &d_boys, &d_recs, &d_npp,
                           &d_qshell, &d_qblock, &d_qscr, &d_counts, &d_offsets, &d_list, &d_fx, &d_dsym,
-                          &d_charges, &d_glist, &d_gfx, &d_gden, &d_scratch})
+                          &d_charges, &d_glist, &d_gfx, &d_gden, &d_scratch, &d_out_x, &d_out_y, &d_ao_shell,
+                          &d_ket_q, &d_kpre, &d_krec, &d_kpair, &d_part_off, &d_part_shell, &d_part_pair,
+                          &d_part_q, &d_dh, &d_jpart, &d_kpart, &d_ket_pair, &d_store, &d_qmap, &d_kcnt,
+                          &d_jcnt, &d_kseg, &d_jseg, &d_klist, &d_jlist, &d_lerr})
             b->release();
     }
 
@@ -240,6 +261,188 @@ struct aimdhf_plan {
         aimdhf::launch_fill_quartets(g, thr, d_offsets.as<long long>(), buf.as<int2>(), st);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    // Sorted ket streams (J-engine) and per-shell partner lists (exchange
+    // owners) from the screening values of the current geometry. Synchronous.
+    void build_owner_lists(double thr, hipStream_t st) {
+        std::vector<double> q(npair);
+        std::vector<int> np(npair);
+        HIPCHECK(hipMemcpyAsync(q.data(), d_qscr.p, sizeof(double) * npair, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(np.data(), d_npp.p, sizeof(int) * npair, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        double qmax = 0.0;
+        for (double v : q) qmax = std::max(qmax, v);
+        // kets by class, q descending (ties: pair index)
+        ket_pair.clear();
+        ket_q.clear();
+        ket_off.assign(7, 0);
+        for (int c = 0; c < 6; ++c) {
+            std::vector<int> ps(cls_pairs.begin() + cls_pair_off[c], cls_pairs.begin() + cls_pair_off[c + 1]);
+            std::stable_sort(ps.begin(), ps.end(), [&](int a, int b) { return q[a] > q[b] || (q[a] == q[b] && a < b); });
+            for (int P : ps) {
+                ket_pair.push_back(P);
+                ket_q.push_back(q[P]);
+            }
+            ket_off[c + 1] = (int)ket_pair.size();
+            nket[c] = ket_off[c + 1] - ket_off[c];
+        }
+        kpre.assign(ket_pair.size() + 1, 0);
+        krec.clear();
+        kpair.clear();
+        for (size_t m = 0; m < ket_pair.size(); ++m) {
+            int P = ket_pair[m];
+            kpre[m + 1] = kpre[m] + np[P];
+            for (int k = 0; k < np[P]; ++k) {
+                krec.push_back(rec_off[P] + (long long)k * rec_size[P]);
+                kpair.push_back(P);
+            }
+        }
+        long long db = 0;
+        for (int c = 0; c < 6; ++c) {
+            int lc = c == 0 ? 0 : c <= 2 ? 1 : 2, ld = c - lc * (lc + 1) / 2;
+            pbase[c] = kpre[ket_off[c]];
+            dh_base2[c] = db;
+            db += (kpre[ket_off[c + 1]] - kpre[ket_off[c]]) * 2 * nherm(lc + ld);
+        }
+        pbase[6] = kpre[ket_off[6]];
+        dh_base2[6] = db;
+        // partner lists: shell X -> shells J of each l with q_XJ qmax >= thr (the exact
+        // test: no quartet of a pair failing it can pass q_P q_Q >= thr), q descending
+        part_off.assign(nshell * 3 + 1, 0);
+        part_shell.clear();
+        part_pair.clear();
+        part_q.clear();
+        std::vector<std::pair<double, int>> tmp;
+        for (int X = 0; X < nshell; ++X)
+            for (int l = 0; l < 3; ++l) {
+                tmp.clear();
+                for (int J = 0; J < nshell; ++J) {
+                    if (shell_l[J] != l) continue;
+                    int P = X >= J ? X * (X + 1) / 2 + J : J * (J + 1) / 2 + X;
+                    if (!(q[P] * qmax < thr)) tmp.push_back({q[P], J});
+                }
+                std::stable_sort(tmp.begin(), tmp.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+                    return a.first > b.first || (a.first == b.first && a.second < b.second);
+                });
+                for (auto& e : tmp) {
+                    int J = e.second;
+                    part_shell.push_back(J);
+                    part_pair.push_back(X >= J ? X * (X + 1) / 2 + J : J * (J + 1) / 2 + X);
+                    part_q.push_back(e.first);
+                }
+                part_off[X * 3 + l + 1] = (int)part_shell.size();
+            }
+        upload(d_ket_q, ket_q);
+        upload(d_ket_pair, ket_pair);
+        upload(d_kpre, kpre);
+        upload(d_krec, krec);
+        upload(d_kpair, kpair);
+        upload(d_part_off, part_off);
+        upload(d_part_shell, part_shell);
+        upload(d_part_pair, part_pair);
+        upload(d_part_q, part_q);
+        d_dh.ensure(sizeof(double) * std::max<long long>(db, 1));
+    }
+
+    // Bytes the integral store of the current list would take (blocks, pair-pair
+    // map and an estimate of the gather lists: about six entries per quartet).
+    long long store_estimate() const {
+        long long dbl = 0;
+        for (int k = 0; k < NQCLASS; ++k) dbl += qcount[k] * aimdhf::store_block(k);
+        long long keys = (long long)npair * (npair + 1) / 2;
+        return 8 * dbl + 4 * keys + 8 * 6 * qtotal;
+    }
+
+    // Evaluate the screened quartets into the store and build the J and K
+    // gather lists. Synchronises the stream.
+    void build_store(hipStream_t st) {
+        long long off = 0;
+        for (int k = 0; k < NQCLASS; ++k) {
+            store_base[k] = off;
+            off += qcount[k] * aimdhf::store_block(k);
+        }
+        store_base[NQCLASS] = off;
+        const long long keys = (long long)npair * (npair + 1) / 2;
+        d_store.ensure(sizeof(double) * std::max<long long>(off, 1));
+        d_qmap.ensure(sizeof(int) * std::max<long long>(keys, 1));
+        d_lerr.ensure(sizeof(int));
+        prof.time(47, st, [&] {
+            aimdhf::launch_fill_int(d_qmap.as<int>(), (size_t)keys, -1, st);
+            aimdhf::launch_fill_int(d_lerr.as<int>(), 1, 0, st);
+            aimdhf::StoreArgs sa;
+            sa.recs = d_recs.as<double>();
+            sa.rec_off = d_rec_off.as<long long>();
+            sa.npp = d_npp.as<int>();
+            sa.boys = d_boys.as<double>();
+            sa.qmap = d_qmap.as<int>();
+            for (int k = 0; k < NQCLASS; ++k) {
+                if (!qcount[k]) continue;
+                sa.store = d_store.as<double>() + store_base[k];
+                aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], st);
+            }
+        });
+        HIPCHECK(hipGetLastError());
+        aimdhf::ListArgs la;
+        la.npair = npair;
+        la.thr = threshold;
+        la.out_x = d_out_x.as<int>();
+        la.out_y = d_out_y.as<int>();
+        la.shell_l = d_shell_l.as<int>();
+        la.shell_ao = d_shell_ao.as<int>();
+        la.pair_a = d_pair_a.as<int>();
+        la.pair_b = d_pair_b.as<int>();
+        la.pair_cls = d_pair_cls.as<int>();
+        la.part_off = d_part_off.as<int>();
+        la.part_shell = d_part_shell.as<int>();
+        la.part_pair = d_part_pair.as<int>();
+        la.part_q = d_part_q.as<double>();
+        la.qscr = d_qscr.as<double>();
+        la.ket_pair = d_ket_pair.as<int>();
+        la.ket_q = d_ket_q.as<double>();
+        for (int c = 0; c < 7; ++c) la.ket_off[c] = ket_off[c];
+        la.qmap = d_qmap.as<int>();
+        for (int k = 0; k < NQCLASS; ++k) la.qcount[k] = qcount[k];
+        la.err = d_lerr.as<int>();
+        const int nout = out_off[9];
+        d_kcnt.ensure(sizeof(long long) * (size_t)nout * 9);
+        d_jcnt.ensure(sizeof(long long) * (size_t)npair * 6);
+        std::vector<long long> kc((size_t)nout * 9), jc((size_t)npair * 6);
+        prof.time(48, st, [&] {
+            aimdhf::launch_klist_count(la, nout, d_kcnt.as<long long>(), st);
+            aimdhf::launch_jlist_count(la, d_jcnt.as<long long>(), st);
+        });
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(kc.data(), d_kcnt.p, sizeof(long long) * kc.size(), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(jc.data(), d_jcnt.p, sizeof(long long) * jc.size(), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        kseg.assign(kc.size() + 1, 0);
+        for (size_t i = 0; i < kc.size(); ++i) kseg[i + 1] = kseg[i] + kc[i];
+        jseg.assign(jc.size() + 1, 0);
+        for (size_t i = 0; i < jc.size(); ++i) jseg[i + 1] = jseg[i] + jc[i];
+        k_entries = kseg.back();
+        j_entries = jseg.back();
+        upload(d_kseg, kseg);
+        upload(d_jseg, jseg);
+        d_klist.ensure(sizeof(long long) * std::max<long long>(k_entries, 1));
+        d_jlist.ensure(sizeof(long long) * std::max<long long>(j_entries, 1));
+        la.kseg = d_kseg.as<long long>();
+        la.klist = d_klist.as<long long>();
+        la.jseg = d_jseg.as<long long>();
+        la.jlist = d_jlist.as<long long>();
+        prof.time(48, st, [&] {
+            aimdhf::launch_klist_fill(la, nout, st);
+            aimdhf::launch_jlist_fill(la, st);
+        });
+        HIPCHECK(hipGetLastError());
+        int err = 0;
+        HIPCHECK(hipMemcpyAsync(&err, d_lerr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (err) {
+            g_err = "integral store: a gather-list quartet is missing from the screened list";
+            throw HipError{AIMDHF_E_STATE};
+        }
+        store_bytes = 8 * off + 4 * keys + 8 * (k_entries + j_entries);
     }
 
     aimdhf::OneElArgs oneel_args() const {
@@ -359,7 +562,29 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
             p->bp_off[i + 1] = p->bp_off[i] + (int)bpl[i].size();
             p->bp_list.insert(p->bp_list.end(), bpl[i].begin(), bpl[i].end());
         }
+        // exchange output blocks (X >= Y) grouped by (l_X, l_Y)
+        {
+            std::vector<std::vector<int>> ox(9), oy(9);
+            for (int X = 0; X < nshell; ++X)
+                for (int Y = 0; Y <= X; ++Y) {
+                    int t = p->shell_l[X] * 3 + p->shell_l[Y];
+                    ox[t].push_back(X);
+                    oy[t].push_back(Y);
+                }
+            for (int t = 0; t < 9; ++t) {
+                p->out_off[t + 1] = p->out_off[t] + (int)ox[t].size();
+                p->out_x.insert(p->out_x.end(), ox[t].begin(), ox[t].end());
+                p->out_y.insert(p->out_y.end(), oy[t].begin(), oy[t].end());
+            }
+            for (int s2 = 0; s2 < nshell; ++s2) p->has_l[p->shell_l[s2]] = 1;
+            p->ao_shell.resize(p->shell_ao[nshell]);
+            for (int s2 = 0; s2 < nshell; ++s2)
+                for (int i = p->shell_ao[s2]; i < p->shell_ao[s2 + 1]; ++i) p->ao_shell[i] = s2;
+        }
         // device copies
+        upload(p->d_out_x, p->out_x);
+        upload(p->d_out_y, p->out_y);
+        upload(p->d_ao_shell, p->ao_shell);
         upload(p->d_shell_atom, p->shell_atom);
         upload(p->d_shell_l, p->shell_l);
         upload(p->d_shell_prim, p->shell_prim);
@@ -423,6 +648,21 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
         p->qtotal = 0;
         for (long long c : p->qcount) p->qtotal += c;
         p->threshold = thr;
+        p->build_owner_lists(thr, st);
+        p->stored = false;
+        p->store_bytes = p->k_entries = p->j_entries = 0;
+        if (p->jk_mode != 1 && p->nao < (1 << 14)) {
+            long long limit = p->store_limit;
+            if (limit <= 0) {
+                size_t fr = 0, tot = 0;
+                HIPCHECK(hipMemGetInfo(&fr, &tot));
+                limit = (long long)(fr / 2);
+            }
+            if (p->jk_mode == 2 || p->store_estimate() <= limit) {
+                p->build_store(st);
+                p->stored = true;
+            }
+        }
         p->have_geometry = true;
     } catch (const HipError& e) {
         p->have_geometry = false;
@@ -472,33 +712,141 @@ int aimdhf_quartets(aimdhf_plan* p, int64_t* out, int64_t cap, int64_t* count) {
 
 int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K, void* stream) {
     if (!p || !dens || !J || !K) return fail(AIMDHF_E_ARG, "NULL argument");
-    if (nset < 1 || nset > 8) return fail(AIMDHF_E_ARG, "nset must be in 1..8");
+    if (nset < 1 || nset > 2) return fail(AIMDHF_E_ARG, "nset must be 1 or 2");
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
+    if (p->stored) {
+        try {
+            const size_t nn = (size_t)p->nao * p->nao;
+            p->d_dsym.ensure(sizeof(double) * nset * nn);
+            p->prof.time(45, st, [&] { aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st); });
+            aimdhf::JSArgs ja;
+            ja.nao = p->nao;
+            ja.nset = nset;
+            ja.cls_pairs = p->d_cls_pairs.as<int>();
+            ja.pair_a = p->d_pair_a.as<int>();
+            ja.pair_b = p->d_pair_b.as<int>();
+            ja.shell_ao = p->d_shell_ao.as<int>();
+            ja.jseg = p->d_jseg.as<long long>();
+            ja.jlist = p->d_jlist.as<long long>();
+            ja.store = p->d_store.as<double>();
+            for (int k = 0; k < NQCLASS; ++k) ja.store_base[k] = p->store_base[k];
+            ja.dens = p->d_dsym.as<double>();
+            ja.J = J;
+            for (int c = 0; c < 6; ++c) {
+                if (!p->ncls[c]) continue;
+                ja.cls_off = p->cls_pair_off[c];
+                p->prof.time(49 + c, st, [&] { aimdhf::launch_jstore(c, ja, p->ncls[c], st); });
+            }
+            aimdhf::KSArgs ka;
+            ka.nao = p->nao;
+            ka.nset = nset;
+            for (int t = 0; t < 10; ++t) ka.out_off[t] = p->out_off[t];
+            ka.out_x = p->d_out_x.as<int>();
+            ka.out_y = p->d_out_y.as<int>();
+            ka.shell_ao = p->d_shell_ao.as<int>();
+            ka.kseg = p->d_kseg.as<long long>();
+            ka.klist = p->d_klist.as<long long>();
+            ka.store = p->d_store.as<double>();
+            for (int k = 0; k < NQCLASS; ++k) ka.store_base[k] = p->store_base[k];
+            ka.dens = p->d_dsym.as<double>();
+            ka.K = K;
+            for (int t = 0; t < 9; ++t) {
+                int nout = p->out_off[t + 1] - p->out_off[t];
+                if (nout > 0) p->prof.time(55 + t, st, [&] { aimdhf::launch_kstore(t, ka, nout, st); });
+            }
+            HIPCHECK(hipGetLastError());
+        } catch (const HipError& e) {
+            return e.code;
+        }
+        return AIMDHF_OK;
+    }
     try {
-        size_t nn = (size_t)p->nao * p->nao;
-        p->d_fx.ensure(sizeof(long long) * 4 * nset * nn);
+        const size_t nn = (size_t)p->nao * p->nao;
         p->d_dsym.ensure(sizeof(double) * nset * nn);
+        p->d_jpart.ensure(sizeof(double) * 6 * (size_t)p->npair * nset * 36);
+        p->d_kpart.ensure(sizeof(double) * 9 * nset * nn);
         p->prof.time(45, st, [&] {
-            aimdhf::launch_zero64(p->d_fx.as<long long>(), 4 * nset * nn, st);
             aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st);
+            aimdhf::launch_zero_d(p->d_kpart.as<double>(), 9 * nset * nn, st);
         });
-        aimdhf::JKArgs a;
-        a.nao = p->nao;
-        a.nset = nset;
-        a.pair_a = p->d_pair_a.as<int>();
-        a.pair_b = p->d_pair_b.as<int>();
-        a.shell_ao = p->d_shell_ao.as<int>();
-        a.npp = p->d_npp.as<int>();
-        a.rec_off = p->d_rec_off.as<long long>();
-        a.recs = p->d_recs.as<double>();
-        a.boys = p->d_boys.as<double>();
-        a.dens = p->d_dsym.as<double>();
-        a.fx = p->d_fx.as<long long>();
-        for (int k = 0; k < NQCLASS; ++k)
-            if (p->qcount[k] > 0)
-                p->prof.time(k, st, [&] { aimdhf::launch_jk_one(k, a, p->list_of_class(k), p->qcount[k], st); });
-        p->prof.time(45, st, [&] { aimdhf::launch_jk_finalize(p->d_fx.as<long long>(), nset, p->nao, J, K, st); });
+        // J-engine: Hermite densities of the ket streams, then one kernel per (bra class, ket class)
+        aimdhf::DhArgs dh;
+        dh.nao = p->nao;
+        dh.nset = nset;
+        long long ncls_entries[6];
+        for (int c = 0; c < 6; ++c) {
+            dh.pbase[c] = p->pbase[c];
+            dh.dh_base[c] = p->dh_base2[c];
+            ncls_entries[c] = p->pbase[c + 1] - p->pbase[c];
+        }
+        dh.krec = p->d_krec.as<long long>();
+        dh.kpair = p->d_kpair.as<int>();
+        dh.pair_a = p->d_pair_a.as<int>();
+        dh.pair_b = p->d_pair_b.as<int>();
+        dh.shell_ao = p->d_shell_ao.as<int>();
+        dh.recs = p->d_recs.as<double>();
+        dh.dens = p->d_dsym.as<double>();
+        dh.dh = p->d_dh.as<double>();
+        p->prof.time(46, st, [&] { aimdhf::launch_dh(dh, ncls_entries, st); });
+        aimdhf::JEArgs je;
+        je.npair = p->npair;
+        je.thr = p->threshold;
+        je.cls_pairs = p->d_cls_pairs.as<int>();
+        je.cls_pair_off = p->d_cls_pair_off.as<int>();
+        for (int c = 0; c < 6; ++c) je.ncls_host[c] = p->ncls[c];
+        je.qscr = p->d_qscr.as<double>();
+        je.ket_q = p->d_ket_q.as<double>();
+        for (int c = 0; c < 7; ++c) je.ket_off[c] = p->ket_off[c];
+        je.kpre = p->d_kpre.as<long long>();
+        je.krec = p->d_krec.as<long long>();
+        je.dh = p->d_dh.as<double>();
+        for (int c = 0; c < 6; ++c) je.dh_base[c] = p->dh_base2[c];
+        je.recs = p->d_recs.as<double>();
+        je.rec_off = p->d_rec_off.as<long long>();
+        je.npp = p->d_npp.as<int>();
+        je.boys = p->d_boys.as<double>();
+        je.jpart = p->d_jpart.as<double>();
+        for (int c = 0; c < 6; ++c)
+            if (p->ncls[c] > 0) p->prof.time(c, st, [&] { aimdhf::launch_jeng(c, je, nset, p->nket, st); });
+        aimdhf::JAsmArgs ja;
+        ja.npair = p->npair;
+        ja.nao = p->nao;
+        ja.nset = nset;
+        for (int c = 0; c < 6; ++c) ja.cq_valid[c] = p->nket[c] > 0;
+        ja.pair_a = p->d_pair_a.as<int>();
+        ja.pair_b = p->d_pair_b.as<int>();
+        ja.shell_ao = p->d_shell_ao.as<int>();
+        ja.jpart = p->d_jpart.as<double>();
+        ja.J = J;
+        p->prof.time(45, st, [&] { aimdhf::launch_j_assemble(ja, st); });
+        // exchange owners: one kernel per (l_X, l_Y, l_J, l_L)
+        aimdhf::KOArgs ko;
+        ko.nao = p->nao;
+        ko.nset = nset;
+        ko.thr = p->threshold;
+        for (int t = 0; t < 10; ++t) ko.out_off[t] = p->out_off[t];
+        ko.out_x = p->d_out_x.as<int>();
+        ko.out_y = p->d_out_y.as<int>();
+        ko.part_off = p->d_part_off.as<int>();
+        ko.part_shell = p->d_part_shell.as<int>();
+        ko.part_pair = p->d_part_pair.as<int>();
+        ko.part_q = p->d_part_q.as<double>();
+        ko.pair_a = p->d_pair_a.as<int>();
+        ko.rec_off = p->d_rec_off.as<long long>();
+        ko.recs = p->d_recs.as<double>();
+        ko.npp = p->d_npp.as<int>();
+        ko.shell_ao = p->d_shell_ao.as<int>();
+        ko.boys = p->d_boys.as<double>();
+        ko.dens = p->d_dsym.as<double>();
+        ko.kpart = p->d_kpart.as<double>();
+        for (int t = 0; t < 9; ++t) {
+            int nout = p->out_off[t + 1] - p->out_off[t];
+            if (nout > 0) p->prof.time(6 + t, st, [&] { aimdhf::launch_kown(t, ko, nout, p->has_l, st); });
+        }
+        p->prof.time(45, st, [&] {
+            aimdhf::launch_k_assemble(p->d_kpart.as<double>(), nset, p->nao, p->d_ao_shell.as<int>(), K, st);
+        });
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
         return e.code;
@@ -596,6 +944,25 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
     } catch (const HipError& e) {
         return e.code;
     }
+    return AIMDHF_OK;
+}
+
+int aimdhf_set_jk_mode(aimdhf_plan* p, int mode, int64_t max_store_bytes) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    if (mode < 0 || mode > 2) return fail(AIMDHF_E_ARG, "jk mode must be 0 (auto), 1 (direct) or 2 (store)");
+    if (max_store_bytes < 0) return fail(AIMDHF_E_ARG, "max_store_bytes must be >= 0");
+    p->jk_mode = mode;
+    p->store_limit = max_store_bytes;
+    return AIMDHF_OK;
+}
+
+int aimdhf_jk_info(aimdhf_plan* p, int* stored, int64_t* store_bytes, int64_t* k_entries, int64_t* j_entries) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    if (stored) *stored = p->stored ? 1 : 0;
+    if (store_bytes) *store_bytes = p->store_bytes;
+    if (k_entries) *k_entries = p->k_entries;
+    if (j_entries) *j_entries = p->j_entries;
     return AIMDHF_OK;
 }
 

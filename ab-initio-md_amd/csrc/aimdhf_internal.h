@@ -82,15 +82,167 @@ struct GradArgs {
     long long* out;             // (natm, 3) fixed point, hi/lo interleaved
 };
 
+// owner-computes J/K (J-engine + exchange owners)
+struct JEArgs {
+    int npair;
+    double thr;
+    const int* cls_pairs;
+    const int* cls_pair_off;
+    int ncls_host[6];           // bra pairs per class (launch sizes)
+    const double* qscr;
+    const double* ket_q;        // sorted kets (q descending within each class)
+    int ket_off[7];             // class ranges of the sorted kets
+    const long long* kpre;      // primitive-stream start of every sorted ket (+1)
+    const long long* krec;      // record offset of every stream entry
+    const double* dh;           // Hermite densities of the stream entries
+    long long dh_base[6];       // start of each class's Hermite densities
+    const double* recs;
+    const long long* rec_off;
+    const int* npp;
+    const double* boys;
+    double* jpart;              // (6, npair, nset, 36)
+};
+
+struct DhArgs {
+    int nao;
+    int nset;
+    long long pbase[6];         // first stream entry of each class
+    long long dh_base[6];
+    const long long* krec;
+    const int* kpair;
+    const int* pair_a;
+    const int* pair_b;
+    const int* shell_ao;
+    const double* recs;
+    const double* dens;
+    double* dh;
+};
+
+struct JAsmArgs {
+    int npair;
+    int nao;
+    int nset;
+    int cq_valid[6];
+    const int* pair_a;
+    const int* pair_b;
+    const int* shell_ao;        // nshell + 1
+    const double* jpart;
+    double* J;
+};
+
+struct KOArgs {
+    int nao;
+    int nset;
+    double thr;
+    int out_off[10];            // output blocks per (l_X, l_Y) type
+    const int* out_x;
+    const int* out_y;
+    const int* part_off;        // (nshell * 3 + 1)
+    const int* part_shell;
+    const int* part_pair;
+    const double* part_q;
+    const int* pair_a;
+    const long long* rec_off;
+    const double* recs;
+    const int* npp;
+    const int* shell_ao;
+    const double* boys;
+    const double* dens;
+    double* kpart;              // (9, nset, nao, nao)
+};
+
+// integral store (per geometry) and its gather lists
+struct StoreArgs {
+    const double* recs;
+    const long long* rec_off;
+    const int* npp;
+    const double* boys;
+    double* store;              // this class's blocks
+    int* qmap;                  // pair-pair key -> index within its class
+};
+
+struct ListArgs {
+    int npair;
+    double thr;
+    const int* out_x;
+    const int* out_y;
+    const int* shell_l;
+    const int* shell_ao;
+    const int* pair_a;
+    const int* pair_b;
+    const int* pair_cls;
+    const int* part_off;
+    const int* part_shell;
+    const int* part_pair;
+    const double* part_q;
+    const double* qscr;
+    const int* ket_pair;
+    const double* ket_q;
+    int ket_off[7];
+    const int* qmap;
+    long long qcount[21];       // quartets per class (bounds of the map values)
+    int* err;                   // set to 1 if a map value is out of its class range
+    const long long* kseg;      // nout * 9 + 1
+    long long* klist;
+    const long long* jseg;      // npair * 6 + 1
+    long long* jlist;
+};
+
+struct KSArgs {
+    int nao;
+    int nset;
+    int out_off[10];
+    const int* out_x;
+    const int* out_y;
+    const int* shell_ao;
+    const long long* kseg;
+    const long long* klist;
+    const double* store;
+    long long store_base[21];
+    const double* dens;         // (nset, nao, nao), symmetric
+    double* K;
+};
+
+struct JSArgs {
+    int nao;
+    int nset;
+    int cls_off;                // first pair of the launched class in cls_pairs
+    const int* cls_pairs;
+    const int* pair_a;
+    const int* pair_b;
+    const int* shell_ao;
+    const long long* jseg;
+    const long long* jlist;
+    const double* store;
+    long long store_base[21];
+    const double* dens;
+    double* J;
+};
+
+void launch_eri_store(int k, const StoreArgs& a, const int2* list, long long nq, hipStream_t st);
+long long store_block(int k);
+void launch_klist_count(const ListArgs& a, int nout, long long* counts, hipStream_t st);
+void launch_klist_fill(const ListArgs& a, int nout, hipStream_t st);
+void launch_jlist_count(const ListArgs& a, long long* counts, hipStream_t st);
+void launch_jlist_fill(const ListArgs& a, hipStream_t st);
+void launch_kstore(int type, const KSArgs& a, int nout, hipStream_t st);
+void launch_jstore(int pclass, const JSArgs& a, int npairs, hipStream_t st);
+void launch_fill_int(int* p, size_t n, int v, hipStream_t st);
+
+void launch_dh(const DhArgs& a, const long long* n_per_class, hipStream_t st);
+void launch_jeng(int pclass, const JEArgs& a, int nset, const int* nket_host, hipStream_t st);
+void launch_j_assemble(const JAsmArgs& a, hipStream_t st);
+void launch_kown(int type, const KOArgs& a, int nout, const int* has_l, hipStream_t st);
+void launch_k_assemble(const double* kpart, int nset, int n, const int* ao_shell, double* K, hipStream_t st);
+void launch_zero_d(double* p, size_t n, hipStream_t st);
+
 void launch_build_pairs(const PairArgs& g, hipStream_t st);
 void launch_schwarz(const PairArgs& g, const int* ncls_host, hipStream_t st);
 void launch_count_quartets(const PairArgs& g, double thr, int* counts, hipStream_t st);
 void launch_fill_quartets(const PairArgs& g, double thr, const long long* offsets, int2* list, hipStream_t st);
 void launch_zero64(long long* p, size_t n, hipStream_t st);
 void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st);
-void launch_jk_one(int k, const JKArgs& a, const int2* list, long long nq, hipStream_t st);
 void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
-void launch_jk(const JKArgs& a, const int2* const* lists, const long long* counts, hipStream_t st);
 void launch_oneel(const OneElArgs& g, double* S, double* T, double* M, double* V, hipStream_t st);
 void launch_grad_one(const OneElArgs& g, const double* D, const double* Wneg, long long* out, hipStream_t st);
 void launch_negate(const double* W, size_t n, double* out, hipStream_t st);

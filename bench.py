@@ -43,6 +43,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--jk-mode", choices=["auto", "direct", "store"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the engine has no multi-GPU path "
@@ -61,7 +62,8 @@ def main() -> int:
     x0 = xyz * ANG_TO_BOHR
     m = masses_au(sym)
     v0 = maxwell_boltzmann(m, 300.0, seed=1)
-    be = get_backend("hf")(sym, basis="6-31g*", conv_tol=1e-10, conv_tol_grad=1e-7)
+    be = get_backend("hf")(sym, basis="6-31g*", conv_tol=1e-10, conv_tol_grad=1e-7,
+                           scf_options={"jk_mode": args.jk_mode})
     md = VelocityVerletNVE(be, m, x0, v0, timestep_fs=0.5)
     eng = be.solver.engine
     md.initialize()
@@ -83,17 +85,22 @@ def main() -> int:
     eng.profile(False)
     ms_per_step = 1e3 * wall / args.steps
 
-    # dominant kernel and its algorithmic work (quartet list of the last timed geometry)
+    # Fock build (J and K kernels + their prologue/epilogue) priced with the algorithmic
+    # work of the unique screened quartets of the last timed geometry
     quartets = eng.quartets()
     npp = eng.pair_npp()
     work = flopmodel.jk_work(be.basis, quartets, npp, nset=1)
-    jk_kinds = list(range(21))
-    k_dom = int(np.argmax(ms_kind[jk_kinds]))
-    avg_ms = ms_kind[k_dom] / max(n_kind[k_dom], 1)
-    flops = work["per_class"][k_dom]
-    achieved = flops / (avg_ms * 1e-3) / 1e12
-    jk_ms = ms_kind[:21].sum() / max(n_kind[:21].max(), 1)
+    info = eng.jk_info()
+    per_build_45 = 1 if info["stored"] else 3        # prologue/epilogue launches per build
+    n_builds = max(int(n_kind[45]) // per_build_45, 1)
+    fock_ms = float(ms_kind[eng.FOCK_KINDS].sum()) / n_builds
+    store_ms = float(ms_kind[eng.STORE_KINDS].sum()) / max(args.steps, 1)
+    flops = work["total"]
+    achieved = flops / (fock_ms * 1e-3) / 1e12
+    jk_ms = fock_ms
     grad_ms = ms_kind[21:42].sum() / max(args.steps, 1)
+    per_kernel = {eng.KIND_NAMES[k]: round(float(ms_kind[k]) / max(int(n_kind[k]), 1), 4)
+                  for k in range(eng.NKIND) if n_kind[k] > 0}
     out = {
         "metric": METRIC,
         "value": round(ms_per_step, 3),
@@ -114,21 +121,26 @@ def main() -> int:
             "quartets": int(len(quartets)),
         },
         "scf_iterations_mean": float(np.mean(iters)),
+        "jk_mode": "store" if info["stored"] else "direct",
+        "jk_builds_per_step": round(n_builds / max(args.steps, 1), 2),
         "jk_build_ms": round(float(jk_ms), 3),
+        "store_build_ms_per_step": round(store_ms, 3),
+        "store_bytes": info["store_bytes"],
         "gradient_kernels_ms_per_step": round(float(grad_ms), 3),
         "eri_quartets_per_s": float(len(quartets) / (jk_ms * 1e-3)) if jk_ms > 0 else None,
         "nve_total_energy_span_Eh": float(np.ptp(e_tot)) if e_tot else None,
         "roofline": {
             "bound": "fp64-valu",
-            "kernel": eng.KIND_NAMES[k_dom],
+            "kernel": "fock_build (jengine_* + exchange_* + prologue/epilogue, per SCF iteration)",
             "achieved": round(achieved, 4),
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 5),
             "traffic": None,
             "flops_per_launch": float(flops),
-            "avg_launch_ms": round(float(avg_ms), 4),
+            "avg_launch_ms": round(float(fock_ms), 4),
         },
+        "avg_ms_per_launch": per_kernel,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sym, md.x, np.mean(iters), args.cpu_threads, be.last_scf.density)

@@ -64,10 +64,14 @@ int aimdhf_plan_destroy(aimdhf_plan* plan);
  * depends on the geometry: primitive-pair records (mu |A-B|^2 <= 50 kept),
  * Schwarz bounds Q = max |(ab|ab)|^(1/2), and the class-binned list of unique
  * shell quartets whose contraction-block pairs satisfy Q_P Q_Q >= threshold.
- * Synchronises the stream (the list sizes come back to the host).
+ * In store mode (aimdhf_set_jk_mode) the screened quartets are also evaluated
+ * here into a device-resident integral store, with the gather lists of the
+ * J and K kernels. Synchronises the stream (the list sizes come back to the host).
  * Replaces: BasisSet.with_positions (aimd/qc/basis.py:242-246), _build_pairs
  * (aimd/qc/integrals.py:525-599), schwarz_bounds (aimd/qc/integrals.py:970-983)
- * and the screening loop of _eri_tensor_kernel (aimd/qc/integrals.py:900-903).
+ * and the screening loop of _eri_tensor_kernel (aimd/qc/integrals.py:900-903);
+ * in store mode also eri_tensor (aimd/qc/integrals.py:986-1006), holding only
+ * the screened unique blocks (8 bytes per integral) instead of nao^4.
  */
 int aimdhf_set_geometry(aimdhf_plan* plan, const double* positions, double schwarz_threshold, void* stream);
 
@@ -86,11 +90,27 @@ int aimdhf_quartets(aimdhf_plan* plan, int64_t* out_host, int64_t cap, int64_t* 
 /*
  * Coulomb and exchange matrices of nset densities (device (nset,nao,nao)):
  *   J_mn = sum_ls (mn|ls) D_ls,   K_mn = sum_ls (ml|ns) D_ls
- * of the symmetric part of each D, integral-direct over the screened
- * quartets, accumulated in exact fixed point (bitwise reproducible).
- * Replaces: eri_tensor + jk_from_eri (aimd/qc/integrals.py:986-1044).
+ * of the symmetric part of each D over the screened quartets. Every output
+ * block is owned by one wavefront that sums its contributions in a fixed
+ * order (no atomics), so J and K are bitwise reproducible run to run.
+ * Direct mode recomputes the integrals (J-engine in Hermite space for J,
+ * per-block exchange owners for K); store mode reads them from the store.
+ * Replaces: jk_from_eri (aimd/qc/integrals.py:1009-1044) and, in direct mode,
+ * eri_tensor (aimd/qc/integrals.py:986-1006).
  */
 int aimdhf_jk(aimdhf_plan* plan, int nset, const double* dens, double* J, double* K, void* stream);
+
+/*
+ * J/K mode for the next aimdhf_set_geometry: 0 auto (store when its estimated
+ * size is within max_store_bytes, 0 = half the free device memory), 1 direct,
+ * 2 store. No counterpart in the reference, which always stores the full
+ * tensor and refuses nao > max_nao (aimd/qc/integrals.py:986-1006).
+ */
+int aimdhf_set_jk_mode(aimdhf_plan* plan, int mode, int64_t max_store_bytes);
+
+/* Whether the current geometry has an integral store, its size in bytes and
+ * the gather-list lengths of the K and J kernels (any pointer may be NULL). */
+int aimdhf_jk_info(aimdhf_plan* plan, int* stored, int64_t* store_bytes, int64_t* k_entries, int64_t* j_entries);
 
 /*
  * One-electron matrices of the current geometry into device buffers:
@@ -122,10 +142,13 @@ int aimdhf_grad_two(aimdhf_plan* plan, const double* Dc, int nx, const double* D
 /*
  * Kernel timing with HIP events recorded on the caller's stream around every
  * launch while enabled (no counterpart in the reference, whose timings are
- * wall-clock dicts, aimd/backends/hf.py:347-354). Kinds: 0..20 J/K kernels of
- * the 21 quartet classes, 21..41 derivative-ERI kernels, 42 geometry set-up,
- * 43 one-electron integrals, 44 one-electron gradient, 45 J/K prologue and
- * epilogue. aimdhf_profile_read synchronises, returns the summed
+ * wall-clock dicts, aimd/backends/hf.py:347-354). Kinds: 0..5 direct J-engine
+ * per bra pair class (ss ps pp ds dp dd), 6..14 direct exchange per output
+ * type (l_X, l_Y), 21..41 derivative-ERI kernels of the 21 quartet classes,
+ * 42 geometry set-up, 43 one-electron integrals, 44 one-electron gradient,
+ * 45 J/K prologue and epilogue, 46 Hermite densities, 47 integral store
+ * evaluation, 48 gather-list build, 49..54 store J per pair class, 55..63
+ * store K per output type. aimdhf_profile_read synchronises, returns the summed
  * milliseconds and launch counts per kind (nkind <= 64) and optionally resets.
  */
 int aimdhf_profile_enable(aimdhf_plan* plan, int on);

@@ -28,12 +28,14 @@ def _hf(s, basis, **kw):
     return get_backend("hf")(s, basis=basis, **{**dict(conv_tol=1e-11, conv_tol_grad=1e-9), **kw})
 
 
-def test_water16_631gs_matches_oracle_and_is_deterministic():
+@pytest.mark.parametrize("jk_mode", ["store", "direct"])
+def test_water16_631gs_matches_oracle_and_is_deterministic(jk_mode):
     s, x = cluster("water16.xyz")
     ref = _fixture("oracle_water16_631gs.json")
-    be = _hf(s, "6-31g*")
+    be = _hf(s, "6-31g*", scf_options={"jk_mode": jk_mode})
     r = be.compute(x)
-    print("(H2O)16", r.energy, ref["energy"], r.info["scf_iterations"], flush=True)
+    assert be.solver.engine.jk_info()["stored"] == (jk_mode == "store")
+    print("(H2O)16", jk_mode, r.energy, ref["energy"], r.info["scf_iterations"], flush=True)
     assert abs(r.energy - ref["energy"]) < 1e-9
     assert np.abs(r.gradient - np.array(ref["gradient"])).max() < 1e-7
     # translation invariance (net force) and bitwise reproducibility

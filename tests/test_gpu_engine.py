@@ -13,12 +13,13 @@ from tests.conftest import METHYL, WATER, bohr, cluster, water_dimer
 pytestmark = pytest.mark.gpu
 
 
-def _engine(s, x, basis):
+def _engine(s, x, basis, mode="auto"):
     import torch
     from aimd_amd.qc.basis import build_basis
     from aimd_amd.qc.engine import Engine
     b = build_basis(s, x, basis)
     e = Engine(b)
+    e.set_jk_mode(mode)
     e.set_geometry(x)
     torch.cuda.synchronize()
     return b, e
@@ -36,12 +37,14 @@ def _geom(name):
     return bohr(WATER, 0.05, 9)
 
 
+@pytest.mark.parametrize("mode", ["direct", "store"])
 @pytest.mark.parametrize("mol,basis", CASES)
-def test_schwarz_quartets_and_jk(oracle_lib, mol, basis):
+def test_schwarz_quartets_and_jk(oracle_lib, mol, basis, mode):
     import torch
     s, x = _geom(mol)
-    print(f"case {mol} {basis}", flush=True)
-    b, e = _engine(s, x, basis)
+    print(f"case {mol} {basis} {mode}", flush=True)
+    b, e = _engine(s, x, basis, mode)
+    assert e.jk_info()["stored"] == (mode == "store")
     ob = oracle_lib.make_basis(s, x, basis)
     pr = oracle_lib.Pairs(ob)
     Qo = pr.schwarz()
