@@ -404,9 +404,12 @@ class DIIS:
         if m == 1:
             return np.ones(1)
         last = self.errors[-1].reshape(-1)
-        delta = torch.stack([e.reshape(-1) - last for e in self.errors[:-1]], dim=1)
-        G = (delta.T @ delta).cpu().numpy()
-        r = (delta.T @ (-last)).cpu().numpy()
+        delta = torch.stack([e.reshape(-1) - last for e in self.errors[:-1]])
+        # Gram matrix and right-hand side as elementwise products + reductions: the
+        # (m x nao^2)(nao^2 x m) shape maps to a single-workgroup GEMM in the BLAS
+        Gr = torch.stack([(delta * d).sum(dim=1) for d in delta] + [-(delta * last).sum(dim=1)])
+        Gr = Gr.cpu().numpy()
+        G, r = Gr[:-1], Gr[-1]
         lam, V = np.linalg.eigh(G)
         sig = np.sqrt(np.maximum(lam, 0.0))
         cut = np.finfo(float).eps * max(delta.shape) * sig.max()
@@ -757,7 +760,7 @@ class SCFSolver:
         spin_t = torch.stack([Ds[0], Ds[-1]])
         P = spin_t[0] + spin_t[1]
         PS_diag = torch.diagonal(P @ S)
-        dip_el = -torch.einsum("xmn,mn->x", I.M, P)
+        dip_el = -(I.M * P).sum(dim=(1, 2))
         s2_t = torch.sum((spin_t[0] @ S) * (spin_t[1] @ S).T) if nspin == 2 else None
         host = [eps_t, C_t, spin_t, F, 0.5 * (W + W.T), PS_diag, dip_el]
         if s2_t is not None:

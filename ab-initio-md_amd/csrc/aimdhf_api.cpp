@@ -185,7 +185,14 @@ struct aimdhf_plan {
     long long store_base[NQCLASS + 1] = {0};
     long long store_bytes = 0, k_entries = 0, j_entries = 0;
     std::vector<long long> kseg, jseg;
+    DevBuf d_gkeep;
     DevBuf d_store, d_qmap, d_kcnt, d_jcnt, d_kseg, d_jseg, d_klist, d_jlist, d_lerr;
+    DevBuf d_kdata, d_jdata, d_kdoff, d_jdoff;
+    long long kdata_total = 0, jdata_total = 0;
+    // work units of the store kernels (lane items per unit: UNIT_ITEMS)
+    static constexpr int UNIT_ITEMS = 512;
+    int kunit_base[10] = {0}, junit_base[7] = {0};
+    DevBuf d_kunit_os, d_kunit_e, d_out_unit, d_kpart2, d_junit_pc, d_junit_e, d_pair_unit, d_jpart2;
 
     ~aimdhf_plan() {
         for (DevBuf* b : {&d_shell_atom, &d_shell_l, &d_shell_prim, &d_shell_ao, &d_prim_exp, &d_prim_coef,
@@ -195,7 +202,9 @@ struct aimdhf_plan {
                           &d_charges, &d_glist, &d_gfx, &d_gden, &d_scratch, &d_out_x, &d_out_y, &d_ao_shell,
                           &d_ket_q, &d_kpre, &d_krec, &d_kpair, &d_part_off, &d_part_shell, &d_part_pair,
                           &d_part_q, &d_dh, &d_jpart, &d_kpart, &d_ket_pair, &d_store, &d_qmap, &d_kcnt,
-                          &d_jcnt, &d_kseg, &d_jseg, &d_klist, &d_jlist, &d_lerr})
+                          &d_jcnt, &d_kseg, &d_jseg, &d_klist, &d_jlist, &d_lerr, &d_gkeep,
+                          &d_kdata, &d_jdata, &d_kdoff, &d_jdoff, &d_kunit_os, &d_kunit_e, &d_out_unit,
+                          &d_kpart2, &d_junit_pc, &d_junit_e, &d_pair_unit, &d_jpart2})
             b->release();
     }
 
@@ -345,13 +354,63 @@ struct aimdhf_plan {
         d_dh.ensure(sizeof(double) * std::max<long long>(db, 1));
     }
 
-    // Bytes the integral store of the current list would take (blocks, pair-pair
-    // map and an estimate of the gather lists: about six entries per quartet).
+    // Bytes the integral store of the current list would take: the blocks, up
+    // to six gather-ordered copies of them (four exchange, two Coulomb), the
+    // pair-pair map and the gather lists (about six entries per quartet).
     long long store_estimate() const {
         long long dbl = 0;
         for (int k = 0; k < NQCLASS; ++k) dbl += qcount[k] * aimdhf::store_block(k);
         long long keys = (long long)npair * (npair + 1) / 2;
-        return 8 * dbl + 4 * keys + 8 * 6 * qtotal;
+        return 8 * 7 * dbl + 4 * keys + 8 * 6 * qtotal;
+    }
+
+    aimdhf::KSArgs ks_args(int nset, const double* dens, double* K) const {
+        aimdhf::KSArgs ka;
+        ka.nao = nao;
+        ka.nset = nset;
+        for (int t = 0; t < 10; ++t) ka.out_off[t] = out_off[t];
+        ka.out_x = d_out_x.as<int>();
+        ka.out_y = d_out_y.as<int>();
+        ka.shell_ao = d_shell_ao.as<int>();
+        ka.kseg = d_kseg.as<long long>();
+        ka.klist = d_klist.as<long long>();
+        ka.store = d_store.as<double>();
+        for (int k = 0; k < NQCLASS; ++k) ka.store_base[k] = store_base[k];
+        ka.kdata = d_kdata.as<double>();
+        ka.kdoff = d_kdoff.as<long long>();
+        for (int t = 0; t < 10; ++t) ka.unit_base[t] = kunit_base[t];
+        ka.unit_os = d_kunit_os.as<int>();
+        ka.unit_e = d_kunit_e.as<long long>();
+        ka.out_unit = d_out_unit.as<int>();
+        ka.kpart = d_kpart2.as<double>();
+        ka.dens = dens;
+        ka.K = K;
+        return ka;
+    }
+
+    aimdhf::JSArgs js_args(int nset, const double* dens, double* J) const {
+        aimdhf::JSArgs ja;
+        ja.nao = nao;
+        ja.nset = nset;
+        ja.cls_off = 0;
+        ja.cls_pairs = d_cls_pairs.as<int>();
+        ja.pair_a = d_pair_a.as<int>();
+        ja.pair_b = d_pair_b.as<int>();
+        ja.shell_ao = d_shell_ao.as<int>();
+        ja.jseg = d_jseg.as<long long>();
+        ja.jlist = d_jlist.as<long long>();
+        ja.store = d_store.as<double>();
+        for (int k = 0; k < NQCLASS; ++k) ja.store_base[k] = store_base[k];
+        ja.jdata = d_jdata.as<double>();
+        ja.jdoff = d_jdoff.as<long long>();
+        for (int c = 0; c < 7; ++c) ja.unit_base[c] = junit_base[c];
+        ja.unit_pc = d_junit_pc.as<int>();
+        ja.unit_e = d_junit_e.as<long long>();
+        ja.pair_unit = d_pair_unit.as<int>();
+        ja.jpart = d_jpart2.as<double>();
+        ja.dens = dens;
+        ja.J = J;
+        return ja;
     }
 
     // Evaluate the screened quartets into the store and build the J and K
@@ -442,7 +501,98 @@ struct aimdhf_plan {
             g_err = "integral store: a gather-list quartet is missing from the screened list";
             throw HipError{AIMDHF_E_STATE};
         }
-        store_bytes = 8 * off + 4 * keys + 8 * (k_entries + j_entries);
+        // gather-ordered copies: segment offsets, then one copy kernel per type / class
+        static const int ncp[6] = {1, 3, 9, 6, 18, 36};
+        std::vector<long long> kdoff((size_t)nout * 9), jdoff((size_t)npair * 6);
+        long long ko = 0;
+        for (int o = 0; o < nout; ++o) {
+            const int nx = ncart(shell_l[out_x[o]]), ny = ncart(shell_l[out_y[o]]);
+            for (int sub = 0; sub < 9; ++sub) {
+                const size_t i = (size_t)o * 9 + sub;
+                kdoff[i] = ko;
+                ko += (kseg[i + 1] - kseg[i]) * nx * ncart(sub / 3) * ny * ncart(sub % 3);
+            }
+        }
+        long long jo = 0;
+        for (int P = 0; P < npair; ++P)
+            for (int c = 0; c < 6; ++c) {
+                const size_t i = (size_t)P * 6 + c;
+                jdoff[i] = jo;
+                jo += (jseg[i + 1] - jseg[i]) * ncp[pair_cls[P]] * ncp[c];
+            }
+        kdata_total = ko;
+        jdata_total = jo;
+        // work units: each segment cut into pieces of at most UNIT_ITEMS lane items
+        {
+            std::vector<int> uos, out_unit(nout + 1);
+            std::vector<long long> ue;
+            for (int o = 0; o < nout; ++o) {
+                out_unit[o] = (int)uos.size();
+                const int nx = ncart(shell_l[out_x[o]]), ny = ncart(shell_l[out_y[o]]);
+                for (int sub = 0; sub < 9; ++sub) {
+                    const size_t i = (size_t)o * 9 + sub;
+                    const int blk = nx * ncart(sub / 3) * ny * ncart(sub % 3);
+                    const int ns = blk > 36 ? ny * ncart(sub % 3) : 1;
+                    const long long per = std::max(1, UNIT_ITEMS / ns);
+                    for (long long e = kseg[i]; e < kseg[i + 1]; e += per) {
+                        uos.push_back(o << 4 | sub);
+                        ue.push_back(e);
+                        ue.push_back(std::min(e + per, kseg[i + 1]));
+                    }
+                }
+            }
+            out_unit[nout] = (int)uos.size();
+            for (int t = 0; t < 10; ++t) kunit_base[t] = out_unit[out_off[t]];
+            upload(d_kunit_os, uos);
+            upload(d_kunit_e, ue);
+            upload(d_out_unit, out_unit);
+            d_kpart2.ensure(sizeof(double) * aimdhf::KPART_STRIDE * std::max<size_t>(uos.size(), 1));
+        }
+        {
+            std::vector<int> upc, pair_unit(npair + 1);
+            std::vector<long long> ue;
+            for (int c = 0; c < 6; ++c)
+                for (int idx = cls_pair_off[c]; idx < cls_pair_off[c + 1]; ++idx) {
+                    pair_unit[idx] = (int)upc.size();
+                    const int P = cls_pairs[idx];
+                    for (int cq = 0; cq < 6; ++cq) {
+                        const size_t i = (size_t)P * 6 + cq;
+                        const int blk = ncp[c] * ncp[cq];
+                        const int ns = blk > 36 ? ncp[cq] : 1;
+                        const long long per = std::max(1, UNIT_ITEMS / ns);
+                        for (long long e = jseg[i]; e < jseg[i + 1]; e += per) {
+                            upc.push_back(P << 3 | cq);
+                            ue.push_back(e);
+                            ue.push_back(std::min(e + per, jseg[i + 1]));
+                        }
+                    }
+                }
+            pair_unit[npair] = (int)upc.size();
+            for (int c = 0; c < 7; ++c) junit_base[c] = pair_unit[cls_pair_off[c]];
+            upload(d_junit_pc, upc);
+            upload(d_junit_e, ue);
+            upload(d_pair_unit, pair_unit);
+            d_jpart2.ensure(sizeof(double) * aimdhf::KPART_STRIDE * std::max<size_t>(upc.size(), 1));
+        }
+        upload(d_kdoff, kdoff);
+        upload(d_jdoff, jdoff);
+        d_kdata.ensure(sizeof(double) * std::max<long long>(ko, 1));
+        d_jdata.ensure(sizeof(double) * std::max<long long>(jo, 1));
+        prof.time(47, st, [&] {
+            aimdhf::KSArgs ka = ks_args(1, nullptr, nullptr);
+            for (int t = 0; t < 9; ++t) {
+                const int no = out_off[t + 1] - out_off[t];
+                if (no > 0) aimdhf::launch_kcopy(t, ka, no, st);
+            }
+            aimdhf::JSArgs ja = js_args(1, nullptr, nullptr);
+            for (int c = 0; c < 6; ++c) {
+                if (!ncls[c]) continue;
+                ja.cls_off = cls_pair_off[c];
+                aimdhf::launch_jcopy(c, ja, ncls[c], st);
+            }
+        });
+        HIPCHECK(hipGetLastError());
+        store_bytes = 8 * (off + ko + jo) + 4 * keys + 8 * (k_entries + j_entries + kdoff.size() + jdoff.size());
     }
 
     aimdhf::OneElArgs oneel_args() const {
@@ -450,6 +600,13 @@ struct aimdhf_plan {
         g.npair = npair;
         g.nao = nao;
         g.natm = natm;
+        for (int c = 0; c < 6; ++c) {
+            g.ncls_host[c] = ncls[c];
+            g.cls_off_host[c] = cls_pair_off[c];
+        }
+        g.cls_pairs = d_cls_pairs.as<int>();
+        g.pair_a = d_pair_a.as<int>();
+        g.pair_b = d_pair_b.as<int>();
         g.shell_atom = d_shell_atom.as<int>();
         g.shell_l = d_shell_l.as<int>();
         g.shell_prim = d_shell_prim.as<int>();
@@ -720,37 +877,13 @@ int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K
             const size_t nn = (size_t)p->nao * p->nao;
             p->d_dsym.ensure(sizeof(double) * nset * nn);
             p->prof.time(45, st, [&] { aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st); });
-            aimdhf::JSArgs ja;
-            ja.nao = p->nao;
-            ja.nset = nset;
-            ja.cls_pairs = p->d_cls_pairs.as<int>();
-            ja.pair_a = p->d_pair_a.as<int>();
-            ja.pair_b = p->d_pair_b.as<int>();
-            ja.shell_ao = p->d_shell_ao.as<int>();
-            ja.jseg = p->d_jseg.as<long long>();
-            ja.jlist = p->d_jlist.as<long long>();
-            ja.store = p->d_store.as<double>();
-            for (int k = 0; k < NQCLASS; ++k) ja.store_base[k] = p->store_base[k];
-            ja.dens = p->d_dsym.as<double>();
-            ja.J = J;
+            aimdhf::JSArgs ja = p->js_args(nset, p->d_dsym.as<double>(), J);
             for (int c = 0; c < 6; ++c) {
                 if (!p->ncls[c]) continue;
                 ja.cls_off = p->cls_pair_off[c];
                 p->prof.time(49 + c, st, [&] { aimdhf::launch_jstore(c, ja, p->ncls[c], st); });
             }
-            aimdhf::KSArgs ka;
-            ka.nao = p->nao;
-            ka.nset = nset;
-            for (int t = 0; t < 10; ++t) ka.out_off[t] = p->out_off[t];
-            ka.out_x = p->d_out_x.as<int>();
-            ka.out_y = p->d_out_y.as<int>();
-            ka.shell_ao = p->d_shell_ao.as<int>();
-            ka.kseg = p->d_kseg.as<long long>();
-            ka.klist = p->d_klist.as<long long>();
-            ka.store = p->d_store.as<double>();
-            for (int k = 0; k < NQCLASS; ++k) ka.store_base[k] = p->store_base[k];
-            ka.dens = p->d_dsym.as<double>();
-            ka.K = K;
+            aimdhf::KSArgs ka = p->ks_args(nset, p->d_dsym.as<double>(), K);
             for (int t = 0; t < 9; ++t) {
                 int nout = p->out_off[t + 1] - p->out_off[t];
                 if (nout > 0) p->prof.time(55 + t, st, [&] { aimdhf::launch_kstore(t, ka, nout, st); });
@@ -932,9 +1065,13 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
         a.dc = dc;
         a.dx = dx;
         a.out = p->d_gfx.as<long long>();
+        long long gtot = 0;
+        for (int k = 0; k < NQCLASS; ++k) gtot += p->gcount[k];
+        p->d_gkeep.ensure(std::max<long long>(gtot, 1));
         long long off = 0;
         for (int k = 0; k < NQCLASS; ++k) {
             const int2* lst = p->d_glist.as<int2>() + off;
+            a.keep = p->d_gkeep.as<unsigned char>() + off;
             if (p->gcount[k] > 0)
                 p->prof.time(21 + k, st, [&] { aimdhf::launch_grad_one_class(k, a, lst, p->gcount[k], st); });
             off += p->gcount[k];

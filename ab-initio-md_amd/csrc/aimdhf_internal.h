@@ -50,6 +50,11 @@ struct OneElArgs {
     int npair;                  // shell pairs s1 >= s2
     int nao;
     int natm;
+    int ncls_host[6];           // pairs per class
+    int cls_off_host[6];        // first pair of each class in cls_pairs
+    const int* cls_pairs;
+    const int* pair_a;          // l_A >= l_B orientation
+    const int* pair_b;
     const int* shell_atom;
     const int* shell_l;
     const int* shell_prim;
@@ -79,6 +84,7 @@ struct GradArgs {
     const double* qscr;
     const double* dc;           // (nao, nao), symmetric
     const double* dx;           // (nx, nao, nao), symmetric
+    unsigned char* keep;        // per quartet of the launched class: passes the screening
     long long* out;             // (natm, 3) fixed point, hi/lo interleaved
 };
 
@@ -151,7 +157,8 @@ struct KOArgs {
     double* kpart;              // (9, nset, nao, nao)
 };
 
-// integral store (per geometry) and its gather lists
+// integral store (per geometry) and its gather lists; doubles per unit partial
+constexpr int KPART_STRIDE = 72;
 struct StoreArgs {
     const double* recs;
     const long long* rec_off;
@@ -199,6 +206,13 @@ struct KSArgs {
     const long long* klist;
     const double* store;
     long long store_base[21];
+    double* kdata;              // gather-ordered, reoriented copies (per output segment)
+    const long long* kdoff;     // nout * 9 start of each segment's copies
+    int unit_base[10];          // work units per output type
+    const int* unit_os;         // per unit: output << 4 | subclass
+    const long long* unit_e;    // per unit: entry range [e0, e1)
+    const int* out_unit;        // per output: unit range (nout + 1)
+    double* kpart;              // per unit partial block (KPART_STRIDE doubles)
     const double* dens;         // (nset, nao, nao), symmetric
     double* K;
 };
@@ -215,6 +229,13 @@ struct JSArgs {
     const long long* jlist;
     const double* store;
     long long store_base[21];
+    double* jdata;              // gather-ordered, oriented copies (per pair segment)
+    const long long* jdoff;     // npair * 6 start of each segment's copies
+    int unit_base[7];           // work units per bra pair class
+    const int* unit_pc;         // per unit: pair << 3 | partner class
+    const long long* unit_e;    // per unit: entry range [e0, e1)
+    const int* pair_unit;       // per cls_pairs position: unit range (npair + 1)
+    double* jpart;              // per unit partial block (KPART_STRIDE doubles)
     const double* dens;
     double* J;
 };
@@ -227,6 +248,8 @@ void launch_jlist_count(const ListArgs& a, long long* counts, hipStream_t st);
 void launch_jlist_fill(const ListArgs& a, hipStream_t st);
 void launch_kstore(int type, const KSArgs& a, int nout, hipStream_t st);
 void launch_jstore(int pclass, const JSArgs& a, int npairs, hipStream_t st);
+void launch_kcopy(int type, const KSArgs& a, int nout, hipStream_t st);
+void launch_jcopy(int pclass, const JSArgs& a, int npairs, hipStream_t st);
 void launch_fill_int(int* p, size_t n, int v, hipStream_t st);
 
 void launch_dh(const DhArgs& a, const long long* n_per_class, hipStream_t st);

@@ -103,32 +103,59 @@ __device__ __forceinline__ int qclass(int cp, int cq) {
     return hi * (hi + 1) / 2 + lo;
 }
 
-__global__ void k_count_quartets(PairArgs g, double thr, int* counts) {
-    int P = blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= g.npair) return;
-    int c[NQCLASS];
-    for (int i = 0; i < NQCLASS; ++i) c[i] = 0;
-    double qp = g.qscr[P];
-    int cp = g.pair_cls[P];
-    for (int Q = 0; Q <= P; ++Q) {
+// One wavefront per pair P, lanes over Q <= P in chunks of 64: per-class
+// counts (count) and the class-binned list in ascending Q within (P, class)
+// (fill, ordered compaction by ballot).
+__global__ __launch_bounds__(64) void k_count_quartets(PairArgs g, double thr, int* counts) {
+    const int P = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double qp = g.qscr[P];
+    const int cp = g.pair_cls[P];
+    int c[6] = {0, 0, 0, 0, 0, 0};
+    for (int Q = lane; Q <= P; Q += 64) {
         if (qp * g.qscr[Q] < thr) continue;
-        c[qclass(cp, g.pair_cls[Q])]++;
+        const int cq = g.pair_cls[Q];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) c[k] += (cq == k);
     }
-    for (int i = 0; i < NQCLASS; ++i) counts[(size_t)P * NQCLASS + i] = c[i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int v = c[k];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+        c[k] = v;
+    }
+    if (lane < NQCLASS) {
+        int v = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (qclass(cp, k) == lane) v += c[k];
+        counts[(size_t)P * NQCLASS + lane] = v;
+    }
 }
 
-__global__ void k_fill_quartets(PairArgs g, double thr, const long long* offsets, int2* list) {
-    int P = blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= g.npair) return;
-    long long o[NQCLASS];
-    for (int i = 0; i < NQCLASS; ++i) o[i] = offsets[(size_t)P * NQCLASS + i];
-    double qp = g.qscr[P];
-    int cp = g.pair_cls[P];
-    for (int Q = 0; Q <= P; ++Q) {
-        if (qp * g.qscr[Q] < thr) continue;
-        int cq = g.pair_cls[Q];
-        int k = qclass(cp, cq);
-        list[o[k]++] = cp >= cq ? make_int2(P, Q) : make_int2(Q, P);
+__global__ __launch_bounds__(64) void k_fill_quartets(PairArgs g, double thr, const long long* offsets, int2* list) {
+    const int P = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double qp = g.qscr[P];
+    const int cp = g.pair_cls[P];
+    long long o[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[k] = offsets[(size_t)P * NQCLASS + qclass(cp, k)];
+    const unsigned long long below = (lane ? (~0ull >> (64 - lane)) : 0ull);
+    for (int Q0 = 0; Q0 <= P; Q0 += 64) {
+        const int Q = Q0 + lane;
+        const bool in = Q <= P && !(qp * g.qscr[Q] < thr);
+        const int cq = in ? g.pair_cls[Q] : -1;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const unsigned long long m = __ballot(cq == k);
+            if (cq == k) {
+                const long long pos = o[k] + __popcll(m & below);
+                list[pos] = cp >= cq ? make_int2(P, Q) : make_int2(Q, P);
+            }
+            o[k] += __popcll(m);
+        }
     }
 }
 
@@ -203,11 +230,11 @@ void launch_schwarz(const PairArgs& g, const int* ncls_host, hipStream_t st) {
 }
 
 void launch_count_quartets(const PairArgs& g, double thr, int* counts, hipStream_t st) {
-    if (g.npair) k_count_quartets<<<GRID(g.npair, 64), 0, st>>>(g, thr, counts);
+    if (g.npair) k_count_quartets<<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, counts);
 }
 
 void launch_fill_quartets(const PairArgs& g, double thr, const long long* offsets, int2* list, hipStream_t st) {
-    if (g.npair) k_fill_quartets<<<GRID(g.npair, 64), 0, st>>>(g, thr, offsets, list);
+    if (g.npair) k_fill_quartets<<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, offsets, list);
 }
 
 void launch_zero64(long long* p, size_t n, hipStream_t st) {

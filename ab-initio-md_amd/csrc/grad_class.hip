@@ -1,13 +1,16 @@
 // Derivative-ERI gradient kernel of one quartet class (compiled once per
 // class with -DAIMDHF_QCLASS=k, like jk_class.hip).
 //
-// Work item = (quartet, j): j < NCD is the bra pass for ket component j,
-// NCD <= j < NCD + NAB the ket pass for bra component j - NCD. Quartets are
-// skipped exactly as aimd/qc/integrals.py:1279-1349 does: all four centres on
-// one atom, or Q_P Q_Q max|Gamma| below the threshold; a pass is replaced by
-// translational invariance when its pair sits on one atom. Contributions are
-// accumulated per workgroup in LDS and flushed once, as exact fixed-point
-// integers, so the gradient is bitwise reproducible.
+// A pre-pass (one thread per quartet) applies the screening of
+// aimd/qc/integrals.py:1279-1329 exactly: quartets with all four centres on
+// one atom, or with Q_P Q_Q max|Gamma| below the threshold, are dropped.
+// Work item = (quartet, j): j < NIB is the bra pass for ket components
+// j*NCB .. j*NCB+NCB-1, NIB <= j the ket pass for NCK bra components, NCB /
+// NCK as many as fit the register budget (one R evaluation per primitive
+// quartet and item); a pass is replaced by translational invariance when its
+// pair sits on one atom. Contributions are accumulated per workgroup in LDS
+// and flushed once, as exact fixed-point integers, so the gradient is
+// bitwise reproducible.
 
 #include "aimdhf_internal.h"
 #include "md_grad.h"
@@ -28,7 +31,9 @@ constexpr int lb_of(int pc) { return pc - la_of(pc) * (la_of(pc) + 1) / 2; }
 constexpr int LA = la_of(pc_hi()), LB = lb_of(pc_hi()), LC = la_of(pc_lo()), LD = lb_of(pc_lo());
 using Qt = Quartet<LA, LB, LC, LD>;
 constexpr int NAB = Qt::NAB, NCD = Qt::NCD, NB = Qt::NB, ND = Qt::ND;
-constexpr int NT = NCD + NAB;
+constexpr int NCB = grad_ncol<LA, LB, LC, LD>(), NCK = grad_ncol<LC, LD, LA, LB>();
+constexpr int NIB = NCD / NCB, NIK = NAB / NCK;
+constexpr int NT = NIB + NIK;
 
 __device__ __forceinline__ void lds_add(long long* lds, int atom, int d, double x) {
     long long h, l;
@@ -48,23 +53,14 @@ __device__ __forceinline__ double gamma_el(const GradArgs& a, int m, int n, int 
 }
 
 template <int Q>
-__global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __restrict__ list, long long nq) {
-    extern __shared__ long long lds[];
-    for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
-    __syncthreads();
-    const long long nitems = nq * NT;
-    for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
-         it += (long long)gridDim.x * blockDim.x) {
-        const long long q = it / NT;
-        const int j = (int)(it % NT);
-        const int2 pq = list[q];
-        const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
-        const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
-        if (aA == aB && aB == aC && aC == aD) continue;
-        const bool need_bra = aA != aB, need_ket = aC != aD;
-        const bool bra = j < NCD;
-        if (bra && !(need_bra || (!need_ket && NAB >= NCD))) continue;
-        if (!bra && !(need_ket || (!need_bra && NAB < NCD))) continue;
+__global__ __launch_bounds__(256) void k_grad_keep(GradArgs a, const int2* __restrict__ list, long long nq) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int2 pq = list[q];
+    const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
+    const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
+    unsigned char keep = 0;
+    if (!(aA == aB && aB == aC && aC == aD)) {
         double deg = 0.5;
         if (sA != sB) deg *= 2.0;
         if (sC != sD) deg *= 2.0;
@@ -74,17 +70,47 @@ __global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __re
         for (int ab = 0; ab < NAB; ++ab)
             for (int cd = 0; cd < NCD; ++cd)
                 gmax = fmax(gmax, fabs(gamma_el(a, oA + ab / NB, oB + ab % NB, oC + cd / ND, oD + cd % ND, deg)));
-        if (a.qscr[pq.x] * a.qscr[pq.y] * gmax < a.thr) continue;
+        keep = !(a.qscr[pq.x] * a.qscr[pq.y] * gmax < a.thr);
+    }
+    a.keep[q] = keep;
+}
+
+template <int Q>
+__global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __restrict__ list, long long nq) {
+    extern __shared__ long long lds[];
+    for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
+    __syncthreads();
+    const long long nitems = nq * NT;
+    for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
+         it += (long long)gridDim.x * blockDim.x) {
+        const long long q = it / NT;
+        const int j = (int)(it % NT);
+        if (!a.keep[q]) continue;
+        const int2 pq = list[q];
+        const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
+        const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
+        const bool need_bra = aA != aB, need_ket = aC != aD;
+        const bool bra = j < NIB;
+        if (bra && !(need_bra || (!need_ket && NAB >= NCD))) continue;
+        if (!bra && !(need_ket || (!need_bra && NAB < NCD))) continue;
+        double deg = 0.5;
+        if (sA != sB) deg *= 2.0;
+        if (sC != sD) deg *= 2.0;
+        if (pq.x != pq.y) deg *= 2.0;
+        const int oA = a.shell_ao[sA], oB = a.shell_ao[sB], oC = a.shell_ao[sC], oD = a.shell_ao[sD];
         const double* rb = a.recs + a.rec_off[pq.x];
         const double* rk = a.recs + a.rec_off[pq.y];
         double g1[3], g2[3];
         if (bra) {
-            const int cd = j;
-            double gam[NAB];
+            const int kc0 = j * NCB;
+            double gam[NAB * NCB];
             for (int ab = 0; ab < NAB; ++ab)
-                gam[ab] = gamma_el(a, oA + ab / NB, oB + ab % NB, oC + cd / ND, oD + cd % ND, deg);
-            grad_pass_kc<LA, LB, LC, LD>(rb, a.npp[pq.x], rk, a.npp[pq.y], a.boys, a.pos + 3 * aA, a.pos + 3 * aB,
-                                         cd, gam, g1, g2);
+#pragma unroll
+                for (int c = 0; c < NCB; ++c)
+                    gam[ab * NCB + c] =
+                        gamma_el(a, oA + ab / NB, oB + ab % NB, oC + (kc0 + c) / ND, oD + (kc0 + c) % ND, deg);
+            grad_pass<LA, LB, LC, LD, NCB>(rb, a.npp[pq.x], rk, a.npp[pq.y], a.boys, a.pos + 3 * aA,
+                                           a.pos + 3 * aB, kc0, gam, g1, g2);
             for (int d = 0; d < 3; ++d) {
                 if (need_bra) {
                     lds_add(lds, aA, d, g1[d]);
@@ -96,12 +122,15 @@ __global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __re
                 }
             }
         } else {
-            const int ab = j - NCD;
-            double gam[NCD];
+            const int ab0 = (j - NIB) * NCK;
+            double gam[NCD * NCK];
             for (int cd = 0; cd < NCD; ++cd)
-                gam[cd] = gamma_el(a, oA + ab / NB, oB + ab % NB, oC + cd / ND, oD + cd % ND, deg);
-            grad_pass_kc<LC, LD, LA, LB>(rk, a.npp[pq.y], rb, a.npp[pq.x], a.boys, a.pos + 3 * aC, a.pos + 3 * aD,
-                                         ab, gam, g1, g2);
+#pragma unroll
+                for (int c = 0; c < NCK; ++c)
+                    gam[cd * NCK + c] =
+                        gamma_el(a, oA + (ab0 + c) / NB, oB + (ab0 + c) % NB, oC + cd / ND, oD + cd % ND, deg);
+            grad_pass<LC, LD, LA, LB, NCK>(rk, a.npp[pq.y], rb, a.npp[pq.x], a.boys, a.pos + 3 * aC,
+                                           a.pos + 3 * aD, ab0, gam, g1, g2);
             for (int d = 0; d < 3; ++d) {
                 if (need_ket) {
                     lds_add(lds, aC, d, g1[d]);
@@ -127,8 +156,9 @@ __global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __re
 void CAT(launch_grad_class_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list, long long nq, hipStream_t st) {
     long long items = nq * NT;
     if (items <= 0) return;
+    k_grad_keep<QC><<<dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st>>>(a, list, nq);
     long long blocks = (items + 127) / 128;
-    if (blocks > 4096) blocks = 4096;
+    if (blocks > 2048) blocks = 2048;
     k_grad_class<QC><<<dim3((unsigned)blocks), dim3(128), sizeof(long long) * 6 * a.natm, st>>>(a, list, nq);
 }
 

@@ -27,7 +27,7 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-template <int CQ, int NSET>
+template <int CP_, int CQ, int NSET>
 __global__ __launch_bounds__(64) void k_jeng(JEArgs a) {
     constexpr int LC = la_of(CQ), LD = lb_of(CQ);
     constexpr int NHB = NH(LA + LB);
@@ -78,8 +78,8 @@ void CAT(launch_jeng_, AIMDHF_PCLASS)(const JEArgs& a, int nset, const int* nket
     static_for<0, 6>([&](auto cc) {
         constexpr int CQ = decltype(cc)::value;
         if (nket_host[CQ] <= 0) return;
-        if (nset == 1) k_jeng<CQ, 1><<<dim3(nbra), dim3(64), 0, st>>>(a);
-        else k_jeng<CQ, 2><<<dim3(nbra), dim3(64), 0, st>>>(a);
+        if (nset == 1) k_jeng<AIMDHF_PCLASS, CQ, 1><<<dim3(nbra), dim3(64), 0, st>>>(a);
+        else k_jeng<AIMDHF_PCLASS, CQ, 2><<<dim3(nbra), dim3(64), 0, st>>>(a);
     });
 }
 

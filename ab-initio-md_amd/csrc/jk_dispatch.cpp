@@ -57,10 +57,12 @@ void launch_kown(int type, const KOArgs& a, int nout, const int* has_l, hipStrea
 DECLS(0) DECLS(1) DECLS(2) DECLS(3) DECLS(4) DECLS(5) DECLS(6) DECLS(7) DECLS(8) DECLS(9) DECLS(10)
 DECLS(11) DECLS(12) DECLS(13) DECLS(14) DECLS(15) DECLS(16) DECLS(17) DECLS(18) DECLS(19) DECLS(20)
 #undef DECLS
-#define DECLJS(c) void launch_jstore_##c(const JSArgs&, int, hipStream_t);
+#define DECLJS(c) void launch_jstore_##c(const JSArgs&, int, hipStream_t); \
+    void launch_jcopy_##c(const JSArgs&, int, hipStream_t);
 DECLJS(0) DECLJS(1) DECLJS(2) DECLJS(3) DECLJS(4) DECLJS(5)
 #undef DECLJS
-#define DECLKS(x, y) void launch_kstore_##x##_##y(const KSArgs&, int, hipStream_t);
+#define DECLKS(x, y) void launch_kstore_##x##_##y(const KSArgs&, int, hipStream_t); \
+    void launch_kcopy_##x##_##y(const KSArgs&, int, hipStream_t);
 DECLKS(0, 0) DECLKS(0, 1) DECLKS(0, 2) DECLKS(1, 0) DECLKS(1, 1) DECLKS(1, 2) DECLKS(2, 0) DECLKS(2, 1) DECLKS(2, 2)
 #undef DECLKS
 
@@ -73,6 +75,21 @@ void launch_eri_store(int k, const StoreArgs& a, const int2* list, long long nq,
         launch_eri_store_15, launch_eri_store_16, launch_eri_store_17, launch_eri_store_18, launch_eri_store_19,
         launch_eri_store_20};
     fns[k](a, list, nq, st);
+}
+
+void launch_jcopy(int pclass, const JSArgs& a, int npairs, hipStream_t st) {
+    typedef void (*Fn)(const JSArgs&, int, hipStream_t);
+    static const Fn fns[6] = {launch_jcopy_0, launch_jcopy_1, launch_jcopy_2,
+                              launch_jcopy_3, launch_jcopy_4, launch_jcopy_5};
+    fns[pclass](a, npairs, st);
+}
+
+void launch_kcopy(int type, const KSArgs& a, int nout, hipStream_t st) {
+    typedef void (*Fn)(const KSArgs&, int, hipStream_t);
+    static const Fn fns[9] = {launch_kcopy_0_0, launch_kcopy_0_1, launch_kcopy_0_2,
+                              launch_kcopy_1_0, launch_kcopy_1_1, launch_kcopy_1_2,
+                              launch_kcopy_2_0, launch_kcopy_2_1, launch_kcopy_2_2};
+    fns[type](a, nout, st);
 }
 
 // doubles per stored block of quartet class k (heavier pair as bra)

@@ -1,14 +1,17 @@
 // Coulomb from the integral store for bra pairs of class AIMDHF_PCLASS
-// (compiled once per class): one wave per pair P, lanes over its gather list
-// (every partner Q with q_P q_Q >= threshold, built once per geometry in the
-// order of the q-sorted partners of each class). J_P[ab] = sum_Q w_Q
-// sum_cd (ab|cd) D_cd, w_Q = 2 for C != D; the per-lane blocks are summed by
-// a fixed butterfly, so J is bitwise reproducible run to run.
-
-#include <type_traits>
+// (compiled once per class).
+//
+// The gather list of every pair P (each partner Q with q_P q_Q >= threshold,
+// by partner class) is cut into work units of at most a fixed number of lane
+// items, one wavefront each. Per geometry, k_jcopy lays each unit's stored
+// quartets out in gather order, oriented (P|Q) with the partner's component
+// index major: jdata[seg + e * BLK + cd NAB + ab] = (ab|cd). Per SCF
+// iteration, k_jstore forms sum_cd (ab|cd) w_Q D_cd (w_Q = 2 for C != D) per
+// unit (lanes summed by a fixed butterfly) and k_jreduce adds each pair's
+// unit partials in unit order into J_P, so J is bitwise reproducible.
 
 #include "aimdhf_internal.h"
-#include "md_jk2.h"
+#include "md_core.h"
 
 #ifndef AIMDHF_PCLASS
 #error "compile with -DAIMDHF_PCLASS=<0..5>"
@@ -31,102 +34,133 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-template <int CQ, bool SW, int NSET>
-__device__ __forceinline__ void j_item(const JSArgs& a, const double* __restrict__ S, int cs, int oC, int oD, double w,
-                                       double* Jacc) {
-    constexpr int LC = la_of(CQ), LD = lb_of(CQ);
-    constexpr int NDc = NC(LD), NCD = NC(LC) * NDc;
-    const size_t n = a.nao, nn = n * n;
-    constexpr int BA = SW ? LC : LA, BB = SW ? LD : LB, KA = SW ? LA : LC, KB = SW ? LB : LD;
-    if constexpr (store_split<BA, BB, KA, KB>()) {
-        constexpr int NBRA = NC(BA) * NC(BB), NKET = NC(KA) * NC(KB);
-        double v[NBRA];
-#pragma unroll
-        for (int i = 0; i < NBRA; ++i) v[i] = S[i * NKET + cs];
-        if constexpr (!SW) {
-            // column cs = cd of the partner
-            const int c = cs / NDc, d = cs % NDc;
-#pragma unroll
-            for (int s = 0; s < NSET; ++s) {
-                const double dv = w * a.dens[s * nn + (size_t)(oC + c) * n + oD + d];
-#pragma unroll
-                for (int ab = 0; ab < NAB; ++ab) Jacc[s * NAB + ab] += v[ab] * dv;
-            }
-        } else {
-            // column cs = ab of the owner; rows over the partner's cd
-#pragma unroll
-            for (int s = 0; s < NSET; ++s) {
-                double t = 0.0;
-#pragma unroll
-                for (int cd = 0; cd < NCD; ++cd)
-                    t += v[cd] * (w * a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc]);
-#pragma unroll
-                for (int ab = 0; ab < NAB; ++ab)
-                    if (ab == cs) Jacc[s * NAB + ab] += t;
-            }
+template <int CQ>
+struct Part {
+    static constexpr int NDc = NC(lb_of(CQ)), NCD = NC(la_of(CQ)) * NDc, BLK = NAB * NCD;
+    static constexpr bool SPLIT = BLK > 36;
+    static constexpr int K = qcls(CP, CQ);
+};
+
+template <class F>
+__device__ __forceinline__ void with_part(int cq, F&& f) {
+    static_for<0, 6>([&](auto qq) {
+        constexpr int C = decltype(qq)::value;
+        if (cq == C) f(Part<C>{});
+    });
+}
+
+template <int CP_>
+__global__ __launch_bounds__(64) void k_jcopy(JSArgs a) {
+    const int u = a.unit_base[CP] + blockIdx.x;
+    const int code = a.unit_pc[u];
+    const int P = code >> 3, cq = code & 7;
+    const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
+    const int lane = threadIdx.x;
+    with_part(cq, [&](auto pt) {
+        using PT = decltype(pt);
+        constexpr int NCD = PT::NCD, BLK = PT::BLK;
+        const size_t seg = (size_t)P * 6 + cq;
+        const double* base = a.store + a.store_base[PT::K];
+        double* out = a.jdata + a.jdoff[seg] + (u0 - a.jseg[seg]) * BLK;
+        const long long nt = (u1 - u0) * BLK;
+        for (long long t = lane; t < nt; t += 64) {
+            const long long e = t / BLK;
+            const int el = (int)(t % BLK);
+            const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
+            const long long qi = (long long)(ent & 0xffffffffull);
+            const bool sw = (ent >> 60) & 1;
+            const int cd = el / NAB, ab = el % NAB;
+            out[t] = base[qi * BLK + (sw ? cd * NAB + ab : ab * NCD + cd)];
         }
-    } else {
-        double blk[NAB * NCD];
-#pragma unroll
-        for (int i = 0; i < NAB * NCD; ++i) blk[i] = S[i];
-        double dq[NSET * NCD];
-#pragma unroll
-        for (int s = 0; s < NSET; ++s)
-#pragma unroll
-            for (int cd = 0; cd < NCD; ++cd)
-                dq[s * NCD + cd] = w * a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc];
-        j_store_full<LA, LB, LC, LD, SW, NSET>(blk, dq, Jacc);
-    }
+    });
 }
 
 template <int CP_, int NSET>
 __global__ __launch_bounds__(64) void k_jstore(JSArgs a) {
-    const int P = a.cls_pairs[a.cls_off + blockIdx.x];
+    const int u = a.unit_base[CP] + blockIdx.x;
+    const int code = a.unit_pc[u];
+    const int P = code >> 3, cq = code & 7;
+    const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
     const int lane = threadIdx.x;
+    const size_t n = a.nao, nn = n * n;
     double Jacc[NSET * NAB];
 #pragma unroll
     for (int i = 0; i < NSET * NAB; ++i) Jacc[i] = 0.0;
-    static_for<0, 6>([&](auto qq) {
-        constexpr int CQ = decltype(qq)::value;
-        const long long e0 = a.jseg[(size_t)P * 6 + CQ], e1 = a.jseg[(size_t)P * 6 + CQ + 1];
-        if (e0 == e1) return;
-        constexpr int k = qcls(CP, CQ);
-        constexpr int LC = la_of(CQ), LD = lb_of(CQ);
-        constexpr int BA = CP >= CQ ? LA : LC, BB = CP >= CQ ? LB : LD;
-        constexpr int KA = CP >= CQ ? LC : LA, KB = CP >= CQ ? LD : LB;
-        constexpr int BLK = NC(BA) * NC(BB) * NC(KA) * NC(KB);
-        constexpr int NS = store_split<BA, BB, KA, KB>() ? NC(KA) * NC(KB) : 1;
-        const double* base = a.store + a.store_base[k];
-        const long long nt = (e1 - e0) * NS;
-        for (long long t = lane; t < nt; t += 64) {
-            const unsigned long long ent = (unsigned long long)a.jlist[e0 + t / NS];
-            const int cs = (int)(t % NS);
-            const long long qi = (long long)(ent & 0xffffffffull);
-            const int oC = (int)((ent >> 32) & 0x3fff), oD = (int)((ent >> 46) & 0x3fff);
-            const bool sw = (ent >> 60) & 1;
-            const double w = ((ent >> 61) & 1) ? 2.0 : 1.0;
-            const double* S = base + qi * BLK;
-            if constexpr (CP == CQ) {
-                if (sw) j_item<CQ, true, NSET>(a, S, cs, oC, oD, w, Jacc);
-                else j_item<CQ, false, NSET>(a, S, cs, oC, oD, w, Jacc);
-            } else {
-                j_item<CQ, (CP < CQ), NSET>(a, S, cs, oC, oD, w, Jacc);
+    with_part(cq, [&](auto pt) {
+        using PT = decltype(pt);
+        constexpr int NDc = PT::NDc, NCD = PT::NCD, BLK = PT::BLK;
+        const size_t seg = (size_t)P * 6 + cq;
+        const double* blk0 = a.jdata + a.jdoff[seg] + (u0 - a.jseg[seg]) * BLK;
+        if constexpr (PT::SPLIT) {
+            // lanes over (entry, cd) columns of NAB contiguous values
+            const long long nt = (u1 - u0) * NCD;
+            for (long long t = lane; t < nt; t += 64) {
+                const long long e = t / NCD;
+                const int cd = (int)(t % NCD);
+                const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
+                const int oC = (int)((ent >> 32) & 0x3fff), oD = (int)((ent >> 46) & 0x3fff);
+                const double w = ((ent >> 61) & 1) ? 2.0 : 1.0;
+                const double* col = blk0 + e * BLK + cd * NAB;
+                double v[NAB];
+#pragma unroll
+                for (int i = 0; i < NAB; ++i) v[i] = col[i];
+#pragma unroll
+                for (int s = 0; s < NSET; ++s) {
+                    const double dv = w * a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc];
+#pragma unroll
+                    for (int ab = 0; ab < NAB; ++ab) Jacc[s * NAB + ab] += v[ab] * dv;
+                }
+            }
+        } else {
+            for (long long e = lane; e < u1 - u0; e += 64) {
+                const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
+                const int oC = (int)((ent >> 32) & 0x3fff), oD = (int)((ent >> 46) & 0x3fff);
+                const double w = ((ent >> 61) & 1) ? 2.0 : 1.0;
+                const double* b = blk0 + e * BLK;
+                double v[BLK];
+#pragma unroll
+                for (int i = 0; i < BLK; ++i) v[i] = b[i];
+#pragma unroll
+                for (int s = 0; s < NSET; ++s) {
+                    double dq[NCD];
+#pragma unroll
+                    for (int cd = 0; cd < NCD; ++cd)
+                        dq[cd] = w * a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc];
+#pragma unroll
+                    for (int ab = 0; ab < NAB; ++ab) {
+                        double t2 = 0.0;
+#pragma unroll
+                        for (int cd = 0; cd < NCD; ++cd) t2 += v[cd * NAB + ab] * dq[cd];
+                        Jacc[s * NAB + ab] += t2;
+                    }
+                }
             }
         }
     });
+    double* part = a.jpart + (size_t)u * KPART_STRIDE;
+#pragma unroll
+    for (int i = 0; i < NSET * NAB; ++i) {
+        const double v = wave_sum(Jacc[i]);
+        if (lane == 0) part[i] = v;
+    }
+}
+
+// one thread per (pair of this class, set, element): unit partials in unit order
+template <int CP_, int NSET>
+__global__ void k_jreduce(JSArgs a, int npairs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs * NSET * NAB) return;
+    const int idx = a.cls_off + t / (NSET * NAB), i = t % (NSET * NAB);
+    double v = 0.0;
+    for (int u = a.pair_unit[idx]; u < a.pair_unit[idx + 1]; ++u) v += a.jpart[(size_t)u * KPART_STRIDE + i];
+    const int P = a.cls_pairs[idx];
     const int A = a.pair_a[P], B = a.pair_b[P];
     const int oA = a.shell_ao[A], oB = a.shell_ao[B];
     const size_t n = a.nao, nn = n * n;
     constexpr int NBc = NC(LB);
-#pragma unroll
-    for (int i = 0; i < NSET * NAB; ++i) {
-        const double v = wave_sum(Jacc[i]);
-        const int s = i / NAB, x = (i % NAB) / NBc, y = i % NBc;
-        if (lane == 0) {
-            a.J[s * nn + (size_t)(oA + x) * n + oB + y] = v;
-            if (A != B) a.J[s * nn + (size_t)(oB + y) * n + oA + x] = v;
-        }
-    }
+    const int s = i / NAB, x = (i % NAB) / NBc, y = i % NBc;
+    a.J[s * nn + (size_t)(oA + x) * n + oB + y] = v;
+    if (A != B) a.J[s * nn + (size_t)(oB + y) * n + oA + x] = v;
 }
 
 }  // namespace
@@ -134,10 +168,23 @@ __global__ __launch_bounds__(64) void k_jstore(JSArgs a) {
 #define CAT2(a, b) a##b
 #define CAT(a, b) CAT2(a, b)
 
+// units of this class: a.unit_base[CP] .. a.unit_base[CP + 1]; npairs pairs from a.cls_off
 void CAT(launch_jstore_, AIMDHF_PCLASS)(const JSArgs& a, int npairs, hipStream_t st) {
     if (npairs <= 0) return;
-    if (a.nset == 1) k_jstore<CP, 1><<<dim3(npairs), dim3(64), 0, st>>>(a);
-    else k_jstore<CP, 2><<<dim3(npairs), dim3(64), 0, st>>>(a);
+    const int nu = a.unit_base[CP + 1] - a.unit_base[CP];
+    const int nr = npairs * a.nset * NAB;
+    if (a.nset == 1) {
+        if (nu > 0) k_jstore<CP, 1><<<dim3(nu), dim3(64), 0, st>>>(a);
+        k_jreduce<CP, 1><<<dim3((nr + 127) / 128), dim3(128), 0, st>>>(a, npairs);
+    } else {
+        if (nu > 0) k_jstore<CP, 2><<<dim3(nu), dim3(64), 0, st>>>(a);
+        k_jreduce<CP, 2><<<dim3((nr + 127) / 128), dim3(128), 0, st>>>(a, npairs);
+    }
+}
+
+void CAT(launch_jcopy_, AIMDHF_PCLASS)(const JSArgs& a, int npairs, hipStream_t st) {
+    const int nu = a.unit_base[CP + 1] - a.unit_base[CP];
+    if (npairs > 0 && nu > 0) k_jcopy<CP><<<dim3(nu), dim3(64), 0, st>>>(a);
 }
 
 }  // namespace aimdhf

@@ -51,7 +51,7 @@ constexpr int nsplit() {
     return split_by_cd<LA, LB, LC, LD>() ? NC(LC) * NC(LD) : 1;
 }
 
-template <int LJ, int LL, int NSET>
+template <int LX_, int LY_, int LJ, int LL, int NSET>
 __global__ __launch_bounds__(64) void k_kown(KOArgs a) {
     constexpr int NS = nsplit<LJ, LL>();
     __shared__ int incl[64];
@@ -143,8 +143,8 @@ void CAT3X(launch_kown_, AIMDHF_LX, AIMDHF_LY)(const KOArgs& a, int nout, const 
         static_for<0, 3>([&](auto ll) {
             constexpr int LL = decltype(ll)::value;
             if (!has_l[LL]) return;
-            if (a.nset == 1) k_kown<LJ, LL, 1><<<dim3(nout), dim3(64), 0, st>>>(a);
-            else k_kown<LJ, LL, 2><<<dim3(nout), dim3(64), 0, st>>>(a);
+            if (a.nset == 1) k_kown<LX, LY, LJ, LL, 1><<<dim3(nout), dim3(64), 0, st>>>(a);
+            else k_kown<LX, LY, LJ, LL, 2><<<dim3(nout), dim3(64), 0, st>>>(a);
         });
     });
 }

@@ -1,15 +1,19 @@
 // Exchange from the integral store for output blocks (X, Y) of angular
-// momenta (AIMDHF_LX, AIMDHF_LY) (compiled once per type): one wave per
-// output block, lanes over its gather list (built once per geometry, in the
-// fixed order of the per-shell partner lists), each entry one stored quartet
-// (XJ|YL) or one stored-ket column of it. Per-lane partial blocks are summed
-// by a fixed butterfly and written to K[X][Y] and K[Y][X]; no atomics, so
-// the result is bitwise reproducible run to run.
-
-#include <type_traits>
+// momenta (AIMDHF_LX, AIMDHF_LY) (compiled once per type).
+//
+// The gather list of every output block is cut into work units of at most a
+// fixed number of lane items (entries, or (entry, yl) columns for blocks
+// above 36 doubles), one wavefront each, so heavy and light blocks balance.
+// Per geometry, k_kcopy lays each unit's stored quartets out in gather
+// order, reoriented to (XJ|YL) with the (y, l) index major:
+// kdata[seg + e * BLK + (y NL + l) NX NJ + x NJ + j] = (xj|yl). Per SCF
+// iteration, k_kstore streams them and writes one partial block per unit
+// (lanes summed by a fixed butterfly); k_kreduce adds each block's unit
+// partials in unit order and writes K[X][Y] and K[Y][X]. No atomics, so the
+// result is bitwise reproducible run to run.
 
 #include "aimdhf_internal.h"
-#include "md_jk2.h"
+#include "md_core.h"
 
 #if !defined(AIMDHF_LX) || !defined(AIMDHF_LY)
 #error "compile with -DAIMDHF_LX=<0..2> -DAIMDHF_LY=<0..2>"
@@ -31,101 +35,149 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-template <bool DYN, bool VAL, class F>
-__device__ __forceinline__ void with_flag(bool v, F&& f) {
-    if constexpr (DYN) {
-        if (v) f(std::true_type{});
-        else f(std::false_type{});
-    } else {
-        f(std::integral_constant<bool, VAL>{});
-    }
+template <int LJ, int LL>
+struct Sub {
+    static constexpr int NJ = NC(LJ), NL = NC(LL), NXJ = NX * NJ, NYL = NY * NL, BLK = NXJ * NYL;
+    static constexpr bool SPLIT = BLK > 36;
+    static constexpr int NS = SPLIT ? NYL : 1;
+    static constexpr int K = qcls(pcls(LX, LJ), pcls(LY, LL));
+};
+
+// run f(Sub<LJ, LL>) for the runtime subclass index sub = LJ * 3 + LL
+template <class F>
+__device__ __forceinline__ void with_sub(int sub, F&& f) {
+    static_for<0, 9>([&](auto ss) {
+        constexpr int S = decltype(ss)::value;
+        if (sub == S) f(Sub<S / 3, S % 3>{});
+    });
 }
 
-template <int LJ, int LL, bool FB, bool FK, bool SW, int NSET>
-__device__ __forceinline__ void k_item(const KSArgs& a, const double* __restrict__ S, int cs, int oJ, int oL,
-                                       double* Kacc) {
-    constexpr int NJ = NC(LJ), NL = NC(LL);
-    // stored bra / ket records
-    constexpr int BA = SW ? (LY > LL ? LY : LL) : (LX > LJ ? LX : LJ);
-    constexpr int BB = SW ? (LY > LL ? LL : LY) : (LX > LJ ? LJ : LX);
-    constexpr int KA = SW ? (LX > LJ ? LX : LJ) : (LY > LL ? LY : LL);
-    constexpr int KB = SW ? (LX > LJ ? LJ : LX) : (LY > LL ? LL : LY);
-    constexpr int NAB = NC(BA) * NC(BB), NCD = NC(KA) * NC(KB);
-    const size_t n = a.nao, nn = n * n;
-    if constexpr (store_split<BA, BB, KA, KB>()) {
-        double v[NAB];
-#pragma unroll
-        for (int i = 0; i < NAB; ++i) v[i] = S[i * NCD + cs];
-        k_store_col<LX, LJ, LY, LL, FB, FK, SW, NSET>(v, cs, a.dens, n, nn, oJ, oL, Kacc);
-    } else {
-        double blk[NAB * NCD];
-#pragma unroll
-        for (int i = 0; i < NAB * NCD; ++i) blk[i] = S[i];
-        double djl[NSET * NJ * NL];
-#pragma unroll
-        for (int s = 0; s < NSET; ++s)
-#pragma unroll
-            for (int j = 0; j < NJ; ++j)
-#pragma unroll
-                for (int l = 0; l < NL; ++l) djl[(s * NJ + j) * NL + l] = a.dens[s * nn + (size_t)(oJ + j) * n + oL + l];
-        k_store_full<LX, LJ, LY, LL, FB, FK, SW, NSET>(blk, djl, Kacc);
-    }
+template <int LX_, int LY_>
+__global__ __launch_bounds__(64) void k_kcopy(KSArgs a) {
+    const int u = a.unit_base[OT] + blockIdx.x;
+    const int code = a.unit_os[u];
+    const int o = code >> 4, sub = code & 15;
+    const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
+    const int lane = threadIdx.x;
+    with_sub(sub, [&](auto sb) {
+        using SB = decltype(sb);
+        constexpr int NJ = SB::NJ, NL = SB::NL, NXJ = SB::NXJ, NYL = SB::NYL, BLK = SB::BLK;
+        const size_t seg = (size_t)o * 9 + sub;
+        const double* base = a.store + a.store_base[SB::K];
+        double* out = a.kdata + a.kdoff[seg] + (u0 - a.kseg[seg]) * BLK;
+        const long long nt = (u1 - u0) * BLK;
+        for (long long t = lane; t < nt; t += 64) {
+            const long long e = t / BLK;
+            const int el = (int)(t % BLK);
+            const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
+            const long long qi = (long long)(ent & 0xffffffffull);
+            const bool sw = (ent >> 60) & 1, fb = (ent >> 61) & 1, fk = (ent >> 62) & 1;
+            const int yl = el / NXJ, xj = el % NXJ;
+            const int x = xj / NJ, j = xj % NJ, y = yl / NL, l = yl % NL;
+            const int i1 = fb ? j * NX + x : x * NJ + j;
+            const int i2 = fk ? l * NY + y : y * NL + l;
+            out[t] = base[qi * BLK + (sw ? i2 * NXJ + i1 : i1 * NYL + i2)];
+        }
+    });
 }
 
 template <int LX_, int LY_, int NSET>
 __global__ __launch_bounds__(64) void k_kstore(KSArgs a) {
-    const int o = a.out_off[OT] + blockIdx.x;
-    const int X = a.out_x[o], Y = a.out_y[o];
+    const int u = a.unit_base[OT] + blockIdx.x;
+    const int code = a.unit_os[u];
+    const int o = code >> 4, sub = code & 15;
+    const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
     const int lane = threadIdx.x;
+    const size_t n = a.nao, nn = n * n;
     double Kacc[NSET * NX * NY];
 #pragma unroll
     for (int i = 0; i < NSET * NX * NY; ++i) Kacc[i] = 0.0;
-    static_for<0, 3>([&](auto jj) {
-        constexpr int LJ = decltype(jj)::value;
-        static_for<0, 3>([&](auto ll) {
-            constexpr int LL = decltype(ll)::value;
-            constexpr int sub = LJ * 3 + LL;
-            const long long e0 = a.kseg[(size_t)o * 9 + sub], e1 = a.kseg[(size_t)o * 9 + sub + 1];
-            if (e0 == e1) return;
-            constexpr int c1 = pcls(LX, LJ), c2 = pcls(LY, LL), k = qcls(c1, c2);
-            // stored ket size (the same for both orientations when c1 == c2)
-            constexpr int BA = c1 >= c2 ? (LX > LJ ? LX : LJ) : (LY > LL ? LY : LL);
-            constexpr int BB = c1 >= c2 ? (LX > LJ ? LJ : LX) : (LY > LL ? LL : LY);
-            constexpr int KA = c1 >= c2 ? (LY > LL ? LY : LL) : (LX > LJ ? LX : LJ);
-            constexpr int KB = c1 >= c2 ? (LY > LL ? LL : LY) : (LX > LJ ? LJ : LX);
-            constexpr int BLK = NC(BA) * NC(BB) * NC(KA) * NC(KB);
-            constexpr int NS = store_split<BA, BB, KA, KB>() ? NC(KA) * NC(KB) : 1;
-            const double* base = a.store + a.store_base[k];
-            const long long nt = (e1 - e0) * NS;
+    with_sub(sub, [&](auto sb) {
+        using SB = decltype(sb);
+        constexpr int NJ = SB::NJ, NL = SB::NL, NXJ = SB::NXJ, NYL = SB::NYL, BLK = SB::BLK;
+        const size_t seg = (size_t)o * 9 + sub;
+        const double* blk0 = a.kdata + a.kdoff[seg] + (u0 - a.kseg[seg]) * BLK;
+        if constexpr (SB::SPLIT) {
+            // lanes over (entry, yl) columns of NXJ contiguous values
+            const long long nt = (u1 - u0) * NYL;
             for (long long t = lane; t < nt; t += 64) {
-                const unsigned long long ent = (unsigned long long)a.klist[e0 + t / NS];
-                const int cs = (int)(t % NS);
-                const long long qi = (long long)(ent & 0xffffffffull);
+                const long long e = t / NYL;
+                const int yl = (int)(t % NYL);
+                const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
                 const int oJ = (int)((ent >> 32) & 0x3fff), oL = (int)((ent >> 46) & 0x3fff);
-                const bool sw = (ent >> 60) & 1, fb = (ent >> 61) & 1, fk = (ent >> 62) & 1;
-                const double* S = base + qi * BLK;
-                with_flag<(c1 == c2), (c1 < c2)>(sw, [&](auto swc) {
-                    with_flag<(LX == LJ), (LX < LJ)>(fb, [&](auto fbc) {
-                        with_flag<(LY == LL), (LY < LL)>(fk, [&](auto fkc) {
-                            k_item<LJ, LL, decltype(fbc)::value, decltype(fkc)::value, decltype(swc)::value, NSET>(
-                                a, S, cs, oJ, oL, Kacc);
-                        });
-                    });
-                });
+                const double* col = blk0 + e * BLK + yl * NXJ;
+                double v[NXJ];
+#pragma unroll
+                for (int i = 0; i < NXJ; ++i) v[i] = col[i];
+                const int y = yl / NL, l = yl % NL;
+#pragma unroll
+                for (int s = 0; s < NSET; ++s) {
+                    double dj[NJ];
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j) dj[j] = a.dens[s * nn + (size_t)(oJ + j) * n + oL + l];
+#pragma unroll
+                    for (int x = 0; x < NX; ++x) {
+                        double t2 = 0.0;
+#pragma unroll
+                        for (int j = 0; j < NJ; ++j) t2 += v[x * NJ + j] * dj[j];
+#pragma unroll
+                        for (int yy = 0; yy < NY; ++yy)
+                            if (yy == y) Kacc[(s * NX + x) * NY + yy] += t2;
+                    }
+                }
             }
-        });
+        } else {
+            for (long long e = lane; e < u1 - u0; e += 64) {
+                const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
+                const int oJ = (int)((ent >> 32) & 0x3fff), oL = (int)((ent >> 46) & 0x3fff);
+                const double* b = blk0 + e * BLK;
+                double v[BLK];
+#pragma unroll
+                for (int i = 0; i < BLK; ++i) v[i] = b[i];
+#pragma unroll
+                for (int s = 0; s < NSET; ++s) {
+                    double djl[NJ * NL];
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                        for (int l = 0; l < NL; ++l) djl[j * NL + l] = a.dens[s * nn + (size_t)(oJ + j) * n + oL + l];
+#pragma unroll
+                    for (int x = 0; x < NX; ++x)
+#pragma unroll
+                        for (int y = 0; y < NY; ++y) {
+                            double t2 = 0.0;
+#pragma unroll
+                            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                                for (int l = 0; l < NL; ++l) t2 += v[(y * NL + l) * NXJ + x * NJ + j] * djl[j * NL + l];
+                            Kacc[(s * NX + x) * NY + y] += t2;
+                        }
+                }
+            }
+        }
     });
-    const size_t n = a.nao, nn = n * n;
-    const int oX = a.shell_ao[X], oY = a.shell_ao[Y];
+    double* part = a.kpart + (size_t)u * KPART_STRIDE;
 #pragma unroll
     for (int i = 0; i < NSET * NX * NY; ++i) {
         const double v = wave_sum(Kacc[i]);
-        const int s = i / (NX * NY), x = (i % (NX * NY)) / NY, y = i % NY;
-        if (lane == 0) {
-            a.K[s * nn + (size_t)(oX + x) * n + oY + y] = v;
-            if (X != Y) a.K[s * nn + (size_t)(oY + y) * n + oX + x] = v;
-        }
+        if (lane == 0) part[i] = v;
     }
+}
+
+// one thread per (output block of this type, set, element): unit partials in unit order
+template <int LX_, int LY_, int NSET>
+__global__ void k_kreduce(KSArgs a, int nout) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nout * NSET * NX * NY) return;
+    const int o = a.out_off[OT] + t / (NSET * NX * NY), i = t % (NSET * NX * NY);
+    double v = 0.0;
+    for (int u = a.out_unit[o]; u < a.out_unit[o + 1]; ++u) v += a.kpart[(size_t)u * KPART_STRIDE + i];
+    const int X = a.out_x[o], Y = a.out_y[o];
+    const size_t n = a.nao, nn = n * n;
+    const int s = i / (NX * NY), x = (i % (NX * NY)) / NY, y = i % NY;
+    const int oX = a.shell_ao[X], oY = a.shell_ao[Y];
+    a.K[s * nn + (size_t)(oX + x) * n + oY + y] = v;
+    if (X != Y) a.K[s * nn + (size_t)(oY + y) * n + oX + x] = v;
 }
 
 }  // namespace
@@ -133,10 +185,23 @@ __global__ __launch_bounds__(64) void k_kstore(KSArgs a) {
 #define CAT3(a, b, c) a##b##_##c
 #define CAT3X(a, b, c) CAT3(a, b, c)
 
+// units of this type: a.unit_base[OT] .. a.unit_base[OT + 1]; nout output blocks
 void CAT3X(launch_kstore_, AIMDHF_LX, AIMDHF_LY)(const KSArgs& a, int nout, hipStream_t st) {
     if (nout <= 0) return;
-    if (a.nset == 1) k_kstore<LX, LY, 1><<<dim3(nout), dim3(64), 0, st>>>(a);
-    else k_kstore<LX, LY, 2><<<dim3(nout), dim3(64), 0, st>>>(a);
+    const int nu = a.unit_base[OT + 1] - a.unit_base[OT];
+    const int nr = nout * a.nset * NX * NY;
+    if (a.nset == 1) {
+        if (nu > 0) k_kstore<LX, LY, 1><<<dim3(nu), dim3(64), 0, st>>>(a);
+        k_kreduce<LX, LY, 1><<<dim3((nr + 127) / 128), dim3(128), 0, st>>>(a, nout);
+    } else {
+        if (nu > 0) k_kstore<LX, LY, 2><<<dim3(nu), dim3(64), 0, st>>>(a);
+        k_kreduce<LX, LY, 2><<<dim3((nr + 127) / 128), dim3(128), 0, st>>>(a, nout);
+    }
+}
+
+void CAT3X(launch_kcopy_, AIMDHF_LX, AIMDHF_LY)(const KSArgs& a, int nout, hipStream_t st) {
+    const int nu = a.unit_base[OT + 1] - a.unit_base[OT];
+    if (nout > 0 && nu > 0) k_kcopy<LX, LY><<<dim3(nu), dim3(64), 0, st>>>(a);
 }
 
 }  // namespace aimdhf

@@ -9,8 +9,11 @@
 // the ket-summed Hermite integrals X[h] = sum_k (-1)^|k| E^{cd}_k R_{h+k} of
 // degree up to l_A+l_B+1. The ket centres come from a second pass with the
 // pair roles swapped, or from translational invariance when a pair sits on
-// one atom. Each pass handles one Cartesian component pair kc of its ket
-// (one thread per (quartet, kc)), with gam[ab] = Gamma[ab][kc].
+// one atom. A pass covers NCOL consecutive Cartesian component pairs
+// kc0 .. kc0+NCOL-1 of its ket, with gam[ab * NCOL + j] = Gamma[ab][kc0 + j]:
+// the ket-summed Hermite integrals X[h][j] are built once per bra primitive
+// pair for all NCOL columns and then contracted with Gamma, so R is evaluated
+// once per primitive quartet and item.
 
 #pragma once
 #include "md_eri.h"
@@ -18,10 +21,21 @@
 
 namespace md {
 
+// columns per pass item: the largest divisor of the ket's component count
+// keeping the X block and the R table within 112 doubles
 template <int LA, int LB, int LC, int LD>
-MD_HD void grad_pass_kc(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
-                        const double* __restrict__ tab, const double* Apos, const double* Bpos, int kc,
-                        const double* gam, double* gA, double* gB) {
+MD_HD constexpr int grad_ncol() {
+    constexpr int nkc = NC(LC) * NC(LD), nh = NH(LA + LB + 1), nr = NH(LA + LB + LC + LD + 1);
+    int best = 1;
+    for (int c = 1; c <= nkc; ++c)
+        if (nkc % c == 0 && nh * c + nr <= 112) best = c;
+    return best;
+}
+
+template <int LA, int LB, int LC, int LD, int NCOL>
+MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
+                     const double* __restrict__ tab, const double* Apos, const double* Bpos, int kc0,
+                     const double* gam, double* gA, double* gB) {
     using Q = Quartet<LA, LB, LC, LD>;
     constexpr int NAB = Q::NAB, NKC = Q::NCD, NHK = Q::NHK;
     constexpr int NHB1 = NH(Q::LAB + 1), L1 = Q::L + 1;
@@ -33,16 +47,16 @@ MD_HD void grad_pass_kc(const double* __restrict__ rb, int nb, const double* __r
     static_for<0, NAB>([&](auto aa) {
         constexpr int ab = decltype(aa)::value;
         constexpr int ia = ab / NB, ib = ab % NB;
-        fcart[ab] = gam[ab] * cart_factor(cart_x(LA, ia), cart_y(LA, ia), cart_z(LA, ia)) *
+        fcart[ab] = cart_factor(cart_x(LA, ia), cart_y(LA, ia), cart_z(LA, ia)) *
                     cart_factor(cart_x(LB, ib), cart_y(LB, ib), cart_z(LB, ib));
     });
     for (int ibp = 0; ibp < nb; ++ibp) {
         const double* Bp = rb + ibp * Q::RSB;
         const double p = Bp[R_P], a = Bp[R_A], b = Bp[R_B], cab = Bp[R_C];
         const double px = Bp[R_PX], py = Bp[R_PY], pz = Bp[R_PZ];
-        double X[NHB1];
+        double X[NHB1 * NCOL];
 #pragma unroll
-        for (int i = 0; i < NHB1; ++i) X[i] = 0.0;
+        for (int i = 0; i < NHB1 * NCOL; ++i) X[i] = 0.0;
         for (int ik = 0; ik < nk; ++ik) {
             const double* K = rk + ik * Q::RSK;
             const double q = K[R_P];
@@ -55,11 +69,14 @@ MD_HD void grad_pass_kc(const double* __restrict__ rb, int nb, const double* __r
                 constexpr int k = decltype(kk)::value;
                 constexpr int tk = HT.t[k], uk = HT.u[k], vk = HT.v[k];
                 constexpr bool odd = (tk + uk + vk) & 1;
-                const double e = odd ? -K[REC_HDR + k * NKC + kc] : K[REC_HDR + k * NKC + kc];
-                static_for<0, NHB1>([&](auto hh) {
-                    constexpr int h = decltype(hh)::value;
-                    X[h] += R[hidx(HT.t[h] + tk, HT.u[h] + uk, HT.v[h] + vk)] * e;
-                });
+#pragma unroll
+                for (int j = 0; j < NCOL; ++j) {
+                    const double e = odd ? -K[REC_HDR + k * NKC + kc0 + j] : K[REC_HDR + k * NKC + kc0 + j];
+                    static_for<0, NHB1>([&](auto hh) {
+                        constexpr int h = decltype(hh)::value;
+                        X[h * NCOL + j] += R[hidx(HT.t[h] + tk, HT.u[h] + uk, HT.v[h] + vk)] * e;
+                    });
+                }
             });
         }
         const double mu = a * b / p;
@@ -81,7 +98,10 @@ MD_HD void grad_pass_kc(const double* __restrict__ rb, int nb, const double* __r
                 constexpr int ox = t > ax + bx, oy = u > ay + by, oz = v > az + bz;
                 if constexpr (ox + oy + oz <= 1 && t <= ax + bx + 1 && u <= ay + by + 1 && v <= az + bz + 1) {
                     auto E = [](const double* e, int i, int j, int tt) { return e[(i * JS + j) * TS + tt]; };
-                    const double y = f * X[h];
+                    double xg = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NCOL; ++j) xg += gam[ab * NCOL + j] * X[h * NCOL + j];
+                    const double y = f * xg;
                     const double e0x = E(ex, ax, bx, t), e0y = E(ey, ay, by, u), e0z = E(ez, az, bz, v);
                     double dax = ta * E(ex, ax + 1, bx, t), dbx = tb * E(ex, ax, bx + 1, t);
                     double day = ta * E(ey, ay + 1, by, u), dby = tb * E(ey, ay, by + 1, u);
@@ -103,6 +123,14 @@ MD_HD void grad_pass_kc(const double* __restrict__ rb, int nb, const double* __r
         });
         for (int d = 0; d < 3; ++d) { gA[d] += sA[d]; gB[d] += sB[d]; }
     }
+}
+
+// single ket component kc, gam[ab] = Gamma[ab][kc]
+template <int LA, int LB, int LC, int LD>
+MD_HD void grad_pass_kc(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
+                        const double* __restrict__ tab, const double* Apos, const double* Bpos, int kc,
+                        const double* gam, double* gA, double* gB) {
+    grad_pass<LA, LB, LC, LD, 1>(rb, nb, rk, nk, tab, Apos, Bpos, kc, gam, gA, gB);
 }
 
 }  // namespace md

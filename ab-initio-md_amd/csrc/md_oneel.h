@@ -1,10 +1,14 @@
-// One-electron integrals and their nuclear derivatives for one shell pair
-// (runtime l <= 2), McMurchie-Davidson as aimd/qc/integrals.py:146-501:
+// One-electron integrals and their nuclear derivatives for one shell pair,
+// McMurchie-Davidson as aimd/qc/integrals.py:146-501:
 //   S, T, dipole          overlap-type (E^{ij}_0, kinetic via E^{i,j+2}, E^{ij}_1)
 //   V                     -sum_C Z_C (2 pi/p) sum_h E_h R_h(p, P - C)
 //   d/dR [W S + D T]      shifted expansions 2a E^{i+1,j} - i E^{i-1,j}
 //   d/dR [D V]            basis centres + the Hellmann-Feynman term (-R_{h+e_d})
-// __host__ __device__ so the check harness runs the same code on the CPU.
+// Templated on the two angular momenta so every table has a compile-time
+// shape (register-resident on the GPU); the runtime-l entry points at the end
+// dispatch to them. The nuclear terms take a strided subset of the point
+// charges (c = c0, c0 + cs, ...) so a wavefront can split the sum over its
+// lanes. __host__ __device__ so the check harness runs the same code.
 
 #pragma once
 #include "md_core.h"
@@ -12,29 +16,37 @@
 
 namespace md {
 
-constexpr int OE_IM = 4, OE_JM = 5, OE_TM = 10;   // E buffer: i <= l+1, j <= l+2, t <= i+j+1
-
-struct E1 {
-    double e[OE_IM][OE_JM][OE_TM];
+// E^{ij}_t for i <= IM, j <= JM (zero for t > i + j)
+template <int IM, int JM>
+struct E1T {
+    double e[IM + 1][JM + 1][IM + JM + 2];
     MD_HD double operator()(int i, int j, int t) const { return e[i][j][t]; }
 };
 
-// E^{ij}_t for i <= imax, j <= jmax (t <= i+j), zero elsewhere
-MD_HD void herm_e_rt(int imax, int jmax, double a, double b, double xab, double e00, E1& E) {
-    double p = a + b;
-    double xpa = -b * xab / p, xpb = a * xab / p, h = 0.5 / p;
-    for (int i = 0; i < OE_IM; ++i)
-        for (int j = 0; j < OE_JM; ++j)
-            for (int t = 0; t < OE_TM; ++t) E.e[i][j][t] = 0.0;
+template <int IM, int JM>
+MD_HD void herm_e_t(double a, double b, double xab, double e00, E1T<IM, JM>& E) {
+    const double p = a + b;
+    const double xpa = -b * xab / p, xpb = a * xab / p, h = 0.5 / p;
+#pragma unroll
+    for (int i = 0; i <= IM; ++i)
+#pragma unroll
+        for (int j = 0; j <= JM; ++j)
+#pragma unroll
+            for (int t = 0; t <= IM + JM + 1; ++t) E.e[i][j][t] = 0.0;
     E.e[0][0][0] = e00;
-    for (int j = 1; j <= jmax; ++j)
+#pragma unroll
+    for (int j = 1; j <= JM; ++j)
+#pragma unroll
         for (int t = 0; t <= j; ++t) {
             double v = xpb * E.e[0][j - 1][t] + (t + 1) * E.e[0][j - 1][t + 1];
             if (t > 0) v += h * E.e[0][j - 1][t - 1];
             E.e[0][j][t] = v;
         }
-    for (int i = 1; i <= imax; ++i)
-        for (int j = 0; j <= jmax; ++j)
+#pragma unroll
+    for (int i = 1; i <= IM; ++i)
+#pragma unroll
+        for (int j = 0; j <= JM; ++j)
+#pragma unroll
             for (int t = 0; t <= i + j; ++t) {
                 double v = xpa * E.e[i - 1][j][t] + (t + 1) * E.e[i - 1][j][t + 1];
                 if (t > 0) v += h * E.e[i - 1][j][t - 1];
@@ -42,31 +54,37 @@ MD_HD void herm_e_rt(int imax, int jmax, double a, double b, double xab, double 
             }
 }
 
-MD_HD int cart_comp(int l, int i, int d) {
-    return d == 0 ? cart_x(l, i) : (d == 1 ? cart_y(l, i) : cart_z(l, i));
-}
-
-MD_HD double kin1d(const E1& E, int i, int j, double b) {
-    double v = -2.0 * b * (2 * j + 1) * E(i, j, 0) + 4.0 * b * b * E(i, j + 2, 0);
-    if (j >= 2) v += j * (j - 1) * E(i, j - 2, 0);
+template <class E>
+MD_HD double kin1d(const E& Et, int i, int j, double b) {
+    double v = -2.0 * b * (2 * j + 1) * Et(i, j, 0) + 4.0 * b * b * Et(i, j + 2, 0);
+    if (j >= 2) v += j * (j - 1) * Et(i, j - 2, 0);
     return -0.5 * v;
 }
 
-// R_{tuv}(alpha, x, y, z) for t+u+v <= L (runtime L <= 5), in place, into R[NH(5)]
-MD_HD void hermite_r_rt(int L, double alpha, double x, double y, double z, const double* tab, double* R) {
-    double T = alpha * (x * x + y * y + z * z);
-    double F[6];
+MD_HD constexpr int cart_comp(int l, int i, int d) {
+    return d == 0 ? cart_x(l, i) : (d == 1 ? cart_y(l, i) : cart_z(l, i));
+}
+
+// R_{tuv}(alpha, x, y, z) for t+u+v <= L, in place
+template <int L>
+MD_HD void hermite_r_t(double alpha, double x, double y, double z, const double* tab, double* R) {
+    const double T = alpha * (x * x + y * y + z * z);
+    double F[L + 1];
     F[L] = boys_top(L, T, tab);
-    double ex = L > 0 ? exp(-T) : 0.0;
+    const double ex = L > 0 ? exp(-T) : 0.0;
+#pragma unroll
     for (int n = L - 1; n >= 0; --n) F[n] = (2.0 * T * F[n + 1] + ex) * (1.0 / (2 * n + 1));
-    double m2a = -2.0 * alpha;
-    double pw[6];
+    const double m2a = -2.0 * alpha;
+    double pw[L + 1];
     pw[0] = 1.0;
+#pragma unroll
     for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * m2a;
     R[0] = pw[L] * F[L];
+#pragma unroll
     for (int n = L - 1; n >= 0; --n) {
+#pragma unroll
         for (int h = NH(L - n) - 1; h >= 1; --h) {
-            int t = HT.t[h], u = HT.u[h], v = HT.v[h];
+            const int t = HT.t[h], u = HT.u[h], v = HT.v[h];
             double r;
             if (t > 0) {
                 r = x * R[hidx(t - 1, u, v)];
@@ -89,226 +107,314 @@ struct ShellView {
     const double *ex, *coef, *pos;
 };
 
-// S, T and dipole (origin O) block of shells si >= sj, written to both triangles
-MD_HD void overlap_type_pair(const ShellView& B, int n, int si, int sj, const double* O, double* S, double* T,
-                             double* M) {
-    const int li = B.l[si], lj = B.l[sj], ai = B.atom[si], aj = B.atom[sj];
-    const int oi = B.ao[si], oj = B.ao[sj], nci = NC(li), ncj = NC(lj);
-    double s[36], t[36], m[3][36];
-    for (int k = 0; k < 36; ++k) { s[k] = t[k] = m[0][k] = m[1][k] = m[2][k] = 0.0; }
+// S, T and dipole (origin O) of shells si (l = LI), sj (l = LJ) for primitive pairs
+// k = k0, k0 + ks, ... (row-major over si's and sj's primitives), accumulated into
+// s, t, m[3] (NC(LI) x NC(LJ) each)
+template <int LI, int LJ>
+MD_HD void overlap_type_acc(const ShellView& B, int si, int sj, const double* O, int k0, int ks, double* s, double* t,
+                            double* m) {
+    constexpr int NI = NC(LI), NJ = NC(LJ);
+    const int ai = B.atom[si], aj = B.atom[sj];
+    const int pi0 = B.prim[si], npj = B.prim[sj + 1] - B.prim[sj], npp = (B.prim[si + 1] - pi0) * npj;
     const double sq_pi = sqrt(MD_PI);
-    E1 E[3];
-    for (int ka = B.prim[si]; ka < B.prim[si + 1]; ++ka) {
-        double a = B.ex[ka];
-        for (int kb = B.prim[sj]; kb < B.prim[sj + 1]; ++kb) {
-            double b = B.ex[kb];
-            double p = a + b, mu = a * b / p, sq = sq_pi / sqrt(p);
-            for (int d = 0; d < 3; ++d) {
-                double xab = B.pos[3 * ai + d] - B.pos[3 * aj + d];
-                double e00 = exp(-mu * xab * xab) * sq;
-                if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
-                herm_e_rt(li, lj + 2, a, b, xab, e00, E[d]);
-            }
-            double P[3];
-            for (int d = 0; d < 3; ++d) P[d] = (a * B.pos[3 * ai + d] + b * B.pos[3 * aj + d]) / p - O[d];
-            for (int ia = 0; ia < nci; ++ia) {
-                int ix = cart_x(li, ia), iy = cart_y(li, ia), iz = cart_z(li, ia);
-                double fa = cart_factor(ix, iy, iz);
-                for (int ib = 0; ib < ncj; ++ib) {
-                    int jx = cart_x(lj, ib), jy = cart_y(lj, ib), jz = cart_z(lj, ib);
-                    double f = fa * cart_factor(jx, jy, jz);
-                    double sx = E[0](ix, jx, 0), sy = E[1](iy, jy, 0), sz = E[2](iz, jz, 0);
-                    double kx = kin1d(E[0], ix, jx, b), ky = kin1d(E[1], iy, jy, b), kz = kin1d(E[2], iz, jz, b);
-                    int k = ia * ncj + ib;
-                    s[k] += f * sx * sy * sz;
-                    t[k] += f * (kx * sy * sz + sx * ky * sz + sx * sy * kz);
-                    m[0][k] += f * (E[0](ix, jx, 1) + P[0] * sx) * sy * sz;
-                    m[1][k] += f * sx * (E[1](iy, jy, 1) + P[1] * sy) * sz;
-                    m[2][k] += f * sx * sy * (E[2](iz, jz, 1) + P[2] * sz);
-                }
+    for (int k = k0; k < npp; k += ks) {
+        const int ka = pi0 + k / npj, kb = B.prim[sj] + k % npj;
+        const double a = B.ex[ka], b = B.ex[kb];
+        const double p = a + b, mu = a * b / p, sq = sq_pi / sqrt(p);
+        E1T<LI, LJ + 2> E[3];
+        double P[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double xab = B.pos[3 * ai + d] - B.pos[3 * aj + d];
+            double e00 = exp(-mu * xab * xab) * sq;
+            if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
+            herm_e_t<LI, LJ + 2>(a, b, xab, e00, E[d]);
+            P[d] = (a * B.pos[3 * ai + d] + b * B.pos[3 * aj + d]) / p - O[d];
+        }
+#pragma unroll
+        for (int ia = 0; ia < NI; ++ia) {
+            const int ix = cart_x(LI, ia), iy = cart_y(LI, ia), iz = cart_z(LI, ia);
+            const double fa = cart_factor(ix, iy, iz);
+#pragma unroll
+            for (int ib = 0; ib < NJ; ++ib) {
+                const int jx = cart_x(LJ, ib), jy = cart_y(LJ, ib), jz = cart_z(LJ, ib);
+                const double f = fa * cart_factor(jx, jy, jz);
+                const double sx = E[0](ix, jx, 0), sy = E[1](iy, jy, 0), sz = E[2](iz, jz, 0);
+                const double kx = kin1d(E[0], ix, jx, b), ky = kin1d(E[1], iy, jy, b), kz = kin1d(E[2], iz, jz, b);
+                const int q = ia * NJ + ib;
+                s[q] += f * sx * sy * sz;
+                t[q] += f * (kx * sy * sz + sx * ky * sz + sx * sy * kz);
+                m[0 * NI * NJ + q] += f * (E[0](ix, jx, 1) + P[0] * sx) * sy * sz;
+                m[1 * NI * NJ + q] += f * sx * (E[1](iy, jy, 1) + P[1] * sy) * sz;
+                m[2 * NI * NJ + q] += f * sx * sy * (E[2](iz, jz, 1) + P[2] * sz);
             }
         }
     }
-    const size_t nn = (size_t)n * n;
-    for (int ia = 0; ia < nci; ++ia)
-        for (int ib = 0; ib < ncj; ++ib) {
-            int k = ia * ncj + ib;
-            size_t u = (size_t)(oi + ia) * n + oj + ib, v = (size_t)(oj + ib) * n + oi + ia;
-            S[u] = S[v] = s[k];
-            T[u] = T[v] = t[k];
-            for (int d = 0; d < 3; ++d) M[d * nn + u] = M[d * nn + v] = m[d][k];
-        }
 }
 
-// V block of shells si >= sj over point charges (Z_c, C_c)
-MD_HD void nuclear_pair(const ShellView& B, int n, int si, int sj, int nc, const double* Z, const double* C,
-                        const double* tab, double* V) {
-    const int li = B.l[si], lj = B.l[sj], ai = B.atom[si], aj = B.atom[sj];
-    const int oi = B.ao[si], oj = B.ao[sj], nci = NC(li), ncj = NC(lj);
-    const int L = li + lj, nh = NH(L);
-    double v[36];
-    for (int k = 0; k < 36; ++k) v[k] = 0.0;
-    E1 E[3];
-    double R[NH(5)], Rs[NH(4)];
+// V block contributions of point charges c = c0, c0 + cs, ... < nc into v (NC(LI) x NC(LJ))
+template <int LI, int LJ>
+MD_HD void nuclear_acc(const ShellView& B, int si, int sj, int c0, int cs, int nc, const double* Z, const double* C,
+                       const double* tab, double* v) {
+    constexpr int NI = NC(LI), NJ = NC(LJ), L = LI + LJ, NHL = NH(L);
+    const int ai = B.atom[si], aj = B.atom[sj];
     for (int ka = B.prim[si]; ka < B.prim[si + 1]; ++ka) {
-        double a = B.ex[ka];
+        const double a = B.ex[ka];
         for (int kb = B.prim[sj]; kb < B.prim[sj + 1]; ++kb) {
-            double b = B.ex[kb];
-            double p = a + b, mu = a * b / p;
+            const double b = B.ex[kb];
+            const double p = a + b, mu = a * b / p;
+            E1T<LI, LJ> E[3];
+#pragma unroll
             for (int d = 0; d < 3; ++d) {
-                double xab = B.pos[3 * ai + d] - B.pos[3 * aj + d];
+                const double xab = B.pos[3 * ai + d] - B.pos[3 * aj + d];
                 double e00 = exp(-mu * xab * xab);
                 if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
-                herm_e_rt(li, lj, a, b, xab, e00, E[d]);
+                herm_e_t<LI, LJ>(a, b, xab, e00, E[d]);
             }
-            double bp = b / p;
-            for (int h = 0; h < nh; ++h) Rs[h] = 0.0;
-            for (int c = 0; c < nc; ++c) {
-                double X = (B.pos[3 * ai + 0] - C[3 * c + 0]) - bp * (B.pos[3 * ai + 0] - B.pos[3 * aj + 0]);
-                double Y = (B.pos[3 * ai + 1] - C[3 * c + 1]) - bp * (B.pos[3 * ai + 1] - B.pos[3 * aj + 1]);
-                double Zc = (B.pos[3 * ai + 2] - C[3 * c + 2]) - bp * (B.pos[3 * ai + 2] - B.pos[3 * aj + 2]);
-                hermite_r_rt(L, p, X, Y, Zc, tab, R);
-                double w = -Z[c] * 2.0 * MD_PI / p;
-                for (int h = 0; h < nh; ++h) Rs[h] += w * R[h];
+            const double bp = b / p;
+            double Rs[NHL];
+#pragma unroll
+            for (int h = 0; h < NHL; ++h) Rs[h] = 0.0;
+            for (int c = c0; c < nc; c += cs) {
+                const double X = (B.pos[3 * ai + 0] - C[3 * c + 0]) - bp * (B.pos[3 * ai + 0] - B.pos[3 * aj + 0]);
+                const double Y = (B.pos[3 * ai + 1] - C[3 * c + 1]) - bp * (B.pos[3 * ai + 1] - B.pos[3 * aj + 1]);
+                const double Zc = (B.pos[3 * ai + 2] - C[3 * c + 2]) - bp * (B.pos[3 * ai + 2] - B.pos[3 * aj + 2]);
+                double R[NHL];
+                hermite_r_t<L>(p, X, Y, Zc, tab, R);
+                const double w = -Z[c] * 2.0 * MD_PI / p;
+#pragma unroll
+                for (int h = 0; h < NHL; ++h) Rs[h] += w * R[h];
             }
-            for (int ia = 0; ia < nci; ++ia) {
-                int ix = cart_x(li, ia), iy = cart_y(li, ia), iz = cart_z(li, ia);
-                double fa = cart_factor(ix, iy, iz);
-                for (int ib = 0; ib < ncj; ++ib) {
-                    int jx = cart_x(lj, ib), jy = cart_y(lj, ib), jz = cart_z(lj, ib);
+#pragma unroll
+            for (int ia = 0; ia < NI; ++ia) {
+                const int ix = cart_x(LI, ia), iy = cart_y(LI, ia), iz = cart_z(LI, ia);
+                const double fa = cart_factor(ix, iy, iz);
+#pragma unroll
+                for (int ib = 0; ib < NJ; ++ib) {
+                    const int jx = cart_x(LJ, ib), jy = cart_y(LJ, ib), jz = cart_z(LJ, ib);
                     double acc = 0.0;
-                    for (int h = 0; h < nh; ++h)
-                        acc += E[0](ix, jx, HT.t[h]) * E[1](iy, jy, HT.u[h]) * E[2](iz, jz, HT.v[h]) * Rs[h];
-                    v[ia * ncj + ib] += fa * cart_factor(jx, jy, jz) * acc;
+#pragma unroll
+                    for (int h = 0; h < NHL; ++h)
+                        if (HT.t[h] <= ix + jx && HT.u[h] <= iy + jy && HT.v[h] <= iz + jz)
+                            acc += E[0](ix, jx, HT.t[h]) * E[1](iy, jy, HT.u[h]) * E[2](iz, jz, HT.v[h]) * Rs[h];
+                    v[ia * NJ + ib] += fa * cart_factor(jx, jy, jz) * acc;
                 }
             }
         }
     }
-    for (int ia = 0; ia < nci; ++ia)
-        for (int ib = 0; ib < ncj; ++ib) {
-            V[(size_t)(oi + ia) * n + oj + ib] = v[ia * ncj + ib];
-            V[(size_t)(oj + ib) * n + oi + ia] = v[ia * ncj + ib];
-        }
 }
 
-// d/dR of [sum_mn Wm_mn S_mn + sum_mn D_mn T_mn] from shells si > sj on different
-// atoms: g (3 values) for atom ai; atom aj gets -g (integrals.py:332-404).
-MD_HD void ovlkin_grad_pair(const ShellView& B, int n, int si, int sj, const double* Wm, const double* D, double* g) {
-    const int li = B.l[si], lj = B.l[sj], ai = B.atom[si], aj = B.atom[sj];
-    const int oi = B.ao[si], oj = B.ao[sj], nci = NC(li), ncj = NC(lj);
+// d/dR_{atom(si)} of [sum Wm S + sum D T] over the primitive pairs k = k0, k0 + ks, ...
+// of shells si, sj on different atoms (atom(sj) gets -g) (integrals.py:332-404)
+template <int LI, int LJ>
+MD_HD void ovlkin_grad_acc(const ShellView& B, int n, int si, int sj, const double* Wm, const double* D, int k0, int ks,
+                           double* g) {
+    constexpr int NI = NC(LI), NJ = NC(LJ);
+    const int ai = B.atom[si], aj = B.atom[sj];
+    const int oi = B.ao[si], oj = B.ao[sj];
+    const int pi0 = B.prim[si], npj = B.prim[sj + 1] - B.prim[sj], npp = (B.prim[si + 1] - pi0) * npj;
     const double sq_pi = sqrt(MD_PI);
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    E1 E[3];
-    for (int ka = B.prim[si]; ka < B.prim[si + 1]; ++ka) {
-        double a = B.ex[ka];
-        for (int kb = B.prim[sj]; kb < B.prim[sj + 1]; ++kb) {
-            double b = B.ex[kb];
-            double p = a + b, mu = a * b / p, sq = sq_pi / sqrt(p);
-            for (int d = 0; d < 3; ++d) {
-                double xab = B.pos[3 * ai + d] - B.pos[3 * aj + d];
-                double e00 = exp(-mu * xab * xab) * sq;
-                if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
-                herm_e_rt(li + 1, lj + 2, a, b, xab, e00, E[d]);
+    double wmn[NI * NJ], dmn[NI * NJ];
+#pragma unroll
+    for (int ia = 0; ia < NI; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < NJ; ++ib) {
+            const double f = 2.0 * cart_factor(cart_x(LI, ia), cart_y(LI, ia), cart_z(LI, ia)) *
+                             cart_factor(cart_x(LJ, ib), cart_y(LJ, ib), cart_z(LJ, ib));
+            wmn[ia * NJ + ib] = f * Wm[(size_t)(oi + ia) * n + oj + ib];
+            dmn[ia * NJ + ib] = f * D[(size_t)(oi + ia) * n + oj + ib];
+        }
+    for (int k = k0; k < npp; k += ks) {
+        const int ka = pi0 + k / npj, kb = B.prim[sj] + k % npj;
+        const double a = B.ex[ka], b = B.ex[kb];
+        const double p = a + b, mu = a * b / p, sq = sq_pi / sqrt(p);
+        E1T<LI + 1, LJ + 2> E[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double xab = B.pos[3 * ai + d] - B.pos[3 * aj + d];
+            double e00 = exp(-mu * xab * xab) * sq;
+            if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
+            herm_e_t<LI + 1, LJ + 2>(a, b, xab, e00, E[d]);
+        }
+#pragma unroll
+        for (int ia = 0; ia < NI; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NJ; ++ib) {
+                double S1[3], K1[3], dS[3], dK[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const int i = cart_comp(LI, ia, d), j = cart_comp(LJ, ib, d);
+                    S1[d] = E[d](i, j, 0);
+                    K1[d] = kin1d(E[d], i, j, b);
+                    dS[d] = 2.0 * a * E[d](i + 1, j, 0);
+                    dK[d] = 2.0 * a * kin1d(E[d], i + 1, j, b);
+                    if (i > 0) {
+                        dS[d] -= i * E[d](i - 1, j, 0);
+                        dK[d] -= i * kin1d(E[d], i - 1, j, b);
+                    }
+                }
+                const double w = wmn[ia * NJ + ib], dm = dmn[ia * NJ + ib];
+                g[0] += w * dS[0] * S1[1] * S1[2] + dm * (dK[0] * S1[1] * S1[2] + dS[0] * (K1[1] * S1[2] + S1[1] * K1[2]));
+                g[1] += w * S1[0] * dS[1] * S1[2] + dm * (dK[1] * S1[0] * S1[2] + dS[1] * (K1[0] * S1[2] + S1[0] * K1[2]));
+                g[2] += w * S1[0] * S1[1] * dS[2] + dm * (dK[2] * S1[0] * S1[1] + dS[2] * (K1[0] * S1[1] + S1[0] * K1[1]));
             }
-            for (int ia = 0; ia < nci; ++ia) {
-                double fa = cart_factor(cart_x(li, ia), cart_y(li, ia), cart_z(li, ia));
-                for (int ib = 0; ib < ncj; ++ib) {
-                    double f = 2.0 * fa * cart_factor(cart_x(lj, ib), cart_y(lj, ib), cart_z(lj, ib));
-                    double wmn = f * Wm[(size_t)(oi + ia) * n + oj + ib];
-                    double dmn = f * D[(size_t)(oi + ia) * n + oj + ib];
-                    double S1[3], K1[3], dS[3], dK[3];
-                    for (int d = 0; d < 3; ++d) {
-                        int i = cart_comp(li, ia, d), j = cart_comp(lj, ib, d);
-                        S1[d] = E[d](i, j, 0);
-                        K1[d] = kin1d(E[d], i, j, b);
-                        dS[d] = 2.0 * a * E[d](i + 1, j, 0);
-                        dK[d] = 2.0 * a * kin1d(E[d], i + 1, j, b);
-                        if (i > 0) {
-                            dS[d] -= i * E[d](i - 1, j, 0);
-                            dK[d] -= i * kin1d(E[d], i - 1, j, b);
+    }
+}
+
+// d/dR sum D V of shells si, sj for operator centres c = c0, c0 + cs, ... < natm
+// (integrals.py:407-501): add(c, d, x) for each centre, gA / gB += the basis-centre
+// terms of atom(si) / atom(sj)
+template <int LI, int LJ, class Add>
+MD_HD void nuclear_grad_acc(const ShellView& B, int n, int si, int sj, int c0, int cs, int natm, const double* Z,
+                            const double* D, const double* tab, Add add, double* gA, double* gB) {
+    constexpr int NI = NC(LI), NJ = NC(LJ), L = LI + LJ, NHL = NH(L), NH1 = NH(L + 1);
+    const int ai = B.atom[si], aj = B.atom[sj];
+    const int oi = B.ao[si], oj = B.ao[sj];
+    const double sym = (si != sj) ? 2.0 : 1.0;
+    double wd[NI * NJ];
+#pragma unroll
+    for (int ia = 0; ia < NI; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < NJ; ++ib)
+            wd[ia * NJ + ib] = sym * cart_factor(cart_x(LI, ia), cart_y(LI, ia), cart_z(LI, ia)) *
+                               cart_factor(cart_x(LJ, ib), cart_y(LJ, ib), cart_z(LJ, ib)) *
+                               D[(size_t)(oi + ia) * n + oj + ib];
+    for (int c = c0; c < natm; c += cs) {
+        double gc[3] = {0.0, 0.0, 0.0};
+        for (int ka = B.prim[si]; ka < B.prim[si + 1]; ++ka) {
+            const double a = B.ex[ka];
+            for (int kb = B.prim[sj]; kb < B.prim[sj + 1]; ++kb) {
+                const double b = B.ex[kb];
+                const double p = a + b, mu = a * b / p;
+                E1T<LI + 1, LJ> E[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double xab = B.pos[3 * ai + d] - B.pos[3 * aj + d];
+                    double e00 = exp(-mu * xab * xab);
+                    if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
+                    herm_e_t<LI + 1, LJ>(a, b, xab, e00, E[d]);
+                }
+                double dE[NHL], dA[3][NH1];
+#pragma unroll
+                for (int h = 0; h < NH1; ++h) dA[0][h] = dA[1][h] = dA[2][h] = 0.0;
+#pragma unroll
+                for (int h = 0; h < NHL; ++h) dE[h] = 0.0;
+#pragma unroll
+                for (int ia = 0; ia < NI; ++ia) {
+                    const int ix = cart_x(LI, ia), iy = cart_y(LI, ia), iz = cart_z(LI, ia);
+#pragma unroll
+                    for (int ib = 0; ib < NJ; ++ib) {
+                        const int jx = cart_x(LJ, ib), jy = cart_y(LJ, ib), jz = cart_z(LJ, ib);
+                        const double w = wd[ia * NJ + ib];
+#pragma unroll
+                        for (int h = 0; h < NH1; ++h) {
+                            const int t = HT.t[h], u = HT.u[h], v = HT.v[h];
+                            if (t > ix + jx + 1 || u > iy + jy + 1 || v > iz + jz + 1) continue;
+                            const double ex = E[0](ix, jx, t), ey = E[1](iy, jy, u), ez = E[2](iz, jz, v);
+                            if (h < NHL) dE[h] += w * ex * ey * ez;
+                            double dx = 2.0 * a * E[0](ix + 1, jx, t);
+                            if (ix > 0) dx -= ix * E[0](ix - 1, jx, t);
+                            double dy = 2.0 * a * E[1](iy + 1, jy, u);
+                            if (iy > 0) dy -= iy * E[1](iy - 1, jy, u);
+                            double dz = 2.0 * a * E[2](iz + 1, jz, v);
+                            if (iz > 0) dz -= iz * E[2](iz - 1, jz, v);
+                            dA[0][h] += w * dx * ey * ez;
+                            dA[1][h] += w * ex * dy * ez;
+                            dA[2][h] += w * ex * ey * dz;
                         }
                     }
-                    g0 += wmn * dS[0] * S1[1] * S1[2] + dmn * (dK[0] * S1[1] * S1[2] + dS[0] * (K1[1] * S1[2] + S1[1] * K1[2]));
-                    g1 += wmn * S1[0] * dS[1] * S1[2] + dmn * (dK[1] * S1[0] * S1[2] + dS[1] * (K1[0] * S1[2] + S1[0] * K1[2]));
-                    g2 += wmn * S1[0] * S1[1] * dS[2] + dmn * (dK[2] * S1[0] * S1[1] + dS[2] * (K1[0] * S1[1] + S1[0] * K1[1]));
+                }
+                const double bp = b / p;
+                const double X = (B.pos[3 * ai + 0] - B.pos[3 * c + 0]) - bp * (B.pos[3 * ai + 0] - B.pos[3 * aj + 0]);
+                const double Y = (B.pos[3 * ai + 1] - B.pos[3 * c + 1]) - bp * (B.pos[3 * ai + 1] - B.pos[3 * aj + 1]);
+                const double Zc = (B.pos[3 * ai + 2] - B.pos[3 * c + 2]) - bp * (B.pos[3 * ai + 2] - B.pos[3 * aj + 2]);
+                double R[NH1];
+                hermite_r_t<L + 1>(p, X, Y, Zc, tab, R);
+                const double pref = -Z[c] * 2.0 * MD_PI / p;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    double ga = 0.0, gcd = 0.0;
+#pragma unroll
+                    for (int h = 0; h < NH1; ++h) ga += dA[d][h] * R[h];
+#pragma unroll
+                    for (int h = 0; h < NHL; ++h)
+                        gcd -= dE[h] * R[hidx(HT.t[h] + (d == 0), HT.u[h] + (d == 1), HT.v[h] + (d == 2))];
+                    ga *= pref;
+                    gcd *= pref;
+                    gA[d] += ga;
+                    gc[d] += gcd;
+                    gB[d] -= ga + gcd;
                 }
             }
         }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) add(c, d, gc[d]);
     }
-    g[0] = g0;
-    g[1] = g1;
-    g[2] = g2;
 }
 
-// d/dR sum D V for shells si >= sj: calls add(atom, dir, value) for the basis
-// centres and every operator centre c (integrals.py:407-501)
+// ------------------------------------------------------------ runtime l
+template <class F>
+MD_HD void with_ls(int li, int lj, F&& f) {
+    static_for<0, 3>([&](auto ii) {
+        static_for<0, 3>([&](auto jj) {
+            if (li == decltype(ii)::value && lj == decltype(jj)::value) f(ii, jj);
+        });
+    });
+}
+
+// S, T and dipole block of shells si, sj, written to both triangles
+MD_HD void overlap_type_pair(const ShellView& B, int n, int si, int sj, const double* O, double* S, double* T,
+                             double* M) {
+    with_ls(B.l[si], B.l[sj], [&](auto ii, auto jj) {
+        constexpr int LI = decltype(ii)::value, LJ = decltype(jj)::value, NI = NC(LI), NJ = NC(LJ);
+        double s[NI * NJ] = {}, t[NI * NJ] = {}, m[3 * NI * NJ] = {};
+        overlap_type_acc<LI, LJ>(B, si, sj, O, 0, 1, s, t, m);
+        const size_t nn = (size_t)n * n;
+        const int oi = B.ao[si], oj = B.ao[sj];
+        for (int q = 0; q < NI * NJ; ++q) {
+            const size_t u = (size_t)(oi + q / NJ) * n + oj + q % NJ, v = (size_t)(oj + q % NJ) * n + oi + q / NJ;
+            S[u] = S[v] = s[q];
+            T[u] = T[v] = t[q];
+            for (int d = 0; d < 3; ++d) M[d * nn + u] = M[d * nn + v] = m[d * NI * NJ + q];
+        }
+    });
+}
+
+// V block of shells si, sj over point charges (Z_c, C_c), both triangles
+MD_HD void nuclear_pair(const ShellView& B, int n, int si, int sj, int nc, const double* Z, const double* C,
+                        const double* tab, double* V) {
+    with_ls(B.l[si], B.l[sj], [&](auto ii, auto jj) {
+        constexpr int LI = decltype(ii)::value, LJ = decltype(jj)::value, NI = NC(LI), NJ = NC(LJ);
+        double v[NI * NJ] = {};
+        nuclear_acc<LI, LJ>(B, si, sj, 0, 1, nc, Z, C, tab, v);
+        const int oi = B.ao[si], oj = B.ao[sj];
+        for (int q = 0; q < NI * NJ; ++q) {
+            V[(size_t)(oi + q / NJ) * n + oj + q % NJ] = v[q];
+            V[(size_t)(oj + q % NJ) * n + oi + q / NJ] = v[q];
+        }
+    });
+}
+
+// d/dR of [sum Wm S + sum D T] of shells si != sj on different atoms: g for atom(si)
+MD_HD void ovlkin_grad_pair(const ShellView& B, int n, int si, int sj, const double* Wm, const double* D, double* g) {
+    g[0] = g[1] = g[2] = 0.0;
+    with_ls(B.l[si], B.l[sj], [&](auto ii, auto jj) {
+        ovlkin_grad_acc<decltype(ii)::value, decltype(jj)::value>(B, n, si, sj, Wm, D, 0, 1, g);
+    });
+}
+
+// d/dR sum D V of shells si, sj: add(atom, dir, value) for every centre
 template <class Add>
 MD_HD void nuclear_grad_pair(const ShellView& B, int n, int si, int sj, int natm, const double* Z, const double* D,
                              const double* tab, Add add) {
-    const int li = B.l[si], lj = B.l[sj], ai = B.atom[si], aj = B.atom[sj];
-    const int oi = B.ao[si], oj = B.ao[sj], nci = NC(li), ncj = NC(lj);
-    const int L = li + lj, nh = NH(L), nh1 = NH(L + 1);
-    const double sym = (si != sj) ? 2.0 : 1.0;
-    E1 E[3];
-    double R[NH(5)], dE[NH(5)], dA[3][NH(5)];
-    for (int ka = B.prim[si]; ka < B.prim[si + 1]; ++ka) {
-        double a = B.ex[ka];
-        for (int kb = B.prim[sj]; kb < B.prim[sj + 1]; ++kb) {
-            double b = B.ex[kb];
-            double p = a + b, mu = a * b / p;
-            for (int d = 0; d < 3; ++d) {
-                double xab = B.pos[3 * ai + d] - B.pos[3 * aj + d];
-                double e00 = exp(-mu * xab * xab);
-                if (d == 0) e00 *= B.coef[ka] * B.coef[kb];
-                herm_e_rt(li + 1, lj, a, b, xab, e00, E[d]);
-            }
-            for (int h = 0; h < nh1; ++h) dE[h] = dA[0][h] = dA[1][h] = dA[2][h] = 0.0;
-            for (int ia = 0; ia < nci; ++ia) {
-                int ix = cart_x(li, ia), iy = cart_y(li, ia), iz = cart_z(li, ia);
-                double fa = cart_factor(ix, iy, iz);
-                for (int ib = 0; ib < ncj; ++ib) {
-                    int jx = cart_x(lj, ib), jy = cart_y(lj, ib), jz = cart_z(lj, ib);
-                    double w = sym * fa * cart_factor(jx, jy, jz) * D[(size_t)(oi + ia) * n + oj + ib];
-                    if (w == 0.0) continue;
-                    for (int h = 0; h < nh1; ++h) {
-                        int t = HT.t[h], u = HT.u[h], v = HT.v[h];
-                        double ex = E[0](ix, jx, t), ey = E[1](iy, jy, u), ez = E[2](iz, jz, v);
-                        if (h < nh) dE[h] += w * ex * ey * ez;
-                        double dx = 2.0 * a * E[0](ix + 1, jx, t);
-                        if (ix > 0) dx -= ix * E[0](ix - 1, jx, t);
-                        double dy = 2.0 * a * E[1](iy + 1, jy, u);
-                        if (iy > 0) dy -= iy * E[1](iy - 1, jy, u);
-                        double dz = 2.0 * a * E[2](iz + 1, jz, v);
-                        if (iz > 0) dz -= iz * E[2](iz - 1, jz, v);
-                        dA[0][h] += w * dx * ey * ez;
-                        dA[1][h] += w * ex * dy * ez;
-                        dA[2][h] += w * ex * ey * dz;
-                    }
-                }
-            }
-            double bp = b / p;
-            for (int c = 0; c < natm; ++c) {
-                double X = (B.pos[3 * ai + 0] - B.pos[3 * c + 0]) - bp * (B.pos[3 * ai + 0] - B.pos[3 * aj + 0]);
-                double Y = (B.pos[3 * ai + 1] - B.pos[3 * c + 1]) - bp * (B.pos[3 * ai + 1] - B.pos[3 * aj + 1]);
-                double Zc = (B.pos[3 * ai + 2] - B.pos[3 * c + 2]) - bp * (B.pos[3 * ai + 2] - B.pos[3 * aj + 2]);
-                hermite_r_rt(L + 1, p, X, Y, Zc, tab, R);
-                double pref = -Z[c] * 2.0 * MD_PI / p;
-                for (int d = 0; d < 3; ++d) {
-                    double gA = 0.0, gC = 0.0;
-                    for (int h = 0; h < nh1; ++h) gA += dA[d][h] * R[h];
-                    for (int h = 0; h < nh; ++h) {
-                        int t = HT.t[h] + (d == 0), u = HT.u[h] + (d == 1), v = HT.v[h] + (d == 2);
-                        gC -= dE[h] * R[hidx(t, u, v)];
-                    }
-                    gA *= pref;
-                    gC *= pref;
-                    add(ai, d, gA);
-                    add(c, d, gC);
-                    add(aj, d, -(gA + gC));
-                }
-            }
+    with_ls(B.l[si], B.l[sj], [&](auto ii, auto jj) {
+        double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+        nuclear_grad_acc<decltype(ii)::value, decltype(jj)::value>(B, n, si, sj, 0, 1, natm, Z, D, tab, add, gA, gB);
+        for (int d = 0; d < 3; ++d) {
+            add(B.atom[si], d, gA[d]);
+            add(B.atom[sj], d, gB[d]);
         }
-    }
+    });
 }
 
 }  // namespace md

@@ -72,12 +72,13 @@ def main() -> int:
     torch.cuda.synchronize()
     eng.profile(True)
     eng.profile_read(reset=True)
-    iters, e_tot = [], []
+    iters, e_tot, tim = [], [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = md.step()
         iters.append(r.info["scf_iterations"])
+        tim.append(r.info["timings"])
         e_tot.append(md.total_energy())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -119,8 +120,10 @@ def main() -> int:
                         "SCF conv 1e-10 Eh / 1e-7, Schwarz 1e-12, gradient screening 1e-14, precision fp64",
             "geometry": "tests/golden/water16.xyz",
             "quartets": int(len(quartets)),
+            "gradient_quartets": int(eng.class_counts()[1].sum()),
         },
         "scf_iterations_mean": float(np.mean(iters)),
+        "phase_ms": {k: round(1e3 * float(np.mean([t[k] for t in tim])), 3) for k in tim[0]} if tim else {},
         "jk_mode": "store" if info["stored"] else "direct",
         "jk_builds_per_step": round(n_builds / max(args.steps, 1), 2),
         "jk_build_ms": round(float(jk_ms), 3),

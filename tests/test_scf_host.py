@@ -1,0 +1,69 @@
+"""
+Host-side SCF pieces on CPU tensors (no GPU): the DIIS solver against the
+linear error model and the blow-up rule of the reference's own tests
+(tests/test_qc_scf.py:763-796 of the reference), and option validation.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from aimd_amd.qc import scf as S
+
+
+def _sym(n, rng):
+    a = rng.standard_normal((n, n))
+    return torch.tensor(0.5 * (a + a.T))[None]
+
+
+def test_diis_is_exact_for_a_linear_error_model():
+    rng = np.random.default_rng(4)
+    F_star = _sym(5, rng)
+    G = _sym(5, rng)
+    d = S.DIIS(4)
+    d.push(F_star + G, G)                 # error proportional to F - F*
+    d.push(F_star - 0.5 * G, -0.5 * G)
+    np.testing.assert_allclose(d.extrapolate().numpy(), F_star.numpy(), atol=1e-12)
+    d.push(F_star - 0.5 * G, -0.5 * G)    # duplicate -> singular system, handled
+    out = d.extrapolate().numpy()
+    assert np.all(np.isfinite(out))
+    np.testing.assert_allclose(out, F_star.numpy(), atol=1e-10)
+    for _ in range(6):
+        d.push(F_star, torch.zeros_like(G))
+    assert len(d) == 4                    # bounded history
+    np.testing.assert_allclose(d.extrapolate().numpy(), F_star.numpy(), atol=1e-12)
+
+
+def test_diis_matches_least_squares_on_random_vectors():
+    # anchored Pulay DIIS = min |e_m + sum_i c_i (e_i - e_m)| (numpy lstsq), c_m = 1 - sum c_i
+    rng = np.random.default_rng(11)
+    d = S.DIIS(6)
+    errs = [_sym(7, rng) for _ in range(5)]
+    for e in errs:
+        d.push(_sym(7, rng), e)
+    c = d.coefficients()
+    E = np.stack([e.numpy().ravel() for e in errs])
+    A = (E[:-1] - E[-1]).T
+    x, *_ = np.linalg.lstsq(A, -E[-1], rcond=None)
+    np.testing.assert_allclose(c[:-1], x, rtol=1e-9, atol=1e-12)
+    assert abs(c.sum() - 1.0) < 1e-12
+
+
+def test_diis_drops_the_oldest_vectors_when_the_extrapolation_blows_up():
+    rng = np.random.default_rng(7)
+    u, w = _sym(4, rng), _sym(4, rng)
+    F_old, F_new = _sym(4, rng), _sym(4, rng)
+    d = S.DIIS(4)
+    d.push(F_old, u + 1e-7 * w)
+    d.push(F_new, u)
+    assert np.abs(d.coefficients()).max() > d.max_coefficient
+    np.testing.assert_array_equal(d.extrapolate().numpy(), F_new.numpy())
+    assert len(d) == 1
+
+
+@pytest.mark.parametrize("bad", [dict(max_iter=0), dict(conv_energy=0.0), dict(guess="minao"), dict(diis_space=1),
+                                 dict(damping=1.0), dict(level_shift=-0.1), dict(orthogonalization="lowdin"),
+                                 dict(blas_threads=0), dict(jk_mode="tensor"), dict(max_store_bytes=-1)])
+def test_option_validation(bad):
+    with pytest.raises(ValueError):
+        S.SCFOptions(**bad)
