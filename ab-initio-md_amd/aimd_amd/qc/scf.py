@@ -38,6 +38,7 @@ GWH_K = 1.75
 MIN_NUCLEAR_SEPARATION = 1e-5
 _GUESSES = ("sad", "core", "gwh")
 _ORTHO = ("auto", "symmetric", "canonical")
+_DENSITY_SOLVERS = ("purify", "eigh")
 _REFERENCES = {"rhf": "rhf", "restricted": "rhf", "uhf": "uhf", "unrestricted": "uhf"}
 
 
@@ -75,6 +76,7 @@ class SCFOptions:
     blas_threads: int | None = 1
     jk_mode: str = "auto"
     max_store_bytes: int = 0
+    density_solver: str = "purify"
 
     def __post_init__(self) -> None:
         if not (self.conv_energy > 0.0 and self.conv_commutator > 0.0):
@@ -103,6 +105,8 @@ class SCFOptions:
             raise ValueError(f"unknown jk_mode {self.jk_mode!r}; use one of {sorted(qengine.Engine.JK_MODES)}")
         if self.max_store_bytes < 0:
             raise ValueError("max_store_bytes must be >= 0")
+        if self.density_solver not in _DENSITY_SOLVERS:
+            raise ValueError(f"unknown density_solver {self.density_solver!r}; use one of {_DENSITY_SOLVERS}")
 
 
 def electron_counts(nuclear_charge: int, charge: int, multiplicity: int,
@@ -145,6 +149,43 @@ def check_nuclear_separation(positions: np.ndarray, atomic_numbers: np.ndarray) 
         raise ValueError(
             f"atoms {i[k]} and {j[k]} are {r[i[k], j[k]]:.3g} bohr apart (coincident "
             f"nuclei, < {MIN_NUCLEAR_SEPARATION:g} bohr): the nuclear repulsion is infinite")
+
+
+def purified_projector(F: torch.Tensor, nocc: int, tol: float = 1e-11, max_iter: int = 120,
+                       check_every: int = 8) -> torch.Tensor | None:
+    """
+    Projector onto the eigenvectors of the nocc lowest eigenvalues of symmetric F
+    by trace-correcting purification (Niklasson, Phys. Rev. B 66, 155115 (2002)):
+    X0 = (e_max - F) / (e_max - e_min) with Gershgorin bounds, then X <- X^2 while
+    tr X > nocc, else X <- 2X - X^2. Branches are chosen on the device; the
+    idempotency error tr(X - X^2) is read back every check_every steps. Once it
+    is below tol, two more steps square the remaining error away. Returns None if
+    it has not converged after max_iter steps (no gap at the Fermi level).
+    """
+    n = F.shape[0]
+    eye = torch.eye(n, dtype=F.dtype, device=F.device)
+    if nocc <= 0:
+        return torch.zeros_like(F)
+    if nocc >= n:
+        return eye
+    a = F.abs()
+    d = torch.diagonal(F)
+    r = a.sum(dim=1) - torch.diagonal(a)
+    emin, emax = (d - r).min(), (d + r).max()
+    X = (emax * eye - F) / (emax - emin)
+    it = 0
+    while it < max_iter:
+        for _ in range(check_every):
+            X2 = X @ X
+            X = torch.where(torch.trace(X) > nocc, X2, 2.0 * X - X2)
+        it += check_every
+        err = float((torch.trace(X) - torch.sum(X * X)).abs())
+        if err < tol:
+            for _ in range(2):
+                X2 = X @ X
+                X = torch.where(torch.trace(X) > nocc, X2, 2.0 * X - X2)
+            return 0.5 * (X + X.T)
+    return None
 
 
 def orthogonalizer(S: torch.Tensor, method: str = "auto",
@@ -686,6 +727,24 @@ class SCFSolver:
         F = I.H + g * J.sum(dim=0) - K
         return 0.5 * (F + F.transpose(1, 2))
 
+    def _next_density(self, Fo: torch.Tensor, X: torch.Tensor, nocc: tuple[int, ...],
+                      Ds: torch.Tensor) -> torch.Tensor:
+        """Aufbau densities of the orthogonal-basis Fock matrices Fo (nspin, nmo, nmo)."""
+        Dn = torch.empty_like(Ds)
+        if self.options.density_solver == "purify":
+            for s in range(Fo.shape[0]):
+                P = purified_projector(Fo[s], nocc[s])
+                if P is None:
+                    break
+                Dn[s] = X @ P @ X.T
+            else:
+                return Dn
+        _, Cp = torch.linalg.eigh(Fo)
+        for s in range(Fo.shape[0]):
+            C = X @ Cp[s][:, :nocc[s]]
+            Dn[s] = C @ C.T
+        return Dn
+
     def _iterate(self, I: SCFIntegrals, Ds: torch.Tensor) -> dict[str, Any]:
         opt = self.options
         nspin = Ds.shape[0]
@@ -732,11 +791,7 @@ class SCFSolver:
             if stabilize and opt.level_shift > 0.0:
                 Do = XtS @ Ds @ XtS.T
                 Fo = Fo + opt.level_shift * (eye - Do)
-            _, Cp = torch.linalg.eigh(Fo)
-            Dn = torch.empty_like(Ds)
-            for s in range(nspin):
-                C = X @ Cp[s][:, :nocc[s]]
-                Dn[s] = C @ C.T
+            Dn = self._next_density(Fo, X, nocc, Ds)
             if stabilize and opt.damping > 0.0:
                 Dn = (1.0 - opt.damping) * Dn + opt.damping * Ds
             Ds = Dn

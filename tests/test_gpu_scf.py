@@ -77,3 +77,15 @@ def test_water8_cluster_against_oracle(oracle_lib):
     st, g = oracle_lib.energy_and_gradient(s, x, "6-31g*", conv_energy=1e-11, conv_commutator=1e-9, stored=False)
     assert abs(r.energy - st.energy) < 1e-9
     assert np.abs(r.gradient - g).max() < 1e-7
+
+
+@pytest.mark.parametrize("mol,basis,mult", [(WATER, "6-31g*", 1), (METHYL, "6-31g*", 2)])
+def test_density_solvers_agree(mol, basis, mult):
+    # purification and eigenvectors give the same aufbau projector, so the same SCF state
+    s, x = bohr(mol, 0.05, 11)
+    print(f"case {s} {basis} {mult}", flush=True)
+    r1 = _hf(s, basis, multiplicity=mult, scf_options={"density_solver": "purify"}).compute(x)
+    r2 = _hf(s, basis, multiplicity=mult, scf_options={"density_solver": "eigh"}).compute(x)
+    assert r1.converged and r2.converged
+    assert abs(r1.energy - r2.energy) < 1e-10
+    assert np.abs(r1.gradient - r2.gradient).max() < 1e-8

@@ -61,9 +61,31 @@ def test_diis_drops_the_oldest_vectors_when_the_extrapolation_blows_up():
     assert len(d) == 1
 
 
+@pytest.mark.parametrize("n,nocc,gap", [(30, 7, 0.3), (60, 25, 0.05), (40, 0, 1.0), (40, 40, 1.0)])
+def test_purified_projector_matches_the_eigenvector_projector(n, nocc, gap):
+    rng = np.random.default_rng(n + nocc)
+    w = np.sort(rng.uniform(-20.0, 5.0, n))
+    w[nocc:] += gap + (w[nocc - 1] - w[nocc] if 0 < nocc < n else 0.0)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    F = torch.tensor((Q * w) @ Q.T)
+    P = S.purified_projector(F, nocc)
+    assert P is not None
+    ref = Q[:, :nocc] @ Q[:, :nocc].T
+    np.testing.assert_allclose(P.numpy(), ref, atol=1e-11)
+    assert abs(float(torch.trace(P)) - nocc) < 1e-10
+
+
+def test_purified_projector_reports_a_missing_gap():
+    # degenerate HOMO/LUMO: no projector exists, purification must not claim one
+    w = np.array([-3.0, -1.0, -1.0, 2.0])
+    F = torch.tensor(np.diag(w))
+    assert S.purified_projector(F, 2, max_iter=40) is None
+
+
 @pytest.mark.parametrize("bad", [dict(max_iter=0), dict(conv_energy=0.0), dict(guess="minao"), dict(diis_space=1),
                                  dict(damping=1.0), dict(level_shift=-0.1), dict(orthogonalization="lowdin"),
-                                 dict(blas_threads=0), dict(jk_mode="tensor"), dict(max_store_bytes=-1)])
+                                 dict(blas_threads=0), dict(jk_mode="tensor"), dict(max_store_bytes=-1),
+                                 dict(density_solver="lanczos")])
 def test_option_validation(bad):
     with pytest.raises(ValueError):
         S.SCFOptions(**bad)
