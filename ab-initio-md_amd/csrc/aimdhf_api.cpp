@@ -247,29 +247,25 @@ struct aimdhf_plan {
     // counts[k] receives the per-class sizes. Synchronises the stream.
     void build_list(double thr, DevBuf& buf, std::vector<long long>& counts, hipStream_t st) {
         aimdhf::PairArgs g = pair_args();
-        aimdhf::launch_count_quartets(g, thr, d_counts.as<int>(), st);
+        const int nv = NQCLASS * aimdhf::NCOSTBIN;
+        d_counts.ensure(sizeof(int) * (size_t)npair * nv);
+        d_offsets.ensure(sizeof(long long) * ((size_t)npair * nv + 2 * nv));
+        long long* offs = d_offsets.as<long long>();
+        long long* totals = offs + (size_t)npair * nv;
+        long long* base = totals + nv;
+        aimdhf::launch_count_quartets(g, thr, d_counts.as<int>(), offs, totals, base, st);
         HIPCHECK(hipGetLastError());
-        std::vector<int> cnt((size_t)npair * NQCLASS);
-        HIPCHECK(hipMemcpyAsync(cnt.data(), d_counts.p, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, st));
+        std::vector<long long> tot(nv);
+        HIPCHECK(hipMemcpyAsync(tot.data(), totals, sizeof(long long) * nv, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        std::vector<long long> tot(NQCLASS, 0);
-        for (int P = 0; P < npair; ++P)
-            for (int k = 0; k < NQCLASS; ++k) tot[k] += cnt[(size_t)P * NQCLASS + k];
-        std::vector<long long> run(NQCLASS, 0);
-        for (int k = 1; k < NQCLASS; ++k) run[k] = run[k - 1] + tot[k - 1];
-        long long total = run[NQCLASS - 1] + tot[NQCLASS - 1];
-        std::vector<long long> offs((size_t)npair * NQCLASS);
-        for (int P = 0; P < npair; ++P)
-            for (int k = 0; k < NQCLASS; ++k) {
-                offs[(size_t)P * NQCLASS + k] = run[k];
-                run[k] += cnt[(size_t)P * NQCLASS + k];
-            }
-        counts = tot;
+        counts.assign(NQCLASS, 0);
+        long long total = 0;
+        for (int k = 0; k < NQCLASS; ++k)
+            for (int b = 0; b < aimdhf::NCOSTBIN; ++b) counts[k] += tot[k * aimdhf::NCOSTBIN + b];
+        for (long long c : counts) total += c;
         buf.ensure(sizeof(int2) * std::max<long long>(total, 1));
-        HIPCHECK(hipMemcpyAsync(d_offsets.p, offs.data(), sizeof(long long) * offs.size(), hipMemcpyHostToDevice, st));
-        aimdhf::launch_fill_quartets(g, thr, d_offsets.as<long long>(), buf.as<int2>(), st);
+        aimdhf::launch_fill_quartets(g, thr, offs, base, buf.as<int2>(), st);
         HIPCHECK(hipGetLastError());
-        HIPCHECK(hipStreamSynchronize(st));
     }
 
     // Sorted ket streams (J-engine) and per-shell partner lists (exchange
@@ -767,8 +763,6 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
         p->d_qshell.ensure(sizeof(double) * p->npair);
         p->d_qscr.ensure(sizeof(double) * p->npair);
         p->d_qblock.ensure(sizeof(double) * p->nbpair);
-        p->d_counts.ensure(sizeof(int) * (size_t)p->npair * NQCLASS);
-        p->d_offsets.ensure(sizeof(long long) * (size_t)p->npair * NQCLASS);
         p->qcount.assign(NQCLASS, 0);
     } catch (const HipError& e) {
         delete p;

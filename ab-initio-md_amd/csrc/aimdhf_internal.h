@@ -261,8 +261,12 @@ void launch_zero_d(double* p, size_t n, hipStream_t st);
 
 void launch_build_pairs(const PairArgs& g, hipStream_t st);
 void launch_schwarz(const PairArgs& g, const int* ncls_host, hipStream_t st);
-void launch_count_quartets(const PairArgs& g, double thr, int* counts, hipStream_t st);
-void launch_fill_quartets(const PairArgs& g, double thr, const long long* offsets, int2* list, hipStream_t st);
+// quartet lists: per class, NCOSTBIN bins of primitive-quartet cost (heaviest first)
+constexpr int NCOSTBIN = 5;
+void launch_count_quartets(const PairArgs& g, double thr, int* counts, long long* offs, long long* totals,
+                           long long* base, hipStream_t st);
+void launch_fill_quartets(const PairArgs& g, double thr, const long long* offs, const long long* base, int2* list,
+                          hipStream_t st);
 void launch_zero64(long long* p, size_t n, hipStream_t st);
 void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st);
 void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);

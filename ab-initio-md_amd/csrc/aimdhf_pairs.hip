@@ -103,58 +103,118 @@ __device__ __forceinline__ int qclass(int cp, int cq) {
     return hi * (hi + 1) / 2 + lo;
 }
 
-// One wavefront per pair P, lanes over Q <= P in chunks of 64: per-class
-// counts (count) and the class-binned list in ascending Q within (P, class)
-// (fill, ordered compaction by ballot).
+// Within a class, quartets are further binned by cost npp_P * npp_Q (kept
+// primitive pairs), heaviest bin first, so the lanes of a wavefront run
+// loops of similar length; within a bin the order is P-major, Q ascending.
+__device__ __forceinline__ int cost_bin(int c) {
+    return c > 256 ? 0 : c > 64 ? 1 : c > 16 ? 2 : c > 4 ? 3 : 4;
+}
+
+// One wavefront per pair P, lanes over Q <= P in chunks of 64: counts per
+// (class, bin) into counts[vc * npair + P], vc = class * NCOSTBIN + bin.
 __global__ __launch_bounds__(64) void k_count_quartets(PairArgs g, double thr, int* counts) {
     const int P = blockIdx.x;
     const int lane = threadIdx.x;
     const double qp = g.qscr[P];
     const int cp = g.pair_cls[P];
-    int c[6] = {0, 0, 0, 0, 0, 0};
+    const int np = g.npp[P];
+    int c[6 * NCOSTBIN];
+#pragma unroll
+    for (int i = 0; i < 6 * NCOSTBIN; ++i) c[i] = 0;
     for (int Q = lane; Q <= P; Q += 64) {
         if (qp * g.qscr[Q] < thr) continue;
-        const int cq = g.pair_cls[Q];
+        const int cq = g.pair_cls[Q], b = cost_bin(np * g.npp[Q]);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) c[k] += (cq == k);
+        for (int i = 0; i < 6 * NCOSTBIN; ++i) c[i] += (cq * NCOSTBIN + b == i);
     }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        int v = c[k];
+    for (int i = 0; i < 6 * NCOSTBIN; ++i) {
+        int v = c[i];
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-        c[k] = v;
+        c[i] = v;
     }
-    if (lane < NQCLASS) {
+    for (int vc = lane; vc < NQCLASS * NCOSTBIN; vc += 64) {
+        const int k = vc / NCOSTBIN, b = vc % NCOSTBIN;
         int v = 0;
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if (qclass(cp, k) == lane) v += c[k];
-        counts[(size_t)P * NQCLASS + lane] = v;
+        for (int cq = 0; cq < 6; ++cq)
+            if (qclass(cp, cq) == k) v += c[cq * NCOSTBIN + b];
+        counts[(size_t)vc * g.npair + P] = v;
     }
 }
 
-__global__ __launch_bounds__(64) void k_fill_quartets(PairArgs g, double thr, const long long* offsets, int2* list) {
+// Exclusive scan of each (class, bin) column over P: one 1024-thread block per
+// column; totals[vc] receives the column sum.
+__global__ __launch_bounds__(1024) void k_scan_columns(const int* counts, int npair, long long* offs,
+                                                       long long* totals) {
+    __shared__ long long wsum[16];
+    const int vc = blockIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int* col = counts + (size_t)vc * npair;
+    long long* out = offs + (size_t)vc * npair;
+    long long run = 0;
+    for (int base = 0; base < npair; base += 1024) {
+        const int i = base + t;
+        const long long x = i < npair ? col[i] : 0;
+        long long v = x;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const long long y = __shfl_up(v, off, 64);
+            if (lane >= off) v += y;
+        }
+        if (lane == 63) wsum[w] = v;
+        __syncthreads();
+        long long pre = 0;
+        for (int j = 0; j < w; ++j) pre += wsum[j];
+        long long tot = 0;
+        for (int j = 0; j < 16; ++j) tot += wsum[j];
+        if (i < npair) out[i] = run + pre + v - x;
+        run += tot;
+        __syncthreads();
+    }
+    if (t == 0) totals[vc] = run;
+}
+
+// base[vc] = exclusive scan of the column totals (class-major, bin-minor)
+__global__ void k_scan_totals(const long long* totals, int n, long long* base) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        long long r = 0;
+        for (int i = 0; i < n; ++i) {
+            base[i] = r;
+            r += totals[i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_fill_quartets(PairArgs g, double thr, const long long* offs,
+                                                      const long long* base, int2* list) {
     const int P = blockIdx.x;
     const int lane = threadIdx.x;
     const double qp = g.qscr[P];
     const int cp = g.pair_cls[P];
-    long long o[6];
+    const int np = g.npp[P];
+    long long o[6 * NCOSTBIN];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) o[k] = offsets[(size_t)P * NQCLASS + qclass(cp, k)];
+    for (int cq = 0; cq < 6; ++cq)
+#pragma unroll
+        for (int b = 0; b < NCOSTBIN; ++b) {
+            const int vc = qclass(cp, cq) * NCOSTBIN + b;
+            o[cq * NCOSTBIN + b] = base[vc] + offs[(size_t)vc * g.npair + P];
+        }
     const unsigned long long below = (lane ? (~0ull >> (64 - lane)) : 0ull);
     for (int Q0 = 0; Q0 <= P; Q0 += 64) {
         const int Q = Q0 + lane;
         const bool in = Q <= P && !(qp * g.qscr[Q] < thr);
-        const int cq = in ? g.pair_cls[Q] : -1;
+        const int key = in ? g.pair_cls[Q] * NCOSTBIN + cost_bin(np * g.npp[Q]) : -1;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const unsigned long long m = __ballot(cq == k);
-            if (cq == k) {
-                const long long pos = o[k] + __popcll(m & below);
-                list[pos] = cp >= cq ? make_int2(P, Q) : make_int2(Q, P);
+        for (int i = 0; i < 6 * NCOSTBIN; ++i) {
+            const unsigned long long m = __ballot(key == i);
+            if (key == i) {
+                const int cq = i / NCOSTBIN;
+                list[o[i] + __popcll(m & below)] = cp >= cq ? make_int2(P, Q) : make_int2(Q, P);
             }
-            o[k] += __popcll(m);
+            o[i] += __popcll(m);
         }
     }
 }
@@ -229,12 +289,17 @@ void launch_schwarz(const PairArgs& g, const int* ncls_host, hipStream_t st) {
     if (g.nbpair) k_block_q<<<GRID(g.nbpair, 64), 0, st>>>(g);
 }
 
-void launch_count_quartets(const PairArgs& g, double thr, int* counts, hipStream_t st) {
-    if (g.npair) k_count_quartets<<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, counts);
+void launch_count_quartets(const PairArgs& g, double thr, int* counts, long long* offs, long long* totals,
+                           long long* base, hipStream_t st) {
+    if (!g.npair) return;
+    k_count_quartets<<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, counts);
+    k_scan_columns<<<dim3(NQCLASS * NCOSTBIN), dim3(1024), 0, st>>>(counts, g.npair, offs, totals);
+    k_scan_totals<<<dim3(1), dim3(1), 0, st>>>(totals, NQCLASS * NCOSTBIN, base);
 }
 
-void launch_fill_quartets(const PairArgs& g, double thr, const long long* offsets, int2* list, hipStream_t st) {
-    if (g.npair) k_fill_quartets<<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, offsets, list);
+void launch_fill_quartets(const PairArgs& g, double thr, const long long* offs, const long long* base, int2* list,
+                          hipStream_t st) {
+    if (g.npair) k_fill_quartets<<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, offs, base, list);
 }
 
 void launch_zero64(long long* p, size_t n, hipStream_t st) {
