@@ -34,7 +34,7 @@ SIGNATURES = {
     "aimdhf_quartets": (_I, [_P, _P, _L, ctypes.POINTER(_L)]),
     "aimdhf_jk": (_I, [_P, _I, _P, _P, _P, _P]),
     "aimdhf_set_jk_mode": (_I, [_P, _I, _L]),
-    "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P]),
+    "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_grad_one": (_I, [_P, _P, _P, _P, _P]),
     "aimdhf_grad_two": (_I, [_P, _P, _I, _P, _D, _D, _P, _P]),
@@ -42,6 +42,7 @@ SIGNATURES = {
     "aimdhf_profile_read": (_I, [_P, _P, _P, _I, _I]),
     "aimdhf_class_counts": (_I, [_P, _P, _P]),
     "aimdhf_pair_npp": (_I, [_P, _P, _L]),
+    "aimdhf_grad_quartets": (_I, [_P, _P, _L, ctypes.POINTER(_L)]),
 }
 
 _lib = None

@@ -108,13 +108,14 @@ class Engine:
         self.jk_mode = mode
 
     def jk_info(self) -> dict:
-        """Store state of the current geometry: stored, bytes, K and J gather-list entries."""
+        """Store state of the current geometry: stored, bytes, K and J gather-list entries and
+        the integrals one J/K build streams (gather_doubles)."""
         st = ctypes.c_int()
-        b, ke, je = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        b, ke, je, gd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         _lib.check(_lib.lib().aimdhf_jk_info(self._h, ctypes.byref(st), ctypes.byref(b), ctypes.byref(ke),
-                                             ctypes.byref(je)))
+                                             ctypes.byref(je), ctypes.byref(gd)))
         return {"stored": bool(st.value), "store_bytes": int(b.value), "k_entries": int(ke.value),
-                "j_entries": int(je.value)}
+                "j_entries": int(je.value), "gather_doubles": int(gd.value)}
 
     def jk(self, dens: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
         """J, K of (nset, nao, nao) (or (nao, nao)) densities; symmetric part of each D is used."""
@@ -157,25 +158,27 @@ class Engine:
     PAIR_CLASS_NAMES = ["ss", "ps", "pp", "ds", "dp", "dd"]
     KIND_NAMES = ([f"jengine_bra_{c}" for c in PAIR_CLASS_NAMES]
                   + [f"exchange_{'spd'[t // 3]}{'spd'[t % 3]}" for t in range(9)]
-                  + [f"unused_{k}" for k in range(15, 21)]
+                  + ["gather_copies"] + [f"unused_{k}" for k in range(16, 21)]
                   + [f"grad_class_{k}" for k in range(21)]
                   + ["geometry", "one_electron", "grad_one", "jk_prologue_epilogue", "jk_hermite_density",
                      "eri_store", "gather_lists"]
                   + [f"jstore_{c}" for c in PAIR_CLASS_NAMES]
-                  + [f"kstore_{'spd'[t // 3]}{'spd'[t % 3]}" for t in range(9)])
-    NKIND = 64
+                  + [f"kstore_{'spd'[t // 3]}{'spd'[t % 3]}" for t in range(9)]
+                  + [f"grad_screen_{k}" for k in range(21)] + [f"unused_{k}" for k in range(85, 96)])
+    NKIND = 96
     FOCK_KINDS = list(range(0, 15)) + [45, 46] + list(range(49, 64))    # per-iteration J/K work, either mode
-    STORE_KINDS = [47, 48]                                              # per-geometry store build
+    STORE_KINDS = [15, 47, 48]                                          # per-geometry store build
+    GRAD_KINDS = list(range(21, 42)) + list(range(64, 85))              # derivative ERIs
 
     def profile(self, on: bool = True) -> None:
         """Record HIP events around every engine launch (on torch's current stream)."""
         _lib.check(_lib.lib().aimdhf_profile_enable(self._h, int(bool(on))))
 
     def profile_read(self, reset: bool = False) -> tuple[np.ndarray, np.ndarray]:
-        """(milliseconds, launch counts) per kind, 64 kinds (see KIND_NAMES)."""
-        ms = np.zeros(64)
-        n = np.zeros(64, np.int64)
-        _lib.check(_lib.lib().aimdhf_profile_read(self._h, _ptr(ms), _ptr(n), 64, int(bool(reset))))
+        """(milliseconds, launch counts) per kind, NKIND kinds (see KIND_NAMES)."""
+        ms = np.zeros(self.NKIND)
+        n = np.zeros(self.NKIND, np.int64)
+        _lib.check(_lib.lib().aimdhf_profile_read(self._h, _ptr(ms), _ptr(n), self.NKIND, int(bool(reset))))
         return ms, n
 
     def class_counts(self) -> tuple[np.ndarray, np.ndarray]:
@@ -184,6 +187,14 @@ class Engine:
         g = np.zeros(21, np.int64)
         _lib.check(_lib.lib().aimdhf_class_counts(self._h, _ptr(e), _ptr(g)))
         return e, g
+
+    def grad_quartets(self) -> np.ndarray:
+        """Quartets differentiated by the last grad_two (after its Gamma screening), (n, 2), P >= Q."""
+        cnt = ctypes.c_int64()
+        _lib.check(_lib.lib().aimdhf_grad_quartets(self._h, None, 0, ctypes.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 2), np.int64)
+        _lib.check(_lib.lib().aimdhf_grad_quartets(self._h, _ptr(out), cnt.value, ctypes.byref(cnt)))
+        return out[:cnt.value]
 
     def pair_npp(self) -> np.ndarray:
         npair = self.basis.nshell * (self.basis.nshell + 1) // 2

@@ -13,8 +13,10 @@ box(ab) = prod_d (a_d + b_d + 1):
     per quartet            F   = m_P m_K F_q + m_P F_b
     digestion per contracted integral: 4 (J) + 8 (K) flops per density set.
 
-The gradient counts each pass (bra centres, ket centres) with L+1,
-NH(l_a+l_b+1) and 6 derivative FMAs per element of the extended Hermite box.
+The gradient (grad_work) counts each pass (bra centres, ket centres) with
+L+1, NH(l_a+l_b+1) and 6 derivative FMAs per element of the extended Hermite
+box; a pass is counted when the derivative kernel runs it (a pair on one atom
+is covered by translational invariance).
 Byte model: 8 bytes per primitive-pair record double read once per quartet.
 """
 
@@ -101,3 +103,43 @@ def jk_work(basis: BasisSet, quartets: np.ndarray, npp: np.ndarray, nset: int = 
                                + out["quartets"][c] * nint * (4 + 8) * nset)
     out["total"] = float(out["per_class"].sum())
     return out
+
+
+def grad_work(basis: BasisSet, quartets: np.ndarray, npp: np.ndarray) -> dict:
+    """Per-class and total algorithmic flops of one derivative-ERI contraction over
+    ``quartets`` ((n, 2) pair indices, the quartets that pass the Gamma screening)."""
+    pc = pair_classes(basis)
+    l = basis.shell_l
+    s1, s2 = np.tril_indices(basis.nshell)
+    o = np.lexsort((s2, s1))
+    s1, s2 = s1[o], s2[o]
+    atom = basis.shell_atom
+    one_centre = atom[s1] == atom[s2]                 # per pair: both shells on one atom
+    P, Q = quartets[:, 0], quartets[:, 1]
+    cp, cq = pc[P], pc[Q]
+    bra = np.where(cp >= cq, P, Q)
+    ket = np.where(cp >= cq, Q, P)
+    hi, lo = np.maximum(cp, cq), np.minimum(cp, cq)
+    k = hi * (hi + 1) // 2 + lo
+    mP = npp[bra].astype(np.float64)
+    mK = npp[ket].astype(np.float64)
+    need_b = ~one_centre[bra]
+    need_k = ~one_centre[ket]
+    out = {"per_class": np.zeros(21), "quartets": np.bincount(k, minlength=21)}
+    for c in range(21):
+        sel = k == c
+        if not sel.any():
+            continue
+        la, lb, lc, ld = class_ls(c)
+        nab, ncd = NC(la) * NC(lb), NC(lc) * NC(ld)
+        nb, nk = need_b[sel], need_k[sel]
+        run_b = nb | (~nk & (nab >= ncd))
+        run_k = nk | (~nb & (nab < ncd))
+        mp, mk = mP[sel], mK[sel]
+        f = 0.0
+        f += np.sum(run_b * (mp * mk * per_prim_quartet(la, lb, lc, ld, 1) + mp * per_bra_prim(la, lb, lc, ld, 1)))
+        f += np.sum(run_k * (mk * mp * per_prim_quartet(lc, ld, la, lb, 1) + mk * per_bra_prim(lc, ld, la, lb, 1)))
+        out["per_class"][c] = f
+    out["total"] = float(out["per_class"].sum())
+    return out
+

@@ -90,7 +90,7 @@ std::vector<double> boys_table() {
 // Optional per-launch timing with HIP events on the caller's stream.
 // Kinds as listed at aimdhf_profile_enable in aimdhf.h (0..63).
 struct Profiler {
-    static constexpr int NKIND = 64;
+    static constexpr int NKIND = 96;
     bool on = false;
     struct Rec {
         int kind;
@@ -574,7 +574,7 @@ struct aimdhf_plan {
         upload(d_jdoff, jdoff);
         d_kdata.ensure(sizeof(double) * std::max<long long>(ko, 1));
         d_jdata.ensure(sizeof(double) * std::max<long long>(jo, 1));
-        prof.time(47, st, [&] {
+        prof.time(15, st, [&] {
             aimdhf::KSArgs ka = ks_args(1, nullptr, nullptr);
             for (int t = 0; t < 9; ++t) {
                 const int no = out_off[t + 1] - out_off[t];
@@ -861,6 +861,34 @@ int aimdhf_quartets(aimdhf_plan* p, int64_t* out, int64_t cap, int64_t* count) {
     return AIMDHF_OK;
 }
 
+int aimdhf_grad_quartets(aimdhf_plan* p, int64_t* out, int64_t cap, int64_t* count) {
+    if (!p || !count) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    try {
+        long long tot = 0;
+        for (long long c : p->gcount) tot += c;
+        std::vector<int2> lst(tot);
+        std::vector<unsigned char> keep(tot);
+        if (tot) {
+            HIPCHECK(hipMemcpy(lst.data(), p->d_glist.p, sizeof(int2) * tot, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(keep.data(), p->d_gkeep.p, tot, hipMemcpyDeviceToHost));
+        }
+        long long n = 0;
+        for (long long i = 0; i < tot; ++i) {
+            if (!keep[i]) continue;
+            if (out && n < cap) {
+                out[2 * n] = std::max(lst[i].x, lst[i].y);
+                out[2 * n + 1] = std::min(lst[i].x, lst[i].y);
+            }
+            ++n;
+        }
+        *count = n;
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
 int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K, void* stream) {
     if (!p || !dens || !J || !K) return fail(AIMDHF_E_ARG, "NULL argument");
     if (nset < 1 || nset > 2) return fail(AIMDHF_E_ARG, "nset must be 1 or 2");
@@ -1066,8 +1094,10 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
         for (int k = 0; k < NQCLASS; ++k) {
             const int2* lst = p->d_glist.as<int2>() + off;
             a.keep = p->d_gkeep.as<unsigned char>() + off;
-            if (p->gcount[k] > 0)
+            if (p->gcount[k] > 0) {
+                p->prof.time(64 + k, st, [&] { aimdhf::launch_grad_keep(k, a, lst, p->gcount[k], st); });
                 p->prof.time(21 + k, st, [&] { aimdhf::launch_grad_one_class(k, a, lst, p->gcount[k], st); });
+            }
             off += p->gcount[k];
         }
         aimdhf::launch_fx_to_double(p->d_gfx.as<long long>(), 3 * p->natm, grad, st);
@@ -1087,10 +1117,12 @@ int aimdhf_set_jk_mode(aimdhf_plan* p, int mode, int64_t max_store_bytes) {
     return AIMDHF_OK;
 }
 
-int aimdhf_jk_info(aimdhf_plan* p, int* stored, int64_t* store_bytes, int64_t* k_entries, int64_t* j_entries) {
+int aimdhf_jk_info(aimdhf_plan* p, int* stored, int64_t* store_bytes, int64_t* k_entries, int64_t* j_entries,
+                   int64_t* gather_doubles) {
     if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     if (stored) *stored = p->stored ? 1 : 0;
+    if (gather_doubles) *gather_doubles = p->stored ? p->kdata_total + p->jdata_total : 0;
     if (store_bytes) *store_bytes = p->store_bytes;
     if (k_entries) *k_entries = p->k_entries;
     if (j_entries) *j_entries = p->j_entries;

@@ -270,6 +270,7 @@ void launch_fill_quartets(const PairArgs& g, double thr, const long long* offs, 
 void launch_zero64(long long* p, size_t n, hipStream_t st);
 void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st);
 void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
+void launch_grad_keep(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
 void launch_oneel(const OneElArgs& g, double* S, double* T, double* M, double* V, hipStream_t st);
 void launch_grad_one(const OneElArgs& g, const double* D, const double* Wneg, long long* out, hipStream_t st);
 void launch_negate(const double* W, size_t n, double* out, hipStream_t st);

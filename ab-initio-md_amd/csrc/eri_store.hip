@@ -29,7 +29,9 @@ constexpr int BLK = Qt::NAB * Qt::NCD;
 constexpr int NSPLIT = SPLIT ? Qt::NCD : 1;
 
 template <int LA_, int LB_, int LC_, int LD_>
-__global__ __launch_bounds__(64) void k_eri_store(StoreArgs a, const int2* __restrict__ list, long long nq) {
+__global__ __launch_bounds__(256) void k_eri_store(StoreArgs a, const int2* __restrict__ list, long long nq) {
+    __shared__ double btab[BOYS_NGRID * BOYS_SLICE];
+    boys_stage(a.boys, LA_ + LB_ + LC_ + LD_, btab);
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nq * NSPLIT) return;
     const long long q = t / NSPLIT;
@@ -40,12 +42,12 @@ __global__ __launch_bounds__(64) void k_eri_store(StoreArgs a, const int2* __res
     if constexpr (SPLIT) {
         const int cd = (int)(t % NSPLIT);
         double acc[Qt::NAB];
-        eri_quartet_cd<LA_, LB_, LC_, LD_>(rb, a.npp[pq.x], rk, a.npp[pq.y], a.boys, cd, acc);
+        eri_quartet_cd<LA_, LB_, LC_, LD_, true>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, cd, acc);
 #pragma unroll
         for (int ab = 0; ab < Qt::NAB; ++ab) out[ab * Qt::NCD + cd] = acc[ab];
     } else {
         double acc[BLK];
-        eri_quartet<LA_, LB_, LC_, LD_>(rb, a.npp[pq.x], rk, a.npp[pq.y], a.boys, acc);
+        eri_quartet<LA_, LB_, LC_, LD_, true>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, acc);
 #pragma unroll
         for (int i = 0; i < BLK; ++i) out[i] = acc[i];
     }
@@ -62,7 +64,7 @@ __global__ __launch_bounds__(64) void k_eri_store(StoreArgs a, const int2* __res
 void CAT(launch_eri_store_, AIMDHF_QCLASS)(const StoreArgs& a, const int2* list, long long nq, hipStream_t st) {
     const long long nt = nq * NSPLIT;
     if (nt > 0)
-        k_eri_store<LA, LB, LC, LD><<<dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, st>>>(a, list, nq);
+        k_eri_store<LA, LB, LC, LD><<<dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st>>>(a, list, nq);
 }
 
 }  // namespace aimdhf

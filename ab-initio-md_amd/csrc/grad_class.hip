@@ -76,10 +76,11 @@ __global__ __launch_bounds__(256) void k_grad_keep(GradArgs a, const int2* __res
 }
 
 template <int Q>
-__global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __restrict__ list, long long nq) {
+__global__ __launch_bounds__(256) void k_grad_class(GradArgs a, const int2* __restrict__ list, long long nq) {
     extern __shared__ long long lds[];
+    __shared__ double btab[BOYS_NGRID * BOYS_SLICE];
     for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
-    __syncthreads();
+    boys_stage(a.boys, LA + LB + LC + LD + 1, btab);
     const long long nitems = nq * NT;
     for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
          it += (long long)gridDim.x * blockDim.x) {
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __re
                 for (int c = 0; c < NCB; ++c)
                     gam[ab * NCB + c] =
                         gamma_el(a, oA + ab / NB, oB + ab % NB, oC + (kc0 + c) / ND, oD + (kc0 + c) % ND, deg);
-            grad_pass<LA, LB, LC, LD, NCB>(rb, a.npp[pq.x], rk, a.npp[pq.y], a.boys, a.pos + 3 * aA,
+            grad_pass<LA, LB, LC, LD, NCB, true>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, a.pos + 3 * aA,
                                            a.pos + 3 * aB, kc0, gam, g1, g2);
             for (int d = 0; d < 3; ++d) {
                 if (need_bra) {
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __re
                 for (int c = 0; c < NCK; ++c)
                     gam[cd * NCK + c] =
                         gamma_el(a, oA + (ab0 + c) / NB, oB + (ab0 + c) % NB, oC + cd / ND, oD + cd % ND, deg);
-            grad_pass<LC, LD, LA, LB, NCK>(rk, a.npp[pq.y], rb, a.npp[pq.x], a.boys, a.pos + 3 * aC,
+            grad_pass<LC, LD, LA, LB, NCK, true>(rk, a.npp[pq.y], rb, a.npp[pq.x], btab, a.pos + 3 * aC,
                                            a.pos + 3 * aD, ab0, gam, g1, g2);
             for (int d = 0; d < 3; ++d) {
                 if (need_ket) {
@@ -153,13 +154,16 @@ __global__ __launch_bounds__(128) void k_grad_class(GradArgs a, const int2* __re
 #define CAT2(a, b) a##b
 #define CAT(a, b) CAT2(a, b)
 
+void CAT(launch_grad_keep_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list, long long nq, hipStream_t st) {
+    if (nq > 0) k_grad_keep<QC><<<dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st>>>(a, list, nq);
+}
+
 void CAT(launch_grad_class_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list, long long nq, hipStream_t st) {
     long long items = nq * NT;
     if (items <= 0) return;
-    k_grad_keep<QC><<<dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st>>>(a, list, nq);
-    long long blocks = (items + 127) / 128;
-    if (blocks > 2048) blocks = 2048;
-    k_grad_class<QC><<<dim3((unsigned)blocks), dim3(128), sizeof(long long) * 6 * a.natm, st>>>(a, list, nq);
+    long long blocks = (items + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    k_grad_class<QC><<<dim3((unsigned)blocks), dim3(256), sizeof(long long) * 6 * a.natm, st>>>(a, list, nq);
 }
 
 }  // namespace aimdhf

@@ -4,7 +4,8 @@
 
 namespace aimdhf {
 
-#define DECLG(k) void launch_grad_class_##k(const GradArgs&, const int2*, long long, hipStream_t);
+#define DECLG(k) void launch_grad_class_##k(const GradArgs&, const int2*, long long, hipStream_t); \
+    void launch_grad_keep_##k(const GradArgs&, const int2*, long long, hipStream_t);
 DECLG(0) DECLG(1) DECLG(2) DECLG(3) DECLG(4) DECLG(5) DECLG(6) DECLG(7) DECLG(8) DECLG(9) DECLG(10)
 DECLG(11) DECLG(12) DECLG(13) DECLG(14) DECLG(15) DECLG(16) DECLG(17) DECLG(18) DECLG(19) DECLG(20)
 #undef DECLG
@@ -18,6 +19,18 @@ void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long
         launch_grad_class_12, launch_grad_class_13, launch_grad_class_14, launch_grad_class_15,
         launch_grad_class_16, launch_grad_class_17, launch_grad_class_18, launch_grad_class_19,
         launch_grad_class_20};
+    fns[k](a, list, nq, st);
+}
+
+void launch_grad_keep(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st) {
+    typedef void (*Fn)(const GradArgs&, const int2*, long long, hipStream_t);
+    static const Fn fns[21] = {
+        launch_grad_keep_0, launch_grad_keep_1, launch_grad_keep_2, launch_grad_keep_3,
+        launch_grad_keep_4, launch_grad_keep_5, launch_grad_keep_6, launch_grad_keep_7,
+        launch_grad_keep_8, launch_grad_keep_9, launch_grad_keep_10, launch_grad_keep_11,
+        launch_grad_keep_12, launch_grad_keep_13, launch_grad_keep_14, launch_grad_keep_15,
+        launch_grad_keep_16, launch_grad_keep_17, launch_grad_keep_18, launch_grad_keep_19,
+        launch_grad_keep_20};
     fns[k](a, list, nq, st);
 }
 

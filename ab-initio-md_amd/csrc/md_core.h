@@ -62,12 +62,18 @@ constexpr double BOYS_XSWITCH = 40.0;
 constexpr double MD_PI = 3.141592653589793;
 constexpr double TWO_PI_2_5 = 34.986836655249725;  // 2 pi^(5/2)
 
+// Boys table layouts: the full table (rows of BOYS_NCOL orders from 0), or a
+// per-class slice of BOYS_SLICE orders starting at the class's top order,
+// staged in LDS by the kernels (boys_stage). BS = row stride, C0 = first order.
+constexpr int BOYS_SLICE = 10;
+
 // F_L(T) (single top order), as aimd/qc/boys.py:87-114
+template <int BS = BOYS_NCOL, int C0 = 0>
 MD_HD double boys_top(int L, double T, const double* __restrict__ tab) {
     if (T < BOYS_XSWITCH) {
         int k = (int)(T * 10.0 + 0.5);
         double dx = k * 0.1 - T;
-        const double* row = tab + k * BOYS_NCOL + L;
+        const double* row = tab + k * BS + (L - C0);
         double acc = 0.0, fac = 1.0;
         acc += row[0] * fac; fac *= dx * 1.0;
         acc += row[1] * fac; fac *= dx * 0.5;
@@ -91,9 +97,9 @@ MD_HD double boys_top(int L, double T, const double* __restrict__ tab) {
 }
 
 // F_0..F_L into F (downward recursion from F_L, as the reference's R code does)
-template <int L>
+template <int L, int BS = BOYS_NCOL, int C0 = 0>
 MD_HD void boys_all(double T, const double* __restrict__ tab, double* F) {
-    F[L] = boys_top(L, T, tab);
+    F[L] = boys_top<BS, C0>(L, T, tab);
     if (L > 0) {
         double ex = exp(-T);
         double t2 = 2.0 * T;
@@ -130,12 +136,14 @@ constexpr HermTable make_herm_table() {
 inline constexpr HermTable HT = make_herm_table();
 
 // R[hidx(t,u,v)] = scale * R_{tuv}(alpha, (x,y,z)) for t+u+v <= L
-template <int L>
+// tab: the full table (SLICE false) or the slice starting at order L (SLICE true)
+template <int L, bool SLICE = false>
 MD_HD void hermite_r(double alpha, double x, double y, double z, double scale,
                      const double* __restrict__ tab, double* R) {
     double T = alpha * (x * x + y * y + z * z);
     double F[L + 1];
-    boys_all<L>(T, tab, F);
+    if constexpr (SLICE) boys_all<L, BOYS_SLICE, L>(T, tab, F);
+    else boys_all<L>(T, tab, F);
     double m2a = -2.0 * alpha;
     double pw[L + 1];
     pw[0] = scale;
@@ -236,5 +244,15 @@ MD_HD void fx_split(double x, long long& hi, long long& lo) {
 MD_HD double fx_value(long long hi, long long lo) {
     return (double)hi * (1.0 / 4294967296.0) + (double)lo * (1.0 / 18889465931478580854784.0);  // 2^-32, 2^-74
 }
+
+#ifdef __HIPCC__
+// Copy the BOYS_SLICE orders from Ltop of the full table (global) into an LDS
+// slice (BOYS_NGRID * BOYS_SLICE doubles) with the whole workgroup; ends with a barrier.
+__device__ inline void boys_stage(const double* __restrict__ tab, int Ltop, double* lds) {
+    for (int i = threadIdx.x; i < BOYS_NGRID * BOYS_SLICE; i += blockDim.x)
+        lds[i] = tab[(i / BOYS_SLICE) * BOYS_NCOL + Ltop + i % BOYS_SLICE];
+    __syncthreads();
+}
+#endif
 
 }  // namespace md

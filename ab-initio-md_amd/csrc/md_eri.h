@@ -33,7 +33,7 @@ constexpr bool in_box(int la, int lb, int ab, int t, int u, int v) {
 }
 
 // acc[ab * NCD + cd] = (ab|cd) for bra records rb[0..nb), ket records rk[0..nk)
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool SLICE = false>
 MD_HD void eri_quartet(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
                        const double* __restrict__ tab, double* __restrict__ acc) {
     using Q = Quartet<LA, LB, LC, LD>;
@@ -53,7 +53,7 @@ MD_HD void eri_quartet(const double* __restrict__ rb, int nb, const double* __re
             const double alpha = p * q / s;
             const double pref = TWO_PI_2_5 / (p * q * sqrt(s));
             double R[NH(L)];
-            hermite_r<L>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
+            hermite_r<L, SLICE>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
             static_for<0, NHK>([&](auto kk) {
                 constexpr int k = decltype(kk)::value;
                 constexpr int tk = HT.t[k], uk = HT.u[k], vk = HT.v[k];
@@ -97,7 +97,7 @@ constexpr bool split_by_cd() { return full_footprint<LA, LB, LC, LD>() > 140; }
 
 // acc[ab] = (ab|cd) for one ket Cartesian component pair cd (runtime); used
 // with one thread per (quartet, cd) for the heavy classes.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool SLICE = false>
 MD_HD void eri_quartet_cd(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
                           const double* __restrict__ tab, int cd, double* __restrict__ acc) {
     using Q = Quartet<LA, LB, LC, LD>;
@@ -117,7 +117,7 @@ MD_HD void eri_quartet_cd(const double* __restrict__ rb, int nb, const double* _
             const double alpha = p * q / s;
             const double pref = TWO_PI_2_5 / (p * q * sqrt(s));
             double R[NH(L)];
-            hermite_r<L>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
+            hermite_r<L, SLICE>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
             static_for<0, NHK>([&](auto kk) {
                 constexpr int k = decltype(kk)::value;
                 constexpr int tk = HT.t[k], uk = HT.u[k], vk = HT.v[k];

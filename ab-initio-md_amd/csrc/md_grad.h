@@ -32,7 +32,7 @@ MD_HD constexpr int grad_ncol() {
     return best;
 }
 
-template <int LA, int LB, int LC, int LD, int NCOL>
+template <int LA, int LB, int LC, int LD, int NCOL, bool SLICE = false>
 MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
                      const double* __restrict__ tab, const double* Apos, const double* Bpos, int kc0,
                      const double* gam, double* gA, double* gB) {
@@ -64,7 +64,7 @@ MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __rest
             const double alpha = p * q / s;
             const double pref = TWO_PI_2_5 / (p * q * sqrt(s));
             double R[NH(L1)];
-            hermite_r<L1>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
+            hermite_r<L1, SLICE>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
             static_for<0, NHK>([&](auto kk) {
                 constexpr int k = decltype(kk)::value;
                 constexpr int tk = HT.t[k], uk = HT.u[k], vk = HT.v[k];

@@ -44,6 +44,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--jk-mode", choices=["auto", "direct", "store"], default="auto")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the engine has no multi-GPU path "
@@ -52,6 +54,16 @@ def main() -> int:
 
     import numpy as np
     import torch
+
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 8 * n}))
+        return 0
 
     from aimd_amd.backends import get_backend
     from aimd_amd.md import VelocityVerletNVE, maxwell_boltzmann, masses_au, read_xyz
@@ -97,9 +109,15 @@ def main() -> int:
     fock_ms = float(ms_kind[eng.FOCK_KINDS].sum()) / n_builds
     store_ms = float(ms_kind[eng.STORE_KINDS].sum()) / max(args.steps, 1)
     flops = work["total"]
-    achieved = flops / (fock_ms * 1e-3) / 1e12
     jk_ms = fock_ms
-    grad_ms = ms_kind[21:42].sum() / max(args.steps, 1)
+    grad_ms = float(ms_kind[eng.GRAD_KINDS].sum()) / max(args.steps, 1)
+    # derivative ERIs of the last timed step priced with the gradient work model; the
+    # dominant single kernel of the step is the heaviest derivative-ERI class kernel
+    gw = flopmodel.grad_work(be.basis, eng.grad_quartets(), npp)
+    cls_ms = np.array([ms_kind[21 + k] / max(int(n_kind[21 + k]), 1) for k in range(21)])
+    kd = int(np.argmax(cls_ms))
+    achieved = gw["per_class"][kd] / (cls_ms[kd] * 1e-3) / 1e12
+    grad_main_ms = float(ms_kind[21:42].sum()) / max(args.steps, 1)
     per_kernel = {eng.KIND_NAMES[k]: round(float(ms_kind[k]) / max(int(n_kind[k]), 1), 4)
                   for k in range(eng.NKIND) if n_kind[k] > 0}
     out = {
@@ -127,21 +145,26 @@ def main() -> int:
         "jk_mode": "store" if info["stored"] else "direct",
         "jk_builds_per_step": round(n_builds / max(args.steps, 1), 2),
         "jk_build_ms": round(float(jk_ms), 3),
+        "jk_build_tflops": round(flops / (jk_ms * 1e-3) / 1e12, 3) if jk_ms > 0 else None,
+        "jk_stream_gbs": round(8.0 * info["gather_doubles"] / (jk_ms * 1e-3) / 1e9, 1) if jk_ms > 0 else None,
         "store_build_ms_per_step": round(store_ms, 3),
         "store_bytes": info["store_bytes"],
         "gradient_kernels_ms_per_step": round(float(grad_ms), 3),
+        "gradient_tflops": round(gw["total"] / (grad_main_ms * 1e-3) / 1e12, 3) if grad_main_ms > 0 else None,
         "eri_quartets_per_s": float(len(quartets) / (jk_ms * 1e-3)) if jk_ms > 0 else None,
         "nve_total_energy_span_Eh": float(np.ptp(e_tot)) if e_tot else None,
         "roofline": {
-            "bound": "fp64-valu",
-            "kernel": "fock_build (jengine_* + exchange_* + prologue/epilogue, per SCF iteration)",
+            "bound": "mfma",
+            "kernel": f"k_grad_class<{kd}> (derivative ERIs, quartet class {kd}: "
+                      f"({'/'.join(str(v) for v in flopmodel.class_ls(kd))}), fp64 VALU)",
             "achieved": round(achieved, 4),
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 5),
-            "traffic": None,
-            "flops_per_launch": float(flops),
-            "avg_launch_ms": round(float(fock_ms), 4),
+            "traffic": measured_traffic(kd),
+            "flops_per_launch": float(gw["per_class"][kd]),
+            "avg_launch_ms": round(float(cls_ms[kd]), 4),
+            "note": "FP64 vector kernel priced against MI355X's FP64 peak (78.6 TF/s, vector and matrix)",
         },
         "avg_ms_per_launch": per_kernel,
     }
@@ -149,6 +172,19 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(sym, md.x, np.mean(iters), args.cpu_threads, be.last_scf.density)
     print(json.dumps(out))
     return 0
+
+
+def measured_traffic(kernel_class):
+    """HBM bytes per launch of k_grad_class<k> from its committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a calibration copy), or None if there is none."""
+    path = os.path.join(HERE, "profiles", f"k_grad_class_{kernel_class}_water16_631gs_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path, encoding="utf-8") as f:
+        for line in f:
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(sym, x, mean_iters, threads, P):

@@ -108,9 +108,12 @@ int aimdhf_jk(aimdhf_plan* plan, int nset, const double* dens, double* J, double
  */
 int aimdhf_set_jk_mode(aimdhf_plan* plan, int mode, int64_t max_store_bytes);
 
-/* Whether the current geometry has an integral store, its size in bytes and
- * the gather-list lengths of the K and J kernels (any pointer may be NULL). */
-int aimdhf_jk_info(aimdhf_plan* plan, int* stored, int64_t* store_bytes, int64_t* k_entries, int64_t* j_entries);
+/* Whether the current geometry has an integral store, its size in bytes, the
+ * gather-list lengths of the K and J kernels and the number of integrals
+ * (doubles) their gather-ordered data holds, i.e. what one J/K build streams
+ * (any pointer may be NULL). */
+int aimdhf_jk_info(aimdhf_plan* plan, int* stored, int64_t* store_bytes, int64_t* k_entries, int64_t* j_entries,
+                   int64_t* gather_doubles);
 
 /*
  * One-electron matrices of the current geometry into device buffers:
@@ -147,9 +150,11 @@ int aimdhf_grad_two(aimdhf_plan* plan, const double* Dc, int nx, const double* D
  * type (l_X, l_Y), 21..41 derivative-ERI kernels of the 21 quartet classes,
  * 42 geometry set-up, 43 one-electron integrals, 44 one-electron gradient,
  * 45 J/K prologue and epilogue, 46 Hermite densities, 47 integral store
- * evaluation, 48 gather-list build, 49..54 store J per pair class, 55..63
- * store K per output type. aimdhf_profile_read synchronises, returns the summed
- * milliseconds and launch counts per kind (nkind <= 64) and optionally resets.
+ * evaluation, 15 gather-ordered copies, 48 gather-list build, 49..54 store J
+ * per pair class, 55..63 store K per output type, 64..84 derivative-ERI
+ * screening pre-pass per class. aimdhf_profile_read synchronises, returns the
+ * summed milliseconds and launch counts per kind (nkind <= 96) and optionally
+ * resets.
  */
 int aimdhf_profile_enable(aimdhf_plan* plan, int on);
 int aimdhf_profile_read(aimdhf_plan* plan, double* ms, int64_t* launches, int nkind, int reset);
@@ -157,6 +162,12 @@ int aimdhf_profile_read(aimdhf_plan* plan, double* ms, int64_t* launches, int nk
 /* Per-class sizes (21 each) of the energy quartet list and of the last
  * gradient quartet list (either pointer may be NULL). */
 int aimdhf_class_counts(aimdhf_plan* plan, int64_t* energy_counts, int64_t* gradient_counts);
+
+/* The quartets the last aimdhf_grad_two differentiated (those passing its
+ * Q_P Q_Q max|Gamma| screening) as (P, Q) shell-pair indices, P >= Q, copied
+ * to host memory (2*cap int64); *count receives their number. Synchronous.
+ * No counterpart in the reference (used to price the derivative kernels). */
+int aimdhf_grad_quartets(aimdhf_plan* plan, int64_t* out_host, int64_t cap, int64_t* count);
 
 /* Kept primitive pairs of every shell pair (npair = nshell (nshell+1)/2),
  * copied to host memory. Synchronous. */

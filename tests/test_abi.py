@@ -59,3 +59,10 @@ def test_product_fails_loudly_without_a_gpu():
     b = build_basis(["H", "H"], [[0, 0, 0], [0, 0, 1.4]], "sto-3g")
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         Engine(b)
+
+
+def test_profile_kind_names_cover_every_kind():
+    from aimd_amd.qc.engine import Engine
+    assert len(Engine.KIND_NAMES) == Engine.NKIND
+    assert len(set(Engine.KIND_NAMES)) == Engine.NKIND
+    assert not set(Engine.FOCK_KINDS) & set(Engine.GRAD_KINDS)

@@ -71,3 +71,20 @@ def test_flop_model_counts_water(oracle_lib):
     assert w["quartets"].sum() == len(q)
     assert w["total"] > 0
     assert w["prim_quartets"].sum() == pytest.approx(float((npp[q[:, 0]] * npp[q[:, 1]]).sum()))
+
+
+def test_gradient_flop_model_hand_counts():
+    # H2/STO-3G: pairs 0 = (0,0), 1 = (1,0), 2 = (1,1). An ss|ss pass with the derivative:
+    # per primitive quartet (20+3*1) + 12 + 3*(NH(1)-1) + NH(1)*NH(0)*(1+2*1) = 56,
+    # per bra primitive pair 6 * 2 * n_cd * box = 6 * 2 * 1 * 8 = 96.
+    from aimd_amd.qc import flopmodel
+    from aimd_amd.qc.basis import build_basis
+    b = build_basis(["H", "H"], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.4]]), "sto-3g")
+    npp = np.array([9, 4, 9], np.int32)
+    one = 9 * 4 * 56 + 4 * 96            # (1|0): the two-centre pair needs its pass, the ket is one-centre
+    w = flopmodel.grad_work(b, np.array([[1, 0]]), npp)
+    assert w["total"] == one
+    w = flopmodel.grad_work(b, np.array([[2, 0]]), npp)   # both pairs one-centre: one pass
+    assert w["total"] == 9 * 9 * 56 + 9 * 96
+    w = flopmodel.grad_work(b, np.array([[1, 1]]), npp)   # both two-centre: bra and ket passes
+    assert w["total"] == 2 * (4 * 4 * 56 + 4 * 96)
