@@ -34,6 +34,7 @@ SIGNATURES = {
     "aimdhf_quartets": (_I, [_P, _P, _L, ctypes.POINTER(_L)]),
     "aimdhf_jk": (_I, [_P, _I, _P, _P, _P, _P]),
     "aimdhf_set_jk_mode": (_I, [_P, _I, _L]),
+    "aimdhf_set_precision": (_I, [_P, _I, _D, _D]),
     "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_grad_one": (_I, [_P, _P, _P, _P, _P]),

@@ -107,6 +107,17 @@ class Engine:
         _lib.check(_lib.lib().aimdhf_set_jk_mode(self._h, self.JK_MODES[mode], int(max_store_bytes)))
         self.jk_mode = mode
 
+    PRECISIONS = {"fp64": 0, "mixed": 1}
+
+    def set_precision(self, precision: str = "fp64", jk_switch: float = 0.0, grad_switch: float = 0.0) -> None:
+        """"fp64", or "mixed": FP32 for quartets whose Schwarz bound (J/K store, from the next
+        set_geometry) or Schwarz-Gamma bound (derivative integrals) is below the switch."""
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(self.PRECISIONS)}, got {precision!r}")
+        _lib.check(_lib.lib().aimdhf_set_precision(self._h, self.PRECISIONS[precision], float(jk_switch),
+                                                   float(grad_switch)))
+        self.precision = precision
+
     def jk_info(self) -> dict:
         """Store state of the current geometry: stored, bytes, K and J gather-list entries and
         the integrals one J/K build streams (gather_doubles)."""

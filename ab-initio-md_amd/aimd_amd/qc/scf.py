@@ -77,6 +77,9 @@ class SCFOptions:
     jk_mode: str = "auto"
     max_store_bytes: int = 0
     density_solver: str = "purify"
+    precision: str = "fp64"
+    jk_fp32_switch: float = 1e-9
+    grad_fp32_switch: float = 1e-9
 
     def __post_init__(self) -> None:
         if not (self.conv_energy > 0.0 and self.conv_commutator > 0.0):
@@ -105,6 +108,10 @@ class SCFOptions:
             raise ValueError(f"unknown jk_mode {self.jk_mode!r}; use one of {sorted(qengine.Engine.JK_MODES)}")
         if self.max_store_bytes < 0:
             raise ValueError("max_store_bytes must be >= 0")
+        if self.precision not in qengine.Engine.PRECISIONS:
+            raise ValueError(f"unknown precision {self.precision!r}; use one of {sorted(qengine.Engine.PRECISIONS)}")
+        if not (self.jk_fp32_switch >= 0.0 and self.grad_fp32_switch >= 0.0):
+            raise ValueError("fp32 switch thresholds must be >= 0")
         if self.density_solver not in _DENSITY_SOLVERS:
             raise ValueError(f"unknown density_solver {self.density_solver!r}; use one of {_DENSITY_SOLVERS}")
 
@@ -593,6 +600,8 @@ class SCFSolver:
             raise ValueError(f"{self.n_alpha} alpha electrons do not fit in {self.basis.nao} AOs")
         self.engine = qengine.Engine(self.basis)
         self.engine.set_jk_mode(self.options.jk_mode, self.options.max_store_bytes)
+        self.engine.set_precision(self.options.precision, self.options.jk_fp32_switch,
+                                  self.options.grad_fp32_switch)
         self._ints: SCFIntegrals | None = None
         self._ints_positions: np.ndarray | None = None
         self._ints_options: SCFOptions | None = None

@@ -185,7 +185,10 @@ struct aimdhf_plan {
     long long store_base[NQCLASS + 1] = {0};
     long long store_bytes = 0, k_entries = 0, j_entries = 0;
     std::vector<long long> kseg, jseg;
-    DevBuf d_gkeep;
+    DevBuf d_gkeep, d_gsel, d_nsel, d_gblk;
+    // precision: 0 FP64 everywhere; 1 mixed (FP32 below the switches)
+    int precision = 0;
+    double jk_switch = 0.0, grad_switch = 0.0;
     DevBuf d_store, d_qmap, d_kcnt, d_jcnt, d_kseg, d_jseg, d_klist, d_jlist, d_lerr;
     DevBuf d_kdata, d_jdata, d_kdoff, d_jdoff;
     long long kdata_total = 0, jdata_total = 0;
@@ -202,7 +205,7 @@ struct aimdhf_plan {
                           &d_charges, &d_glist, &d_gfx, &d_gden, &d_scratch, &d_out_x, &d_out_y, &d_ao_shell,
                           &d_ket_q, &d_kpre, &d_krec, &d_kpair, &d_part_off, &d_part_shell, &d_part_pair,
                           &d_part_q, &d_dh, &d_jpart, &d_kpart, &d_ket_pair, &d_store, &d_qmap, &d_kcnt,
-                          &d_jcnt, &d_kseg, &d_jseg, &d_klist, &d_jlist, &d_lerr, &d_gkeep,
+                          &d_jcnt, &d_kseg, &d_jseg, &d_klist, &d_jlist, &d_lerr, &d_gkeep, &d_gsel, &d_nsel, &d_gblk,
                           &d_kdata, &d_jdata, &d_kdoff, &d_jdoff, &d_kunit_os, &d_kunit_e, &d_out_unit,
                           &d_kpart2, &d_junit_pc, &d_junit_e, &d_pair_unit, &d_jpart2})
             b->release();
@@ -1090,10 +1093,19 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
         long long gtot = 0;
         for (int k = 0; k < NQCLASS; ++k) gtot += p->gcount[k];
         p->d_gkeep.ensure(std::max<long long>(gtot, 1));
+        p->d_gsel.ensure(sizeof(int) * 2 * std::max<long long>(gtot, 1));
+        p->d_nsel.ensure(sizeof(int) * 2 * NQCLASS);
+        p->d_gblk.ensure(sizeof(int) * 4 * (gtot / 256 + 2 * NQCLASS));
+        a.fp32_switch = p->precision == 1 ? p->grad_switch : 0.0;
         long long off = 0;
         for (int k = 0; k < NQCLASS; ++k) {
             const int2* lst = p->d_glist.as<int2>() + off;
             a.keep = p->d_gkeep.as<unsigned char>() + off;
+            a.sel64 = p->d_gsel.as<int>() + off;
+            a.sel32 = p->d_gsel.as<int>() + gtot + off;
+            a.nsel = p->d_nsel.as<int>() + 2 * k;
+            a.bcnt = p->d_gblk.as<int>() + 4 * (off / 256 + k);
+            a.boff = a.bcnt + 2 * ((p->gcount[k] + 255) / 256);
             if (p->gcount[k] > 0) {
                 p->prof.time(64 + k, st, [&] { aimdhf::launch_grad_keep(k, a, lst, p->gcount[k], st); });
                 p->prof.time(21 + k, st, [&] { aimdhf::launch_grad_one_class(k, a, lst, p->gcount[k], st); });
@@ -1114,6 +1126,16 @@ int aimdhf_set_jk_mode(aimdhf_plan* p, int mode, int64_t max_store_bytes) {
     if (max_store_bytes < 0) return fail(AIMDHF_E_ARG, "max_store_bytes must be >= 0");
     p->jk_mode = mode;
     p->store_limit = max_store_bytes;
+    return AIMDHF_OK;
+}
+
+int aimdhf_set_precision(aimdhf_plan* p, int mode, double jk_switch, double grad_switch) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    if (mode < 0 || mode > 1) return fail(AIMDHF_E_ARG, "precision mode must be 0 (fp64) or 1 (mixed)");
+    if (!(jk_switch >= 0.0) || !(grad_switch >= 0.0)) return fail(AIMDHF_E_ARG, "switch thresholds must be >= 0");
+    p->precision = mode;
+    p->jk_switch = jk_switch;
+    p->grad_switch = grad_switch;
     return AIMDHF_OK;
 }
 

@@ -84,7 +84,13 @@ struct GradArgs {
     const double* qscr;
     const double* dc;           // (nao, nao), symmetric
     const double* dx;           // (nx, nao, nao), symmetric
-    unsigned char* keep;        // per quartet of the launched class: passes the screening
+    double fp32_switch;         // quartets with Q_P Q_Q max|Gamma| below: FP32 derivative integrals
+    unsigned char* keep;        // per quartet of the launched class: 0 dropped, 1 FP64, 2 FP32
+    int* sel64;                 // selected quartets (indices into the class list), FP64
+    int* sel32;                 // and FP32
+    int* nsel;                  // their counts (2 ints)
+    int* bcnt;                  // pre-pass block counts (2 ints per 256 quartets)
+    int* boff;                  // and their exclusive prefix sums
     long long* out;             // (natm, 3) fixed point, hi/lo interleaved
 };
 

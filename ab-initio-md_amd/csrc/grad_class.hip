@@ -52,41 +52,129 @@ __device__ __forceinline__ double gamma_el(const GradArgs& a, int m, int n, int 
     return v * deg;
 }
 
+// Screening pre-pass, one thread per quartet (256-quartet blocks): keep[q] = 0
+// (dropped), 1 (FP64) or 2 (FP32: Q_P Q_Q max|Gamma| below fp32_switch); each
+// block's counts of 1s and 2s go to bcnt[2 * block + {0, 1}].
 template <int Q>
 __global__ __launch_bounds__(256) void k_grad_keep(GradArgs a, const int2* __restrict__ list, long long nq) {
+    __shared__ int wc[2][4];
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    const int2 pq = list[q];
-    const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
-    const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
-    unsigned char keep = 0;
-    if (!(aA == aB && aB == aC && aC == aD)) {
-        double deg = 0.5;
-        if (sA != sB) deg *= 2.0;
-        if (sC != sD) deg *= 2.0;
-        if (pq.x != pq.y) deg *= 2.0;
-        const int oA = a.shell_ao[sA], oB = a.shell_ao[sB], oC = a.shell_ao[sC], oD = a.shell_ao[sD];
-        double gmax = 0.0;
-        for (int ab = 0; ab < NAB; ++ab)
-            for (int cd = 0; cd < NCD; ++cd)
-                gmax = fmax(gmax, fabs(gamma_el(a, oA + ab / NB, oB + ab % NB, oC + cd / ND, oD + cd % ND, deg)));
-        keep = !(a.qscr[pq.x] * a.qscr[pq.y] * gmax < a.thr);
+    int keep = 0;
+    if (q < nq) {
+        const int2 pq = list[q];
+        const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
+        const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
+        if (!(aA == aB && aB == aC && aC == aD)) {
+            double deg = 0.5;
+            if (sA != sB) deg *= 2.0;
+            if (sC != sD) deg *= 2.0;
+            if (pq.x != pq.y) deg *= 2.0;
+            const int oA = a.shell_ao[sA], oB = a.shell_ao[sB], oC = a.shell_ao[sC], oD = a.shell_ao[sD];
+            double gmax = 0.0;
+            for (int ab = 0; ab < NAB; ++ab)
+                for (int cd = 0; cd < NCD; ++cd)
+                    gmax = fmax(gmax, fabs(gamma_el(a, oA + ab / NB, oB + ab % NB, oC + cd / ND, oD + cd % ND, deg)));
+            const double b = a.qscr[pq.x] * a.qscr[pq.y] * gmax;
+            keep = (b < a.thr) ? 0 : (b < a.fp32_switch ? 2 : 1);
+        }
+        a.keep[q] = (unsigned char)keep;
     }
-    a.keep[q] = keep;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long m1 = __ballot(keep == 1), m2 = __ballot(keep == 2);
+    if (lane == 0) {
+        wc[0][w] = (int)__popcll(m1);
+        wc[1][w] = (int)__popcll(m2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2)
+        a.bcnt[2 * blockIdx.x + threadIdx.x] =
+            wc[threadIdx.x][0] + wc[threadIdx.x][1] + wc[threadIdx.x][2] + wc[threadIdx.x][3];
 }
 
-template <int Q>
-__global__ __launch_bounds__(256) void k_grad_class(GradArgs a, const int2* __restrict__ list, long long nq) {
+// one 1024-thread block: exclusive scan of the per-block counts (both lists) into
+// boff, totals into nsel[0..1]
+__global__ __launch_bounds__(1024) void k_grad_scan(const int* __restrict__ bcnt, int nb, int* __restrict__ boff,
+                                                    int* __restrict__ nsel) {
+    __shared__ int ws[2][16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int run0 = 0, run1 = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + t;
+        const int x0 = i < nb ? bcnt[2 * i] : 0, x1 = i < nb ? bcnt[2 * i + 1] : 0;
+        int v0 = x0, v1 = x1;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y0 = __shfl_up(v0, off, 64), y1 = __shfl_up(v1, off, 64);
+            if (lane >= off) {
+                v0 += y0;
+                v1 += y1;
+            }
+        }
+        if (lane == 63) {
+            ws[0][w] = v0;
+            ws[1][w] = v1;
+        }
+        __syncthreads();
+        int p0 = 0, p1 = 0, t0 = 0, t1 = 0;
+        for (int j = 0; j < 16; ++j) {
+            if (j < w) {
+                p0 += ws[0][j];
+                p1 += ws[1][j];
+            }
+            t0 += ws[0][j];
+            t1 += ws[1][j];
+        }
+        if (i < nb) {
+            boff[2 * i] = run0 + p0 + v0 - x0;
+            boff[2 * i + 1] = run1 + p1 + v1 - x1;
+        }
+        run0 += t0;
+        run1 += t1;
+        __syncthreads();
+    }
+    if (t == 0) {
+        nsel[0] = run0;
+        nsel[1] = run1;
+    }
+}
+
+// ordered scatter of the selected quartets (same 256-quartet blocks as the pre-pass)
+__global__ __launch_bounds__(256) void k_grad_scatter(const unsigned char* __restrict__ keep, long long nq,
+                                                      const int* __restrict__ boff, int* __restrict__ sel64,
+                                                      int* __restrict__ sel32) {
+    __shared__ int wc[2][4];
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = q < nq ? keep[q] : 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long m1 = __ballot(k == 1), m2 = __ballot(k == 2);
+    if (lane == 0) {
+        wc[0][w] = (int)__popcll(m1);
+        wc[1][w] = (int)__popcll(m2);
+    }
+    __syncthreads();
+    int o1 = boff[2 * blockIdx.x], o2 = boff[2 * blockIdx.x + 1];
+    for (int j = 0; j < w; ++j) {
+        o1 += wc[0][j];
+        o2 += wc[1][j];
+    }
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    if (k == 1) sel64[o1 + __popcll(m1 & below)] = (int)q;
+    if (k == 2) sel32[o2 + __popcll(m2 & below)] = (int)q;
+}
+
+// Work item = (selected quartet, pass column group), derivative integrals in Real
+template <int Q, typename Real>
+__global__ __launch_bounds__(256) void k_grad_class(GradArgs a, const int2* __restrict__ list,
+                                                    const int* __restrict__ sel, const int* __restrict__ nsel) {
     extern __shared__ long long lds[];
     __shared__ double btab[BOYS_NGRID * BOYS_SLICE];
     for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
     boys_stage(a.boys, LA + LB + LC + LD + 1, btab);
-    const long long nitems = nq * NT;
+    const long long nitems = (long long)(*nsel) * NT;
     for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
          it += (long long)gridDim.x * blockDim.x) {
-        const long long q = it / NT;
+        const int q = sel[it / NT];
         const int j = (int)(it % NT);
-        if (!a.keep[q]) continue;
         const int2 pq = list[q];
         const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
         const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
@@ -104,14 +192,14 @@ __global__ __launch_bounds__(256) void k_grad_class(GradArgs a, const int2* __re
         double g1[3], g2[3];
         if (bra) {
             const int kc0 = j * NCB;
-            double gam[NAB * NCB];
+            Real gam[NAB * NCB];
             for (int ab = 0; ab < NAB; ++ab)
 #pragma unroll
                 for (int c = 0; c < NCB; ++c)
                     gam[ab * NCB + c] =
-                        gamma_el(a, oA + ab / NB, oB + ab % NB, oC + (kc0 + c) / ND, oD + (kc0 + c) % ND, deg);
-            grad_pass<LA, LB, LC, LD, NCB, true>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, a.pos + 3 * aA,
-                                           a.pos + 3 * aB, kc0, gam, g1, g2);
+                        Real(gamma_el(a, oA + ab / NB, oB + ab % NB, oC + (kc0 + c) / ND, oD + (kc0 + c) % ND, deg));
+            grad_pass<LA, LB, LC, LD, NCB, true, Real>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, a.pos + 3 * aA,
+                                                       a.pos + 3 * aB, kc0, gam, g1, g2);
             for (int d = 0; d < 3; ++d) {
                 if (need_bra) {
                     lds_add(lds, aA, d, g1[d]);
@@ -124,14 +212,14 @@ __global__ __launch_bounds__(256) void k_grad_class(GradArgs a, const int2* __re
             }
         } else {
             const int ab0 = (j - NIB) * NCK;
-            double gam[NCD * NCK];
+            Real gam[NCD * NCK];
             for (int cd = 0; cd < NCD; ++cd)
 #pragma unroll
                 for (int c = 0; c < NCK; ++c)
                     gam[cd * NCK + c] =
-                        gamma_el(a, oA + (ab0 + c) / NB, oB + (ab0 + c) % NB, oC + cd / ND, oD + cd % ND, deg);
-            grad_pass<LC, LD, LA, LB, NCK, true>(rk, a.npp[pq.y], rb, a.npp[pq.x], btab, a.pos + 3 * aC,
-                                           a.pos + 3 * aD, ab0, gam, g1, g2);
+                        Real(gamma_el(a, oA + (ab0 + c) / NB, oB + (ab0 + c) % NB, oC + cd / ND, oD + cd % ND, deg));
+            grad_pass<LC, LD, LA, LB, NCK, true, Real>(rk, a.npp[pq.y], rb, a.npp[pq.x], btab, a.pos + 3 * aC,
+                                                       a.pos + 3 * aD, ab0, gam, g1, g2);
             for (int d = 0; d < 3; ++d) {
                 if (need_ket) {
                     lds_add(lds, aC, d, g1[d]);
@@ -154,16 +242,26 @@ __global__ __launch_bounds__(256) void k_grad_class(GradArgs a, const int2* __re
 #define CAT2(a, b) a##b
 #define CAT(a, b) CAT2(a, b)
 
+// pre-pass + scan + ordered scatter (a.bcnt / a.boff hold 2 ints per 256 quartets)
 void CAT(launch_grad_keep_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list, long long nq, hipStream_t st) {
-    if (nq > 0) k_grad_keep<QC><<<dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st>>>(a, list, nq);
+    if (nq <= 0) return;
+    const int nb = (int)((nq + 255) / 256);
+    k_grad_keep<QC><<<dim3(nb), dim3(256), 0, st>>>(a, list, nq);
+    k_grad_scan<<<dim3(1), dim3(1024), 0, st>>>(a.bcnt, nb, a.boff, a.nsel);
+    k_grad_scatter<<<dim3(nb), dim3(256), 0, st>>>(a.keep, nq, a.boff, a.sel64, a.sel32);
 }
 
+// FP64 items, then FP32 items (the grid is sized for the whole list; each kernel
+// reads its selection count on the device)
 void CAT(launch_grad_class_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list, long long nq, hipStream_t st) {
     long long items = nq * NT;
     if (items <= 0) return;
     long long blocks = (items + 255) / 256;
     if (blocks > 1024) blocks = 1024;
-    k_grad_class<QC><<<dim3((unsigned)blocks), dim3(256), sizeof(long long) * 6 * a.natm, st>>>(a, list, nq);
+    const size_t shm = sizeof(long long) * 6 * a.natm;
+    k_grad_class<QC, double><<<dim3((unsigned)blocks), dim3(256), shm, st>>>(a, list, a.sel64, a.nsel);
+    if (a.fp32_switch > 0.0)
+        k_grad_class<QC, float><<<dim3((unsigned)blocks), dim3(256), shm, st>>>(a, list, a.sel32, a.nsel + 1);
 }
 
 }  // namespace aimdhf

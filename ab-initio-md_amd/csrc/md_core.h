@@ -67,44 +67,50 @@ constexpr double TWO_PI_2_5 = 34.986836655249725;  // 2 pi^(5/2)
 // staged in LDS by the kernels (boys_stage). BS = row stride, C0 = first order.
 constexpr int BOYS_SLICE = 10;
 
-// F_L(T) (single top order), as aimd/qc/boys.py:87-114
-template <int BS = BOYS_NCOL, int C0 = 0>
-MD_HD double boys_top(int L, double T, const double* __restrict__ tab) {
-    if (T < BOYS_XSWITCH) {
-        int k = (int)(T * 10.0 + 0.5);
-        double dx = k * 0.1 - T;
+// exp / sqrt of the working precision (float arithmetic stays float)
+MD_HD inline double md_exp(double x) { return exp(x); }
+MD_HD inline float md_exp(float x) { return expf(x); }
+MD_HD inline double md_sqrt(double x) { return sqrt(x); }
+MD_HD inline float md_sqrt(float x) { return sqrtf(x); }
+
+// F_L(T) (single top order), as aimd/qc/boys.py:87-114, in precision Real
+template <int BS = BOYS_NCOL, int C0 = 0, typename Real = double>
+MD_HD Real boys_top(int L, Real T, const double* __restrict__ tab) {
+    if (T < Real(BOYS_XSWITCH)) {
+        int k = (int)(T * Real(10.0) + Real(0.5));
+        Real dx = Real(k) * Real(0.1) - T;
         const double* row = tab + k * BS + (L - C0);
-        double acc = 0.0, fac = 1.0;
-        acc += row[0] * fac; fac *= dx * 1.0;
-        acc += row[1] * fac; fac *= dx * 0.5;
-        acc += row[2] * fac; fac *= dx * (1.0 / 3.0);
-        acc += row[3] * fac; fac *= dx * 0.25;
-        acc += row[4] * fac; fac *= dx * 0.2;
-        acc += row[5] * fac; fac *= dx * (1.0 / 6.0);
-        acc += row[6] * fac; fac *= dx * (1.0 / 7.0);
-        acc += row[7] * fac; fac *= dx * 0.125;
-        acc += row[8] * fac; fac *= dx * (1.0 / 9.0);
-        acc += row[9] * fac;
+        Real acc = 0, fac = 1;
+        acc += Real(row[0]) * fac; fac *= dx * Real(1.0);
+        acc += Real(row[1]) * fac; fac *= dx * Real(0.5);
+        acc += Real(row[2]) * fac; fac *= dx * Real(1.0 / 3.0);
+        acc += Real(row[3]) * fac; fac *= dx * Real(0.25);
+        acc += Real(row[4]) * fac; fac *= dx * Real(0.2);
+        acc += Real(row[5]) * fac; fac *= dx * Real(1.0 / 6.0);
+        acc += Real(row[6]) * fac; fac *= dx * Real(1.0 / 7.0);
+        acc += Real(row[7]) * fac; fac *= dx * Real(0.125);
+        acc += Real(row[8]) * fac; fac *= dx * Real(1.0 / 9.0);
+        acc += Real(row[9]) * fac;
         return acc;
     }
-    double f = 0.5 * sqrt(MD_PI / T);
+    Real f = Real(0.5) * md_sqrt(Real(MD_PI) / T);
     if (L > 0) {
-        double ex = exp(-T);
-        double inv2x = 0.5 / T;
-        for (int m = 0; m < L; ++m) f = ((2 * m + 1) * f - ex) * inv2x;
+        Real ex = md_exp(-T);
+        Real inv2x = Real(0.5) / T;
+        for (int m = 0; m < L; ++m) f = (Real(2 * m + 1) * f - ex) * inv2x;
     }
     return f;
 }
 
 // F_0..F_L into F (downward recursion from F_L, as the reference's R code does)
-template <int L, int BS = BOYS_NCOL, int C0 = 0>
-MD_HD void boys_all(double T, const double* __restrict__ tab, double* F) {
-    F[L] = boys_top<BS, C0>(L, T, tab);
+template <int L, int BS = BOYS_NCOL, int C0 = 0, typename Real = double>
+MD_HD void boys_all(Real T, const double* __restrict__ tab, Real* F) {
+    F[L] = boys_top<BS, C0, Real>(L, T, tab);
     if (L > 0) {
-        double ex = exp(-T);
-        double t2 = 2.0 * T;
+        Real ex = md_exp(-T);
+        Real t2 = Real(2.0) * T;
 #pragma unroll
-        for (int n = L - 1; n >= 0; --n) F[n] = (t2 * F[n + 1] + ex) * (1.0 / (2 * n + 1));
+        for (int n = L - 1; n >= 0; --n) F[n] = (t2 * F[n + 1] + ex) * (Real(1.0) / Real(2 * n + 1));
     }
 }
 
@@ -137,15 +143,15 @@ inline constexpr HermTable HT = make_herm_table();
 
 // R[hidx(t,u,v)] = scale * R_{tuv}(alpha, (x,y,z)) for t+u+v <= L
 // tab: the full table (SLICE false) or the slice starting at order L (SLICE true)
-template <int L, bool SLICE = false>
-MD_HD void hermite_r(double alpha, double x, double y, double z, double scale,
-                     const double* __restrict__ tab, double* R) {
-    double T = alpha * (x * x + y * y + z * z);
-    double F[L + 1];
-    if constexpr (SLICE) boys_all<L, BOYS_SLICE, L>(T, tab, F);
-    else boys_all<L>(T, tab, F);
-    double m2a = -2.0 * alpha;
-    double pw[L + 1];
+template <int L, bool SLICE = false, typename Real = double>
+MD_HD void hermite_r(Real alpha, Real x, Real y, Real z, Real scale,
+                     const double* __restrict__ tab, Real* R) {
+    Real T = alpha * (x * x + y * y + z * z);
+    Real F[L + 1];
+    if constexpr (SLICE) boys_all<L, BOYS_SLICE, L, Real>(T, tab, F);
+    else boys_all<L, BOYS_NCOL, 0, Real>(T, tab, F);
+    Real m2a = Real(-2.0) * alpha;
+    Real pw[L + 1];
     pw[0] = scale;
 #pragma unroll
     for (int n = 1; n <= L; ++n) pw[n] = pw[n - 1] * m2a;
@@ -157,16 +163,16 @@ MD_HD void hermite_r(double alpha, double x, double y, double z, double scale,
         static_for<1, nhn>([&](auto hh) {
             constexpr int h = nhn - decltype(hh)::value;
             constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
-            double r;
+            Real r;
             if constexpr (t > 0) {
                 r = x * R[hidx(t - 1, u, v)];
-                if constexpr (t > 1) r += (t - 1) * R[hidx(t - 2, u, v)];
+                if constexpr (t > 1) r += Real(t - 1) * R[hidx(t - 2, u, v)];
             } else if constexpr (u > 0) {
                 r = y * R[hidx(0, u - 1, v)];
-                if constexpr (u > 1) r += (u - 1) * R[hidx(0, u - 2, v)];
+                if constexpr (u > 1) r += Real(u - 1) * R[hidx(0, u - 2, v)];
             } else {
                 r = z * R[hidx(0, 0, v - 1)];
-                if constexpr (v > 1) r += (v - 1) * R[hidx(0, 0, v - 2)];
+                if constexpr (v > 1) r += Real(v - 1) * R[hidx(0, 0, v - 2)];
             }
             R[h] = r;
         });
@@ -176,23 +182,23 @@ MD_HD void hermite_r(double alpha, double x, double y, double z, double scale,
 
 // 1-D Hermite expansion E^{ij}_t, i <= IM, j <= JM, t <= i+j; e[(i*(JM+1)+j)*(IM+JM+1)+t]
 // (entries with t > i+j are zero)
-template <int IM, int JM>
-MD_HD void hermite_e(double a, double b, double xab, double e00, double* e) {
+template <int IM, int JM, typename Real = double>
+MD_HD void hermite_e(Real a, Real b, Real xab, Real e00, Real* e) {
     constexpr int TS = IM + JM + 1;
     constexpr int JS = JM + 1;
-    const double p = a + b;
-    const double xpa = -b * xab / p;
-    const double xpb = a * xab / p;
-    const double h = 0.5 / p;
+    const Real p = a + b;
+    const Real xpa = -b * xab / p;
+    const Real xpb = a * xab / p;
+    const Real h = Real(0.5) / p;
 #pragma unroll
-    for (int i = 0; i < (IM + 1) * JS * TS; ++i) e[i] = 0.0;
+    for (int i = 0; i < (IM + 1) * JS * TS; ++i) e[i] = Real(0);
     e[0] = e00;
     static_for<1, JM + 1>([&](auto jj) {
         constexpr int j = decltype(jj)::value;
         static_for<0, j + 1>([&](auto tt) {
             constexpr int t = decltype(tt)::value;
-            double v = xpb * e[(j - 1) * TS + t];
-            if constexpr (t + 1 <= j - 1) v += (t + 1) * e[(j - 1) * TS + t + 1];
+            Real v = xpb * e[(j - 1) * TS + t];
+            if constexpr (t + 1 <= j - 1) v += Real(t + 1) * e[(j - 1) * TS + t + 1];
             if constexpr (t > 0) v += h * e[(j - 1) * TS + t - 1];
             e[j * TS + t] = v;
         });
@@ -203,8 +209,8 @@ MD_HD void hermite_e(double a, double b, double xab, double e00, double* e) {
             constexpr int j = decltype(jj)::value;
             static_for<0, i + j + 1>([&](auto tt) {
                 constexpr int t = decltype(tt)::value;
-                double v = xpa * e[((i - 1) * JS + j) * TS + t];
-                if constexpr (t + 1 <= i - 1 + j) v += (t + 1) * e[((i - 1) * JS + j) * TS + t + 1];
+                Real v = xpa * e[((i - 1) * JS + j) * TS + t];
+                if constexpr (t + 1 <= i - 1 + j) v += Real(t + 1) * e[((i - 1) * JS + j) * TS + t + 1];
                 if constexpr (t > 0) v += h * e[((i - 1) * JS + j) * TS + t - 1];
                 e[(i * JS + j) * TS + t] = v;
             });

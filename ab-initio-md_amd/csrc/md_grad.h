@@ -32,46 +32,46 @@ MD_HD constexpr int grad_ncol() {
     return best;
 }
 
-template <int LA, int LB, int LC, int LD, int NCOL, bool SLICE = false>
+template <int LA, int LB, int LC, int LD, int NCOL, bool SLICE = false, typename Real = double>
 MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
                      const double* __restrict__ tab, const double* Apos, const double* Bpos, int kc0,
-                     const double* gam, double* gA, double* gB) {
+                     const Real* gam, double* gA, double* gB) {
     using Q = Quartet<LA, LB, LC, LD>;
     constexpr int NAB = Q::NAB, NKC = Q::NCD, NHK = Q::NHK;
     constexpr int NHB1 = NH(Q::LAB + 1), L1 = Q::L + 1;
     constexpr int NA = Q::NA, NB = Q::NB;
     constexpr int IS = LA + 2, JS = LB + 2, TS = LA + LB + 3;
-    double xab[3];
-    for (int d = 0; d < 3; ++d) { gA[d] = 0.0; gB[d] = 0.0; xab[d] = Apos[d] - Bpos[d]; }
-    double fcart[NAB];
+    Real xab[3];
+    for (int d = 0; d < 3; ++d) { gA[d] = 0.0; gB[d] = 0.0; xab[d] = Real(Apos[d] - Bpos[d]); }
+    Real fcart[NAB];
     static_for<0, NAB>([&](auto aa) {
         constexpr int ab = decltype(aa)::value;
         constexpr int ia = ab / NB, ib = ab % NB;
-        fcart[ab] = cart_factor(cart_x(LA, ia), cart_y(LA, ia), cart_z(LA, ia)) *
-                    cart_factor(cart_x(LB, ib), cart_y(LB, ib), cart_z(LB, ib));
+        fcart[ab] = Real(cart_factor(cart_x(LA, ia), cart_y(LA, ia), cart_z(LA, ia)) *
+                         cart_factor(cart_x(LB, ib), cart_y(LB, ib), cart_z(LB, ib)));
     });
     for (int ibp = 0; ibp < nb; ++ibp) {
         const double* Bp = rb + ibp * Q::RSB;
-        const double p = Bp[R_P], a = Bp[R_A], b = Bp[R_B], cab = Bp[R_C];
-        const double px = Bp[R_PX], py = Bp[R_PY], pz = Bp[R_PZ];
-        double X[NHB1 * NCOL];
+        const Real p = Real(Bp[R_P]), a = Real(Bp[R_A]), b = Real(Bp[R_B]), cab = Real(Bp[R_C]);
+        const Real px = Real(Bp[R_PX]), py = Real(Bp[R_PY]), pz = Real(Bp[R_PZ]);
+        Real X[NHB1 * NCOL];
 #pragma unroll
-        for (int i = 0; i < NHB1 * NCOL; ++i) X[i] = 0.0;
+        for (int i = 0; i < NHB1 * NCOL; ++i) X[i] = Real(0);
         for (int ik = 0; ik < nk; ++ik) {
             const double* K = rk + ik * Q::RSK;
-            const double q = K[R_P];
-            const double s = p + q;
-            const double alpha = p * q / s;
-            const double pref = TWO_PI_2_5 / (p * q * sqrt(s));
-            double R[NH(L1)];
-            hermite_r<L1, SLICE>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
+            const Real q = Real(K[R_P]);
+            const Real s = p + q;
+            const Real alpha = p * q / s;
+            const Real pref = Real(TWO_PI_2_5) / (p * q * md_sqrt(s));
+            Real R[NH(L1)];
+            hermite_r<L1, SLICE, Real>(alpha, px - Real(K[R_PX]), py - Real(K[R_PY]), pz - Real(K[R_PZ]), pref, tab, R);
             static_for<0, NHK>([&](auto kk) {
                 constexpr int k = decltype(kk)::value;
                 constexpr int tk = HT.t[k], uk = HT.u[k], vk = HT.v[k];
                 constexpr bool odd = (tk + uk + vk) & 1;
 #pragma unroll
                 for (int j = 0; j < NCOL; ++j) {
-                    const double e = odd ? -K[REC_HDR + k * NKC + kc0 + j] : K[REC_HDR + k * NKC + kc0 + j];
+                    const Real e = odd ? -Real(K[REC_HDR + k * NKC + kc0 + j]) : Real(K[REC_HDR + k * NKC + kc0 + j]);
                     static_for<0, NHB1>([&](auto hh) {
                         constexpr int h = decltype(hh)::value;
                         X[h * NCOL + j] += R[hidx(HT.t[h] + tk, HT.u[h] + uk, HT.v[h] + vk)] * e;
@@ -79,39 +79,39 @@ MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __rest
                 }
             });
         }
-        const double mu = a * b / p;
-        double ex[IS * JS * TS], ey[IS * JS * TS], ez[IS * JS * TS];
-        hermite_e<LA + 1, LB + 1>(a, b, xab[0], exp(-mu * xab[0] * xab[0]), ex);
-        hermite_e<LA + 1, LB + 1>(a, b, xab[1], exp(-mu * xab[1] * xab[1]), ey);
-        hermite_e<LA + 1, LB + 1>(a, b, xab[2], exp(-mu * xab[2] * xab[2]), ez);
-        const double ta = 2.0 * a, tb = 2.0 * b;
-        double sA[3] = {0.0, 0.0, 0.0}, sB[3] = {0.0, 0.0, 0.0};
+        const Real mu = a * b / p;
+        Real ex[IS * JS * TS], ey[IS * JS * TS], ez[IS * JS * TS];
+        hermite_e<LA + 1, LB + 1, Real>(a, b, xab[0], md_exp(-mu * xab[0] * xab[0]), ex);
+        hermite_e<LA + 1, LB + 1, Real>(a, b, xab[1], md_exp(-mu * xab[1] * xab[1]), ey);
+        hermite_e<LA + 1, LB + 1, Real>(a, b, xab[2], md_exp(-mu * xab[2] * xab[2]), ez);
+        const Real ta = Real(2.0) * a, tb = Real(2.0) * b;
+        Real sA[3] = {0, 0, 0}, sB[3] = {0, 0, 0};
         static_for<0, NAB>([&](auto aa) {
             constexpr int ab = decltype(aa)::value;
             constexpr int ia = ab / NB, ib = ab % NB;
             constexpr int ax = cart_x(LA, ia), ay = cart_y(LA, ia), az = cart_z(LA, ia);
             constexpr int bx = cart_x(LB, ib), by = cart_y(LB, ib), bz = cart_z(LB, ib);
-            const double f = fcart[ab] * cab;
+            const Real f = fcart[ab] * cab;
             static_for<0, NHB1>([&](auto hh) {
                 constexpr int h = decltype(hh)::value;
                 constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
                 constexpr int ox = t > ax + bx, oy = u > ay + by, oz = v > az + bz;
                 if constexpr (ox + oy + oz <= 1 && t <= ax + bx + 1 && u <= ay + by + 1 && v <= az + bz + 1) {
-                    auto E = [](const double* e, int i, int j, int tt) { return e[(i * JS + j) * TS + tt]; };
-                    double xg = 0.0;
+                    auto E = [](const Real* e, int i, int j, int tt) { return e[(i * JS + j) * TS + tt]; };
+                    Real xg = 0;
 #pragma unroll
                     for (int j = 0; j < NCOL; ++j) xg += gam[ab * NCOL + j] * X[h * NCOL + j];
-                    const double y = f * xg;
-                    const double e0x = E(ex, ax, bx, t), e0y = E(ey, ay, by, u), e0z = E(ez, az, bz, v);
-                    double dax = ta * E(ex, ax + 1, bx, t), dbx = tb * E(ex, ax, bx + 1, t);
-                    double day = ta * E(ey, ay + 1, by, u), dby = tb * E(ey, ay, by + 1, u);
-                    double daz = ta * E(ez, az + 1, bz, v), dbz = tb * E(ez, az, bz + 1, v);
-                    if constexpr (ax > 0) dax -= ax * E(ex, ax - 1, bx, t);
-                    if constexpr (bx > 0) dbx -= bx * E(ex, ax, bx - 1, t);
-                    if constexpr (ay > 0) day -= ay * E(ey, ay - 1, by, u);
-                    if constexpr (by > 0) dby -= by * E(ey, ay, by - 1, u);
-                    if constexpr (az > 0) daz -= az * E(ez, az - 1, bz, v);
-                    if constexpr (bz > 0) dbz -= bz * E(ez, az, bz - 1, v);
+                    const Real y = f * xg;
+                    const Real e0x = E(ex, ax, bx, t), e0y = E(ey, ay, by, u), e0z = E(ez, az, bz, v);
+                    Real dax = ta * E(ex, ax + 1, bx, t), dbx = tb * E(ex, ax, bx + 1, t);
+                    Real day = ta * E(ey, ay + 1, by, u), dby = tb * E(ey, ay, by + 1, u);
+                    Real daz = ta * E(ez, az + 1, bz, v), dbz = tb * E(ez, az, bz + 1, v);
+                    if constexpr (ax > 0) dax -= Real(ax) * E(ex, ax - 1, bx, t);
+                    if constexpr (bx > 0) dbx -= Real(bx) * E(ex, ax, bx - 1, t);
+                    if constexpr (ay > 0) day -= Real(ay) * E(ey, ay - 1, by, u);
+                    if constexpr (by > 0) dby -= Real(by) * E(ey, ay, by - 1, u);
+                    if constexpr (az > 0) daz -= Real(az) * E(ez, az - 1, bz, v);
+                    if constexpr (bz > 0) dbz -= Real(bz) * E(ez, az, bz - 1, v);
                     sA[0] += y * dax * e0y * e0z;
                     sA[1] += y * e0x * day * e0z;
                     sA[2] += y * e0x * e0y * daz;
@@ -121,7 +121,7 @@ MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __rest
                 }
             });
         });
-        for (int d = 0; d < 3; ++d) { gA[d] += sA[d]; gB[d] += sB[d]; }
+        for (int d = 0; d < 3; ++d) { gA[d] += double(sA[d]); gB[d] += double(sB[d]); }
     }
 }
 

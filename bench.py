@@ -44,6 +44,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--jk-mode", choices=["auto", "direct", "store"], default="auto")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -75,7 +76,7 @@ def main() -> int:
     m = masses_au(sym)
     v0 = maxwell_boltzmann(m, 300.0, seed=1)
     be = get_backend("hf")(sym, basis="6-31g*", conv_tol=1e-10, conv_tol_grad=1e-7,
-                           scf_options={"jk_mode": args.jk_mode})
+                           scf_options={"jk_mode": args.jk_mode, "precision": args.precision})
     md = VelocityVerletNVE(be, m, x0, v0, timestep_fs=0.5)
     eng = be.solver.engine
     md.initialize()

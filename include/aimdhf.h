@@ -108,6 +108,17 @@ int aimdhf_jk(aimdhf_plan* plan, int nset, const double* dens, double* J, double
  */
 int aimdhf_set_jk_mode(aimdhf_plan* plan, int mode, int64_t max_store_bytes);
 
+/*
+ * Arithmetic precision. mode 0: FP64 everywhere (default). mode 1 (mixed):
+ * derivative integrals of quartets with Q_P Q_Q max|Gamma| < grad_switch are
+ * evaluated in FP32 (accumulated per quartet in FP32, summed in exact fixed
+ * point), and, in store mode, integrals of quartets with Q_P Q_Q < jk_switch
+ * are held and digested in FP32 (taking effect at the next set_geometry).
+ * Large-bound quartets, the SCF algebra and all sums stay FP64. No
+ * counterpart in the reference (FP64 only).
+ */
+int aimdhf_set_precision(aimdhf_plan* plan, int mode, double jk_switch, double grad_switch);
+
 /* Whether the current geometry has an integral store, its size in bytes, the
  * gather-list lengths of the K and J kernels and the number of integrals
  * (doubles) their gather-ordered data holds, i.e. what one J/K build streams

@@ -115,3 +115,23 @@ def test_water_631gs_nve_trajectory_matches_oracle(oracle_lib):
     assert np.abs(md.x - mo.x).max() < 1e-7
     assert abs(md.result.energy - mo.result.energy) < 1e-8
     assert dev <= devo * 1.01 + 1e-9
+
+
+def test_water16_mixed_precision_within_the_metric_bounds():
+    # mixed: FP32 for quartets below the switches; the metric's bounds are |dE| <= 1 uEh and
+    # force error <= 1e-5 Eh/bohr against the FP64 path, and results stay bitwise reproducible
+    s, x = cluster("water16.xyz")
+    r64 = _hf(s, "6-31g*").compute(x)
+    bem = _hf(s, "6-31g*", scf_options={"precision": "mixed"})
+    rm = bem.compute(x)
+    de, df = abs(rm.energy - r64.energy), np.abs(rm.gradient - r64.gradient).max()
+    print("(H2O)16 mixed vs fp64: dE", de, "max dF", df, flush=True)
+    assert de < 1e-6
+    assert df < 1e-5
+    guess = bem.last_scf.density.copy()
+    bem.set_density_guess(guess)
+    r2 = bem.compute(x)
+    bem.set_density_guess(guess)
+    r3 = bem.compute(x)
+    assert r2.energy == r3.energy
+    np.testing.assert_array_equal(r2.gradient, r3.gradient)
