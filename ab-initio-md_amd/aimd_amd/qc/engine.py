@@ -175,7 +175,9 @@ class Engine:
                      "eri_store", "gather_lists"]
                   + [f"jstore_{c}" for c in PAIR_CLASS_NAMES]
                   + [f"kstore_{'spd'[t // 3]}{'spd'[t % 3]}" for t in range(9)]
-                  + [f"grad_screen_{k}" for k in range(21)] + [f"unused_{k}" for k in range(85, 96)])
+                  + [f"grad_screen_{k}" for k in range(21)]
+                  + ["host_energy_list", "host_owner_lists", "host_store_build", "host_gradient_list"]
+                  + [f"unused_{k}" for k in range(89, 96)])
     NKIND = 96
     FOCK_KINDS = list(range(0, 15)) + [45, 46] + list(range(49, 64))    # per-iteration J/K work, either mode
     STORE_KINDS = [15, 47, 48]                                          # per-geometry store build

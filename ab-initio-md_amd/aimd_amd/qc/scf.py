@@ -195,11 +195,58 @@ def purified_projector(F: torch.Tensor, nocc: int, tol: float = 1e-11, max_iter:
     return None
 
 
+def inverse_sqrt_newton_schulz(S: torch.Tensor, max_iter: int = 60, tol: float = 1e-12,
+                               check_every: int = 6) -> tuple[torch.Tensor, float] | None:
+    """
+    S^{-1/2} of a symmetric positive definite S by the coupled Newton-Schulz
+    iteration (Higham, Functions of Matrices, eq. 6.35): Y0 = S/c, Z0 = 1,
+    T = (3 - Z Y)/2, Y <- Y T, Z <- T Z, with c a Gershgorin bound on the largest
+    eigenvalue; Z -> (S/c)^{-1/2}. Returns (S^{-1/2}, estimate of the smallest
+    eigenvalue of S from 30 power steps on S^{-1}: an upper bound, the Rayleigh
+    quotient never exceeds the largest eigenvalue of S^{-1}), or None if |X S X - 1| does not
+    fall below 1e-10 within max_iter steps.
+    """
+    n = S.shape[0]
+    eye = torch.eye(n, dtype=S.dtype, device=S.device)
+    c = S.abs().sum(dim=1).max()
+    Y = S / c
+    Z = eye.clone()
+    for it in range(1, max_iter + 1):
+        T = 0.5 * (3.0 * eye - Z @ Y)
+        Y = Y @ T
+        Z = T @ Z
+        if it % check_every == 0 and float((Z @ Y - eye).abs().max()) < tol:
+            break
+    X = Z / torch.sqrt(c)
+    X = 0.5 * (X + X.T)
+    if not float((X @ S @ X - eye).abs().max()) < 1e-10:
+        return None
+    X2 = X @ X
+    # fixed pseudo-random start (no symmetry can make it orthogonal to the smallest mode)
+    v = torch.cos(torch.arange(n, dtype=S.dtype, device=S.device) * 0.7548776662466927 + 0.3)
+    v = v / torch.linalg.vector_norm(v)
+    for _ in range(30):
+        v = X2 @ v
+        v = v / torch.linalg.vector_norm(v)
+    lam = float(v @ (X2 @ v))
+    return X, 1.0 / lam
+
+
 def orthogonalizer(S: torch.Tensor, method: str = "auto",
                    threshold: float = LINDEP_THRESHOLD) -> tuple[torch.Tensor, int, float]:
-    """X with X^T S X = 1, removed combinations, smallest eigenvalue (aimd/qc/scf.py:284-303)."""
+    """
+    X with X^T S X = 1, removed combinations, smallest eigenvalue (aimd/qc/scf.py:284-303).
+    The symmetric S^{-1/2} is formed by Newton-Schulz iteration (dense FP64 products)
+    when the smallest eigenvalue is clearly above the threshold (estimate > 1e3 x
+    threshold); otherwise, and for canonical orthogonalisation, from eigh as the
+    reference does, with its linear-dependence handling.
+    """
     if method not in _ORTHO:
         raise ValueError(f"unknown orthogonalization {method!r}; use one of {_ORTHO}")
+    if method in ("auto", "symmetric"):
+        ns = inverse_sqrt_newton_schulz(S)
+        if ns is not None and ns[1] > 1e3 * threshold:
+            return ns[0], 0, ns[1]
     s, U = torch.linalg.eigh(S)
     s_h = s.cpu().numpy()
     smin = float(s_h[0])
@@ -488,12 +535,8 @@ class SCFResult:
     energy: float
     electronic_energy: float
     nuclear_repulsion: float
-    mo_energy: np.ndarray
-    mo_coeff: np.ndarray
-    mo_occ: np.ndarray
     spin_densities: np.ndarray
     fock: np.ndarray
-    energy_weighted_density: np.ndarray
     mulliken_charges: np.ndarray
     dipole: np.ndarray
     s2: float
@@ -511,6 +554,32 @@ class SCFResult:
     integrals: SCFIntegrals | None = None
     device_spin_densities: torch.Tensor | None = None   # (nspin, nao, nao)
     device_fock: torch.Tensor | None = None             # (nspin, nao, nao)
+    # canonical orbitals (mo_energy, mo_coeff, mo_occ, energy_weighted_density):
+    # diagonalised on first access, the MD path never needs them
+    orbitals: Any = None
+
+    def _orbitals(self) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+        if callable(self.orbitals):
+            self.orbitals = self.orbitals()
+        return self.orbitals
+
+    @property
+    def mo_energy(self) -> np.ndarray:
+        return self._orbitals()[0]
+
+    @property
+    def mo_coeff(self) -> np.ndarray:
+        return self._orbitals()[1]
+
+    @property
+    def mo_occ(self) -> np.ndarray:
+        return self._orbitals()[2]
+
+    @property
+    def energy_weighted_density(self) -> np.ndarray:
+        """Canonical-orbital W = sum_s C_o eps_o C_o^T (aimd/qc/scf.py:935-941); the forces use
+        sum_s D_s F_s D_s instead (gradients.energy_weighted_density)."""
+        return self._orbitals()[3]
 
     @property
     def unrestricted(self) -> bool:
@@ -815,42 +884,50 @@ class SCFSolver:
         g = 2.0 / nspin
         nocc = self._nocc()
         X, S = I.X, I.S
-        eps_t, Cp = torch.linalg.eigh(X.T @ F @ X)
-        C_t = X @ Cp
-        W = torch.zeros((I.nao, I.nao), dtype=torch.float64, device=S.device)
-        for s in range(nspin):
-            Co = C_t[s][:, :nocc[s]]
-            W = W + g * (Co * eps_t[s, :nocc[s]]) @ Co.T
+        nmo = I.nmo
+
+        def orbitals():
+            eps_t, Cp = torch.linalg.eigh(X.T @ F @ X)
+            C_t = X @ Cp
+            W = torch.zeros((I.nao, I.nao), dtype=torch.float64, device=S.device)
+            for s in range(nspin):
+                Co = C_t[s][:, :nocc[s]]
+                W = W + g * (Co * eps_t[s, :nocc[s]]) @ Co.T
+            eps, C, Wh = eps_t.cpu().numpy(), C_t.cpu().numpy(), (0.5 * (W + W.T)).cpu().numpy()
+            occ = np.zeros((nspin, nmo))
+            for s in range(nspin):
+                occ[s, :nocc[s]] = g
+            if nspin == 1:
+                eps, C, occ = eps[0], C[0], occ[0]
+            return eps, C, occ, Wh
+
         spin_t = torch.stack([Ds[0], Ds[-1]])
         P = spin_t[0] + spin_t[1]
         PS_diag = torch.diagonal(P @ S)
         dip_el = -(I.M * P).sum(dim=(1, 2))
         s2_t = torch.sum((spin_t[0] @ S) * (spin_t[1] @ S).T) if nspin == 2 else None
-        host = [eps_t, C_t, spin_t, F, 0.5 * (W + W.T), PS_diag, dip_el]
+        host = [spin_t, F, PS_diag, dip_el]
         if s2_t is not None:
             host.append(s2_t)
         host = [t.cpu().numpy() for t in host]
-        eps, C, spin, Fh, Wh, psd, dipel = host[:7]
+        spin, Fh, psd, dipel = host[:4]
         q = basis.nuclear_charges.copy()
         np.subtract.at(q, basis.ao_atom, psd)
         dip = dipel + nuclear_dipole(basis.nuclear_charges, basis.positions)
         sz = 0.5 * (self.n_alpha - self.n_beta)
-        s2 = sz * (sz + 1.0) + self.n_beta - float(host[7]) if nspin == 2 else 0.0
-        occ = np.zeros((nspin, I.nmo))
-        for s in range(nspin):
-            occ[s, :nocc[s]] = g
+        s2 = sz * (sz + 1.0) + self.n_beta - float(host[4]) if nspin == 2 else 0.0
         if nspin == 1:
-            eps, C, occ, Fh = eps[0], C[0], occ[0], Fh[0]
+            Fh = Fh[0]
         e_el = st["e_el"]
         return SCFResult(
             reference=self.reference, converged=st["converged"], iterations=st["it"],
             energy=e_el + I.nuclear_repulsion, electronic_energy=e_el,
-            nuclear_repulsion=I.nuclear_repulsion, mo_energy=eps, mo_coeff=C, mo_occ=occ,
-            spin_densities=spin, fock=Fh, energy_weighted_density=Wh,
+            nuclear_repulsion=I.nuclear_repulsion,
+            spin_densities=spin, fock=Fh,
             mulliken_charges=q, dipole=dip, s2=float(s2), n_alpha=self.n_alpha,
             n_beta=self.n_beta, energy_change=float(st["de"]), commutator_norm=st["err"],
             basis=basis, n_removed=I.n_removed, guess=label, diis_restarts=st["restarts"],
-            history=st["history"], integrals=I, device_spin_densities=Ds, device_fock=F)
+            history=st["history"], integrals=I, device_spin_densities=Ds, device_fock=F, orbitals=orbitals)
 
 
 def run_scf(symbols: Sequence[str], positions: np.ndarray, basis: str | BasisSet = "sto-3g", charge: int = 0,

@@ -5,6 +5,7 @@
 #include "aimdhf_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -133,6 +134,13 @@ struct Profiler {
             pool.push_back(r.b);
         }
         recs.clear();
+    }
+    // host wall-clock phase (milliseconds since t0), recorded when enabled
+    using clock = std::chrono::steady_clock;
+    void host(int kind, clock::time_point t0) {
+        if (!on) return;
+        ms[kind] += std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+        n[kind] += 1;
     }
     void reset() {
         collect();
@@ -798,11 +806,15 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
             aimdhf::launch_schwarz(g, p->ncls.data(), st);
         });
         HIPCHECK(hipGetLastError());
+        auto t0 = Profiler::clock::now();
         p->build_list(thr, p->d_list, p->qcount, st);
         p->qtotal = 0;
         for (long long c : p->qcount) p->qtotal += c;
         p->threshold = thr;
+        p->prof.host(85, t0);
+        t0 = Profiler::clock::now();
         p->build_owner_lists(thr, st);
+        p->prof.host(86, t0);
         p->stored = false;
         p->store_bytes = p->k_entries = p->j_entries = 0;
         if (p->jk_mode != 1 && p->nao < (1 << 14)) {
@@ -813,7 +825,9 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
                 limit = (long long)(fr / 2);
             }
             if (p->jk_mode == 2 || p->store_estimate() <= limit) {
+                auto t1 = Profiler::clock::now();
                 p->build_store(st);
+                p->prof.host(87, t1);
                 p->stored = true;
             }
         }
@@ -1068,7 +1082,9 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
         if (mc < 0.0 || mx < 0.0) return fail(AIMDHF_E_HIP, "max reduction failed");
         double gbound = 4.0 * (mc * mc + std::fabs(kfac) * nx * mx * mx);
         double lthr = gbound > 0.0 ? thr / gbound : 0.0;
+        auto t0 = Profiler::clock::now();
         p->build_list(lthr, p->d_glist, p->gcount, st);
+        p->prof.host(88, t0);
         p->d_gfx.ensure(sizeof(long long) * 6 * p->natm);
         aimdhf::launch_zero64(p->d_gfx.as<long long>(), 6 * p->natm, st);
         aimdhf::GradArgs a;

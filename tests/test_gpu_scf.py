@@ -89,3 +89,24 @@ def test_density_solvers_agree(mol, basis, mult):
     assert r1.converged and r2.converged
     assert abs(r1.energy - r2.energy) < 1e-10
     assert np.abs(r1.gradient - r2.gradient).max() < 1e-8
+
+
+@pytest.mark.parametrize("mol,basis,mult", [(WATER, "6-31g*", 1), (METHYL, "6-31g*", 2)])
+def test_canonical_orbitals_on_demand(mol, basis, mult):
+    # mo_coeff / mo_energy are diagonalised on first access: C^T S C = 1, ascending energies,
+    # and the occupied orbitals rebuild the converged densities
+    s, x = bohr(mol, 0.05, 11)
+    print(f"case {s} {basis} {mult}", flush=True)
+    be = _hf(s, basis, multiplicity=mult)
+    be.compute(x)
+    res = be.last_scf
+    S = res.integrals.S.cpu().numpy()
+    C = np.atleast_3d(res.mo_coeff) if res.unrestricted else res.mo_coeff[None]
+    eps = res.mo_energy if res.unrestricted else res.mo_energy[None]
+    occ = res.mo_occ if res.unrestricted else res.mo_occ[None]
+    for sp in range(C.shape[0]):
+        np.testing.assert_allclose(C[sp].T @ S @ C[sp], np.eye(C.shape[-1]), atol=1e-10)
+        assert np.all(np.diff(eps[sp]) >= -1e-12)
+        n = int(round(occ[sp].sum() / (1.0 if res.unrestricted else 2.0)))
+        Co = C[sp][:, :n]
+        np.testing.assert_allclose(Co @ Co.T, res.spin_densities[sp], atol=1e-8)

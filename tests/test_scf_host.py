@@ -82,6 +82,39 @@ def test_purified_projector_reports_a_missing_gap():
     assert S.purified_projector(F, 2, max_iter=40) is None
 
 
+def test_newton_schulz_inverse_square_root_matches_eigh():
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((40, 40))
+    S = A @ A.T / 40 + 0.05 * np.eye(40)
+    w, U = np.linalg.eigh(S)
+    ref = (U / np.sqrt(w)) @ U.T
+    out = S_module().inverse_sqrt_newton_schulz(torch.tensor(S))
+    assert out is not None
+    X, smin = out
+    np.testing.assert_allclose(X.numpy(), ref, atol=1e-11)
+    assert w[0] * (1 - 1e-12) <= smin <= 1.5 * w[0]      # power-iteration estimate: an upper bound
+    X, n_removed, smin2 = S_module().orthogonalizer(torch.tensor(S))
+    np.testing.assert_allclose(X.numpy(), ref, atol=1e-11)
+    assert n_removed == 0
+
+
+def test_orthogonalizer_falls_back_to_eigh_near_linear_dependence():
+    # smallest eigenvalue 1e-9: below the threshold, symmetric orthogonalisation is refused
+    # ("auto" switches to canonical and removes one combination), as the reference does
+    rng = np.random.default_rng(6)
+    Q, _ = np.linalg.qr(rng.standard_normal((12, 12)))
+    w = np.linspace(1e-9, 2.0, 12)
+    S = (Q * w) @ Q.T
+    X, n_removed, smin = S_module().orthogonalizer(torch.tensor(S), "auto", 1e-7)
+    assert n_removed == 1 and X.shape == (12, 11)
+    with pytest.raises(ValueError):
+        S_module().orthogonalizer(torch.tensor(S), "symmetric", 1e-7)
+
+
+def S_module():
+    return S
+
+
 @pytest.mark.parametrize("bad", [dict(max_iter=0), dict(conv_energy=0.0), dict(guess="minao"), dict(diis_space=1),
                                  dict(damping=1.0), dict(level_shift=-0.1), dict(orthogonalization="lowdin"),
                                  dict(blas_threads=0), dict(jk_mode="tensor"), dict(max_store_bytes=-1),
