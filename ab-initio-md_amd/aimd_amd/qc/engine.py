@@ -120,13 +120,13 @@ class Engine:
 
     def jk_info(self) -> dict:
         """Store state of the current geometry: stored, bytes, K and J gather-list entries and
-        the integrals one J/K build streams (gather_doubles)."""
+        the bytes of integral data one J/K build streams (gather_bytes)."""
         st = ctypes.c_int()
         b, ke, je, gd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         _lib.check(_lib.lib().aimdhf_jk_info(self._h, ctypes.byref(st), ctypes.byref(b), ctypes.byref(ke),
                                              ctypes.byref(je), ctypes.byref(gd)))
         return {"stored": bool(st.value), "store_bytes": int(b.value), "k_entries": int(ke.value),
-                "j_entries": int(je.value), "gather_doubles": int(gd.value)}
+                "j_entries": int(je.value), "gather_bytes": int(gd.value)}
 
     def jk(self, dens: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
         """J, K of (nset, nao, nao) (or (nao, nao)) densities; symmetric part of each D is used."""

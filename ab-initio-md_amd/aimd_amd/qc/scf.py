@@ -78,7 +78,7 @@ class SCFOptions:
     max_store_bytes: int = 0
     density_solver: str = "purify"
     precision: str = "fp64"
-    jk_fp32_switch: float = 1e-9
+    jk_fp32_switch: float = 1e-7
     grad_fp32_switch: float = 1e-9
 
     def __post_init__(self) -> None:

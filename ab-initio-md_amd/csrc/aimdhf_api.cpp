@@ -198,8 +198,8 @@ struct aimdhf_plan {
     int precision = 0;
     double jk_switch = 0.0, grad_switch = 0.0;
     DevBuf d_store, d_qmap, d_kcnt, d_jcnt, d_kseg, d_jseg, d_klist, d_jlist, d_lerr;
-    DevBuf d_kdata, d_jdata, d_kdoff, d_jdoff;
-    long long kdata_total = 0, jdata_total = 0;
+    DevBuf d_kdata, d_jdata, d_kdoff, d_jdoff, d_kdata32, d_jdata32, d_kdoff32, d_jdoff32, d_kbig, d_jbig;
+    long long kdata_total = 0, jdata_total = 0, kdata32_total = 0, jdata32_total = 0;
     // work units of the store kernels (lane items per unit: UNIT_ITEMS)
     static constexpr int UNIT_ITEMS = 512;
     int kunit_base[10] = {0}, junit_base[7] = {0};
@@ -214,7 +214,8 @@ struct aimdhf_plan {
                           &d_ket_q, &d_kpre, &d_krec, &d_kpair, &d_part_off, &d_part_shell, &d_part_pair,
                           &d_part_q, &d_dh, &d_jpart, &d_kpart, &d_ket_pair, &d_store, &d_qmap, &d_kcnt,
                           &d_jcnt, &d_kseg, &d_jseg, &d_klist, &d_jlist, &d_lerr, &d_gkeep, &d_gsel, &d_nsel, &d_gblk,
-                          &d_kdata, &d_jdata, &d_kdoff, &d_jdoff, &d_kunit_os, &d_kunit_e, &d_out_unit,
+                          &d_kdata, &d_jdata, &d_kdoff, &d_jdoff, &d_kdata32, &d_jdata32, &d_kdoff32, &d_jdoff32,
+                          &d_kbig, &d_jbig, &d_kunit_os, &d_kunit_e, &d_out_unit,
                           &d_kpart2, &d_junit_pc, &d_junit_e, &d_pair_unit, &d_jpart2})
             b->release();
     }
@@ -385,6 +386,9 @@ struct aimdhf_plan {
         for (int k = 0; k < NQCLASS; ++k) ka.store_base[k] = store_base[k];
         ka.kdata = d_kdata.as<double>();
         ka.kdoff = d_kdoff.as<long long>();
+        ka.kdata32 = d_kdata32.as<float>();
+        ka.kdoff32 = d_kdoff32.as<long long>();
+        ka.kbig = d_kbig.as<long long>();
         for (int t = 0; t < 10; ++t) ka.unit_base[t] = kunit_base[t];
         ka.unit_os = d_kunit_os.as<int>();
         ka.unit_e = d_kunit_e.as<long long>();
@@ -410,6 +414,9 @@ struct aimdhf_plan {
         for (int k = 0; k < NQCLASS; ++k) ja.store_base[k] = store_base[k];
         ja.jdata = d_jdata.as<double>();
         ja.jdoff = d_jdoff.as<long long>();
+        ja.jdata32 = d_jdata32.as<float>();
+        ja.jdoff32 = d_jdoff32.as<long long>();
+        ja.jbig = d_jbig.as<long long>();
         for (int c = 0; c < 7; ++c) ja.unit_base[c] = junit_base[c];
         ja.unit_pc = d_junit_pc.as<int>();
         ja.unit_e = d_junit_e.as<long long>();
@@ -452,6 +459,7 @@ struct aimdhf_plan {
         aimdhf::ListArgs la;
         la.npair = npair;
         la.thr = threshold;
+        la.sw = (precision == 1 && jk_switch > threshold) ? jk_switch : threshold;
         la.out_x = d_out_x.as<int>();
         la.out_y = d_out_y.as<int>();
         la.shell_l = d_shell_l.as<int>();
@@ -471,17 +479,24 @@ struct aimdhf_plan {
         for (int k = 0; k < NQCLASS; ++k) la.qcount[k] = qcount[k];
         la.err = d_lerr.as<int>();
         const int nout = out_off[9];
-        d_kcnt.ensure(sizeof(long long) * (size_t)nout * 9);
-        d_jcnt.ensure(sizeof(long long) * (size_t)npair * 6);
-        std::vector<long long> kc((size_t)nout * 9), jc((size_t)npair * 6);
+        d_kcnt.ensure(sizeof(long long) * 2 * (size_t)nout * 9);
+        d_jcnt.ensure(sizeof(long long) * 2 * (size_t)npair * 6);
+        std::vector<long long> kc((size_t)nout * 9), jc((size_t)npair * 6), kb(kc.size()), jb(jc.size());
         prof.time(48, st, [&] {
-            aimdhf::launch_klist_count(la, nout, d_kcnt.as<long long>(), st);
-            aimdhf::launch_jlist_count(la, d_jcnt.as<long long>(), st);
+            aimdhf::launch_klist_count(la, nout, d_kcnt.as<long long>(), d_kcnt.as<long long>() + kc.size(), st);
+            aimdhf::launch_jlist_count(la, d_jcnt.as<long long>(), d_jcnt.as<long long>() + jc.size(), st);
         });
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(kc.data(), d_kcnt.p, sizeof(long long) * kc.size(), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(kb.data(), d_kcnt.as<long long>() + kc.size(), sizeof(long long) * kb.size(),
+                                hipMemcpyDeviceToHost, st));
         HIPCHECK(hipMemcpyAsync(jc.data(), d_jcnt.p, sizeof(long long) * jc.size(), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(jb.data(), d_jcnt.as<long long>() + jc.size(), sizeof(long long) * jb.size(),
+                                hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
+        upload(d_kbig, kb);
+        upload(d_jbig, jb);
+        la.kbig = d_kbig.as<long long>();
         kseg.assign(kc.size() + 1, 0);
         for (size_t i = 0; i < kc.size(); ++i) kseg[i + 1] = kseg[i] + kc[i];
         jseg.assign(jc.size() + 1, 0);
@@ -510,25 +525,34 @@ struct aimdhf_plan {
         }
         // gather-ordered copies: segment offsets, then one copy kernel per type / class
         static const int ncp[6] = {1, 3, 9, 6, 18, 36};
-        std::vector<long long> kdoff((size_t)nout * 9), jdoff((size_t)npair * 6);
-        long long ko = 0;
+        std::vector<long long> kdoff((size_t)nout * 9), jdoff((size_t)npair * 6), kdoff32(kdoff.size()),
+            jdoff32(jdoff.size());
+        long long ko = 0, ko32 = 0;
         for (int o = 0; o < nout; ++o) {
             const int nx = ncart(shell_l[out_x[o]]), ny = ncart(shell_l[out_y[o]]);
             for (int sub = 0; sub < 9; ++sub) {
                 const size_t i = (size_t)o * 9 + sub;
+                const long long blk = (long long)nx * ncart(sub / 3) * ny * ncart(sub % 3);
                 kdoff[i] = ko;
-                ko += (kseg[i + 1] - kseg[i]) * nx * ncart(sub / 3) * ny * ncart(sub % 3);
+                kdoff32[i] = ko32;
+                ko += kb[i] * blk;
+                ko32 += (kseg[i + 1] - kseg[i] - kb[i]) * blk;
             }
         }
-        long long jo = 0;
+        long long jo = 0, jo32 = 0;
         for (int P = 0; P < npair; ++P)
             for (int c = 0; c < 6; ++c) {
                 const size_t i = (size_t)P * 6 + c;
+                const long long blk = (long long)ncp[pair_cls[P]] * ncp[c];
                 jdoff[i] = jo;
-                jo += (jseg[i + 1] - jseg[i]) * ncp[pair_cls[P]] * ncp[c];
+                jdoff32[i] = jo32;
+                jo += jb[i] * blk;
+                jo32 += (jseg[i + 1] - jseg[i] - jb[i]) * blk;
             }
         kdata_total = ko;
         jdata_total = jo;
+        kdata32_total = ko32;
+        jdata32_total = jo32;
         // work units: each segment cut into pieces of at most UNIT_ITEMS lane items
         {
             std::vector<int> uos, out_unit(nout + 1);
@@ -541,10 +565,13 @@ struct aimdhf_plan {
                     const int blk = nx * ncart(sub / 3) * ny * ncart(sub % 3);
                     const int ns = blk > 36 ? ny * ncart(sub % 3) : 1;
                     const long long per = std::max(1, UNIT_ITEMS / ns);
-                    for (long long e = kseg[i]; e < kseg[i + 1]; e += per) {
-                        uos.push_back(o << 4 | sub);
-                        ue.push_back(e);
-                        ue.push_back(std::min(e + per, kseg[i + 1]));
+                    for (int f32 = 0; f32 < 2; ++f32) {
+                        const long long e0 = f32 ? kseg[i] + kb[i] : kseg[i], e1 = f32 ? kseg[i + 1] : kseg[i] + kb[i];
+                        for (long long e = e0; e < e1; e += per) {
+                            uos.push_back(o << 5 | f32 << 4 | sub);
+                            ue.push_back(e);
+                            ue.push_back(std::min(e + per, e1));
+                        }
                     }
                 }
             }
@@ -567,10 +594,13 @@ struct aimdhf_plan {
                         const int blk = ncp[c] * ncp[cq];
                         const int ns = blk > 36 ? ncp[cq] : 1;
                         const long long per = std::max(1, UNIT_ITEMS / ns);
-                        for (long long e = jseg[i]; e < jseg[i + 1]; e += per) {
-                            upc.push_back(P << 3 | cq);
-                            ue.push_back(e);
-                            ue.push_back(std::min(e + per, jseg[i + 1]));
+                        for (int f32 = 0; f32 < 2; ++f32) {
+                            const long long e0 = f32 ? jseg[i] + jb[i] : jseg[i], e1 = f32 ? jseg[i + 1] : jseg[i] + jb[i];
+                            for (long long e = e0; e < e1; e += per) {
+                                upc.push_back(P << 4 | f32 << 3 | cq);
+                                ue.push_back(e);
+                                ue.push_back(std::min(e + per, e1));
+                            }
                         }
                     }
                 }
@@ -583,8 +613,12 @@ struct aimdhf_plan {
         }
         upload(d_kdoff, kdoff);
         upload(d_jdoff, jdoff);
+        upload(d_kdoff32, kdoff32);
+        upload(d_jdoff32, jdoff32);
         d_kdata.ensure(sizeof(double) * std::max<long long>(ko, 1));
         d_jdata.ensure(sizeof(double) * std::max<long long>(jo, 1));
+        d_kdata32.ensure(sizeof(float) * std::max<long long>(ko32, 1));
+        d_jdata32.ensure(sizeof(float) * std::max<long long>(jo32, 1));
         prof.time(15, st, [&] {
             aimdhf::KSArgs ka = ks_args(1, nullptr, nullptr);
             for (int t = 0; t < 9; ++t) {
@@ -599,7 +633,8 @@ struct aimdhf_plan {
             }
         });
         HIPCHECK(hipGetLastError());
-        store_bytes = 8 * (off + ko + jo) + 4 * keys + 8 * (k_entries + j_entries + kdoff.size() + jdoff.size());
+        store_bytes = 8 * (off + ko + jo) + 4 * (ko32 + jo32) + 4 * keys +
+                      8 * (k_entries + j_entries + 2 * kdoff.size() + 2 * jdoff.size());
     }
 
     aimdhf::OneElArgs oneel_args() const {
@@ -1156,11 +1191,12 @@ int aimdhf_set_precision(aimdhf_plan* p, int mode, double jk_switch, double grad
 }
 
 int aimdhf_jk_info(aimdhf_plan* p, int* stored, int64_t* store_bytes, int64_t* k_entries, int64_t* j_entries,
-                   int64_t* gather_doubles) {
+                   int64_t* gather_bytes) {
     if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     if (stored) *stored = p->stored ? 1 : 0;
-    if (gather_doubles) *gather_doubles = p->stored ? p->kdata_total + p->jdata_total : 0;
+    if (gather_bytes)
+        *gather_bytes = p->stored ? 8 * (p->kdata_total + p->jdata_total) + 4 * (p->kdata32_total + p->jdata32_total) : 0;
     if (store_bytes) *store_bytes = p->store_bytes;
     if (k_entries) *k_entries = p->k_entries;
     if (j_entries) *j_entries = p->j_entries;

@@ -177,6 +177,7 @@ struct StoreArgs {
 struct ListArgs {
     int npair;
     double thr;
+    double sw;                  // FP64 / FP32 split (>= thr; = thr: everything FP64)
     const int* out_x;
     const int* out_y;
     const int* shell_l;
@@ -196,6 +197,7 @@ struct ListArgs {
     long long qcount[21];       // quartets per class (bounds of the map values)
     int* err;                   // set to 1 if a map value is out of its class range
     const long long* kseg;      // nout * 9 + 1
+    const long long* kbig;      // entries with q q >= sw per segment (they come first)
     long long* klist;
     const long long* jseg;      // npair * 6 + 1
     long long* jlist;
@@ -212,10 +214,13 @@ struct KSArgs {
     const long long* klist;
     const double* store;
     long long store_base[21];
-    double* kdata;              // gather-ordered, reoriented copies (per output segment)
-    const long long* kdoff;     // nout * 9 start of each segment's copies
+    double* kdata;              // gather-ordered, reoriented copies (per output segment), FP64 part
+    const long long* kdoff;     // nout * 9 start of each segment's FP64 copies
+    float* kdata32;             // FP32 part (entries below the switch)
+    const long long* kdoff32;   // nout * 9 start of each segment's FP32 copies
+    const long long* kbig;      // FP64 entries per segment (they come first)
     int unit_base[10];          // work units per output type
-    const int* unit_os;         // per unit: output << 4 | subclass
+    const int* unit_os;         // per unit: output << 5 | fp32 << 4 | subclass
     const long long* unit_e;    // per unit: entry range [e0, e1)
     const int* out_unit;        // per output: unit range (nout + 1)
     double* kpart;              // per unit partial block (KPART_STRIDE doubles)
@@ -235,10 +240,13 @@ struct JSArgs {
     const long long* jlist;
     const double* store;
     long long store_base[21];
-    double* jdata;              // gather-ordered, oriented copies (per pair segment)
-    const long long* jdoff;     // npair * 6 start of each segment's copies
+    double* jdata;              // gather-ordered, oriented copies (per pair segment), FP64 part
+    const long long* jdoff;     // npair * 6 start of each segment's FP64 copies
+    float* jdata32;             // FP32 part
+    const long long* jdoff32;   // npair * 6 start of each segment's FP32 copies
+    const long long* jbig;      // FP64 entries per segment (they come first)
     int unit_base[7];           // work units per bra pair class
-    const int* unit_pc;         // per unit: pair << 3 | partner class
+    const int* unit_pc;         // per unit: pair << 4 | fp32 << 3 | partner class
     const long long* unit_e;    // per unit: entry range [e0, e1)
     const int* pair_unit;       // per cls_pairs position: unit range (npair + 1)
     double* jpart;              // per unit partial block (KPART_STRIDE doubles)
@@ -248,9 +256,9 @@ struct JSArgs {
 
 void launch_eri_store(int k, const StoreArgs& a, const int2* list, long long nq, hipStream_t st);
 long long store_block(int k);
-void launch_klist_count(const ListArgs& a, int nout, long long* counts, hipStream_t st);
+void launch_klist_count(const ListArgs& a, int nout, long long* counts, long long* big, hipStream_t st);
 void launch_klist_fill(const ListArgs& a, int nout, hipStream_t st);
-void launch_jlist_count(const ListArgs& a, long long* counts, hipStream_t st);
+void launch_jlist_count(const ListArgs& a, long long* counts, long long* big, hipStream_t st);
 void launch_jlist_fill(const ListArgs& a, hipStream_t st);
 void launch_kstore(int type, const KSArgs& a, int nout, hipStream_t st);
 void launch_jstore(int pclass, const JSArgs& a, int npairs, hipStream_t st);

@@ -53,7 +53,7 @@ template <int CP_>
 __global__ __launch_bounds__(64) void k_jcopy(JSArgs a) {
     const int u = a.unit_base[CP] + blockIdx.x;
     const int code = a.unit_pc[u];
-    const int P = code >> 3, cq = code & 7;
+    const int P = code >> 4, f32 = (code >> 3) & 1, cq = code & 7;
     const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
     const int lane = threadIdx.x;
     with_part(cq, [&](auto pt) {
@@ -61,7 +61,8 @@ __global__ __launch_bounds__(64) void k_jcopy(JSArgs a) {
         constexpr int NCD = PT::NCD, BLK = PT::BLK;
         const size_t seg = (size_t)P * 6 + cq;
         const double* base = a.store + a.store_base[PT::K];
-        double* out = a.jdata + a.jdoff[seg] + (u0 - a.jseg[seg]) * BLK;
+        double* out64 = a.jdata + a.jdoff[seg] + (u0 - a.jseg[seg]) * BLK;
+        float* out32 = a.jdata32 + a.jdoff32[seg] + (u0 - a.jseg[seg] - a.jbig[seg]) * BLK;
         const long long nt = (u1 - u0) * BLK;
         for (long long t = lane; t < nt; t += 64) {
             const long long e = t / BLK;
@@ -70,72 +71,89 @@ __global__ __launch_bounds__(64) void k_jcopy(JSArgs a) {
             const long long qi = (long long)(ent & 0xffffffffull);
             const bool sw = (ent >> 60) & 1;
             const int cd = el / NAB, ab = el % NAB;
-            out[t] = base[qi * BLK + (sw ? cd * NAB + ab : ab * NCD + cd)];
+            const double v = base[qi * BLK + (sw ? cd * NAB + ab : ab * NCD + cd)];
+            if (f32) out32[t] = (float)v;
+            else out64[t] = v;
         }
     });
+}
+
+// one work unit of partner class PT in storage / arithmetic precision Real
+template <typename Real, class PT, int NSET>
+__device__ __forceinline__ void j_unit(const JSArgs& a, const Real* __restrict__ blk0, long long u0, long long u1,
+                                       int lane, double* Jacc) {
+    constexpr int NDc = PT::NDc, NCD = PT::NCD, BLK = PT::BLK;
+    const size_t n = a.nao, nn = n * n;
+    Real acc[NSET * NAB];
+#pragma unroll
+    for (int i = 0; i < NSET * NAB; ++i) acc[i] = Real(0);
+    if constexpr (PT::SPLIT) {
+        // lanes over (entry, cd) columns of NAB contiguous values
+        const long long nt = (u1 - u0) * NCD;
+        for (long long t = lane; t < nt; t += 64) {
+            const long long e = t / NCD;
+            const int cd = (int)(t % NCD);
+            const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
+            const int oC = (int)((ent >> 32) & 0x3fff), oD = (int)((ent >> 46) & 0x3fff);
+            const Real w = ((ent >> 61) & 1) ? Real(2) : Real(1);
+            const Real* col = blk0 + e * BLK + cd * NAB;
+            Real v[NAB];
+#pragma unroll
+            for (int i = 0; i < NAB; ++i) v[i] = col[i];
+#pragma unroll
+            for (int s = 0; s < NSET; ++s) {
+                const Real dv = w * Real(a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc]);
+#pragma unroll
+                for (int ab = 0; ab < NAB; ++ab) acc[s * NAB + ab] += v[ab] * dv;
+            }
+        }
+    } else {
+        for (long long e = lane; e < u1 - u0; e += 64) {
+            const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
+            const int oC = (int)((ent >> 32) & 0x3fff), oD = (int)((ent >> 46) & 0x3fff);
+            const Real w = ((ent >> 61) & 1) ? Real(2) : Real(1);
+            const Real* b = blk0 + e * BLK;
+            Real v[BLK];
+#pragma unroll
+            for (int i = 0; i < BLK; ++i) v[i] = b[i];
+#pragma unroll
+            for (int s = 0; s < NSET; ++s) {
+                Real dq[NCD];
+#pragma unroll
+                for (int cd = 0; cd < NCD; ++cd)
+                    dq[cd] = w * Real(a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc]);
+#pragma unroll
+                for (int ab = 0; ab < NAB; ++ab) {
+                    Real t2 = 0;
+#pragma unroll
+                    for (int cd = 0; cd < NCD; ++cd) t2 += v[cd * NAB + ab] * dq[cd];
+                    acc[s * NAB + ab] += t2;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NSET * NAB; ++i) Jacc[i] += double(acc[i]);
 }
 
 template <int CP_, int NSET>
 __global__ __launch_bounds__(64) void k_jstore(JSArgs a) {
     const int u = a.unit_base[CP] + blockIdx.x;
     const int code = a.unit_pc[u];
-    const int P = code >> 3, cq = code & 7;
+    const int P = code >> 4, f32 = (code >> 3) & 1, cq = code & 7;
     const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
     const int lane = threadIdx.x;
-    const size_t n = a.nao, nn = n * n;
     double Jacc[NSET * NAB];
 #pragma unroll
     for (int i = 0; i < NSET * NAB; ++i) Jacc[i] = 0.0;
     with_part(cq, [&](auto pt) {
         using PT = decltype(pt);
-        constexpr int NDc = PT::NDc, NCD = PT::NCD, BLK = PT::BLK;
         const size_t seg = (size_t)P * 6 + cq;
-        const double* blk0 = a.jdata + a.jdoff[seg] + (u0 - a.jseg[seg]) * BLK;
-        if constexpr (PT::SPLIT) {
-            // lanes over (entry, cd) columns of NAB contiguous values
-            const long long nt = (u1 - u0) * NCD;
-            for (long long t = lane; t < nt; t += 64) {
-                const long long e = t / NCD;
-                const int cd = (int)(t % NCD);
-                const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
-                const int oC = (int)((ent >> 32) & 0x3fff), oD = (int)((ent >> 46) & 0x3fff);
-                const double w = ((ent >> 61) & 1) ? 2.0 : 1.0;
-                const double* col = blk0 + e * BLK + cd * NAB;
-                double v[NAB];
-#pragma unroll
-                for (int i = 0; i < NAB; ++i) v[i] = col[i];
-#pragma unroll
-                for (int s = 0; s < NSET; ++s) {
-                    const double dv = w * a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc];
-#pragma unroll
-                    for (int ab = 0; ab < NAB; ++ab) Jacc[s * NAB + ab] += v[ab] * dv;
-                }
-            }
-        } else {
-            for (long long e = lane; e < u1 - u0; e += 64) {
-                const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
-                const int oC = (int)((ent >> 32) & 0x3fff), oD = (int)((ent >> 46) & 0x3fff);
-                const double w = ((ent >> 61) & 1) ? 2.0 : 1.0;
-                const double* b = blk0 + e * BLK;
-                double v[BLK];
-#pragma unroll
-                for (int i = 0; i < BLK; ++i) v[i] = b[i];
-#pragma unroll
-                for (int s = 0; s < NSET; ++s) {
-                    double dq[NCD];
-#pragma unroll
-                    for (int cd = 0; cd < NCD; ++cd)
-                        dq[cd] = w * a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc];
-#pragma unroll
-                    for (int ab = 0; ab < NAB; ++ab) {
-                        double t2 = 0.0;
-#pragma unroll
-                        for (int cd = 0; cd < NCD; ++cd) t2 += v[cd * NAB + ab] * dq[cd];
-                        Jacc[s * NAB + ab] += t2;
-                    }
-                }
-            }
-        }
+        if (f32)
+            j_unit<float, PT, NSET>(a, a.jdata32 + a.jdoff32[seg] + (u0 - a.jseg[seg] - a.jbig[seg]) * PT::BLK, u0, u1,
+                                    lane, Jacc);
+        else
+            j_unit<double, PT, NSET>(a, a.jdata + a.jdoff[seg] + (u0 - a.jseg[seg]) * PT::BLK, u0, u1, lane, Jacc);
     });
     double* part = a.jpart + (size_t)u * KPART_STRIDE;
 #pragma unroll

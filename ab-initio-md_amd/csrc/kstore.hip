@@ -56,7 +56,7 @@ template <int LX_, int LY_>
 __global__ __launch_bounds__(64) void k_kcopy(KSArgs a) {
     const int u = a.unit_base[OT] + blockIdx.x;
     const int code = a.unit_os[u];
-    const int o = code >> 4, sub = code & 15;
+    const int o = code >> 5, f32 = (code >> 4) & 1, sub = code & 15;
     const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
     const int lane = threadIdx.x;
     with_sub(sub, [&](auto sb) {
@@ -64,7 +64,8 @@ __global__ __launch_bounds__(64) void k_kcopy(KSArgs a) {
         constexpr int NJ = SB::NJ, NL = SB::NL, NXJ = SB::NXJ, NYL = SB::NYL, BLK = SB::BLK;
         const size_t seg = (size_t)o * 9 + sub;
         const double* base = a.store + a.store_base[SB::K];
-        double* out = a.kdata + a.kdoff[seg] + (u0 - a.kseg[seg]) * BLK;
+        double* out64 = a.kdata + a.kdoff[seg] + (u0 - a.kseg[seg]) * BLK;
+        float* out32 = a.kdata32 + a.kdoff32[seg] + (u0 - a.kseg[seg] - a.kbig[seg]) * BLK;
         const long long nt = (u1 - u0) * BLK;
         for (long long t = lane; t < nt; t += 64) {
             const long long e = t / BLK;
@@ -76,85 +77,103 @@ __global__ __launch_bounds__(64) void k_kcopy(KSArgs a) {
             const int x = xj / NJ, j = xj % NJ, y = yl / NL, l = yl % NL;
             const int i1 = fb ? j * NX + x : x * NJ + j;
             const int i2 = fk ? l * NY + y : y * NL + l;
-            out[t] = base[qi * BLK + (sw ? i2 * NXJ + i1 : i1 * NYL + i2)];
+            const double v = base[qi * BLK + (sw ? i2 * NXJ + i1 : i1 * NYL + i2)];
+            if (f32) out32[t] = (float)v;
+            else out64[t] = v;
         }
     });
+}
+
+// one work unit of subclass SB in storage / arithmetic precision Real; its
+// per-lane sums are added to Kacc in FP64
+template <typename Real, class SB, int NSET>
+__device__ __forceinline__ void k_unit(const KSArgs& a, const Real* __restrict__ blk0, long long u0, long long u1,
+                                       int lane, double* Kacc) {
+    constexpr int NJ = SB::NJ, NL = SB::NL, NXJ = SB::NXJ, NYL = SB::NYL, BLK = SB::BLK;
+    const size_t n = a.nao, nn = n * n;
+    Real acc[NSET * NX * NY];
+#pragma unroll
+    for (int i = 0; i < NSET * NX * NY; ++i) acc[i] = Real(0);
+    if constexpr (SB::SPLIT) {
+        // lanes over (entry, yl) columns of NXJ contiguous values
+        const long long nt = (u1 - u0) * NYL;
+        for (long long t = lane; t < nt; t += 64) {
+            const long long e = t / NYL;
+            const int yl = (int)(t % NYL);
+            const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
+            const int oJ = (int)((ent >> 32) & 0x3fff), oL = (int)((ent >> 46) & 0x3fff);
+            const Real* col = blk0 + e * BLK + yl * NXJ;
+            Real v[NXJ];
+#pragma unroll
+            for (int i = 0; i < NXJ; ++i) v[i] = col[i];
+            const int y = yl / NL, l = yl % NL;
+#pragma unroll
+            for (int s = 0; s < NSET; ++s) {
+                Real dj[NJ];
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) dj[j] = Real(a.dens[s * nn + (size_t)(oJ + j) * n + oL + l]);
+#pragma unroll
+                for (int x = 0; x < NX; ++x) {
+                    Real t2 = 0;
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j) t2 += v[x * NJ + j] * dj[j];
+#pragma unroll
+                    for (int yy = 0; yy < NY; ++yy)
+                        if (yy == y) acc[(s * NX + x) * NY + yy] += t2;
+                }
+            }
+        }
+    } else {
+        for (long long e = lane; e < u1 - u0; e += 64) {
+            const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
+            const int oJ = (int)((ent >> 32) & 0x3fff), oL = (int)((ent >> 46) & 0x3fff);
+            const Real* b = blk0 + e * BLK;
+            Real v[BLK];
+#pragma unroll
+            for (int i = 0; i < BLK; ++i) v[i] = b[i];
+#pragma unroll
+            for (int s = 0; s < NSET; ++s) {
+                Real djl[NJ * NL];
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                    for (int l = 0; l < NL; ++l) djl[j * NL + l] = Real(a.dens[s * nn + (size_t)(oJ + j) * n + oL + l]);
+#pragma unroll
+                for (int x = 0; x < NX; ++x)
+#pragma unroll
+                    for (int y = 0; y < NY; ++y) {
+                        Real t2 = 0;
+#pragma unroll
+                        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                            for (int l = 0; l < NL; ++l) t2 += v[(y * NL + l) * NXJ + x * NJ + j] * djl[j * NL + l];
+                        acc[(s * NX + x) * NY + y] += t2;
+                    }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NSET * NX * NY; ++i) Kacc[i] += double(acc[i]);
 }
 
 template <int LX_, int LY_, int NSET>
 __global__ __launch_bounds__(64) void k_kstore(KSArgs a) {
     const int u = a.unit_base[OT] + blockIdx.x;
     const int code = a.unit_os[u];
-    const int o = code >> 4, sub = code & 15;
+    const int o = code >> 5, f32 = (code >> 4) & 1, sub = code & 15;
     const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
     const int lane = threadIdx.x;
-    const size_t n = a.nao, nn = n * n;
     double Kacc[NSET * NX * NY];
 #pragma unroll
     for (int i = 0; i < NSET * NX * NY; ++i) Kacc[i] = 0.0;
     with_sub(sub, [&](auto sb) {
         using SB = decltype(sb);
-        constexpr int NJ = SB::NJ, NL = SB::NL, NXJ = SB::NXJ, NYL = SB::NYL, BLK = SB::BLK;
         const size_t seg = (size_t)o * 9 + sub;
-        const double* blk0 = a.kdata + a.kdoff[seg] + (u0 - a.kseg[seg]) * BLK;
-        if constexpr (SB::SPLIT) {
-            // lanes over (entry, yl) columns of NXJ contiguous values
-            const long long nt = (u1 - u0) * NYL;
-            for (long long t = lane; t < nt; t += 64) {
-                const long long e = t / NYL;
-                const int yl = (int)(t % NYL);
-                const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
-                const int oJ = (int)((ent >> 32) & 0x3fff), oL = (int)((ent >> 46) & 0x3fff);
-                const double* col = blk0 + e * BLK + yl * NXJ;
-                double v[NXJ];
-#pragma unroll
-                for (int i = 0; i < NXJ; ++i) v[i] = col[i];
-                const int y = yl / NL, l = yl % NL;
-#pragma unroll
-                for (int s = 0; s < NSET; ++s) {
-                    double dj[NJ];
-#pragma unroll
-                    for (int j = 0; j < NJ; ++j) dj[j] = a.dens[s * nn + (size_t)(oJ + j) * n + oL + l];
-#pragma unroll
-                    for (int x = 0; x < NX; ++x) {
-                        double t2 = 0.0;
-#pragma unroll
-                        for (int j = 0; j < NJ; ++j) t2 += v[x * NJ + j] * dj[j];
-#pragma unroll
-                        for (int yy = 0; yy < NY; ++yy)
-                            if (yy == y) Kacc[(s * NX + x) * NY + yy] += t2;
-                    }
-                }
-            }
-        } else {
-            for (long long e = lane; e < u1 - u0; e += 64) {
-                const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
-                const int oJ = (int)((ent >> 32) & 0x3fff), oL = (int)((ent >> 46) & 0x3fff);
-                const double* b = blk0 + e * BLK;
-                double v[BLK];
-#pragma unroll
-                for (int i = 0; i < BLK; ++i) v[i] = b[i];
-#pragma unroll
-                for (int s = 0; s < NSET; ++s) {
-                    double djl[NJ * NL];
-#pragma unroll
-                    for (int j = 0; j < NJ; ++j)
-#pragma unroll
-                        for (int l = 0; l < NL; ++l) djl[j * NL + l] = a.dens[s * nn + (size_t)(oJ + j) * n + oL + l];
-#pragma unroll
-                    for (int x = 0; x < NX; ++x)
-#pragma unroll
-                        for (int y = 0; y < NY; ++y) {
-                            double t2 = 0.0;
-#pragma unroll
-                            for (int j = 0; j < NJ; ++j)
-#pragma unroll
-                                for (int l = 0; l < NL; ++l) t2 += v[(y * NL + l) * NXJ + x * NJ + j] * djl[j * NL + l];
-                            Kacc[(s * NX + x) * NY + y] += t2;
-                        }
-                }
-            }
-        }
+        if (f32)
+            k_unit<float, SB, NSET>(a, a.kdata32 + a.kdoff32[seg] + (u0 - a.kseg[seg] - a.kbig[seg]) * SB::BLK, u0, u1,
+                                    lane, Kacc);
+        else
+            k_unit<double, SB, NSET>(a, a.kdata + a.kdoff[seg] + (u0 - a.kseg[seg]) * SB::BLK, u0, u1, lane, Kacc);
     });
     double* part = a.kpart + (size_t)u * KPART_STRIDE;
 #pragma unroll

@@ -147,7 +147,7 @@ def main() -> int:
         "jk_builds_per_step": round(n_builds / max(args.steps, 1), 2),
         "jk_build_ms": round(float(jk_ms), 3),
         "jk_build_tflops": round(flops / (jk_ms * 1e-3) / 1e12, 3) if jk_ms > 0 else None,
-        "jk_stream_gbs": round(8.0 * info["gather_doubles"] / (jk_ms * 1e-3) / 1e9, 1) if jk_ms > 0 else None,
+        "jk_stream_gbs": round(info["gather_bytes"] / (jk_ms * 1e-3) / 1e9, 1) if jk_ms > 0 else None,
         "store_build_ms_per_step": round(store_ms, 3),
         "store_bytes": info["store_bytes"],
         "gradient_kernels_ms_per_step": round(float(grad_ms), 3),

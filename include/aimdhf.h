@@ -120,11 +120,11 @@ int aimdhf_set_jk_mode(aimdhf_plan* plan, int mode, int64_t max_store_bytes);
 int aimdhf_set_precision(aimdhf_plan* plan, int mode, double jk_switch, double grad_switch);
 
 /* Whether the current geometry has an integral store, its size in bytes, the
- * gather-list lengths of the K and J kernels and the number of integrals
- * (doubles) their gather-ordered data holds, i.e. what one J/K build streams
- * (any pointer may be NULL). */
+ * gather-list lengths of the K and J kernels and the bytes of their
+ * gather-ordered integral data (FP64 and, in mixed precision, FP32), i.e. what
+ * one J/K build streams (any pointer may be NULL). */
 int aimdhf_jk_info(aimdhf_plan* plan, int* stored, int64_t* store_bytes, int64_t* k_entries, int64_t* j_entries,
-                   int64_t* gather_doubles);
+                   int64_t* gather_bytes);
 
 /*
  * One-electron matrices of the current geometry into device buffers:
@@ -163,7 +163,9 @@ int aimdhf_grad_two(aimdhf_plan* plan, const double* Dc, int nx, const double* D
  * 45 J/K prologue and epilogue, 46 Hermite densities, 47 integral store
  * evaluation, 15 gather-ordered copies, 48 gather-list build, 49..54 store J
  * per pair class, 55..63 store K per output type, 64..84 derivative-ERI
- * screening pre-pass per class. aimdhf_profile_read synchronises, returns the
+ * screening pre-pass per class; 85..88 host wall-clock phases (energy list,
+ * owner lists, store build incl. its device work, gradient list; milliseconds
+ * of the calling thread, not events). aimdhf_profile_read synchronises, returns the
  * summed milliseconds and launch counts per kind (nkind <= 96) and optionally
  * resets.
  */
