@@ -35,6 +35,7 @@ SIGNATURES = {
     "aimdhf_jk": (_I, [_P, _I, _P, _P, _P, _P]),
     "aimdhf_set_jk_mode": (_I, [_P, _I, _L]),
     "aimdhf_set_precision": (_I, [_P, _I, _D, _D]),
+    "aimdhf_set_streams": (_I, [_P, _I]),
     "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_grad_one": (_I, [_P, _P, _P, _P, _P]),

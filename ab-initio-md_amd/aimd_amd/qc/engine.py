@@ -118,6 +118,11 @@ class Engine:
                                                    float(grad_switch)))
         self.precision = precision
 
+    def set_streams(self, concurrent: bool = True) -> None:
+        """Spread independent launches over extra library streams (default) or keep every
+        launch on torch's current stream (per-kernel timings then exclude overlap)."""
+        _lib.check(_lib.lib().aimdhf_set_streams(self._h, int(bool(concurrent))))
+
     def jk_info(self) -> dict:
         """Store state of the current geometry: stored, bytes, K and J gather-list entries and
         the bytes of integral data one J/K build streams (gather_bytes)."""

@@ -78,8 +78,8 @@ class SCFOptions:
     max_store_bytes: int = 0
     density_solver: str = "purify"
     precision: str = "fp64"
-    jk_fp32_switch: float = 1e-7
-    grad_fp32_switch: float = 1e-9
+    jk_fp32_switch: float = 1e-5
+    grad_fp32_switch: float = 1e-7
 
     def __post_init__(self) -> None:
         if not (self.conv_energy > 0.0 and self.conv_commutator > 0.0):

@@ -152,10 +152,53 @@ struct Profiler {
     }
 };
 
+// Fan-out of independent launches over the caller's stream and NAUX extra
+// streams (default priority, created on first use): fork() makes the extra
+// streams wait for the caller's work so far, on(i) picks a stream round-robin,
+// join() makes the caller's stream wait for everything launched on them.
+struct Fan {
+    static constexpr int NAUX = 3;
+    hipStream_t aux[NAUX] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[NAUX] = {};
+    hipStream_t base = nullptr;
+    bool serial = false;        // every launch on the caller's stream
+    void init() {
+        if (fork_ev) return;
+        for (int i = 0; i < NAUX; ++i) {
+            HIPCHECK(hipStreamCreateWithFlags(&aux[i], hipStreamNonBlocking));
+            HIPCHECK(hipEventCreateWithFlags(&join_ev[i], hipEventDisableTiming));
+        }
+        HIPCHECK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    }
+    void fork(hipStream_t st) {
+        base = st;
+        if (serial) return;
+        init();
+        HIPCHECK(hipEventRecord(fork_ev, st));
+        for (int i = 0; i < NAUX; ++i) HIPCHECK(hipStreamWaitEvent(aux[i], fork_ev, 0));
+    }
+    hipStream_t on(int i) const { return serial || i % (NAUX + 1) == 0 ? base : aux[i % (NAUX + 1) - 1]; }
+    void join() {
+        if (serial) return;
+        for (int i = 0; i < NAUX; ++i) {
+            HIPCHECK(hipEventRecord(join_ev[i], aux[i]));
+            HIPCHECK(hipStreamWaitEvent(base, join_ev[i], 0));
+        }
+    }
+    ~Fan() {
+        for (int i = 0; i < NAUX; ++i) {
+            if (aux[i]) (void)hipStreamDestroy(aux[i]);
+            if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+        }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+    }
+};
+
 }  // namespace
 
 struct aimdhf_plan {
     Profiler prof;
+    Fan fan;
     int natm = 0, nshell = 0, nao = 0, npair = 0, nblock = 0, nbpair = 0, lmax = 0;
     std::vector<int> shell_atom, shell_l, shell_prim, shell_ao, shell_block;
     std::vector<double> prim_exp, prim_coef, charges, pos;
@@ -449,11 +492,13 @@ struct aimdhf_plan {
             sa.npp = d_npp.as<int>();
             sa.boys = d_boys.as<double>();
             sa.qmap = d_qmap.as<int>();
-            for (int k = 0; k < NQCLASS; ++k) {
+            fan.fork(st);
+            for (int k = 0, i = 0; k < NQCLASS; ++k) {
                 if (!qcount[k]) continue;
                 sa.store = d_store.as<double>() + store_base[k];
-                aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], st);
+                aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], fan.on(i++));
             }
+            fan.join();
         });
         HIPCHECK(hipGetLastError());
         aimdhf::ListArgs la;
@@ -620,17 +665,20 @@ struct aimdhf_plan {
         d_kdata32.ensure(sizeof(float) * std::max<long long>(ko32, 1));
         d_jdata32.ensure(sizeof(float) * std::max<long long>(jo32, 1));
         prof.time(15, st, [&] {
+            fan.fork(st);
+            int i = 0;
             aimdhf::KSArgs ka = ks_args(1, nullptr, nullptr);
             for (int t = 0; t < 9; ++t) {
                 const int no = out_off[t + 1] - out_off[t];
-                if (no > 0) aimdhf::launch_kcopy(t, ka, no, st);
+                if (no > 0) aimdhf::launch_kcopy(t, ka, no, fan.on(i++));
             }
             aimdhf::JSArgs ja = js_args(1, nullptr, nullptr);
             for (int c = 0; c < 6; ++c) {
                 if (!ncls[c]) continue;
                 ja.cls_off = cls_pair_off[c];
-                aimdhf::launch_jcopy(c, ja, ncls[c], st);
+                aimdhf::launch_jcopy(c, ja, ncls[c], fan.on(i++));
             }
+            fan.join();
         });
         HIPCHECK(hipGetLastError());
         store_bytes = 8 * (off + ko + jo) + 4 * (ko32 + jo32) + 4 * keys +
@@ -951,17 +999,24 @@ int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K
             const size_t nn = (size_t)p->nao * p->nao;
             p->d_dsym.ensure(sizeof(double) * nset * nn);
             p->prof.time(45, st, [&] { aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st); });
+            // the 15 J and K launches are independent (disjoint outputs): spread over the fan's streams
+            p->fan.fork(st);
+            int i = 0;
+            aimdhf::KSArgs ka = p->ks_args(nset, p->d_dsym.as<double>(), K);
+            for (int t = 8; t >= 0; --t) {
+                int nout = p->out_off[t + 1] - p->out_off[t];
+                if (nout <= 0) continue;
+                hipStream_t s2 = p->fan.on(i++);
+                p->prof.time(55 + t, s2, [&] { aimdhf::launch_kstore(t, ka, nout, s2); });
+            }
             aimdhf::JSArgs ja = p->js_args(nset, p->d_dsym.as<double>(), J);
-            for (int c = 0; c < 6; ++c) {
+            for (int c = 5; c >= 0; --c) {
                 if (!p->ncls[c]) continue;
                 ja.cls_off = p->cls_pair_off[c];
-                p->prof.time(49 + c, st, [&] { aimdhf::launch_jstore(c, ja, p->ncls[c], st); });
+                hipStream_t s2 = p->fan.on(i++);
+                p->prof.time(49 + c, s2, [&] { aimdhf::launch_jstore(c, ja, p->ncls[c], s2); });
             }
-            aimdhf::KSArgs ka = p->ks_args(nset, p->d_dsym.as<double>(), K);
-            for (int t = 0; t < 9; ++t) {
-                int nout = p->out_off[t + 1] - p->out_off[t];
-                if (nout > 0) p->prof.time(55 + t, st, [&] { aimdhf::launch_kstore(t, ka, nout, st); });
-            }
+            p->fan.join();
             HIPCHECK(hipGetLastError());
         } catch (const HipError& e) {
             return e.code;
@@ -1149,8 +1204,10 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
         p->d_gblk.ensure(sizeof(int) * 4 * (gtot / 256 + 2 * NQCLASS));
         a.fp32_switch = p->precision == 1 ? p->grad_switch : 0.0;
         long long off = 0;
-        for (int k = 0; k < NQCLASS; ++k) {
+        p->fan.fork(st);
+        for (int k = 0, i = 0; k < NQCLASS; ++k) {
             const int2* lst = p->d_glist.as<int2>() + off;
+            hipStream_t s2 = p->gcount[k] > 0 ? p->fan.on(i++) : st;
             a.keep = p->d_gkeep.as<unsigned char>() + off;
             a.sel64 = p->d_gsel.as<int>() + off;
             a.sel32 = p->d_gsel.as<int>() + gtot + off;
@@ -1158,11 +1215,12 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
             a.bcnt = p->d_gblk.as<int>() + 4 * (off / 256 + k);
             a.boff = a.bcnt + 2 * ((p->gcount[k] + 255) / 256);
             if (p->gcount[k] > 0) {
-                p->prof.time(64 + k, st, [&] { aimdhf::launch_grad_keep(k, a, lst, p->gcount[k], st); });
-                p->prof.time(21 + k, st, [&] { aimdhf::launch_grad_one_class(k, a, lst, p->gcount[k], st); });
+                p->prof.time(64 + k, s2, [&] { aimdhf::launch_grad_keep(k, a, lst, p->gcount[k], s2); });
+                p->prof.time(21 + k, s2, [&] { aimdhf::launch_grad_one_class(k, a, lst, p->gcount[k], s2); });
             }
             off += p->gcount[k];
         }
+        p->fan.join();
         aimdhf::launch_fx_to_double(p->d_gfx.as<long long>(), 3 * p->natm, grad, st);
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
@@ -1177,6 +1235,12 @@ int aimdhf_set_jk_mode(aimdhf_plan* p, int mode, int64_t max_store_bytes) {
     if (max_store_bytes < 0) return fail(AIMDHF_E_ARG, "max_store_bytes must be >= 0");
     p->jk_mode = mode;
     p->store_limit = max_store_bytes;
+    return AIMDHF_OK;
+}
+
+int aimdhf_set_streams(aimdhf_plan* p, int concurrent) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    p->fan.serial = concurrent == 0;
     return AIMDHF_OK;
 }
 

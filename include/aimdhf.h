@@ -119,6 +119,16 @@ int aimdhf_set_jk_mode(aimdhf_plan* plan, int mode, int64_t max_store_bytes);
  */
 int aimdhf_set_precision(aimdhf_plan* plan, int mode, double jk_switch, double grad_switch);
 
+/*
+ * Independent launches (the J and K kernels of a build, the store classes,
+ * the derivative-ERI classes) run on the caller's stream and three extra
+ * library streams of default priority, joined back into the caller's stream
+ * (concurrent = 1, default), or all on the caller's stream (0: each kernel's
+ * timing is then its own, e.g. for per-kernel profiling). Results are
+ * identical either way. No counterpart in the reference.
+ */
+int aimdhf_set_streams(aimdhf_plan* plan, int concurrent);
+
 /* Whether the current geometry has an integral store, its size in bytes, the
  * gather-list lengths of the K and J kernels and the bytes of their
  * gather-ordered integral data (FP64 and, in mixed precision, FP32), i.e. what
