@@ -21,6 +21,7 @@ differences, formed on the device and solved on the host (m <= diis_space).
 
 from __future__ import annotations
 
+import ctypes
 import time
 import warnings
 from dataclasses import dataclass, field, fields, replace
@@ -30,6 +31,7 @@ import numpy as np
 import torch
 
 from aimd_amd.elements import normalize_symbol
+from aimd_amd import _lib
 from aimd_amd.qc import engine as qengine
 from aimd_amd.qc.basis import BasisSet, build_basis
 
@@ -230,6 +232,75 @@ def inverse_sqrt_newton_schulz(S: torch.Tensor, max_iter: int = 60, tol: float =
         v = v / torch.linalg.vector_norm(v)
     lam = float(v @ (X2 @ v))
     return X, 1.0 / lam
+
+
+class TC2Graph:
+    """
+    STEPS purification steps of an n x n matrix with nocc electrons captured once as
+    a HIP graph: each step is a dense FP64 product X2 = X X (BLAS) and the engine's
+    fused update (aimdhf_tc2_step), ping-ponging between two static buffers so the
+    result of an even number of steps lands back in ``X``. The update kernel needs
+    no reduction pass and clears no memory, so replays are exact repeats.
+    """
+    STEPS = 8
+
+    def __init__(self, n: int, nocc: int, device: torch.device) -> None:
+        self.n, self.nocc = n, float(nocc)
+        self.X = torch.zeros((n, n), dtype=torch.float64, device=device)
+        self.B = torch.empty_like(self.X)
+        self.X2 = torch.empty_like(self.X)
+        side = torch.cuda.Stream(device=device)
+        side.wait_stream(torch.cuda.current_stream(device))
+        with torch.cuda.stream(side):
+            self._steps()
+        torch.cuda.current_stream(device).wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._steps()
+
+    def _steps(self) -> None:
+        L = _lib.lib()
+        st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        src, dst = self.X, self.B
+        for _ in range(self.STEPS):
+            torch.matmul(src, src, out=self.X2)
+            _lib.check(L.aimdhf_tc2_step(self.n, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(self.X2.data_ptr()),
+                                         ctypes.c_void_p(dst.data_ptr()), self.nocc, st))
+            src, dst = dst, src
+
+    def replay(self) -> torch.Tensor:
+        self.graph.replay()
+        return self.X
+
+
+_TC2_GRAPHS: dict[tuple, TC2Graph] = {}
+
+
+def purified_projector_device(F: torch.Tensor, nocc: int, tol: float = 1e-11,
+                              max_replays: int = 15) -> torch.Tensor | None:
+    """purified_projector on the GPU with the captured TC2Graph (same steps, same branch
+    rule, the idempotency error read back after every TC2Graph.STEPS steps)."""
+    n = F.shape[0]
+    if nocc <= 0:
+        return torch.zeros_like(F)
+    if nocc >= n:
+        return torch.eye(n, dtype=F.dtype, device=F.device)
+    key = (n, int(nocc), F.device)
+    g = _TC2_GRAPHS.get(key)
+    if g is None:
+        g = _TC2_GRAPHS[key] = TC2Graph(n, nocc, F.device)
+    eye = torch.eye(n, dtype=F.dtype, device=F.device)
+    a = F.abs()
+    d = torch.diagonal(F)
+    r = a.sum(dim=1) - torch.diagonal(a)
+    emin, emax = (d - r).min(), (d + r).max()
+    g.X.copy_((emax * eye - F) / (emax - emin))
+    for _ in range(max_replays):
+        X = g.replay()
+        if float((torch.trace(X) - torch.sum(X * X)).abs()) < tol:
+            X = g.replay()
+            return 0.5 * (X + X.T)
+    return None
 
 
 def orthogonalizer(S: torch.Tensor, method: str = "auto",
@@ -810,8 +881,9 @@ class SCFSolver:
         """Aufbau densities of the orthogonal-basis Fock matrices Fo (nspin, nmo, nmo)."""
         Dn = torch.empty_like(Ds)
         if self.options.density_solver == "purify":
+            purify = purified_projector_device if Fo.is_cuda else purified_projector
             for s in range(Fo.shape[0]):
-                P = purified_projector(Fo[s], nocc[s])
+                P = purify(Fo[s], nocc[s])
                 if P is None:
                     break
                 Dn[s] = X @ P @ X.T

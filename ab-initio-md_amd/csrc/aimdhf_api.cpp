@@ -1238,6 +1238,14 @@ int aimdhf_set_jk_mode(aimdhf_plan* p, int mode, int64_t max_store_bytes) {
     return AIMDHF_OK;
 }
 
+int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, void* stream) {
+    if (n <= 0 || !X || !X2 || !Xout) return fail(AIMDHF_E_ARG, "bad tc2 arguments");
+    aimdhf::launch_tc2_step(n, X, X2, Xout, nocc, (hipStream_t)stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(AIMDHF_E_HIP, std::string("tc2 step: ") + hipGetErrorString(e));
+    return AIMDHF_OK;
+}
+
 int aimdhf_set_streams(aimdhf_plan* p, int concurrent) {
     if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
     p->fan.serial = concurrent == 0;

@@ -266,6 +266,7 @@ void launch_kcopy(int type, const KSArgs& a, int nout, hipStream_t st);
 void launch_jcopy(int pclass, const JSArgs& a, int npairs, hipStream_t st);
 void launch_fill_int(int* p, size_t n, int v, hipStream_t st);
 
+void launch_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, hipStream_t st);
 void launch_dh(const DhArgs& a, const long long* n_per_class, hipStream_t st);
 void launch_jeng(int pclass, const JEArgs& a, int nset, const int* nket_host, hipStream_t st);
 void launch_j_assemble(const JAsmArgs& a, hipStream_t st);

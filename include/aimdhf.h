@@ -120,6 +120,15 @@ int aimdhf_set_jk_mode(aimdhf_plan* plan, int mode, int64_t max_store_bytes);
 int aimdhf_set_precision(aimdhf_plan* plan, int mode, double jk_switch, double grad_switch);
 
 /*
+ * One trace-correcting purification step on device matrices (n x n, row-major):
+ * Xout = X2 if tr X > nocc, else 2 X - X2, where X2 = X X (computed by the
+ * caller). Needs no plan; issues no memory clearing, so it may be captured in a
+ * graph. Building block of the SCF density from the orthogonal-basis Fock matrix,
+ * which replaces the eigenvector aufbau of aimd/qc/scf.py:913-921.
+ */
+int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, void* stream);
+
+/*
  * Independent launches (the J and K kernels of a build, the store classes,
  * the derivative-ERI classes) run on the caller's stream and three extra
  * library streams of default priority, joined back into the caller's stream

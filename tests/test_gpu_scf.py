@@ -110,3 +110,28 @@ def test_canonical_orbitals_on_demand(mol, basis, mult):
         n = int(round(occ[sp].sum() / (1.0 if res.unrestricted else 2.0)))
         Co = C[sp][:, :n]
         np.testing.assert_allclose(Co @ Co.T, res.spin_densities[sp], atol=1e-8)
+
+
+def test_graph_purification_matches_eigenvectors_on_every_replay():
+    # the captured TC2 graph (BLAS product + fused update) against the eigenvector projector,
+    # replayed several times with eager GPU work in between: every replay must agree
+    import torch
+    from aimd_amd.qc import scf as S
+    rng = np.random.default_rng(8)
+    n, nocc = 120, 37
+    w = np.sort(rng.uniform(-15.0, 4.0, n))
+    w[nocc:] += 0.4 + (w[nocc - 1] - w[nocc])
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ref = Q[:, :nocc] @ Q[:, :nocc].T
+    F = torch.tensor((Q * w) @ Q.T, device="cuda")
+    outs = []
+    for k in range(3):
+        P = S.purified_projector_device(F, nocc)
+        assert P is not None
+        outs.append(P.cpu().numpy())
+        torch.cuda.synchronize()
+        _ = (torch.randn(500, 500, device="cuda", dtype=torch.float64) @ torch.randn(500, 500, device="cuda",
+                                                                                     dtype=torch.float64)).sum()
+    for o in outs:
+        np.testing.assert_allclose(o, ref, atol=1e-10)
+        np.testing.assert_array_equal(o, outs[0])
