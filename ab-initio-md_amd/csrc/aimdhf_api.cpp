@@ -244,7 +244,7 @@ struct aimdhf_plan {
     DevBuf d_kdata, d_jdata, d_kdoff, d_jdoff, d_kdata32, d_jdata32, d_kdoff32, d_jdoff32, d_kbig, d_jbig;
     long long kdata_total = 0, jdata_total = 0, kdata32_total = 0, jdata32_total = 0;
     // work units of the store kernels (lane items per unit: UNIT_ITEMS)
-    static constexpr int UNIT_ITEMS = 512;
+    static constexpr int UNIT_ITEMS = 1024;
     int kunit_base[10] = {0}, junit_base[7] = {0};
     DevBuf d_kunit_os, d_kunit_e, d_out_unit, d_kpart2, d_junit_pc, d_junit_e, d_pair_unit, d_jpart2;
 
@@ -433,6 +433,11 @@ struct aimdhf_plan {
         ka.kdoff32 = d_kdoff32.as<long long>();
         ka.kbig = d_kbig.as<long long>();
         for (int t = 0; t < 10; ++t) ka.unit_base[t] = kunit_base[t];
+        long long pb = 0;
+        for (int t = 0; t < 9; ++t) {
+            ka.part_base[t] = pb;
+            pb += (long long)(kunit_base[t + 1] - kunit_base[t]) * std::max(nset, 1) * ncart(t / 3) * ncart(t % 3);
+        }
         ka.unit_os = d_kunit_os.as<int>();
         ka.unit_e = d_kunit_e.as<long long>();
         ka.out_unit = d_out_unit.as<int>();
@@ -461,6 +466,12 @@ struct aimdhf_plan {
         ja.jdoff32 = d_jdoff32.as<long long>();
         ja.jbig = d_jbig.as<long long>();
         for (int c = 0; c < 7; ++c) ja.unit_base[c] = junit_base[c];
+        static const int ncp[6] = {1, 3, 9, 6, 18, 36};
+        long long pb = 0;
+        for (int c = 0; c < 6; ++c) {
+            ja.part_base[c] = pb;
+            pb += (long long)(junit_base[c + 1] - junit_base[c]) * std::max(nset, 1) * ncp[c];
+        }
         ja.unit_pc = d_junit_pc.as<int>();
         ja.unit_e = d_junit_e.as<long long>();
         ja.pair_unit = d_pair_unit.as<int>();
@@ -625,7 +636,7 @@ struct aimdhf_plan {
             upload(d_kunit_os, uos);
             upload(d_kunit_e, ue);
             upload(d_out_unit, out_unit);
-            d_kpart2.ensure(sizeof(double) * aimdhf::KPART_STRIDE * std::max<size_t>(uos.size(), 1));
+            d_kpart2.ensure(sizeof(double) * 2 * 36 * std::max<size_t>(uos.size(), 1));
         }
         {
             std::vector<int> upc, pair_unit(npair + 1);
@@ -654,7 +665,7 @@ struct aimdhf_plan {
             upload(d_junit_pc, upc);
             upload(d_junit_e, ue);
             upload(d_pair_unit, pair_unit);
-            d_jpart2.ensure(sizeof(double) * aimdhf::KPART_STRIDE * std::max<size_t>(upc.size(), 1));
+            d_jpart2.ensure(sizeof(double) * 2 * 36 * std::max<size_t>(upc.size(), 1));
         }
         upload(d_kdoff, kdoff);
         upload(d_jdoff, jdoff);

@@ -163,8 +163,7 @@ struct KOArgs {
     double* kpart;              // (9, nset, nao, nao)
 };
 
-// integral store (per geometry) and its gather lists; doubles per unit partial
-constexpr int KPART_STRIDE = 72;
+// integral store (per geometry) and its gather lists
 struct StoreArgs {
     const double* recs;
     const long long* rec_off;
@@ -223,7 +222,8 @@ struct KSArgs {
     const int* unit_os;         // per unit: output << 5 | fp32 << 4 | subclass
     const long long* unit_e;    // per unit: entry range [e0, e1)
     const int* out_unit;        // per output: unit range (nout + 1)
-    double* kpart;              // per unit partial block (KPART_STRIDE doubles)
+    double* kpart;              // per unit partial block (nset NX NY doubles, per type region)
+    long long part_base[9];     // start of each type's partial region
     const double* dens;         // (nset, nao, nao), symmetric
     double* K;
 };
@@ -249,7 +249,8 @@ struct JSArgs {
     const int* unit_pc;         // per unit: pair << 4 | fp32 << 3 | partner class
     const long long* unit_e;    // per unit: entry range [e0, e1)
     const int* pair_unit;       // per cls_pairs position: unit range (npair + 1)
-    double* jpart;              // per unit partial block (KPART_STRIDE doubles)
+    double* jpart;              // per unit partial block (nset NAB doubles, per class region)
+    long long part_base[6];     // start of each class's partial region
     const double* dens;
     double* J;
 };

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(64) void k_jstore(JSArgs a) {
         else
             j_unit<double, PT, NSET>(a, a.jdata + a.jdoff[seg] + (u0 - a.jseg[seg]) * PT::BLK, u0, u1, lane, Jacc);
     });
-    double* part = a.jpart + (size_t)u * KPART_STRIDE;
+    double* part = a.jpart + a.part_base[CP] + (size_t)(u - a.unit_base[CP]) * (NSET * NAB);
 #pragma unroll
     for (int i = 0; i < NSET * NAB; ++i) {
         const double v = wave_sum(Jacc[i]);
@@ -170,7 +170,8 @@ __global__ void k_jreduce(JSArgs a, int npairs) {
     if (t >= npairs * NSET * NAB) return;
     const int idx = a.cls_off + t / (NSET * NAB), i = t % (NSET * NAB);
     double v = 0.0;
-    for (int u = a.pair_unit[idx]; u < a.pair_unit[idx + 1]; ++u) v += a.jpart[(size_t)u * KPART_STRIDE + i];
+    for (int u = a.pair_unit[idx]; u < a.pair_unit[idx + 1]; ++u)
+        v += a.jpart[a.part_base[CP] + (size_t)(u - a.unit_base[CP]) * (NSET * NAB) + i];
     const int P = a.cls_pairs[idx];
     const int A = a.pair_a[P], B = a.pair_b[P];
     const int oA = a.shell_ao[A], oB = a.shell_ao[B];

@@ -8,7 +8,8 @@
 // order, reoriented to (XJ|YL) with the (y, l) index major:
 // kdata[seg + e * BLK + (y NL + l) NX NJ + x NJ + j] = (xj|yl). Per SCF
 // iteration, k_kstore streams them and writes one partial block per unit
-// (lanes summed by a fixed butterfly); k_kreduce adds each block's unit
+// (lanes summed by a fixed butterfly; nset NX NY doubles per unit, each
+// type in its own region); k_kreduce adds each block's unit
 // partials in unit order and writes K[X][Y] and K[Y][X]. No atomics, so the
 // result is bitwise reproducible run to run.
 
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(64) void k_kstore(KSArgs a) {
         else
             k_unit<double, SB, NSET>(a, a.kdata + a.kdoff[seg] + (u0 - a.kseg[seg]) * SB::BLK, u0, u1, lane, Kacc);
     });
-    double* part = a.kpart + (size_t)u * KPART_STRIDE;
+    double* part = a.kpart + a.part_base[OT] + (size_t)(u - a.unit_base[OT]) * (NSET * NX * NY);
 #pragma unroll
     for (int i = 0; i < NSET * NX * NY; ++i) {
         const double v = wave_sum(Kacc[i]);
@@ -190,7 +191,8 @@ __global__ void k_kreduce(KSArgs a, int nout) {
     if (t >= nout * NSET * NX * NY) return;
     const int o = a.out_off[OT] + t / (NSET * NX * NY), i = t % (NSET * NX * NY);
     double v = 0.0;
-    for (int u = a.out_unit[o]; u < a.out_unit[o + 1]; ++u) v += a.kpart[(size_t)u * KPART_STRIDE + i];
+    for (int u = a.out_unit[o]; u < a.out_unit[o + 1]; ++u)
+        v += a.kpart[a.part_base[OT] + (size_t)(u - a.unit_base[OT]) * (NSET * NX * NY) + i];
     const int X = a.out_x[o], Y = a.out_y[o];
     const size_t n = a.nao, nn = n * n;
     const int s = i / (NX * NY), x = (i % (NX * NY)) / NY, y = i % NY;
