@@ -59,30 +59,22 @@ __global__ void k_build_pairs(PairArgs g) {
 }
 
 // ---------------------------------------------------------------- Schwarz
-template <int LA, int LB>
-__device__ double schwarz_diag(const double* rec, int n, const double* tab) {
-    using Qt = Quartet<LA, LB, LA, LB>;
-    double m = 0.0;
-    if constexpr (split_by_cd<LA, LB, LA, LB>()) {
-        for (int cd = 0; cd < Qt::NCD; ++cd) {
-            double acc[Qt::NAB];
-            eri_quartet_cd<LA, LB, LA, LB>(rec, n, rec, n, tab, cd, acc);
-            m = fmax(m, fabs(acc[cd]));
-        }
-    } else {
-        double acc[Qt::NAB * Qt::NCD];
-        eri_quartet<LA, LB, LA, LB>(rec, n, rec, n, tab, acc);
-        for (int ab = 0; ab < Qt::NAB; ++ab) m = fmax(m, fabs(acc[ab * Qt::NCD + ab]));
-    }
-    return sqrt(m);
-}
-
+// One thread per (shell pair, component ab): the diagonal integral (ab|ab) from
+// the ab column of (AB|AB); qshell[P] receives max |(ab|ab)| through an integer
+// max on the bit pattern (non-negative doubles order like their bits, so the
+// result does not depend on the order), and k_block_q takes the square root.
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void k_schwarz(PairArgs g) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= g.ncls_pairs[pair_class(LA, LB)]) return;
-    int P = g.cls_pairs[g.cls_pair_off[pair_class(LA, LB)] + i];
-    g.qshell[P] = schwarz_diag<LA, LB>(g.recs + g.rec_off[P], g.npp[P], g.boys);
+    using Qt = Quartet<LA, LB, LA, LB>;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int np = g.ncls_pairs[pair_class(LA, LB)];
+    if (t >= np * Qt::NCD) return;
+    const int i = t / Qt::NCD, cd = t % Qt::NCD;
+    const int P = g.cls_pairs[g.cls_pair_off[pair_class(LA, LB)] + i];
+    const double* rec = g.recs + g.rec_off[P];
+    double acc[Qt::NAB];
+    eri_quartet_cd<LA, LB, LA, LB>(rec, g.npp[P], rec, g.npp[P], g.boys, cd, acc);
+    atomicMax((unsigned long long*)&g.qshell[P], (unsigned long long)__double_as_longlong(fabs(acc[cd])));
 }
 
 // block-pair bound = max over its shell pairs; every shell pair screens with it
@@ -91,6 +83,7 @@ __global__ void k_block_q(PairArgs g) {
     if (bp >= g.nbpair) return;
     double m = 0.0;
     for (int i = g.bp_off[bp]; i < g.bp_off[bp + 1]; ++i) m = fmax(m, g.qshell[g.bp_list[i]]);
+    m = sqrt(m);
     g.qblock[bp] = m;
     for (int i = g.bp_off[bp]; i < g.bp_off[bp + 1]; ++i) g.qscr[g.bp_list[i]] = m;
 }
@@ -280,12 +273,13 @@ void launch_build_pairs(const PairArgs& g, hipStream_t st) {
 }
 
 void launch_schwarz(const PairArgs& g, const int* ncls_host, hipStream_t st) {
-    if (ncls_host[0]) k_schwarz<0, 0><<<GRID(ncls_host[0], 64), 0, st>>>(g);
-    if (ncls_host[1]) k_schwarz<1, 0><<<GRID(ncls_host[1], 64), 0, st>>>(g);
-    if (ncls_host[2]) k_schwarz<1, 1><<<GRID(ncls_host[2], 64), 0, st>>>(g);
-    if (ncls_host[3]) k_schwarz<2, 0><<<GRID(ncls_host[3], 64), 0, st>>>(g);
-    if (ncls_host[4]) k_schwarz<2, 1><<<GRID(ncls_host[4], 64), 0, st>>>(g);
-    if (ncls_host[5]) k_schwarz<2, 2><<<GRID(ncls_host[5], 64), 0, st>>>(g);
+    if (g.npair) k_zero64<<<GRID(g.npair, 256), 0, st>>>((long long*)g.qshell, g.npair);
+    if (ncls_host[0]) k_schwarz<0, 0><<<GRID(ncls_host[0] * 1, 64), 0, st>>>(g);
+    if (ncls_host[1]) k_schwarz<1, 0><<<GRID(ncls_host[1] * 3, 64), 0, st>>>(g);
+    if (ncls_host[2]) k_schwarz<1, 1><<<GRID(ncls_host[2] * 9, 64), 0, st>>>(g);
+    if (ncls_host[3]) k_schwarz<2, 0><<<GRID(ncls_host[3] * 6, 64), 0, st>>>(g);
+    if (ncls_host[4]) k_schwarz<2, 1><<<GRID(ncls_host[4] * 18, 64), 0, st>>>(g);
+    if (ncls_host[5]) k_schwarz<2, 2><<<GRID(ncls_host[5] * 36, 64), 0, st>>>(g);
     if (g.nbpair) k_block_q<<<GRID(g.nbpair, 64), 0, st>>>(g);
 }
 
