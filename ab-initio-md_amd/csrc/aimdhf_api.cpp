@@ -579,8 +579,11 @@ struct aimdhf_plan {
             g_err = "integral store: a gather-list quartet is missing from the screened list";
             throw HipError{AIMDHF_E_STATE};
         }
-        // gather-ordered copies: segment offsets, then one copy kernel per type / class
+        // gather-ordered copies: segment offsets, then one copy kernel per type / class.
+        // A part of n entries holds n * ns lane items of blk / ns values in
+        // tiles of 64 items (blocks above 36 values are split into ns columns).
         static const int ncp[6] = {1, 3, 9, 6, 18, 36};
+        auto tiled = [](long long n, long long blk, long long ns) { return (n * ns + 63) / 64 * 64 * (blk / ns); };
         std::vector<long long> kdoff((size_t)nout * 9), jdoff((size_t)npair * 6), kdoff32(kdoff.size()),
             jdoff32(jdoff.size());
         long long ko = 0, ko32 = 0;
@@ -589,10 +592,11 @@ struct aimdhf_plan {
             for (int sub = 0; sub < 9; ++sub) {
                 const size_t i = (size_t)o * 9 + sub;
                 const long long blk = (long long)nx * ncart(sub / 3) * ny * ncart(sub % 3);
+                const long long ns = blk > 36 ? ny * ncart(sub % 3) : 1;
                 kdoff[i] = ko;
                 kdoff32[i] = ko32;
-                ko += kb[i] * blk;
-                ko32 += (kseg[i + 1] - kseg[i] - kb[i]) * blk;
+                ko += tiled(kb[i], blk, ns);
+                ko32 += tiled(kseg[i + 1] - kseg[i] - kb[i], blk, ns);
             }
         }
         long long jo = 0, jo32 = 0;
@@ -600,10 +604,11 @@ struct aimdhf_plan {
             for (int c = 0; c < 6; ++c) {
                 const size_t i = (size_t)P * 6 + c;
                 const long long blk = (long long)ncp[pair_cls[P]] * ncp[c];
+                const long long ns = blk > 36 ? ncp[c] : 1;
                 jdoff[i] = jo;
                 jdoff32[i] = jo32;
-                jo += jb[i] * blk;
-                jo32 += (jseg[i + 1] - jseg[i] - jb[i]) * blk;
+                jo += tiled(jb[i], blk, ns);
+                jo32 += tiled(jseg[i + 1] - jseg[i] - jb[i], blk, ns);
             }
         kdata_total = ko;
         jdata_total = jo;

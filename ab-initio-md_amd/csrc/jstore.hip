@@ -5,7 +5,11 @@
 // by partner class) is cut into work units of at most a fixed number of lane
 // items, one wavefront each. Per geometry, k_jcopy lays each unit's stored
 // quartets out in gather order, oriented (P|Q) with the partner's component
-// index major: jdata[seg + e * BLK + cd NAB + ab] = (ab|cd). Per SCF
+// index major. A lane item (an entry, or an (entry, cd) column when split)
+// holds W consecutive values of that order; items are stored in
+// lane-interleaved tiles of 64, value major: item T, value v at
+// seg + (T / 64) 64 W + v 64 + T % 64, so every load of the digestion and
+// every store of the copy covers 64 consecutive words. Per SCF
 // iteration, k_jstore forms sum_cd (ab|cd) w_Q D_cd (w_Q = 2 for C != D) per
 // unit (lanes summed by a fixed butterfly) and k_jreduce adds each pair's
 // unit partials in unit order into J_P, so J is bitwise reproducible.
@@ -38,6 +42,8 @@ template <int CQ>
 struct Part {
     static constexpr int NDc = NC(lb_of(CQ)), NCD = NC(la_of(CQ)) * NDc, BLK = NAB * NCD;
     static constexpr bool SPLIT = BLK > 36;
+    static constexpr int NS = SPLIT ? NCD : 1;       // lane items per entry
+    static constexpr int W = BLK / NS;               // values per lane item
     static constexpr int K = qcls(CP, CQ);
 };
 
@@ -58,82 +64,103 @@ __global__ __launch_bounds__(64) void k_jcopy(JSArgs a) {
     const int lane = threadIdx.x;
     with_part(cq, [&](auto pt) {
         using PT = decltype(pt);
-        constexpr int NCD = PT::NCD, BLK = PT::BLK;
+        constexpr int NCD = PT::NCD, BLK = PT::BLK, NS = PT::NS, W = PT::W;
         const size_t seg = (size_t)P * 6 + cq;
         const double* base = a.store + a.store_base[PT::K];
-        double* out64 = a.jdata + a.jdoff[seg] + (u0 - a.jseg[seg]) * BLK;
-        float* out32 = a.jdata32 + a.jdoff32[seg] + (u0 - a.jseg[seg] - a.jbig[seg]) * BLK;
-        const long long nt = (u1 - u0) * BLK;
-        for (long long t = lane; t < nt; t += 64) {
-            const long long e = t / BLK;
-            const int el = (int)(t % BLK);
-            const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
-            const long long qi = (long long)(ent & 0xffffffffull);
+        const long long t0 = (u0 - a.jseg[seg] - (f32 ? a.jbig[seg] : 0)) * NS;
+        const long long nt = (u1 - u0) * NS;
+        // lanes over consecutive items, so each value row is written by one
+        // coalesced store; the W source loads of an item are independent
+        for (long long i = lane; i < nt; i += 64) {
+            const unsigned long long ent = (unsigned long long)a.jlist[u0 + i / NS];
+            const int s = (int)(i % NS);
+            const double* blk = base + (long long)(ent & 0xffffffffull) * BLK;
             const bool sw = (ent >> 60) & 1;
-            const int cd = el / NAB, ab = el % NAB;
-            const double v = base[qi * BLK + (sw ? cd * NAB + ab : ab * NCD + cd)];
-            if (f32) out32[t] = (float)v;
-            else out64[t] = v;
+            double val[W];
+#pragma unroll
+            for (int v = 0; v < W; ++v) {
+                // element (cd, ab) of the (P|Q) order
+                const int cd = PT::SPLIT ? s : v / NAB, ab = PT::SPLIT ? v : v % NAB;
+                val[v] = blk[sw ? cd * NAB + ab : ab * NCD + cd];
+            }
+            const long long T = t0 + i;
+            const long long dst = (T >> 6) * (64 * W) + (T & 63);
+            if (f32) {
+                float* out = a.jdata32 + a.jdoff32[seg] + dst;
+#pragma unroll
+                for (int v = 0; v < W; ++v) out[v * 64] = (float)val[v];
+            } else {
+                double* out = a.jdata + a.jdoff[seg] + dst;
+#pragma unroll
+                for (int v = 0; v < W; ++v) out[v * 64] = val[v];
+            }
         }
     });
 }
 
-// one work unit of partner class PT in storage / arithmetic precision Real
+// items per lane and pass of the digestion: small items are batched so each
+// lane has several independent loads in flight
+template <int W>
+constexpr int batch_items() { return W <= 2 ? 4 : W <= 4 ? 2 : 1; }
+
+// one work unit of partner class PT in storage / arithmetic precision Real:
+// the part's tiles start at tile0, the unit's first lane item is t0 within
+// the part; its per-lane sums are added to acc
 template <typename Real, class PT, int NSET>
-__device__ __forceinline__ void j_unit(const JSArgs& a, const Real* __restrict__ blk0, long long u0, long long u1,
-                                       int lane, double* Jacc) {
-    constexpr int NDc = PT::NDc, NCD = PT::NCD, BLK = PT::BLK;
+__device__ __forceinline__ void j_unit(const JSArgs& a, const Real* __restrict__ tile0, long long t0, long long u0,
+                                       long long u1, int lane, Real* __restrict__ acc) {
+    constexpr int NDc = PT::NDc, NCD = PT::NCD, NS = PT::NS, W = PT::W;
+    constexpr int U = batch_items<W>();
     const size_t n = a.nao, nn = n * n;
-    Real acc[NSET * NAB];
+    const long long nt = (u1 - u0) * NS;
+    for (long long t = lane; t < nt; t += 64 * U) {
+        // loads of all U items first; an item past the end repeats item t
+        // with its density weight zeroed
+        unsigned long long ent[U];
+        Real v[U][W], wk[U];
+        int sk[U];
 #pragma unroll
-    for (int i = 0; i < NSET * NAB; ++i) acc[i] = Real(0);
-    if constexpr (PT::SPLIT) {
-        // lanes over (entry, cd) columns of NAB contiguous values
-        const long long nt = (u1 - u0) * NCD;
-        for (long long t = lane; t < nt; t += 64) {
-            const long long e = t / NCD;
-            const int cd = (int)(t % NCD);
-            const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
-            const int oC = (int)((ent >> 32) & 0x3fff), oD = (int)((ent >> 46) & 0x3fff);
-            const Real w = ((ent >> 61) & 1) ? Real(2) : Real(1);
-            const Real* col = blk0 + e * BLK + cd * NAB;
-            Real v[NAB];
+        for (int k = 0; k < U; ++k) {
+            const long long tk = t + 64 * k;
+            const long long tc = tk < nt ? tk : t;
+            ent[k] = (unsigned long long)a.jlist[u0 + tc / NS];
+            wk[k] = tk >= nt ? Real(0) : ((ent[k] >> 61) & 1) ? Real(2) : Real(1);
+            sk[k] = (int)(tc % NS);
+            const long long T = t0 + tc;
+            const Real* p = tile0 + (T >> 6) * (64 * W) + (T & 63);
 #pragma unroll
-            for (int i = 0; i < NAB; ++i) v[i] = col[i];
-#pragma unroll
-            for (int s = 0; s < NSET; ++s) {
-                const Real dv = w * Real(a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc]);
-#pragma unroll
-                for (int ab = 0; ab < NAB; ++ab) acc[s * NAB + ab] += v[ab] * dv;
-            }
+            for (int i = 0; i < W; ++i) v[k][i] = p[i * 64];
         }
-    } else {
-        for (long long e = lane; e < u1 - u0; e += 64) {
-            const unsigned long long ent = (unsigned long long)a.jlist[u0 + e];
-            const int oC = (int)((ent >> 32) & 0x3fff), oD = (int)((ent >> 46) & 0x3fff);
-            const Real w = ((ent >> 61) & 1) ? Real(2) : Real(1);
-            const Real* b = blk0 + e * BLK;
-            Real v[BLK];
 #pragma unroll
-            for (int i = 0; i < BLK; ++i) v[i] = b[i];
+        for (int k = 0; k < U; ++k) {
+            const int oC = (int)((ent[k] >> 32) & 0x3fff), oD = (int)((ent[k] >> 46) & 0x3fff);
+            if constexpr (PT::SPLIT) {
+                // item = (entry, cd) column of NAB values
+                const int cd = sk[k];
 #pragma unroll
-            for (int s = 0; s < NSET; ++s) {
-                Real dq[NCD];
+                for (int s = 0; s < NSET; ++s) {
+                    const Real dv = wk[k] * Real(a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc]);
 #pragma unroll
-                for (int cd = 0; cd < NCD; ++cd)
-                    dq[cd] = w * Real(a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc]);
+                    for (int ab = 0; ab < NAB; ++ab) acc[s * NAB + ab] += v[k][ab] * dv;
+                }
+            } else {
 #pragma unroll
-                for (int ab = 0; ab < NAB; ++ab) {
-                    Real t2 = 0;
+                for (int s = 0; s < NSET; ++s) {
+                    Real dq[NCD];
 #pragma unroll
-                    for (int cd = 0; cd < NCD; ++cd) t2 += v[cd * NAB + ab] * dq[cd];
-                    acc[s * NAB + ab] += t2;
+                    for (int cd = 0; cd < NCD; ++cd)
+                        dq[cd] = wk[k] * Real(a.dens[s * nn + (size_t)(oC + cd / NDc) * n + oD + cd % NDc]);
+#pragma unroll
+                    for (int ab = 0; ab < NAB; ++ab) {
+                        Real t2 = 0;
+#pragma unroll
+                        for (int cd = 0; cd < NCD; ++cd) t2 += v[k][cd * NAB + ab] * dq[cd];
+                        acc[s * NAB + ab] += t2;
+                    }
                 }
             }
         }
     }
-#pragma unroll
-    for (int i = 0; i < NSET * NAB; ++i) Jacc[i] += double(acc[i]);
 }
 
 template <int CP_, int NSET>
@@ -149,11 +176,17 @@ __global__ __launch_bounds__(64) void k_jstore(JSArgs a) {
     with_part(cq, [&](auto pt) {
         using PT = decltype(pt);
         const size_t seg = (size_t)P * 6 + cq;
-        if (f32)
-            j_unit<float, PT, NSET>(a, a.jdata32 + a.jdoff32[seg] + (u0 - a.jseg[seg] - a.jbig[seg]) * PT::BLK, u0, u1,
-                                    lane, Jacc);
-        else
-            j_unit<double, PT, NSET>(a, a.jdata + a.jdoff[seg] + (u0 - a.jseg[seg]) * PT::BLK, u0, u1, lane, Jacc);
+        if (f32) {
+            // FP32 sums per unit, folded into the FP64 block
+            float acc[NSET * NAB];
+#pragma unroll
+            for (int i = 0; i < NSET * NAB; ++i) acc[i] = 0.0f;
+            j_unit<float, PT, NSET>(a, a.jdata32 + a.jdoff32[seg], (u0 - a.jseg[seg] - a.jbig[seg]) * PT::NS, u0, u1,
+                                    lane, acc);
+#pragma unroll
+            for (int i = 0; i < NSET * NAB; ++i) Jacc[i] = double(acc[i]);
+        } else
+            j_unit<double, PT, NSET>(a, a.jdata + a.jdoff[seg], (u0 - a.jseg[seg]) * PT::NS, u0, u1, lane, Jacc);
     });
     double* part = a.jpart + a.part_base[CP] + (size_t)(u - a.unit_base[CP]) * (NSET * NAB);
 #pragma unroll

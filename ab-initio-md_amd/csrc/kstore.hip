@@ -5,8 +5,12 @@
 // fixed number of lane items (entries, or (entry, yl) columns for blocks
 // above 36 doubles), one wavefront each, so heavy and light blocks balance.
 // Per geometry, k_kcopy lays each unit's stored quartets out in gather
-// order, reoriented to (XJ|YL) with the (y, l) index major:
-// kdata[seg + e * BLK + (y NL + l) NX NJ + x NJ + j] = (xj|yl). Per SCF
+// order, reoriented to (XJ|YL) with the (y, l) index major. A lane item
+// (an entry, or an (entry, yl) column when split) holds W consecutive values
+// of that order; items are stored in lane-interleaved tiles of 64, value
+// major: item T, value v at seg + (T / 64) 64 W + v 64 + T % 64, so every
+// load of the digestion and every store of the copy covers 64 consecutive
+// words. Per SCF
 // iteration, k_kstore streams them and writes one partial block per unit
 // (lanes summed by a fixed butterfly; nset NX NY doubles per unit, each
 // type in its own region); k_kreduce adds each block's unit
@@ -40,7 +44,8 @@ template <int LJ, int LL>
 struct Sub {
     static constexpr int NJ = NC(LJ), NL = NC(LL), NXJ = NX * NJ, NYL = NY * NL, BLK = NXJ * NYL;
     static constexpr bool SPLIT = BLK > 36;
-    static constexpr int NS = SPLIT ? NYL : 1;
+    static constexpr int NS = SPLIT ? NYL : 1;     // lane items per entry
+    static constexpr int W = BLK / NS;               // values per lane item
     static constexpr int K = qcls(pcls(LX, LJ), pcls(LY, LL));
 };
 
@@ -63,98 +68,122 @@ __global__ __launch_bounds__(64) void k_kcopy(KSArgs a) {
     with_sub(sub, [&](auto sb) {
         using SB = decltype(sb);
         constexpr int NJ = SB::NJ, NL = SB::NL, NXJ = SB::NXJ, NYL = SB::NYL, BLK = SB::BLK;
+        constexpr int NS = SB::NS, W = SB::W;
         const size_t seg = (size_t)o * 9 + sub;
         const double* base = a.store + a.store_base[SB::K];
-        double* out64 = a.kdata + a.kdoff[seg] + (u0 - a.kseg[seg]) * BLK;
-        float* out32 = a.kdata32 + a.kdoff32[seg] + (u0 - a.kseg[seg] - a.kbig[seg]) * BLK;
-        const long long nt = (u1 - u0) * BLK;
-        for (long long t = lane; t < nt; t += 64) {
-            const long long e = t / BLK;
-            const int el = (int)(t % BLK);
-            const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
-            const long long qi = (long long)(ent & 0xffffffffull);
+        const long long t0 = (u0 - a.kseg[seg] - (f32 ? a.kbig[seg] : 0)) * NS;
+        const long long nt = (u1 - u0) * NS;
+        // lanes over consecutive items, so each value row is written by one
+        // coalesced store; the W source loads of an item are independent
+        for (long long i = lane; i < nt; i += 64) {
+            const unsigned long long ent = (unsigned long long)a.klist[u0 + i / NS];
+            const int s = (int)(i % NS);
+            const double* blk = base + (long long)(ent & 0xffffffffull) * BLK;
             const bool sw = (ent >> 60) & 1, fb = (ent >> 61) & 1, fk = (ent >> 62) & 1;
-            const int yl = el / NXJ, xj = el % NXJ;
-            const int x = xj / NJ, j = xj % NJ, y = yl / NL, l = yl % NL;
-            const int i1 = fb ? j * NX + x : x * NJ + j;
-            const int i2 = fk ? l * NY + y : y * NL + l;
-            const double v = base[qi * BLK + (sw ? i2 * NXJ + i1 : i1 * NYL + i2)];
-            if (f32) out32[t] = (float)v;
-            else out64[t] = v;
+            double val[W];
+#pragma unroll
+            for (int v = 0; v < W; ++v) {
+                // element (xj, yl) of the (XJ|YL) order
+                const int yl = SB::SPLIT ? s : v / NXJ, xj = SB::SPLIT ? v : v % NXJ;
+                const int x = xj / NJ, j = xj % NJ, y = yl / NL, l = yl % NL;
+                const int i1 = fb ? j * NX + x : x * NJ + j;
+                const int i2 = fk ? l * NY + y : y * NL + l;
+                val[v] = blk[sw ? i2 * NXJ + i1 : i1 * NYL + i2];
+            }
+            const long long T = t0 + i;
+            const long long dst = (T >> 6) * (64 * W) + (T & 63);
+            if (f32) {
+                float* out = a.kdata32 + a.kdoff32[seg] + dst;
+#pragma unroll
+                for (int v = 0; v < W; ++v) out[v * 64] = (float)val[v];
+            } else {
+                double* out = a.kdata + a.kdoff[seg] + dst;
+#pragma unroll
+                for (int v = 0; v < W; ++v) out[v * 64] = val[v];
+            }
         }
     });
 }
 
-// one work unit of subclass SB in storage / arithmetic precision Real; its
-// per-lane sums are added to Kacc in FP64
+// items per lane and pass of the digestion: small items are batched so each
+// lane has several independent loads in flight
+template <int W>
+constexpr int batch_items() { return W <= 2 ? 4 : W <= 4 ? 2 : 1; }
+
+// one work unit of subclass SB in storage / arithmetic precision Real: the
+// part's tiles start at tile0, the unit's first lane item is t0 within the
+// part; its per-lane sums are added to acc
 template <typename Real, class SB, int NSET>
-__device__ __forceinline__ void k_unit(const KSArgs& a, const Real* __restrict__ blk0, long long u0, long long u1,
-                                       int lane, double* Kacc) {
-    constexpr int NJ = SB::NJ, NL = SB::NL, NXJ = SB::NXJ, NYL = SB::NYL, BLK = SB::BLK;
+__device__ __forceinline__ void k_unit(const KSArgs& a, const Real* __restrict__ tile0, long long t0, long long u0,
+                                       long long u1, int lane, Real* __restrict__ acc) {
+    constexpr int NJ = SB::NJ, NL = SB::NL, NXJ = SB::NXJ, NYL = SB::NYL, NS = SB::NS, W = SB::W;
+    constexpr int U = batch_items<W>();
     const size_t n = a.nao, nn = n * n;
-    Real acc[NSET * NX * NY];
+    const long long nt = (u1 - u0) * NS;
+    for (long long t = lane; t < nt; t += 64 * U) {
+        // loads of all U items first; an item past the end repeats item t
+        // with its density weight zeroed
+        unsigned long long ent[U];
+        Real v[U][W], wk[U];
+        int sk[U];
 #pragma unroll
-    for (int i = 0; i < NSET * NX * NY; ++i) acc[i] = Real(0);
-    if constexpr (SB::SPLIT) {
-        // lanes over (entry, yl) columns of NXJ contiguous values
-        const long long nt = (u1 - u0) * NYL;
-        for (long long t = lane; t < nt; t += 64) {
-            const long long e = t / NYL;
-            const int yl = (int)(t % NYL);
-            const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
-            const int oJ = (int)((ent >> 32) & 0x3fff), oL = (int)((ent >> 46) & 0x3fff);
-            const Real* col = blk0 + e * BLK + yl * NXJ;
-            Real v[NXJ];
+        for (int k = 0; k < U; ++k) {
+            const long long tk = t + 64 * k;
+            const long long tc = tk < nt ? tk : t;
+            wk[k] = tk < nt ? Real(1) : Real(0);
+            ent[k] = (unsigned long long)a.klist[u0 + tc / NS];
+            sk[k] = (int)(tc % NS);
+            const long long T = t0 + tc;
+            const Real* p = tile0 + (T >> 6) * (64 * W) + (T & 63);
 #pragma unroll
-            for (int i = 0; i < NXJ; ++i) v[i] = col[i];
-            const int y = yl / NL, l = yl % NL;
+            for (int i = 0; i < W; ++i) v[k][i] = p[i * 64];
+        }
 #pragma unroll
-            for (int s = 0; s < NSET; ++s) {
-                Real dj[NJ];
+        for (int k = 0; k < U; ++k) {
+            const int oJ = (int)((ent[k] >> 32) & 0x3fff), oL = (int)((ent[k] >> 46) & 0x3fff);
+            if constexpr (SB::SPLIT) {
+                // item = (entry, yl) column of NXJ values
+                const int y = sk[k] / NL, l = sk[k] % NL;
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) dj[j] = Real(a.dens[s * nn + (size_t)(oJ + j) * n + oL + l]);
+                for (int s = 0; s < NSET; ++s) {
+                    Real dj[NJ];
 #pragma unroll
-                for (int x = 0; x < NX; ++x) {
-                    Real t2 = 0;
+                    for (int j = 0; j < NJ; ++j) dj[j] = wk[k] * Real(a.dens[s * nn + (size_t)(oJ + j) * n + oL + l]);
 #pragma unroll
-                    for (int j = 0; j < NJ; ++j) t2 += v[x * NJ + j] * dj[j];
+                    for (int x = 0; x < NX; ++x) {
+                        Real t2 = 0;
 #pragma unroll
-                    for (int yy = 0; yy < NY; ++yy)
-                        if (yy == y) acc[(s * NX + x) * NY + yy] += t2;
+                        for (int j = 0; j < NJ; ++j) t2 += v[k][x * NJ + j] * dj[j];
+#pragma unroll
+                        for (int yy = 0; yy < NY; ++yy)
+                            if (yy == y) acc[(s * NX + x) * NY + yy] += t2;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < NSET; ++s) {
+                    Real djl[NJ * NL];
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                        for (int l = 0; l < NL; ++l)
+                            djl[j * NL + l] = wk[k] * Real(a.dens[s * nn + (size_t)(oJ + j) * n + oL + l]);
+#pragma unroll
+                    for (int x = 0; x < NX; ++x)
+#pragma unroll
+                        for (int y = 0; y < NY; ++y) {
+                            Real t2 = 0;
+#pragma unroll
+                            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                                for (int l = 0; l < NL; ++l)
+                                    t2 += v[k][(y * NL + l) * NXJ + x * NJ + j] * djl[j * NL + l];
+                            acc[(s * NX + x) * NY + y] += t2;
+                        }
                 }
             }
         }
-    } else {
-        for (long long e = lane; e < u1 - u0; e += 64) {
-            const unsigned long long ent = (unsigned long long)a.klist[u0 + e];
-            const int oJ = (int)((ent >> 32) & 0x3fff), oL = (int)((ent >> 46) & 0x3fff);
-            const Real* b = blk0 + e * BLK;
-            Real v[BLK];
-#pragma unroll
-            for (int i = 0; i < BLK; ++i) v[i] = b[i];
-#pragma unroll
-            for (int s = 0; s < NSET; ++s) {
-                Real djl[NJ * NL];
-#pragma unroll
-                for (int j = 0; j < NJ; ++j)
-#pragma unroll
-                    for (int l = 0; l < NL; ++l) djl[j * NL + l] = Real(a.dens[s * nn + (size_t)(oJ + j) * n + oL + l]);
-#pragma unroll
-                for (int x = 0; x < NX; ++x)
-#pragma unroll
-                    for (int y = 0; y < NY; ++y) {
-                        Real t2 = 0;
-#pragma unroll
-                        for (int j = 0; j < NJ; ++j)
-#pragma unroll
-                            for (int l = 0; l < NL; ++l) t2 += v[(y * NL + l) * NXJ + x * NJ + j] * djl[j * NL + l];
-                        acc[(s * NX + x) * NY + y] += t2;
-                    }
-            }
-        }
     }
-#pragma unroll
-    for (int i = 0; i < NSET * NX * NY; ++i) Kacc[i] += double(acc[i]);
 }
 
 template <int LX_, int LY_, int NSET>
@@ -170,11 +199,17 @@ __global__ __launch_bounds__(64) void k_kstore(KSArgs a) {
     with_sub(sub, [&](auto sb) {
         using SB = decltype(sb);
         const size_t seg = (size_t)o * 9 + sub;
-        if (f32)
-            k_unit<float, SB, NSET>(a, a.kdata32 + a.kdoff32[seg] + (u0 - a.kseg[seg] - a.kbig[seg]) * SB::BLK, u0, u1,
-                                    lane, Kacc);
-        else
-            k_unit<double, SB, NSET>(a, a.kdata + a.kdoff[seg] + (u0 - a.kseg[seg]) * SB::BLK, u0, u1, lane, Kacc);
+        if (f32) {
+            // FP32 sums per unit, folded into the FP64 block
+            float acc[NSET * NX * NY];
+#pragma unroll
+            for (int i = 0; i < NSET * NX * NY; ++i) acc[i] = 0.0f;
+            k_unit<float, SB, NSET>(a, a.kdata32 + a.kdoff32[seg], (u0 - a.kseg[seg] - a.kbig[seg]) * SB::NS, u0, u1,
+                                    lane, acc);
+#pragma unroll
+            for (int i = 0; i < NSET * NX * NY; ++i) Kacc[i] = double(acc[i]);
+        } else
+            k_unit<double, SB, NSET>(a, a.kdata + a.kdoff[seg], (u0 - a.kseg[seg]) * SB::NS, u0, u1, lane, Kacc);
     });
     double* part = a.kpart + a.part_base[OT] + (size_t)(u - a.unit_base[OT]) * (NSET * NX * NY);
 #pragma unroll
