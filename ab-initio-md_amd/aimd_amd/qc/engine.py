@@ -142,6 +142,21 @@ class Engine:
         _lib.check(_lib.lib().aimdhf_jk(self._h, D.shape[0], _ptr(D), _ptr(J), _ptr(K), _stream()))
         return (J[0], K[0]) if single else (J, K)
 
+    def jk_delta(self, ddens: torch.Tensor, threshold: float) -> tuple[torch.Tensor, torch.Tensor]:
+        """J, K of density differences for an incremental Fock build.
+
+        In store mode, work units whose largest Schwarz product times max |dD|
+        is below ``threshold`` are skipped, and in mixed precision the integrals
+        are digested in FP32. In direct mode this is :meth:`jk`.
+        """
+        single = ddens.dim() == 2
+        D = ddens.reshape(-1, self.nao, self.nao).to(device=self.device, dtype=torch.float64).contiguous()
+        J = torch.empty_like(D)
+        K = torch.empty_like(D)
+        _lib.check(_lib.lib().aimdhf_jk_delta(self._h, D.shape[0], _ptr(D), _ptr(J), _ptr(K), float(threshold),
+                                              _stream()))
+        return (J[0], K[0]) if single else (J, K)
+
     # ------------------------------------------------------- one-electron
     def one_electron(self, S: torch.Tensor, T: torch.Tensor, V: torch.Tensor, M: torch.Tensor) -> None:
         """Fill S, T, V (nao, nao) and the dipole integrals M (3, nao, nao) of the current geometry."""
@@ -182,7 +197,8 @@ class Engine:
                   + [f"kstore_{'spd'[t // 3]}{'spd'[t % 3]}" for t in range(9)]
                   + [f"grad_screen_{k}" for k in range(21)]
                   + ["host_energy_list", "host_owner_lists", "host_store_build", "host_gradient_list"]
-                  + [f"unused_{k}" for k in range(89, 96)])
+                  + ["host_store_counts", "host_store_bands", "host_store_fill", "host_store_units"]
+                  + [f"unused_{k}" for k in range(93, 96)])
     NKIND = 96
     FOCK_KINDS = list(range(0, 15)) + [45, 46] + list(range(49, 64))    # per-iteration J/K work, either mode
     STORE_KINDS = [15, 47, 48]                                          # per-geometry store build

@@ -58,7 +58,12 @@ class SCFOptions:
     have no counterpart in the reference: they choose between recomputing the
     screened integrals every J/K build and evaluating them once per geometry
     into device memory (auto: store when it fits in max_store_bytes, 0 = half
-    the free device memory).
+    the free device memory). ``incremental_fock`` (default: on in mixed
+    precision, off in FP64) builds J and K of every iteration after the first
+    from the density difference, J(D) = J(D_prev) + J(D - D_prev), skipping in
+    store mode the work units whose largest Schwarz product times max |dD| is
+    below ``incremental_threshold`` (default: the Schwarz threshold) and, in
+    mixed precision, digesting the difference in FP32.
     """
     conv_energy: float = 1e-10
     conv_commutator: float = 1e-7
@@ -82,6 +87,8 @@ class SCFOptions:
     precision: str = "fp64"
     jk_fp32_switch: float = 1e-5
     grad_fp32_switch: float = 1e-7
+    incremental_fock: bool | None = None
+    incremental_threshold: float | None = None
 
     def __post_init__(self) -> None:
         if not (self.conv_energy > 0.0 and self.conv_commutator > 0.0):
@@ -114,6 +121,9 @@ class SCFOptions:
             raise ValueError(f"unknown precision {self.precision!r}; use one of {sorted(qengine.Engine.PRECISIONS)}")
         if not (self.jk_fp32_switch >= 0.0 and self.grad_fp32_switch >= 0.0):
             raise ValueError("fp32 switch thresholds must be >= 0")
+        if self.incremental_threshold is not None and not (self.incremental_threshold >= 0.0
+                                                            and np.isfinite(self.incremental_threshold)):
+            raise ValueError("incremental_threshold must be finite and >= 0")
         if self.density_solver not in _DENSITY_SOLVERS:
             raise ValueError(f"unknown density_solver {self.density_solver!r}; use one of {_DENSITY_SOLVERS}")
 
@@ -870,11 +880,17 @@ class SCFSolver:
                 SCFConvergenceWarning, stacklevel=stacklevel)
         return res
 
-    def _fock(self, I: SCFIntegrals, Ds: torch.Tensor) -> torch.Tensor:
+    def _fock(self, I: SCFIntegrals, Ds: torch.Tensor, jk: tuple | None = None) -> torch.Tensor:
         g = 2.0 / Ds.shape[0]
-        J, K = I.engine.jk(Ds)
+        J, K = I.engine.jk(Ds) if jk is None else jk
         F = I.H + g * J.sum(dim=0) - K
         return 0.5 * (F + F.transpose(1, 2))
+
+    @property
+    def incremental(self) -> bool:
+        """Whether SCF iterations after the first build J/K from density differences."""
+        inc = self.options.incremental_fock
+        return self.options.precision == "mixed" if inc is None else bool(inc)
 
     def _next_density(self, Fo: torch.Tensor, X: torch.Tensor, nocc: tuple[int, ...],
                       Ds: torch.Tensor) -> torch.Tensor:
@@ -910,8 +926,20 @@ class SCFSolver:
         history: list[dict[str, float]] = []
         converged = False
         eye = torch.eye(I.nmo, dtype=torch.float64, device=S.device)
+        incremental = self.incremental
+        dthr = opt.schwarz_threshold if opt.incremental_threshold is None else opt.incremental_threshold
+        jk = None
+        D_built = None
         for it in range(1, opt.max_iter + 1):
-            F = self._fock(I, Ds)
+            if incremental:
+                # J(D) = J(D_built) + J(D - D_built): the first build is complete
+                if jk is None:
+                    jk = I.engine.jk(Ds)
+                else:
+                    dJ, dK = I.engine.jk_delta(Ds - D_built, dthr)
+                    jk = (jk[0] + dJ, jk[1] + dK)
+                D_built = Ds
+            F = self._fock(I, Ds, jk)
             e_dev = 0.5 * g * torch.sum(Ds * (H + F))
             FDS = F @ Ds @ S
             err = X.T @ (FDS - FDS.transpose(1, 2)) @ X

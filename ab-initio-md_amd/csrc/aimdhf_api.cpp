@@ -39,13 +39,16 @@ struct HipError {
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    // grows with 1/8 slack: per-geometry sizes drift along a trajectory, and each
+    // reallocation (hipFree synchronises the device) would stall the pipeline
     void ensure(size_t n) {
         if (n <= bytes) return;
         if (p) HIPCHECK(hipFree(p));
         p = nullptr;
         bytes = 0;
-        HIPCHECK(hipMalloc(&p, std::max<size_t>(n, 256)));
-        bytes = std::max<size_t>(n, 256);
+        const size_t want = std::max<size_t>(n + n / 8, 256);
+        HIPCHECK(hipMalloc(&p, want));
+        bytes = want;
     }
     void release() {
         if (p) hipFree(p);
@@ -235,7 +238,6 @@ struct aimdhf_plan {
     bool stored = false;
     long long store_base[NQCLASS + 1] = {0};
     long long store_bytes = 0, k_entries = 0, j_entries = 0;
-    std::vector<long long> kseg, jseg;
     DevBuf d_gkeep, d_gsel, d_nsel, d_gblk;
     // precision: 0 FP64 everywhere; 1 mixed (FP32 below the switches)
     int precision = 0;
@@ -247,6 +249,12 @@ struct aimdhf_plan {
     static constexpr int UNIT_ITEMS = 1024;
     int kunit_base[10] = {0}, junit_base[7] = {0};
     DevBuf d_kunit_os, d_kunit_e, d_out_unit, d_kpart2, d_junit_pc, d_junit_e, d_pair_unit, d_jpart2;
+    // density-difference builds: per-unit largest Schwarz products, max |dD|,
+    // and in mixed precision FP32 copies of the FP64 parts
+    DevBuf d_kunit_q, d_junit_q, d_dmax, d_kdata32b, d_jdata32b;
+    bool have_32b = false;
+    DevBuf d_tab;                   // store-table temporaries (sizes, unit counts, scan blocks)
+    long long* h_pin = nullptr;     // pinned host words for the totals and the error flag
 
     ~aimdhf_plan() {
         for (DevBuf* b : {&d_shell_atom, &d_shell_l, &d_shell_prim, &d_shell_ao, &d_prim_exp, &d_prim_coef,
@@ -259,8 +267,10 @@ struct aimdhf_plan {
                           &d_jcnt, &d_kseg, &d_jseg, &d_klist, &d_jlist, &d_lerr, &d_gkeep, &d_gsel, &d_nsel, &d_gblk,
                           &d_kdata, &d_jdata, &d_kdoff, &d_jdoff, &d_kdata32, &d_jdata32, &d_kdoff32, &d_jdoff32,
                           &d_kbig, &d_jbig, &d_kunit_os, &d_kunit_e, &d_out_unit,
-                          &d_kpart2, &d_junit_pc, &d_junit_e, &d_pair_unit, &d_jpart2})
+                          &d_kpart2, &d_junit_pc, &d_junit_e, &d_pair_unit, &d_jpart2, &d_kunit_q, &d_junit_q,
+                          &d_dmax, &d_kdata32b, &d_jdata32b, &d_tab})
             b->release();
+        if (h_pin) (void)hipHostFree(h_pin);
     }
 
     aimdhf::PairArgs pair_args() const {
@@ -444,6 +454,11 @@ struct aimdhf_plan {
         ka.kpart = d_kpart2.as<double>();
         ka.dens = dens;
         ka.K = K;
+        ka.delta = 0;
+        ka.unit_q = d_kunit_q.as<double>();
+        ka.dmax = d_dmax.as<double>();
+        ka.dthr = 0.0;
+        ka.kdata32b = have_32b ? d_kdata32b.as<float>() : nullptr;
         return ka;
     }
 
@@ -478,6 +493,11 @@ struct aimdhf_plan {
         ja.jpart = d_jpart2.as<double>();
         ja.dens = dens;
         ja.J = J;
+        ja.delta = 0;
+        ja.unit_q = d_junit_q.as<double>();
+        ja.dmax = d_dmax.as<double>();
+        ja.dthr = 0.0;
+        ja.jdata32b = have_32b ? d_jdata32b.as<float>() : nullptr;
         return ja;
     }
 
@@ -494,28 +514,43 @@ struct aimdhf_plan {
         d_store.ensure(sizeof(double) * std::max<long long>(off, 1));
         d_qmap.ensure(sizeof(int) * std::max<long long>(keys, 1));
         d_lerr.ensure(sizeof(int));
-        prof.time(47, st, [&] {
-            aimdhf::launch_fill_int(d_qmap.as<int>(), (size_t)keys, -1, st);
-            aimdhf::launch_fill_int(d_lerr.as<int>(), 1, 0, st);
+        // the integrals run on the library streams while the caller's stream builds
+        // the gather lists (they need only the pair-pair map); joined before the copies
+        fan.fork(st);
+        {
             aimdhf::StoreArgs sa;
             sa.recs = d_recs.as<double>();
             sa.rec_off = d_rec_off.as<long long>();
             sa.npp = d_npp.as<int>();
             sa.boys = d_boys.as<double>();
-            sa.qmap = d_qmap.as<int>();
-            fan.fork(st);
             for (int k = 0, i = 0; k < NQCLASS; ++k) {
                 if (!qcount[k]) continue;
                 sa.store = d_store.as<double>() + store_base[k];
-                aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], fan.on(i++));
+                hipStream_t s2 = fan.on(1 + i++ % Fan::NAUX);
+                prof.time(47, s2, [&] { aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], s2); });
             }
-            fan.join();
+        }
+        prof.time(48, st, [&] {
+            aimdhf::launch_fill_int(d_qmap.as<int>(), (size_t)keys, -1, st);
+            aimdhf::launch_fill_int(d_lerr.as<int>(), 1, 0, st);
+            aimdhf::QmapArgs qa;
+            qa.list = d_list.as<int2>();
+            qa.qoff[0] = 0;
+            for (int k = 0; k < NQCLASS; ++k) qa.qoff[k + 1] = qa.qoff[k] + qcount[k];
+            qa.qmap = d_qmap.as<int>();
+            aimdhf::launch_qmap(qa, st);
         });
         HIPCHECK(hipGetLastError());
         aimdhf::ListArgs la;
         la.npair = npair;
         la.thr = threshold;
-        la.sw = (precision == 1 && jk_switch > threshold) ? jk_switch : threshold;
+        // mixed precision: bands two decades apart from the FP64 / FP32 switch down, the
+        // first held in FP64; FP64: one band (finer units only slow the complete builds)
+        const bool split = precision == 1 && jk_switch > threshold;
+        const int nbig = split ? 1 : aimdhf::NBAND;   // leading bands held in FP64
+        for (int b = 0; b < aimdhf::NBAND - 1; ++b)
+            la.band_lo[b] = split ? std::max(jk_switch * std::pow(1e-2, b), threshold) : threshold;
+        la.band_lo[aimdhf::NBAND - 1] = threshold;
         la.out_x = d_out_x.as<int>();
         la.out_y = d_out_y.as<int>();
         la.shell_l = d_shell_l.as<int>();
@@ -535,151 +570,150 @@ struct aimdhf_plan {
         for (int k = 0; k < NQCLASS; ++k) la.qcount[k] = qcount[k];
         la.err = d_lerr.as<int>();
         const int nout = out_off[9];
-        d_kcnt.ensure(sizeof(long long) * 2 * (size_t)nout * 9);
-        d_jcnt.ensure(sizeof(long long) * 2 * (size_t)npair * 6);
-        std::vector<long long> kc((size_t)nout * 9), jc((size_t)npair * 6), kb(kc.size()), jb(jc.size());
+        constexpr int NB = aimdhf::NBAND;
+        const long long nks = (long long)nout * 9, njs = (long long)npair * 6;
+        if (nks * NB + 1 > aimdhf::scan_max_items() || njs * NB + 1 > aimdhf::scan_max_items()) {
+            g_err = "integral store: too many gather segments for the table scans";
+            throw HipError{AIMDHF_E_UNSUPPORTED};
+        }
+        // device tables: per-band counts -> sizes -> scans -> units; only totals come back
+        d_kcnt.ensure(sizeof(long long) * (NB + 1) * std::max<long long>(nks, 1));   // bands, then totals
+        d_jcnt.ensure(sizeof(long long) * (NB + 1) * std::max<long long>(njs, 1));
+        d_kseg.ensure(sizeof(long long) * (nks + 1));
+        d_jseg.ensure(sizeof(long long) * (njs + 1));
+        d_kdoff.ensure(sizeof(long long) * (nks + 1));
+        d_kdoff32.ensure(sizeof(long long) * (nks + 1));
+        d_jdoff.ensure(sizeof(long long) * (njs + 1));
+        d_jdoff32.ensure(sizeof(long long) * (njs + 1));
+        d_kbig.ensure(sizeof(long long) * std::max<long long>(nks, 1));
+        d_jbig.ensure(sizeof(long long) * std::max<long long>(njs, 1));
+        const long long nku_slots = nks * NB + 1, nju_slots = njs * NB + 1;
+        const long long nbsum = (std::max(nku_slots, nju_slots) + 4095) / 4096 + 1;
+        const long long tab_n = 3 * nks + (nks * NB) + nku_slots + 3 * njs + (njs * NB) + nju_slots + nbsum + 32;
+        d_tab.ensure(sizeof(long long) * tab_n);
+        d_out_unit.ensure(sizeof(int) * (nout + 1));
+        d_pair_unit.ensure(sizeof(int) * (npair + 1));
+        aimdhf::TabArgs ta;
+        ta.nout = nout;
+        ta.npair = npair;
+        ta.nbig = nbig;
+        ta.unit_items = UNIT_ITEMS;
+        for (int t = 0; t < 10; ++t) ta.out_off[t] = out_off[t];
+        for (int c = 0; c < 7; ++c) ta.cls_pair_off[c] = cls_pair_off[c];
+        ta.out_x = d_out_x.as<int>();
+        ta.out_y = d_out_y.as<int>();
+        ta.shell_l = d_shell_l.as<int>();
+        ta.pair_cls = d_pair_cls.as<int>();
+        ta.cls_pairs = d_cls_pairs.as<int>();
+        ta.kband = d_kcnt.as<long long>();
+        ta.jband = d_jcnt.as<long long>();
+        long long* tp = d_tab.as<long long>();
+        auto take = [&](long long n) { long long* r = tp; tp += n; return r; };
+        ta.kcount = take(nks);
+        ta.ksz64 = take(nks);
+        ta.ksz32 = take(nks);
+        ta.knu = take(nks * NB);
+        ta.kustart = take(nku_slots);
+        ta.jcount = take(njs);
+        ta.jsz64 = take(njs);
+        ta.jsz32 = take(njs);
+        ta.jnu = take(njs * NB);
+        ta.justart = take(nju_slots);
+        long long* bsum = take(nbsum);
+        long long* tot = take(32);      // totals and check flag of launch_tab_totals
+        ta.kbig = d_kbig.as<long long>();
+        ta.jbig = d_jbig.as<long long>();
+        ta.kseg = d_kseg.as<long long>();
+        ta.kdoff = d_kdoff.as<long long>();
+        ta.kdoff32 = d_kdoff32.as<long long>();
+        ta.jseg = d_jseg.as<long long>();
+        ta.jdoff = d_jdoff.as<long long>();
+        ta.jdoff32 = d_jdoff32.as<long long>();
+        ta.out_unit = d_out_unit.as<int>();
+        ta.pair_unit = d_pair_unit.as<int>();
+        ta.kunit_os = nullptr;
+        ta.kunit_e = nullptr;
+        ta.junit_pc = nullptr;
+        ta.junit_e = nullptr;
+        la.kband = d_kcnt.as<long long>();
+        if (!h_pin) HIPCHECK(hipHostMalloc((void**)&h_pin, sizeof(long long) * 64));
+        auto th = Profiler::clock::now();
         prof.time(48, st, [&] {
-            aimdhf::launch_klist_count(la, nout, d_kcnt.as<long long>(), d_kcnt.as<long long>() + kc.size(), st);
-            aimdhf::launch_jlist_count(la, d_jcnt.as<long long>(), d_jcnt.as<long long>() + jc.size(), st);
+            aimdhf::launch_klist_count(la, nout, d_kcnt.as<long long>() + NB * nks, d_kcnt.as<long long>(), st);
+            aimdhf::launch_jlist_count(la, d_jcnt.as<long long>() + NB * njs, d_jcnt.as<long long>(), st);
+            aimdhf::launch_seg_sizes(ta, st);
+            aimdhf::launch_scan(ta.kcount, ta.kseg, nks, bsum, st);
+            aimdhf::launch_scan(ta.ksz64, ta.kdoff, nks, bsum, st);
+            aimdhf::launch_scan(ta.ksz32, ta.kdoff32, nks, bsum, st);
+            aimdhf::launch_scan(ta.knu, ta.kustart, nks * NB, bsum, st);
+            aimdhf::launch_scan(ta.jcount, ta.jseg, njs, bsum, st);
+            aimdhf::launch_scan(ta.jsz64, ta.jdoff, njs, bsum, st);
+            aimdhf::launch_scan(ta.jsz32, ta.jdoff32, njs, bsum, st);
+            aimdhf::launch_scan(ta.jnu, ta.justart, njs * NB, bsum, st);
+            aimdhf::launch_tab_totals(ta, tot, st);
         });
         HIPCHECK(hipGetLastError());
-        HIPCHECK(hipMemcpyAsync(kc.data(), d_kcnt.p, sizeof(long long) * kc.size(), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(kb.data(), d_kcnt.as<long long>() + kc.size(), sizeof(long long) * kb.size(),
-                                hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(jc.data(), d_jcnt.p, sizeof(long long) * jc.size(), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(jb.data(), d_jcnt.as<long long>() + jc.size(), sizeof(long long) * jb.size(),
-                                hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(h_pin, tot, sizeof(long long) * 26, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        upload(d_kbig, kb);
-        upload(d_jbig, jb);
-        la.kbig = d_kbig.as<long long>();
-        kseg.assign(kc.size() + 1, 0);
-        for (size_t i = 0; i < kc.size(); ++i) kseg[i + 1] = kseg[i] + kc[i];
-        jseg.assign(jc.size() + 1, 0);
-        for (size_t i = 0; i < jc.size(); ++i) jseg[i + 1] = jseg[i] + jc[i];
-        k_entries = kseg.back();
-        j_entries = jseg.back();
-        upload(d_kseg, kseg);
-        upload(d_jseg, jseg);
-        d_klist.ensure(sizeof(long long) * std::max<long long>(k_entries, 1));
-        d_jlist.ensure(sizeof(long long) * std::max<long long>(j_entries, 1));
-        la.kseg = d_kseg.as<long long>();
-        la.klist = d_klist.as<long long>();
-        la.jseg = d_jseg.as<long long>();
-        la.jlist = d_jlist.as<long long>();
-        prof.time(48, st, [&] {
-            aimdhf::launch_klist_fill(la, nout, st);
-            aimdhf::launch_jlist_fill(la, st);
-        });
-        HIPCHECK(hipGetLastError());
-        int err = 0;
-        HIPCHECK(hipMemcpyAsync(&err, d_lerr.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        if (err) {
-            g_err = "integral store: a gather-list quartet is missing from the screened list";
+        if (h_pin[25]) {
+            g_err = "integral store: inconsistent gather-segment tables";
             throw HipError{AIMDHF_E_STATE};
         }
-        // gather-ordered copies: segment offsets, then one copy kernel per type / class.
-        // A part of n entries holds n * ns lane items of blk / ns values in
-        // tiles of 64 items (blocks above 36 values are split into ns columns).
-        static const int ncp[6] = {1, 3, 9, 6, 18, 36};
-        auto tiled = [](long long n, long long blk, long long ns) { return (n * ns + 63) / 64 * 64 * (blk / ns); };
-        std::vector<long long> kdoff((size_t)nout * 9), jdoff((size_t)npair * 6), kdoff32(kdoff.size()),
-            jdoff32(jdoff.size());
-        long long ko = 0, ko32 = 0;
-        for (int o = 0; o < nout; ++o) {
-            const int nx = ncart(shell_l[out_x[o]]), ny = ncart(shell_l[out_y[o]]);
-            for (int sub = 0; sub < 9; ++sub) {
-                const size_t i = (size_t)o * 9 + sub;
-                const long long blk = (long long)nx * ncart(sub / 3) * ny * ncart(sub % 3);
-                const long long ns = blk > 36 ? ny * ncart(sub % 3) : 1;
-                kdoff[i] = ko;
-                kdoff32[i] = ko32;
-                ko += tiled(kb[i], blk, ns);
-                ko32 += tiled(kseg[i + 1] - kseg[i] - kb[i], blk, ns);
-            }
-        }
-        long long jo = 0, jo32 = 0;
-        for (int P = 0; P < npair; ++P)
-            for (int c = 0; c < 6; ++c) {
-                const size_t i = (size_t)P * 6 + c;
-                const long long blk = (long long)ncp[pair_cls[P]] * ncp[c];
-                const long long ns = blk > 36 ? ncp[c] : 1;
-                jdoff[i] = jo;
-                jdoff32[i] = jo32;
-                jo += tiled(jb[i], blk, ns);
-                jo32 += tiled(jseg[i + 1] - jseg[i] - jb[i], blk, ns);
-            }
+        prof.host(89, th);
+        th = Profiler::clock::now();
+        k_entries = h_pin[0];
+        j_entries = h_pin[1];
+        const long long ko = h_pin[2], ko32 = h_pin[3], jo = h_pin[4], jo32 = h_pin[5];
+        const long long nku = h_pin[6], nju = h_pin[7];
+        for (int t = 0; t < 10; ++t) kunit_base[t] = (int)h_pin[8 + t];
+        for (int c = 0; c < 7; ++c) junit_base[c] = (int)h_pin[18 + c];
         kdata_total = ko;
         jdata_total = jo;
         kdata32_total = ko32;
         jdata32_total = jo32;
-        // work units: each segment cut into pieces of at most UNIT_ITEMS lane items
-        {
-            std::vector<int> uos, out_unit(nout + 1);
-            std::vector<long long> ue;
-            for (int o = 0; o < nout; ++o) {
-                out_unit[o] = (int)uos.size();
-                const int nx = ncart(shell_l[out_x[o]]), ny = ncart(shell_l[out_y[o]]);
-                for (int sub = 0; sub < 9; ++sub) {
-                    const size_t i = (size_t)o * 9 + sub;
-                    const int blk = nx * ncart(sub / 3) * ny * ncart(sub % 3);
-                    const int ns = blk > 36 ? ny * ncart(sub % 3) : 1;
-                    const long long per = std::max(1, UNIT_ITEMS / ns);
-                    for (int f32 = 0; f32 < 2; ++f32) {
-                        const long long e0 = f32 ? kseg[i] + kb[i] : kseg[i], e1 = f32 ? kseg[i + 1] : kseg[i] + kb[i];
-                        for (long long e = e0; e < e1; e += per) {
-                            uos.push_back(o << 5 | f32 << 4 | sub);
-                            ue.push_back(e);
-                            ue.push_back(std::min(e + per, e1));
-                        }
-                    }
-                }
-            }
-            out_unit[nout] = (int)uos.size();
-            for (int t = 0; t < 10; ++t) kunit_base[t] = out_unit[out_off[t]];
-            upload(d_kunit_os, uos);
-            upload(d_kunit_e, ue);
-            upload(d_out_unit, out_unit);
-            d_kpart2.ensure(sizeof(double) * 2 * 36 * std::max<size_t>(uos.size(), 1));
-        }
-        {
-            std::vector<int> upc, pair_unit(npair + 1);
-            std::vector<long long> ue;
-            for (int c = 0; c < 6; ++c)
-                for (int idx = cls_pair_off[c]; idx < cls_pair_off[c + 1]; ++idx) {
-                    pair_unit[idx] = (int)upc.size();
-                    const int P = cls_pairs[idx];
-                    for (int cq = 0; cq < 6; ++cq) {
-                        const size_t i = (size_t)P * 6 + cq;
-                        const int blk = ncp[c] * ncp[cq];
-                        const int ns = blk > 36 ? ncp[cq] : 1;
-                        const long long per = std::max(1, UNIT_ITEMS / ns);
-                        for (int f32 = 0; f32 < 2; ++f32) {
-                            const long long e0 = f32 ? jseg[i] + jb[i] : jseg[i], e1 = f32 ? jseg[i + 1] : jseg[i] + jb[i];
-                            for (long long e = e0; e < e1; e += per) {
-                                upc.push_back(P << 4 | f32 << 3 | cq);
-                                ue.push_back(e);
-                                ue.push_back(std::min(e + per, e1));
-                            }
-                        }
-                    }
-                }
-            pair_unit[npair] = (int)upc.size();
-            for (int c = 0; c < 7; ++c) junit_base[c] = pair_unit[cls_pair_off[c]];
-            upload(d_junit_pc, upc);
-            upload(d_junit_e, ue);
-            upload(d_pair_unit, pair_unit);
-            d_jpart2.ensure(sizeof(double) * 2 * 36 * std::max<size_t>(upc.size(), 1));
-        }
-        upload(d_kdoff, kdoff);
-        upload(d_jdoff, jdoff);
-        upload(d_kdoff32, kdoff32);
-        upload(d_jdoff32, jdoff32);
+        d_klist.ensure(sizeof(long long) * std::max<long long>(k_entries, 1));
+        d_jlist.ensure(sizeof(long long) * std::max<long long>(j_entries, 1));
+        d_kunit_os.ensure(sizeof(int) * std::max<long long>(nku, 1));
+        d_kunit_e.ensure(sizeof(long long) * 2 * std::max<long long>(nku, 1));
+        d_junit_pc.ensure(sizeof(int) * std::max<long long>(nju, 1));
+        d_junit_e.ensure(sizeof(long long) * 2 * std::max<long long>(nju, 1));
+        d_kpart2.ensure(sizeof(double) * 2 * 36 * std::max<long long>(nku, 1));
+        d_jpart2.ensure(sizeof(double) * 2 * 36 * std::max<long long>(nju, 1));
         d_kdata.ensure(sizeof(double) * std::max<long long>(ko, 1));
         d_jdata.ensure(sizeof(double) * std::max<long long>(jo, 1));
+        have_32b = precision == 1;
+        if (have_32b) {
+            d_kdata32b.ensure(sizeof(float) * std::max<long long>(ko, 1));
+            d_jdata32b.ensure(sizeof(float) * std::max<long long>(jo, 1));
+        }
         d_kdata32.ensure(sizeof(float) * std::max<long long>(ko32, 1));
         d_jdata32.ensure(sizeof(float) * std::max<long long>(jo32, 1));
+        ta.kunit_os = d_kunit_os.as<int>();
+        ta.kunit_e = d_kunit_e.as<long long>();
+        ta.junit_pc = d_junit_pc.as<int>();
+        ta.junit_e = d_junit_e.as<long long>();
+        la.kseg = d_kseg.as<long long>();
+        la.klist = d_klist.as<long long>();
+        la.jseg = d_jseg.as<long long>();
+        la.jlist = d_jlist.as<long long>();
+        prof.host(90, th);
+        th = Profiler::clock::now();
+        prof.time(48, st, [&] {
+            aimdhf::launch_klist_fill(la, nout, st);
+            aimdhf::launch_jlist_fill(la, st);
+            aimdhf::launch_units(ta, st);
+        });
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(h_pin + 32, d_lerr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (*(const int*)(h_pin + 32)) {
+            g_err = "integral store: a gather-list quartet is missing from the screened list";
+            throw HipError{AIMDHF_E_STATE};
+        }
+        prof.host(91, th);
+        th = Profiler::clock::now();
+        prof.host(92, th);
+        fan.join();
         prof.time(15, st, [&] {
             fan.fork(st);
             int i = 0;
@@ -697,8 +731,39 @@ struct aimdhf_plan {
             fan.join();
         });
         HIPCHECK(hipGetLastError());
-        store_bytes = 8 * (off + ko + jo) + 4 * (ko32 + jo32) + 4 * keys +
-                      8 * (k_entries + j_entries + 2 * kdoff.size() + 2 * jdoff.size());
+        // largest Schwarz product of every work unit, for density-difference screening
+        {
+            const int nku = kunit_base[9], nju = junit_base[6];
+            d_kunit_q.ensure(sizeof(double) * std::max(nku, 1));
+            d_junit_q.ensure(sizeof(double) * std::max(nju, 1));
+            d_dmax.ensure(sizeof(double));
+            aimdhf::UnitQArgs ua;
+            ua.list = d_list.as<int2>();
+            long long qo = 0;
+            for (int k = 0; k < NQCLASS; ++k) {
+                ua.qoff[k] = qo;
+                qo += qcount[k];
+            }
+            ua.qscr = d_qscr.as<double>();
+            ua.shell_l = d_shell_l.as<int>();
+            ua.out_x = d_out_x.as<int>();
+            ua.out_y = d_out_y.as<int>();
+            ua.pair_cls = d_pair_cls.as<int>();
+            ua.klist = d_klist.as<long long>();
+            ua.kunit_os = d_kunit_os.as<int>();
+            ua.kunit_e = d_kunit_e.as<long long>();
+            ua.nkunit = nku;
+            ua.kunit_q = d_kunit_q.as<double>();
+            ua.jlist = d_jlist.as<long long>();
+            ua.junit_pc = d_junit_pc.as<int>();
+            ua.junit_e = d_junit_e.as<long long>();
+            ua.njunit = nju;
+            ua.junit_q = d_junit_q.as<double>();
+            prof.time(48, st, [&] { aimdhf::launch_unit_q(ua, st); });
+            HIPCHECK(hipGetLastError());
+        }
+        store_bytes = 8 * (off + ko + jo) + 4 * (ko32 + jo32) + (have_32b ? 4 * (ko + jo) : 0) + 4 * keys +
+                      8 * (k_entries + j_entries + 4 * nks + 4 * njs);
     }
 
     aimdhf::OneElArgs oneel_args() const {
@@ -1005,40 +1070,60 @@ int aimdhf_grad_quartets(aimdhf_plan* p, int64_t* out, int64_t cap, int64_t* cou
     return AIMDHF_OK;
 }
 
+// J and K from the integral store; delta: dens is a density difference and
+// units below dthr (largest Schwarz product times max |dens|) are skipped
+static int jk_from_store(aimdhf_plan* p, int nset, const double* dens, double* J, double* K, bool delta, double dthr,
+                         hipStream_t st) {
+    try {
+        const size_t nn = (size_t)p->nao * p->nao;
+        p->d_dsym.ensure(sizeof(double) * nset * nn);
+        p->prof.time(45, st, [&] {
+            aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st);
+            if (delta) aimdhf::launch_absmax(p->d_dsym.as<double>(), nset * nn, p->d_dmax.as<double>(), st);
+        });
+        // the 15 J and K launches are independent (disjoint outputs): spread over the fan's streams
+        p->fan.fork(st);
+        int i = 0;
+        aimdhf::KSArgs ka = p->ks_args(nset, p->d_dsym.as<double>(), K);
+        aimdhf::JSArgs ja = p->js_args(nset, p->d_dsym.as<double>(), J);
+        ka.delta = ja.delta = delta ? 1 : 0;
+        ka.dthr = ja.dthr = dthr;
+        for (int t = 8; t >= 0; --t) {
+            int nout = p->out_off[t + 1] - p->out_off[t];
+            if (nout <= 0) continue;
+            hipStream_t s2 = p->fan.on(i++);
+            p->prof.time(55 + t, s2, [&] { aimdhf::launch_kstore(t, ka, nout, s2); });
+        }
+        for (int c = 5; c >= 0; --c) {
+            if (!p->ncls[c]) continue;
+            ja.cls_off = p->cls_pair_off[c];
+            hipStream_t s2 = p->fan.on(i++);
+            p->prof.time(49 + c, s2, [&] { aimdhf::launch_jstore(c, ja, p->ncls[c], s2); });
+        }
+        p->fan.join();
+        HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_jk_delta(aimdhf_plan* p, int nset, const double* ddens, double* dJ, double* dK, double threshold,
+                    void* stream) {
+    if (!p || !ddens || !dJ || !dK) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (nset < 1 || nset > 2) return fail(AIMDHF_E_ARG, "nset must be 1 or 2");
+    if (!(threshold >= 0.0) || !std::isfinite(threshold)) return fail(AIMDHF_E_ARG, "threshold must be finite and >= 0");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    if (!p->stored) return aimdhf_jk(p, nset, ddens, dJ, dK, stream);
+    return jk_from_store(p, nset, ddens, dJ, dK, true, threshold, (hipStream_t)stream);
+}
+
 int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K, void* stream) {
     if (!p || !dens || !J || !K) return fail(AIMDHF_E_ARG, "NULL argument");
     if (nset < 1 || nset > 2) return fail(AIMDHF_E_ARG, "nset must be 1 or 2");
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
-    if (p->stored) {
-        try {
-            const size_t nn = (size_t)p->nao * p->nao;
-            p->d_dsym.ensure(sizeof(double) * nset * nn);
-            p->prof.time(45, st, [&] { aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st); });
-            // the 15 J and K launches are independent (disjoint outputs): spread over the fan's streams
-            p->fan.fork(st);
-            int i = 0;
-            aimdhf::KSArgs ka = p->ks_args(nset, p->d_dsym.as<double>(), K);
-            for (int t = 8; t >= 0; --t) {
-                int nout = p->out_off[t + 1] - p->out_off[t];
-                if (nout <= 0) continue;
-                hipStream_t s2 = p->fan.on(i++);
-                p->prof.time(55 + t, s2, [&] { aimdhf::launch_kstore(t, ka, nout, s2); });
-            }
-            aimdhf::JSArgs ja = p->js_args(nset, p->d_dsym.as<double>(), J);
-            for (int c = 5; c >= 0; --c) {
-                if (!p->ncls[c]) continue;
-                ja.cls_off = p->cls_pair_off[c];
-                hipStream_t s2 = p->fan.on(i++);
-                p->prof.time(49 + c, s2, [&] { aimdhf::launch_jstore(c, ja, p->ncls[c], s2); });
-            }
-            p->fan.join();
-            HIPCHECK(hipGetLastError());
-        } catch (const HipError& e) {
-            return e.code;
-        }
-        return AIMDHF_OK;
-    }
+    if (p->stored) return jk_from_store(p, nset, dens, J, K, false, 0.0, st);
     try {
         const size_t nn = (size_t)p->nao * p->nao;
         p->d_dsym.ensure(sizeof(double) * nset * nn);

@@ -170,13 +170,64 @@ struct StoreArgs {
     const int* npp;
     const double* boys;
     double* store;              // this class's blocks
-    int* qmap;                  // pair-pair key -> index within its class
 };
+
+// device tables of the store's gather segments (see store_tables.hip)
+struct TabArgs {
+    int nout, npair, nbig, unit_items;
+    int out_off[10];
+    int cls_pair_off[7];
+    const int* out_x;
+    const int* out_y;
+    const int* shell_l;
+    const int* pair_cls;
+    const int* cls_pairs;
+    const long long* kband;     // (nout * 9, NBAND) entries per segment band
+    const long long* jband;     // (npair * 6, NBAND)
+    long long* kcount;          // per K segment: entries, FP64 entries, tiled sizes
+    long long* kbig;
+    long long* ksz64;
+    long long* ksz32;
+    long long* knu;             // units per (segment, band)
+    long long* jcount;          // per J segment (pair order)
+    long long* jbig;
+    long long* jsz64;
+    long long* jsz32;
+    long long* jnu;             // units per (class-order position, cq, band)
+    long long* kseg;            // exclusive scans, n + 1 entries
+    long long* kdoff;
+    long long* kdoff32;
+    long long* kustart;
+    long long* jseg;
+    long long* jdoff;
+    long long* jdoff32;
+    long long* justart;
+    int* kunit_os;              // unit tables
+    long long* kunit_e;
+    int* out_unit;
+    int* junit_pc;
+    long long* junit_e;
+    int* pair_unit;
+};
+
+// pair-pair map of the screened list: qmap[hi (hi + 1) / 2 + lo] = index of
+// the quartet (hi, lo) within its class list
+struct QmapArgs {
+    const int2* list;           // class-binned quartet lists, class k from qoff[k]
+    long long qoff[22];
+    int* qmap;
+};
+
+// Schwarz-product bands of the store's gather segments: band b holds the
+// entries with q q in [band_lo[b], band_lo[b - 1]) (band 0 unbounded above,
+// band_lo[NBAND - 1] = the screening threshold), in band order, so density-
+// difference builds skip whole units of the low bands
+constexpr int NBAND = 4;
 
 struct ListArgs {
     int npair;
     double thr;
-    double sw;                  // FP64 / FP32 split (>= thr; = thr: everything FP64)
+    double band_lo[NBAND];      // descending lower bounds, the last one thr
     const int* out_x;
     const int* out_y;
     const int* shell_l;
@@ -196,7 +247,7 @@ struct ListArgs {
     long long qcount[21];       // quartets per class (bounds of the map values)
     int* err;                   // set to 1 if a map value is out of its class range
     const long long* kseg;      // nout * 9 + 1
-    const long long* kbig;      // entries with q q >= sw per segment (they come first)
+    const long long* kband;     // nout * 9 * NBAND entries per segment and band
     long long* klist;
     const long long* jseg;      // npair * 6 + 1
     long long* jlist;
@@ -226,6 +277,14 @@ struct KSArgs {
     long long part_base[9];     // start of each type's partial region
     const double* dens;         // (nset, nao, nao), symmetric
     double* K;
+    // density-difference builds (incremental Fock): a unit whose largest
+    // Schwarz product times max |dens| is below dthr contributes nothing;
+    // with kdata32b set, FP64-part units read that FP32 copy (same offsets)
+    int delta;
+    const double* unit_q;       // per unit: largest q_P q_Q of its entries
+    const double* dmax;         // device scalar max |dens|
+    double dthr;
+    float* kdata32b;            // FP32 copy of the FP64 part (mixed precision), or null
 };
 
 struct JSArgs {
@@ -253,19 +312,54 @@ struct JSArgs {
     long long part_base[6];     // start of each class's partial region
     const double* dens;
     double* J;
+    // density-difference builds, as in KSArgs
+    int delta;
+    const double* unit_q;
+    const double* dmax;
+    double dthr;
+    float* jdata32b;            // FP32 copy of the FP64 part (mixed precision), or null
+};
+
+// per-unit largest Schwarz products of the store's work units
+struct UnitQArgs {
+    const int2* list;           // class-binned quartet lists, class k from qoff[k]
+    long long qoff[21];
+    const double* qscr;         // per pair Schwarz value
+    const int* shell_l;
+    const int* out_x;
+    const int* out_y;
+    const int* pair_cls;
+    const long long* klist;
+    const int* kunit_os;
+    const long long* kunit_e;
+    int nkunit;
+    double* kunit_q;
+    const long long* jlist;
+    const int* junit_pc;
+    const long long* junit_e;
+    int njunit;
+    double* junit_q;
 };
 
 void launch_eri_store(int k, const StoreArgs& a, const int2* list, long long nq, hipStream_t st);
 long long store_block(int k);
-void launch_klist_count(const ListArgs& a, int nout, long long* counts, long long* big, hipStream_t st);
+void launch_klist_count(const ListArgs& a, int nout, long long* counts, long long* band, hipStream_t st);
 void launch_klist_fill(const ListArgs& a, int nout, hipStream_t st);
-void launch_jlist_count(const ListArgs& a, long long* counts, long long* big, hipStream_t st);
+void launch_jlist_count(const ListArgs& a, long long* counts, long long* band, hipStream_t st);
 void launch_jlist_fill(const ListArgs& a, hipStream_t st);
 void launch_kstore(int type, const KSArgs& a, int nout, hipStream_t st);
 void launch_jstore(int pclass, const JSArgs& a, int npairs, hipStream_t st);
 void launch_kcopy(int type, const KSArgs& a, int nout, hipStream_t st);
 void launch_jcopy(int pclass, const JSArgs& a, int npairs, hipStream_t st);
 void launch_fill_int(int* p, size_t n, int v, hipStream_t st);
+void launch_unit_q(const UnitQArgs& a, hipStream_t st);
+void launch_qmap(const QmapArgs& a, hipStream_t st);
+void launch_scan(const long long* in, long long* out, long long n, long long* bsum, hipStream_t st);
+long long scan_max_items();
+void launch_seg_sizes(const TabArgs& a, hipStream_t st);
+void launch_units(const TabArgs& a, hipStream_t st);
+void launch_tab_totals(const TabArgs& a, long long* out, hipStream_t st);
+void launch_absmax(const double* x, size_t n, double* out, hipStream_t st);
 
 void launch_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, hipStream_t st);
 void launch_dh(const DhArgs& a, const long long* n_per_class, hipStream_t st);

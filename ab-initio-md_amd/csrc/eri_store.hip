@@ -2,9 +2,7 @@
 // (compiled once per class with -DAIMDHF_QCLASS=k) into the HBM integral
 // store. Quartet t of the class list (bra = heavier pair, ties: higher pair
 // index) keeps its contracted block at store[t * NAB * NCD + ab * NCD + cd],
-// ab / cd over the bra / ket pair records, and its index t in the pair-pair
-// map qmap[hi (hi + 1) / 2 + lo] used to build the gather lists of the
-// per-iteration J and K kernels.
+// ab / cd over the bra / ket pair records.
 
 #include "aimdhf_internal.h"
 #include "md_eri.h"
@@ -50,10 +48,6 @@ __global__ __launch_bounds__(256) void k_eri_store(StoreArgs a, const int2* __re
         eri_quartet<LA_, LB_, LC_, LD_, true>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, acc);
 #pragma unroll
         for (int i = 0; i < BLK; ++i) out[i] = acc[i];
-    }
-    if (!SPLIT || t % NSPLIT == 0) {
-        const long long hi = pq.x > pq.y ? pq.x : pq.y, lo = pq.x > pq.y ? pq.y : pq.x;
-        a.qmap[hi * (hi + 1) / 2 + lo] = (int)q;
     }
 }
 }  // namespace

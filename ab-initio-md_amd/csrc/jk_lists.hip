@@ -7,8 +7,9 @@
 //      of that class with q_P q_Q >= threshold.
 // An entry packs the quartet's index in its class store (bits 0-31), two AO
 // offsets (14 bits each at 32 and 46) and orientation flags (bits 60-62).
-// In mixed precision the entries with q q >= sw (held in FP64) come first in
-// each segment, those below (held in FP32) after them.
+// Each segment is ordered by Schwarz-product band (q q in [band_lo[b],
+// band_lo[b - 1])), each band in the order above; in mixed precision the
+// leading bands are held in FP64 and the rest in FP32.
 
 #include "aimdhf_internal.h"
 
@@ -50,8 +51,8 @@ __device__ __forceinline__ long long pair_key(int P, int Q) {
     return hi * (hi + 1) / 2 + lo;
 }
 
-// per segment: counts[seg] entries with q q >= thr, of which big[seg] have q q >= sw
-__global__ __launch_bounds__(64) void k_klist_count(ListArgs a, long long* counts, long long* big) {
+// per segment: counts[seg] entries with q q >= thr, band[seg * NBAND + b] of them in band b
+__global__ __launch_bounds__(64) void k_klist_count(ListArgs a, long long* counts, long long* band) {
     const int o = blockIdx.x;
     const int X = a.out_x[o], Y = a.out_y[o];
     const int lane = threadIdx.x;
@@ -59,24 +60,33 @@ __global__ __launch_bounds__(64) void k_klist_count(ListArgs a, long long* count
         const int LJ = sub / 3, LL = sub % 3;
         const int p0 = a.part_off[X * 3 + LJ], p1 = a.part_off[X * 3 + LJ + 1];
         const int r0 = a.part_off[Y * 3 + LL], r1 = a.part_off[Y * 3 + LL + 1];
-        int cnt = 0, cb = 0;
+        int cb[NBAND];
+        for (int b = 0; b < NBAND; ++b) cb[b] = 0;
         for (int jj = p0 + lane; jj < p1; jj += 64) {
-            cnt += valid_prefix(a.part_q, r0, r1, a.part_q[jj], a.thr);
-            cb += valid_prefix(a.part_q, r0, r1, a.part_q[jj], a.sw);
+            int prev = 0;
+            for (int b = 0; b < NBAND; ++b) {
+                const int n = valid_prefix(a.part_q, r0, r1, a.part_q[jj], a.band_lo[b]);
+                cb[b] += n - prev;
+                prev = n;
+            }
         }
-        cnt = wave_isum(cnt);
-        cb = wave_isum(cb);
+        int cnt = 0;
+        for (int b = 0; b < NBAND; ++b) {
+            cb[b] = wave_isum(cb[b]);
+            cnt += cb[b];
+        }
         if (lane == 0) {
             counts[(size_t)o * 9 + sub] = cnt;
-            big[(size_t)o * 9 + sub] = cb;
+            for (int b = 0; b < NBAND; ++b) band[((size_t)o * 9 + sub) * NBAND + b] = cb[b];
         }
     }
 }
 
-// Entries of a segment: first every (J, L) with q_XJ q_YL >= sw, then the rest
-// (q q in [thr, sw)), each part in partner order (J by descending q, L prefix).
+// Entries of a segment, band by band; within a band every (J, L) with q_XJ
+// q_YL in the band, J by descending q and, for each J, its L in descending q
+// (a contiguous run of Y's sorted partners).
 __global__ __launch_bounds__(64) void k_klist_fill(ListArgs a) {
-    __shared__ int inb[64], ins[64], nbj[64];
+    __shared__ int incl[NBAND][64], lbeg[NBAND][64];
     const int o = blockIdx.x;
     const int X = a.out_x[o], Y = a.out_y[o];
     const int lX = a.shell_l[X], lY = a.shell_l[Y];
@@ -87,7 +97,9 @@ __global__ __launch_bounds__(64) void k_klist_fill(ListArgs a) {
         const int r0 = a.part_off[Y * 3 + LL], r1 = a.part_off[Y * 3 + LL + 1];
         const int c1 = pcls(lX, LJ), c2 = pcls(lY, LL);
         const size_t seg = (size_t)o * 9 + sub;
-        long long base_b = a.kseg[seg], base_s = a.kseg[seg] + a.kbig[seg];
+        long long base[NBAND];
+        base[0] = a.kseg[seg];
+        for (int b = 1; b < NBAND; ++b) base[b] = base[b - 1] + a.kband[seg * NBAND + b - 1];
         auto put = [&](long long pos, int jidx, int lidx) {
             const int J = a.part_shell[jidx], PXJ = a.part_pair[jidx];
             const int L = a.part_shell[lidx], PYL = a.part_pair[lidx];
@@ -101,60 +113,54 @@ __global__ __launch_bounds__(64) void k_klist_fill(ListArgs a) {
         };
         for (int jb = p0; jb < p1; jb += 64) {
             const int jj = jb + lane;
-            int vt = 0, vb = 0;
-            if (jj < p1) {
-                vt = valid_prefix(a.part_q, r0, r1, a.part_q[jj], a.thr);
-                vb = valid_prefix(a.part_q, r0, r1, a.part_q[jj], a.sw);
-            }
-            int xb = vb, xs = vt - vb;
+            // this J's run of L in each band: [prev prefix, prefix at the band's lower bound)
+            int prev = 0;
+            for (int b = 0; b < NBAND; ++b) {
+                const int n = jj < p1 ? valid_prefix(a.part_q, r0, r1, a.part_q[jj], a.band_lo[b]) : 0;
+                int x = n - prev;
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int tb = __shfl_up(xb, off, 64), ts = __shfl_up(xs, off, 64);
-                if (lane >= off) {
-                    xb += tb;
-                    xs += ts;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int t = __shfl_up(x, off, 64);
+                    if (lane >= off) x += t;
                 }
+                incl[b][lane] = x;
+                lbeg[b][lane] = prev;
+                prev = n;
             }
-            inb[lane] = xb;
-            ins[lane] = xs;
-            nbj[lane] = vb;
             __syncthreads();
-            const int tot_b = inb[63], tot_s = ins[63];
-            for (int t = lane; t < tot_b; t += 64) {
-                int lo = 0, hi = 63;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (inb[mid] > t) hi = mid;
-                    else lo = mid + 1;
+            for (int b = 0; b < NBAND; ++b) {
+                const int tot = incl[b][63];
+                for (int t = lane; t < tot; t += 64) {
+                    int lo = 0, hi = 63;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (incl[b][mid] > t) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    const int before = lo ? incl[b][lo - 1] : 0;
+                    put(base[b] + t, jb + lo, r0 + lbeg[b][lo] + (t - before));
                 }
-                const int prev = lo ? inb[lo - 1] : 0;
-                put(base_b + t, jb + lo, r0 + (t - prev));
+                base[b] += tot;
             }
-            for (int t = lane; t < tot_s; t += 64) {
-                int lo = 0, hi = 63;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (ins[mid] > t) hi = mid;
-                    else lo = mid + 1;
-                }
-                const int prev = lo ? ins[lo - 1] : 0;
-                put(base_s + t, jb + lo, r0 + nbj[lo] + (t - prev));
-            }
-            base_b += tot_b;
-            base_s += tot_s;
             __syncthreads();
         }
     }
 }
 
-// per (P, partner class): counts = partners with q_P q_Q >= thr, big = those >= sw (a prefix)
-__global__ void k_jlist_count(ListArgs a, long long* counts, long long* big) {
+// per (P, partner class): counts = partners with q_P q_Q >= thr, band = those
+// in each band (the partners are in descending q, so each band is a run)
+__global__ void k_jlist_count(ListArgs a, long long* counts, long long* band) {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= a.npair) return;
     const double qp = a.qscr[P];
     for (int c = 0; c < 6; ++c) {
-        counts[(size_t)P * 6 + c] = valid_prefix(a.ket_q, a.ket_off[c], a.ket_off[c + 1], qp, a.thr);
-        big[(size_t)P * 6 + c] = valid_prefix(a.ket_q, a.ket_off[c], a.ket_off[c + 1], qp, a.sw);
+        int prev = 0;
+        for (int b = 0; b < NBAND; ++b) {
+            const int n = valid_prefix(a.ket_q, a.ket_off[c], a.ket_off[c + 1], qp, a.band_lo[b]);
+            band[((size_t)P * 6 + c) * NBAND + b] = n - prev;
+            prev = n;
+        }
+        counts[(size_t)P * 6 + c] = prev;
     }
 }
 
@@ -176,6 +182,68 @@ __global__ __launch_bounds__(64) void k_jlist_fill(ListArgs a) {
     }
 }
 
+__device__ __forceinline__ double wave_dmax(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// largest q_P q_Q over the entries of each K unit (one wavefront per unit)
+__global__ __launch_bounds__(64) void k_kunit_q(UnitQArgs a) {
+    const int u = blockIdx.x;
+    const int code = a.kunit_os[u];
+    const int o = code >> 5, sub = code & 15;
+    const int k = qcls(pcls(a.shell_l[a.out_x[o]], sub / 3), pcls(a.shell_l[a.out_y[o]], sub % 3));
+    const int2* list = a.list + a.qoff[k];
+    double m = 0.0;
+    for (long long e = a.kunit_e[2 * u] + threadIdx.x; e < a.kunit_e[2 * u + 1]; e += 64) {
+        const int2 q = list[(unsigned long long)a.klist[e] & 0xffffffffull];
+        m = fmax(m, a.qscr[q.x] * a.qscr[q.y]);
+    }
+    m = wave_dmax(m);
+    if (threadIdx.x == 0) a.kunit_q[u] = m;
+}
+
+// the same for the J units
+__global__ __launch_bounds__(64) void k_junit_q(UnitQArgs a) {
+    const int u = blockIdx.x;
+    const int code = a.junit_pc[u];
+    const int k = qcls(a.pair_cls[code >> 4], code & 7);
+    const int2* list = a.list + a.qoff[k];
+    double m = 0.0;
+    for (long long e = a.junit_e[2 * u] + threadIdx.x; e < a.junit_e[2 * u + 1]; e += 64) {
+        const int2 q = list[(unsigned long long)a.jlist[e] & 0xffffffffull];
+        m = fmax(m, a.qscr[q.x] * a.qscr[q.y]);
+    }
+    m = wave_dmax(m);
+    if (threadIdx.x == 0) a.junit_q[u] = m;
+}
+
+// qmap entry of every listed quartet (thread per quartet; class by its offset)
+__global__ void k_qmap(QmapArgs a) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.qoff[21]) return;
+    int k = 0;
+    while (k < 20 && g >= a.qoff[k + 1]) ++k;
+    const int2 pq = a.list[g];
+    a.qmap[pair_key(pq.x, pq.y)] = (int)(g - a.qoff[k]);
+}
+
+// out[0] = max |x| (one block; the max does not depend on the order)
+__global__ __launch_bounds__(1024) void k_absmax(const double* __restrict__ x, size_t n, double* out) {
+    __shared__ double part[16];
+    double m = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(x[i]));
+    m = wave_dmax(m);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        m = threadIdx.x < 16 ? part[threadIdx.x] : 0.0;
+        m = wave_dmax(m);
+        if (threadIdx.x == 0) out[0] = m;
+    }
+}
+
 __global__ void k_fill_int(int* p, size_t n, int v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -186,16 +254,30 @@ void launch_fill_int(int* p, size_t n, int v, hipStream_t st) {
     if (n) k_fill_int<<<dim3(2048), dim3(256), 0, st>>>(p, n, v);
 }
 
-void launch_klist_count(const ListArgs& a, int nout, long long* counts, long long* big, hipStream_t st) {
-    if (nout > 0) k_klist_count<<<dim3(nout), dim3(64), 0, st>>>(a, counts, big);
+void launch_unit_q(const UnitQArgs& a, hipStream_t st) {
+    if (a.nkunit > 0) k_kunit_q<<<dim3(a.nkunit), dim3(64), 0, st>>>(a);
+    if (a.njunit > 0) k_junit_q<<<dim3(a.njunit), dim3(64), 0, st>>>(a);
+}
+
+void launch_qmap(const QmapArgs& a, hipStream_t st) {
+    const long long n = a.qoff[21];
+    if (n > 0) k_qmap<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(a);
+}
+
+void launch_absmax(const double* x, size_t n, double* out, hipStream_t st) {
+    k_absmax<<<dim3(1), dim3(1024), 0, st>>>(x, n, out);
+}
+
+void launch_klist_count(const ListArgs& a, int nout, long long* counts, long long* band, hipStream_t st) {
+    if (nout > 0) k_klist_count<<<dim3(nout), dim3(64), 0, st>>>(a, counts, band);
 }
 
 void launch_klist_fill(const ListArgs& a, int nout, hipStream_t st) {
     if (nout > 0) k_klist_fill<<<dim3(nout), dim3(64), 0, st>>>(a);
 }
 
-void launch_jlist_count(const ListArgs& a, long long* counts, long long* big, hipStream_t st) {
-    if (a.npair > 0) k_jlist_count<<<dim3((a.npair + 63) / 64), dim3(64), 0, st>>>(a, counts, big);
+void launch_jlist_count(const ListArgs& a, long long* counts, long long* band, hipStream_t st) {
+    if (a.npair > 0) k_jlist_count<<<dim3((a.npair + 63) / 64), dim3(64), 0, st>>>(a, counts, band);
 }
 
 void launch_jlist_fill(const ListArgs& a, hipStream_t st) {

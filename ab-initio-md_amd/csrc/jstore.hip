@@ -12,7 +12,10 @@
 // every store of the copy covers 64 consecutive words. Per SCF
 // iteration, k_jstore forms sum_cd (ab|cd) w_Q D_cd (w_Q = 2 for C != D) per
 // unit (lanes summed by a fixed butterfly) and k_jreduce adds each pair's
-// unit partials in unit order into J_P, so J is bitwise reproducible.
+// unit partials in unit order into J_P, so J is bitwise reproducible. For a
+// density difference (incremental Fock) units whose largest Schwarz product
+// times max |dD| is below a threshold are skipped, and in mixed precision
+// every unit is digested from FP32 copies.
 
 #include "aimdhf_internal.h"
 #include "md_core.h"
@@ -93,6 +96,11 @@ __global__ __launch_bounds__(64) void k_jcopy(JSArgs a) {
                 double* out = a.jdata + a.jdoff[seg] + dst;
 #pragma unroll
                 for (int v = 0; v < W; ++v) out[v * 64] = val[v];
+                if (a.jdata32b) {
+                    float* out32 = a.jdata32b + a.jdoff[seg] + dst;
+#pragma unroll
+                    for (int v = 0; v < W; ++v) out32[v * 64] = (float)val[v];
+                }
             }
         }
     });
@@ -170,25 +178,33 @@ __global__ __launch_bounds__(64) void k_jstore(JSArgs a) {
     const int P = code >> 4, f32 = (code >> 3) & 1, cq = code & 7;
     const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
     const int lane = threadIdx.x;
+    double* part = a.jpart + a.part_base[CP] + (size_t)(u - a.unit_base[CP]) * (NSET * NAB);
+    if (a.delta && a.unit_q[u] * a.dmax[0] < a.dthr) {
+        // screened out for this density difference
+        for (int i = lane; i < NSET * NAB; i += 64) part[i] = 0.0;
+        return;
+    }
     double Jacc[NSET * NAB];
 #pragma unroll
     for (int i = 0; i < NSET * NAB; ++i) Jacc[i] = 0.0;
     with_part(cq, [&](auto pt) {
         using PT = decltype(pt);
         const size_t seg = (size_t)P * 6 + cq;
-        if (f32) {
+        if (f32 || (a.delta && a.jdata32b)) {
             // FP32 sums per unit, folded into the FP64 block
             float acc[NSET * NAB];
 #pragma unroll
             for (int i = 0; i < NSET * NAB; ++i) acc[i] = 0.0f;
-            j_unit<float, PT, NSET>(a, a.jdata32 + a.jdoff32[seg], (u0 - a.jseg[seg] - a.jbig[seg]) * PT::NS, u0, u1,
-                                    lane, acc);
+            if (f32)
+                j_unit<float, PT, NSET>(a, a.jdata32 + a.jdoff32[seg], (u0 - a.jseg[seg] - a.jbig[seg]) * PT::NS, u0,
+                                        u1, lane, acc);
+            else
+                j_unit<float, PT, NSET>(a, a.jdata32b + a.jdoff[seg], (u0 - a.jseg[seg]) * PT::NS, u0, u1, lane, acc);
 #pragma unroll
             for (int i = 0; i < NSET * NAB; ++i) Jacc[i] = double(acc[i]);
         } else
             j_unit<double, PT, NSET>(a, a.jdata + a.jdoff[seg], (u0 - a.jseg[seg]) * PT::NS, u0, u1, lane, Jacc);
     });
-    double* part = a.jpart + a.part_base[CP] + (size_t)(u - a.unit_base[CP]) * (NSET * NAB);
 #pragma unroll
     for (int i = 0; i < NSET * NAB; ++i) {
         const double v = wave_sum(Jacc[i]);

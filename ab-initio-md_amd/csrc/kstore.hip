@@ -14,7 +14,10 @@
 // iteration, k_kstore streams them and writes one partial block per unit
 // (lanes summed by a fixed butterfly; nset NX NY doubles per unit, each
 // type in its own region); k_kreduce adds each block's unit
-// partials in unit order and writes K[X][Y] and K[Y][X]. No atomics, so the
+// partials in unit order and writes K[X][Y] and K[Y][X]. For a density
+// difference (incremental Fock) units whose largest Schwarz product times
+// max |dD| is below a threshold are skipped, and in mixed precision every
+// unit is digested from FP32 copies. No atomics, so the
 // result is bitwise reproducible run to run.
 
 #include "aimdhf_internal.h"
@@ -100,6 +103,11 @@ __global__ __launch_bounds__(64) void k_kcopy(KSArgs a) {
                 double* out = a.kdata + a.kdoff[seg] + dst;
 #pragma unroll
                 for (int v = 0; v < W; ++v) out[v * 64] = val[v];
+                if (a.kdata32b) {
+                    float* out32 = a.kdata32b + a.kdoff[seg] + dst;
+#pragma unroll
+                    for (int v = 0; v < W; ++v) out32[v * 64] = (float)val[v];
+                }
             }
         }
     });
@@ -193,25 +201,33 @@ __global__ __launch_bounds__(64) void k_kstore(KSArgs a) {
     const int o = code >> 5, f32 = (code >> 4) & 1, sub = code & 15;
     const long long u0 = a.unit_e[2 * u], u1 = a.unit_e[2 * u + 1];
     const int lane = threadIdx.x;
+    double* part = a.kpart + a.part_base[OT] + (size_t)(u - a.unit_base[OT]) * (NSET * NX * NY);
+    if (a.delta && a.unit_q[u] * a.dmax[0] < a.dthr) {
+        // screened out for this density difference
+        for (int i = lane; i < NSET * NX * NY; i += 64) part[i] = 0.0;
+        return;
+    }
     double Kacc[NSET * NX * NY];
 #pragma unroll
     for (int i = 0; i < NSET * NX * NY; ++i) Kacc[i] = 0.0;
     with_sub(sub, [&](auto sb) {
         using SB = decltype(sb);
         const size_t seg = (size_t)o * 9 + sub;
-        if (f32) {
+        if (f32 || (a.delta && a.kdata32b)) {
             // FP32 sums per unit, folded into the FP64 block
             float acc[NSET * NX * NY];
 #pragma unroll
             for (int i = 0; i < NSET * NX * NY; ++i) acc[i] = 0.0f;
-            k_unit<float, SB, NSET>(a, a.kdata32 + a.kdoff32[seg], (u0 - a.kseg[seg] - a.kbig[seg]) * SB::NS, u0, u1,
-                                    lane, acc);
+            if (f32)
+                k_unit<float, SB, NSET>(a, a.kdata32 + a.kdoff32[seg], (u0 - a.kseg[seg] - a.kbig[seg]) * SB::NS, u0,
+                                        u1, lane, acc);
+            else
+                k_unit<float, SB, NSET>(a, a.kdata32b + a.kdoff[seg], (u0 - a.kseg[seg]) * SB::NS, u0, u1, lane, acc);
 #pragma unroll
             for (int i = 0; i < NSET * NX * NY; ++i) Kacc[i] = double(acc[i]);
         } else
             k_unit<double, SB, NSET>(a, a.kdata + a.kdoff[seg], (u0 - a.kseg[seg]) * SB::NS, u0, u1, lane, Kacc);
     });
-    double* part = a.kpart + a.part_base[OT] + (size_t)(u - a.unit_base[OT]) * (NSET * NX * NY);
 #pragma unroll
     for (int i = 0; i < NSET * NX * NY; ++i) {
         const double v = wave_sum(Kacc[i]);

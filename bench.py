@@ -12,7 +12,8 @@ tests/golden/water16.xyz (16 randomly rotated waters in an 8 A cube, O-O >=
 2.7 A, seed 1).
 
 By default the FP64 trajectory runs first; the mixed-precision path (FP32 for
-quartets below the Schwarz switches) then computes the same geometries with
+quartets below the Schwarz switches, and incremental Fock builds whose
+density differences are screened and digested in FP32) then computes the same geometries with
 its own density reuse, and its energies and forces are compared with the
 FP64 ones at every step. ``value`` is the mixed step time when it stays
 within |dE| <= 1e-6 Eh and |dF| <= 1e-5 Eh/bohr at every step, else FP64.
@@ -147,7 +148,14 @@ def main() -> int:
         mixed = {"ms_per_step": round(mixed_ms, 3), "max_abs_dE_Eh": de, "max_abs_dF_Eh_bohr": dfm,
                  "within_bounds": bool(de <= 1e-6 and dfm <= 1e-5),
                  "jk_fp32_switch": bm.solver.options.jk_fp32_switch,
-                 "grad_fp32_switch": bm.solver.options.grad_fp32_switch}
+                 "grad_fp32_switch": bm.solver.options.grad_fp32_switch,
+                 "incremental_fock": bm.solver.incremental,
+                 "incremental_threshold": (bm.solver.options.incremental_threshold
+                                           if bm.solver.options.incremental_threshold is not None
+                                           else bm.solver.options.schwarz_threshold),
+                 "scf_iterations_mean": float(np.mean([o.info["scf_iterations"] for o in outs])),
+                 "phase_ms": {k: round(1e3 * float(np.mean([o.info["timings"][k] for o in outs])), 3)
+                              for k in outs[0].info["timings"]}}
 
     # Fock build (J and K kernels + their prologue/epilogue) priced with the algorithmic
     # work of the unique screened quartets of the last timed geometry
@@ -189,7 +197,8 @@ def main() -> int:
         "config": {
             "workload": "(H2O)16 RHF/6-31G* (304 AOs), velocity-Verlet NVE BOMD step at 0.5 fs, 300 K, "
                         "SCF conv 1e-10 Eh / 1e-7, Schwarz 1e-12, gradient screening 1e-14; value: the mixed "
-                        "path (FP32 below the Schwarz switches) over the FP64 trajectory's geometries when it "
+                        "path (FP32 below the Schwarz switches, incremental Fock with FP32 density-difference "
+                        "builds) over the FP64 trajectory's geometries when it "
                         "stays within |dE| <= 1e-6 Eh and |dF| <= 1e-5 Eh/bohr of FP64 at every step, else FP64; "
                         "per-kernel figures from the FP64 run",
             "geometry": "tests/golden/water16.xyz",

@@ -101,6 +101,19 @@ int aimdhf_quartets(aimdhf_plan* plan, int64_t* out_host, int64_t cap, int64_t* 
 int aimdhf_jk(aimdhf_plan* plan, int nset, const double* dens, double* J, double* K, void* stream);
 
 /*
+ * J and K of density differences dD (device (nset,nao,nao)) for an
+ * incremental Fock build, J(D) = J(D_prev) + J(dD). In store mode a work unit
+ * whose largest Q_P Q_Q times max |dD| is below threshold is skipped (each of
+ * its integral-density products is below threshold), and in mixed precision
+ * every stored integral is digested in FP32, the large-bound ones from FP32
+ * copies made at set_geometry; threshold 0 skips nothing. In direct mode the
+ * same as aimdhf_jk. Replaces the full jk_from_eri rebuild of every SCF
+ * iteration (aimd/qc/scf.py:863-867, aimd/qc/integrals.py:1009-1044).
+ */
+int aimdhf_jk_delta(aimdhf_plan* plan, int nset, const double* ddens, double* dJ, double* dK, double threshold,
+                    void* stream);
+
+/*
  * J/K mode for the next aimdhf_set_geometry: 0 auto (store when its estimated
  * size is within max_store_bytes, 0 = half the free device memory), 1 direct,
  * 2 store. No counterpart in the reference, which always stores the full
@@ -183,8 +196,9 @@ int aimdhf_grad_two(aimdhf_plan* plan, const double* Dc, int nx, const double* D
  * evaluation, 15 gather-ordered copies, 48 gather-list build, 49..54 store J
  * per pair class, 55..63 store K per output type, 64..84 derivative-ERI
  * screening pre-pass per class; 85..88 host wall-clock phases (energy list,
- * owner lists, store build incl. its device work, gradient list; milliseconds
- * of the calling thread, not events). aimdhf_profile_read synchronises, returns the
+ * owner lists, store build incl. its device work, gradient list) and 89..92
+ * sub-phases of the store build (list counts, band tables, list fill, unit
+ * tables; milliseconds of the calling thread, not events). aimdhf_profile_read synchronises, returns the
  * summed milliseconds and launch counts per kind (nkind <= 96) and optionally
  * resets.
  */

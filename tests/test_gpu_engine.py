@@ -103,6 +103,48 @@ def test_one_electron_and_gradient_terms(oracle_lib, mol, basis, mult):
     e.close()
 
 
+@pytest.mark.parametrize("mode,precision", [("store", "fp64"), ("store", "mixed"), ("direct", "fp64")])
+def test_density_difference_builds(oracle_lib, mode, precision):
+    """J/K of a density difference (incremental Fock): linear against the oracle
+    with nothing screened, exactly zero when every unit is screened out, and
+    FP32 digestion in mixed precision within FP32 rounding of the FP64 result."""
+    import torch
+    from aimd_amd.qc.basis import build_basis
+    from aimd_amd.qc.engine import Engine
+    s, x = _geom("water8")
+    print(f"case water8 6-31g* {mode} {precision}", flush=True)
+    b = build_basis(s, x, "6-31g*")
+    e = Engine(b)
+    e.set_jk_mode(mode)
+    e.set_precision(precision, 1e-5, 1e-7)
+    e.set_geometry(x)
+    rng = np.random.default_rng(11)
+    dD = rng.standard_normal((1, b.nao, b.nao)) * 1e-4
+    dD = dD + dD.transpose(0, 2, 1)
+    scale = np.abs(dD).max()
+    dt = torch.tensor(dD, device="cuda")
+    dJ, dK = e.jk_delta(dt, 0.0)
+    Jo, Ko = oracle_lib.Pairs(oracle_lib.make_basis(s, x, "6-31g*")).jk_direct(dD)
+    # FP64: the oracle's 1e-11 at unit density, scaled; FP32 digestion: ~1e-7 relative per term
+    tol = (2e-5 if precision == "mixed" else 1e-10) * scale
+    np.testing.assert_allclose(dJ.cpu().numpy(), Jo, rtol=0, atol=tol)
+    np.testing.assert_allclose(dK.cpu().numpy(), Ko, rtol=0, atol=tol)
+    dJ2, dK2 = e.jk_delta(dt, 1e-300)
+    assert torch.equal(dJ, dJ2) and torch.equal(dK, dK2)
+    if mode == "store":
+        # every Schwarz product is <= ~1e2, so units * max|dD| stay far below 1e3
+        z = e.jk_delta(dt, 1e3)
+        assert not torch.any(z[0]) and not torch.any(z[1])
+        # screening at 1e-9 skips units with q q < 1e-9 / max|dD|: each skipped
+        # integral-density product is below 1e-9 (Schwarz), and an element sums
+        # far fewer than 1e3 of them at full size
+        dJs, dKs = e.jk_delta(dt, 1e-9)
+        assert not (torch.equal(dJs, dJ) and torch.equal(dKs, dK))
+        np.testing.assert_allclose(dJs.cpu().numpy(), Jo, rtol=0, atol=tol + 1e3 * 1e-9)
+        np.testing.assert_allclose(dKs.cpu().numpy(), Ko, rtol=0, atol=tol + 1e3 * 1e-9)
+    e.close()
+
+
 def test_unsupported_angular_momentum_is_a_value_error():
     from aimd_amd.qc.basis import BasisSet
     from aimd_amd.qc.engine import Engine

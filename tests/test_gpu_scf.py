@@ -79,6 +79,23 @@ def test_water8_cluster_against_oracle(oracle_lib):
     assert np.abs(r.gradient - g).max() < 1e-7
 
 
+@pytest.mark.parametrize("case", ["water8", "methyl"])
+def test_incremental_fock_matches_oracle(oracle_lib, case):
+    # J/K of every iteration after the first from the density difference, with
+    # density-difference screening at the Schwarz threshold: the FP64 bar holds
+    if case == "water8":
+        (s, x), basis, mult = cluster("water8.xyz"), "6-31g*", 1
+    else:
+        (s, x), basis, mult = bohr(METHYL, 0.03, 1), "6-31g*", 2
+    print(f"case {case} {basis} {mult}", flush=True)
+    r = _hf(s, basis, multiplicity=mult, scf_options={"incremental_fock": True, "jk_mode": "store"}).compute(x)
+    st, g = oracle_lib.energy_and_gradient(s, x, basis, mult=mult, conv_energy=1e-11, conv_commutator=1e-9,
+                                           stored=False)
+    assert r.converged
+    assert abs(r.energy - st.energy) < 1e-9
+    assert np.abs(r.gradient - g).max() < 1e-7
+
+
 @pytest.mark.parametrize("mol,basis,mult", [(WATER, "6-31g*", 1), (METHYL, "6-31g*", 2)])
 def test_density_solvers_agree(mol, basis, mult):
     # purification and eigenvectors give the same aufbau projector, so the same SCF state

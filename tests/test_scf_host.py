@@ -118,7 +118,21 @@ def S_module():
 @pytest.mark.parametrize("bad", [dict(max_iter=0), dict(conv_energy=0.0), dict(guess="minao"), dict(diis_space=1),
                                  dict(damping=1.0), dict(level_shift=-0.1), dict(orthogonalization="lowdin"),
                                  dict(blas_threads=0), dict(jk_mode="tensor"), dict(max_store_bytes=-1),
-                                 dict(density_solver="lanczos")])
+                                 dict(density_solver="lanczos"), dict(incremental_threshold=-1.0),
+                                 dict(incremental_threshold=float("nan"))])
 def test_option_validation(bad):
     with pytest.raises(ValueError):
         S.SCFOptions(**bad)
+
+
+def test_incremental_fock_defaults_follow_the_precision():
+    class Solver:
+        incremental = S.SCFSolver.incremental
+
+        def __init__(self, **kw):
+            self.options = S.SCFOptions(**kw)
+
+    assert not Solver().incremental
+    assert Solver(precision="mixed").incremental
+    assert Solver(incremental_fock=True).incremental
+    assert not Solver(precision="mixed", incremental_fock=False).incremental
