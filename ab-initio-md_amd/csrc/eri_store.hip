@@ -28,7 +28,7 @@ constexpr int NSPLIT = SPLIT ? Qt::NCD : 1;
 
 template <int LA_, int LB_, int LC_, int LD_>
 __global__ __launch_bounds__(256) void k_eri_store(StoreArgs a, const int2* __restrict__ list, long long nq) {
-    __shared__ double btab[BOYS_NGRID * BOYS_SLICE];
+    __shared__ double btab[BOYS_NGRID * BOYS_SROW];
     boys_stage(a.boys, LA_ + LB_ + LC_ + LD_, btab);
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nq * NSPLIT) return;

@@ -167,7 +167,7 @@ template <int Q, typename Real>
 __global__ __launch_bounds__(256) void k_grad_class(GradArgs a, const int2* __restrict__ list,
                                                     const int* __restrict__ sel, const int* __restrict__ nsel) {
     extern __shared__ long long lds[];
-    __shared__ double btab[BOYS_NGRID * BOYS_SLICE];
+    __shared__ double btab[BOYS_NGRID * BOYS_SROW];
     for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
     boys_stage(a.boys, LA + LB + LC + LD + 1, btab);
     const long long nitems = (long long)(*nsel) * NT;

@@ -64,14 +64,52 @@ constexpr double TWO_PI_2_5 = 34.986836655249725;  // 2 pi^(5/2)
 
 // Boys table layouts: the full table (rows of BOYS_NCOL orders from 0), or a
 // per-class slice of BOYS_SLICE orders starting at the class's top order,
-// staged in LDS by the kernels (boys_stage). BS = row stride, C0 = first order.
+// staged in LDS by the kernels (boys_stage) in rows of BOYS_SROW with
+// exp(-T_k) of the grid point last. BS = row stride, C0 = first order.
 constexpr int BOYS_SLICE = 10;
+constexpr int BOYS_SROW = BOYS_SLICE + 1;
 
 // exp / sqrt of the working precision (float arithmetic stays float)
 MD_HD inline double md_exp(double x) { return exp(x); }
 MD_HD inline float md_exp(float x) { return expf(x); }
 MD_HD inline double md_sqrt(double x) { return sqrt(x); }
 MD_HD inline float md_sqrt(float x) { return sqrtf(x); }
+
+// 1/sqrt(x), x > 0: the hardware estimate refined by Newton steps
+// y <- y (3 - x y^2) / 2, each taking a relative error e to ~1.5 e^2. The
+// FP64 estimate is good to ~5e-8 (measured on gfx950), so FP64 takes two
+// steps (~1e-16; one step leaves ~4e-15, which the cancellation between
+// core-orbital derivative integrals turns into visible force errors); the
+// FP32 estimate is already near 1 ulp and takes one.
+MD_HD inline double md_rsqrt(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    double y = __builtin_amdgcn_rsq(x);
+#else
+    double y = 1.0 / sqrt(x);
+#endif
+    const double hx = 0.5 * x;
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    return fma(y, fma(-hx * y, y, 0.5), y);
+}
+MD_HD inline float md_rsqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const float y = __builtin_amdgcn_rsqf(x);
+#else
+    const float y = 1.0f / sqrtf(x);
+#endif
+    const float h = 0.5f * x * y;
+    return fmaf(y, fmaf(-h, y, 0.5f), y);
+}
+
+// alpha = p q / (p + q) and the ERI prefactor 2 pi^(5/2) / (p q sqrt(p + q))
+// from the pair exponents, their reciprocals (stored in the pair records)
+// and one refined 1/sqrt(p + q): no division
+template <typename Real>
+MD_HD void quartet_factors(Real p, Real q, Real ip, Real iq, Real& alpha, Real& pref) {
+    const Real r = md_rsqrt(p + q);
+    alpha = p * q * (r * r);
+    pref = Real(TWO_PI_2_5) * (ip * iq) * r;
+}
 
 // F_L(T) (single top order), as aimd/qc/boys.py:87-114, in precision Real
 template <int BS = BOYS_NCOL, int C0 = 0, typename Real = double>
@@ -114,6 +152,64 @@ MD_HD void boys_all(Real T, const double* __restrict__ tab, Real* F) {
     }
 }
 
+// The same from a staged slice (rows of BOYS_SROW: orders L .. L+9, then
+// exp(-T_k)). Below the switch, F_L is the 10-term Taylor expansion and
+// exp(-T) = exp(-T_k) exp(dx), dx = T_k - T in [-0.05, 0.05], with exp(dx)
+// its degree-9 Taylor polynomial (remainder < 0.05^10 / 10! ~ 3e-20).
+// Above it, F_L by upward recursion from F_0 = sqrt(pi / T) / 2, with one
+// exp(-T) shared by both recursions.
+template <int L, typename Real = double>
+MD_HD void boys_all_slice(Real T, const double* __restrict__ tab, Real* F) {
+    Real ex;
+    if (T < Real(BOYS_XSWITCH)) {
+        const int k = (int)(T * Real(10.0) + Real(0.5));
+        const Real dx = Real(k) * Real(0.1) - T;
+        const double* row = tab + k * BOYS_SROW;
+        Real acc = 0, fac = 1;
+        acc += Real(row[0]) * fac; fac *= dx * Real(1.0);
+        acc += Real(row[1]) * fac; fac *= dx * Real(0.5);
+        acc += Real(row[2]) * fac; fac *= dx * Real(1.0 / 3.0);
+        acc += Real(row[3]) * fac; fac *= dx * Real(0.25);
+        acc += Real(row[4]) * fac; fac *= dx * Real(0.2);
+        acc += Real(row[5]) * fac; fac *= dx * Real(1.0 / 6.0);
+        acc += Real(row[6]) * fac; fac *= dx * Real(1.0 / 7.0);
+        acc += Real(row[7]) * fac; fac *= dx * Real(0.125);
+        acc += Real(row[8]) * fac; fac *= dx * Real(1.0 / 9.0);
+        acc += Real(row[9]) * fac;
+        F[L] = acc;
+        if constexpr (L > 0) {
+            Real e = Real(1.0 / 362880.0);
+            e = e * dx + Real(1.0 / 40320.0);
+            e = e * dx + Real(1.0 / 5040.0);
+            e = e * dx + Real(1.0 / 720.0);
+            e = e * dx + Real(1.0 / 120.0);
+            e = e * dx + Real(1.0 / 24.0);
+            e = e * dx + Real(1.0 / 6.0);
+            e = e * dx + Real(0.5);
+            e = e * dx + Real(1.0);
+            e = e * dx + Real(1.0);
+            ex = Real(row[BOYS_SLICE]) * e;
+        }
+    } else {
+        // sqrt and division here, not md_rsqrt: with the hardware reciprocal
+        // square root in both this branch and quartet_factors, gfx950 builds
+        // of the derivative kernels gave force errors ~5e-5 that neither use
+        // alone shows (and the host build of the same source does not)
+        Real f = Real(0.5) * md_sqrt(Real(MD_PI) / T);
+        if constexpr (L > 0) {
+            ex = md_exp(-T);
+            const Real inv2x = Real(0.5) / T;
+            for (int m = 0; m < L; ++m) f = (Real(2 * m + 1) * f - ex) * inv2x;
+        }
+        F[L] = f;
+    }
+    if constexpr (L > 0) {
+        const Real t2 = Real(2.0) * T;
+#pragma unroll
+        for (int n = L - 1; n >= 0; --n) F[n] = (t2 * F[n + 1] + ex) * (Real(1.0) / Real(2 * n + 1));
+    }
+}
+
 // compile-time loop: f(std::integral_constant<int, i>) for i in [B, E)
 template <int B, int E, class F>
 MD_HD void static_for(F&& f) {
@@ -148,7 +244,7 @@ MD_HD void hermite_r(Real alpha, Real x, Real y, Real z, Real scale,
                      const double* __restrict__ tab, Real* R) {
     Real T = alpha * (x * x + y * y + z * z);
     Real F[L + 1];
-    if constexpr (SLICE) boys_all<L, BOYS_SLICE, L, Real>(T, tab, F);
+    if constexpr (SLICE) boys_all_slice<L, Real>(T, tab, F);
     else boys_all<L, BOYS_NCOL, 0, Real>(T, tab, F);
     Real m2a = Real(-2.0) * alpha;
     Real pw[L + 1];
@@ -224,7 +320,7 @@ MD_HD void hermite_e(Real a, Real b, Real xab, Real e00, Real* e) {
 //   E[h][ab] = f_a f_b c_a c_b E^{a_x b_x}_t E^{a_y b_y}_u E^{a_z b_z}_v,  h = hidx(t,u,v)
 // (NH(l_A+l_B) x NC(l_A) NC(l_B), row-major), i.e. the contraction
 // coefficients and Cartesian normalisation factors are folded in.
-enum { R_P = 0, R_PX = 1, R_PY = 2, R_PZ = 3, R_A = 4, R_B = 5, R_C = 6, REC_HDR = 8 };
+enum { R_P = 0, R_PX = 1, R_PY = 2, R_PZ = 3, R_A = 4, R_B = 5, R_C = 6, R_IP = 7, REC_HDR = 8 };
 
 constexpr int rec_size(int la, int lb) { return REC_HDR + NH(la + lb) * NC(la) * NC(lb); }
 
@@ -252,11 +348,14 @@ MD_HD double fx_value(long long hi, long long lo) {
 }
 
 #ifdef __HIPCC__
-// Copy the BOYS_SLICE orders from Ltop of the full table (global) into an LDS
-// slice (BOYS_NGRID * BOYS_SLICE doubles) with the whole workgroup; ends with a barrier.
+// Stage the BOYS_SLICE orders from Ltop of the full table (global) and
+// exp(-T_k) of every grid point into an LDS slice (BOYS_NGRID * BOYS_SROW
+// doubles) with the whole workgroup; ends with a barrier.
 __device__ inline void boys_stage(const double* __restrict__ tab, int Ltop, double* lds) {
-    for (int i = threadIdx.x; i < BOYS_NGRID * BOYS_SLICE; i += blockDim.x)
-        lds[i] = tab[(i / BOYS_SLICE) * BOYS_NCOL + Ltop + i % BOYS_SLICE];
+    for (int i = threadIdx.x; i < BOYS_NGRID * BOYS_SROW; i += blockDim.x) {
+        const int k = i / BOYS_SROW, c = i % BOYS_SROW;
+        lds[i] = c < BOYS_SLICE ? tab[k * BOYS_NCOL + Ltop + c] : exp(-0.1 * k);
+    }
     __syncthreads();
 }
 #endif

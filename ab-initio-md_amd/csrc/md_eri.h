@@ -49,9 +49,14 @@ MD_HD void eri_quartet(const double* __restrict__ rb, int nb, const double* __re
         for (int ik = 0; ik < nk; ++ik) {
             const double* K = rk + ik * Q::RSK;
             const double q = K[R_P];
-            const double s = p + q;
-            const double alpha = p * q / s;
-            const double pref = TWO_PI_2_5 / (p * q * sqrt(s));
+            double alpha, pref;
+            if constexpr (SLICE) {
+                quartet_factors(p, q, B[R_IP], K[R_IP], alpha, pref);
+            } else {
+                const double s = p + q;
+                alpha = p * q / s;
+                pref = TWO_PI_2_5 / (p * q * sqrt(s));
+            }
             double R[NH(L)];
             hermite_r<L, SLICE>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
             static_for<0, NHK>([&](auto kk) {
@@ -113,9 +118,14 @@ MD_HD void eri_quartet_cd(const double* __restrict__ rb, int nb, const double* _
         for (int ik = 0; ik < nk; ++ik) {
             const double* K = rk + ik * Q::RSK;
             const double q = K[R_P];
-            const double s = p + q;
-            const double alpha = p * q / s;
-            const double pref = TWO_PI_2_5 / (p * q * sqrt(s));
+            double alpha, pref;
+            if constexpr (SLICE) {
+                quartet_factors(p, q, B[R_IP], K[R_IP], alpha, pref);
+            } else {
+                const double s = p + q;
+                alpha = p * q / s;
+                pref = TWO_PI_2_5 / (p * q * sqrt(s));
+            }
             double R[NH(L)];
             hermite_r<L, SLICE>(alpha, px - K[R_PX], py - K[R_PY], pz - K[R_PZ], pref, tab, R);
             static_for<0, NHK>([&](auto kk) {

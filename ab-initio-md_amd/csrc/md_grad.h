@@ -40,7 +40,7 @@ MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __rest
     constexpr int NAB = Q::NAB, NKC = Q::NCD, NHK = Q::NHK;
     constexpr int NHB1 = NH(Q::LAB + 1), L1 = Q::L + 1;
     constexpr int NA = Q::NA, NB = Q::NB;
-    constexpr int IS = LA + 2, JS = LB + 2, TS = LA + LB + 3;
+    constexpr int IS = LA + 2, JS0 = LB + 1, TS0 = LA + LB + 2;
     Real xab[3];
     for (int d = 0; d < 3; ++d) { gA[d] = 0.0; gB[d] = 0.0; xab[d] = Real(Apos[d] - Bpos[d]); }
     Real fcart[NAB];
@@ -60,9 +60,14 @@ MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __rest
         for (int ik = 0; ik < nk; ++ik) {
             const double* K = rk + ik * Q::RSK;
             const Real q = Real(K[R_P]);
-            const Real s = p + q;
-            const Real alpha = p * q / s;
-            const Real pref = Real(TWO_PI_2_5) / (p * q * md_sqrt(s));
+            Real alpha, pref;
+            if constexpr (SLICE) {
+                quartet_factors(p, q, Real(Bp[R_IP]), Real(K[R_IP]), alpha, pref);
+            } else {
+                const Real s = p + q;
+                alpha = p * q / s;
+                pref = Real(TWO_PI_2_5) / (p * q * md_sqrt(s));
+            }
             Real R[NH(L1)];
             hermite_r<L1, SLICE, Real>(alpha, px - Real(K[R_PX]), py - Real(K[R_PY]), pz - Real(K[R_PZ]), pref, tab, R);
             static_for<0, NHK>([&](auto kk) {
@@ -80,47 +85,55 @@ MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __rest
             });
         }
         const Real mu = a * b / p;
-        Real ex[IS * JS * TS], ey[IS * JS * TS], ez[IS * JS * TS];
-        hermite_e<LA + 1, LB + 1, Real>(a, b, xab[0], md_exp(-mu * xab[0] * xab[0]), ex);
-        hermite_e<LA + 1, LB + 1, Real>(a, b, xab[1], md_exp(-mu * xab[1] * xab[1]), ey);
-        hermite_e<LA + 1, LB + 1, Real>(a, b, xab[2], md_exp(-mu * xab[2] * xab[2]), ez);
-        const Real ta = Real(2.0) * a, tb = Real(2.0) * b;
-        Real sA[3] = {0, 0, 0}, sB[3] = {0, 0, 0};
+        // E^{ij}_t for i <= l_A + 1, j <= l_B: the A derivative needs the row
+        // above; the B derivative follows from translational invariance of the
+        // pair distribution, (d/dA + d/dB) sum_t E_t L_t = sum_t E_t L_{t+1},
+        // i.e. the plain coefficients against the block shifted by one degree
+        Real ex[IS * JS0 * TS0], ey[IS * JS0 * TS0], ez[IS * JS0 * TS0];
+        hermite_e<LA + 1, LB, Real>(a, b, xab[0], md_exp(-mu * xab[0] * xab[0]), ex);
+        hermite_e<LA + 1, LB, Real>(a, b, xab[1], md_exp(-mu * xab[1] * xab[1]), ey);
+        hermite_e<LA + 1, LB, Real>(a, b, xab[2], md_exp(-mu * xab[2] * xab[2]), ez);
+        const Real ta = Real(2.0) * a;
+        Real sA[3] = {0, 0, 0}, sP[3] = {0, 0, 0};
         static_for<0, NAB>([&](auto aa) {
             constexpr int ab = decltype(aa)::value;
             constexpr int ia = ab / NB, ib = ab % NB;
             constexpr int ax = cart_x(LA, ia), ay = cart_y(LA, ia), az = cart_z(LA, ia);
             constexpr int bx = cart_x(LB, ib), by = cart_y(LB, ib), bz = cart_z(LB, ib);
             const Real f = fcart[ab] * cab;
+            auto E = [](const Real* e, int i, int j, int tt) { return e[(i * JS0 + j) * TS0 + tt]; };
+            auto yv = [&](int h) {
+                Real xg = 0;
+#pragma unroll
+                for (int j = 0; j < NCOL; ++j) xg += gam[ab * NCOL + j] * X[h * NCOL + j];
+                return f * xg;
+            };
             static_for<0, NHB1>([&](auto hh) {
                 constexpr int h = decltype(hh)::value;
                 constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
                 constexpr int ox = t > ax + bx, oy = u > ay + by, oz = v > az + bz;
+                // A derivative: one index may exceed the pair's degree by one
                 if constexpr (ox + oy + oz <= 1 && t <= ax + bx + 1 && u <= ay + by + 1 && v <= az + bz + 1) {
-                    auto E = [](const Real* e, int i, int j, int tt) { return e[(i * JS + j) * TS + tt]; };
-                    Real xg = 0;
-#pragma unroll
-                    for (int j = 0; j < NCOL; ++j) xg += gam[ab * NCOL + j] * X[h * NCOL + j];
-                    const Real y = f * xg;
+                    const Real y = yv(h);
                     const Real e0x = E(ex, ax, bx, t), e0y = E(ey, ay, by, u), e0z = E(ez, az, bz, v);
-                    Real dax = ta * E(ex, ax + 1, bx, t), dbx = tb * E(ex, ax, bx + 1, t);
-                    Real day = ta * E(ey, ay + 1, by, u), dby = tb * E(ey, ay, by + 1, u);
-                    Real daz = ta * E(ez, az + 1, bz, v), dbz = tb * E(ez, az, bz + 1, v);
+                    Real dax = ta * E(ex, ax + 1, bx, t), day = ta * E(ey, ay + 1, by, u), daz = ta * E(ez, az + 1, bz, v);
                     if constexpr (ax > 0) dax -= Real(ax) * E(ex, ax - 1, bx, t);
-                    if constexpr (bx > 0) dbx -= Real(bx) * E(ex, ax, bx - 1, t);
                     if constexpr (ay > 0) day -= Real(ay) * E(ey, ay - 1, by, u);
-                    if constexpr (by > 0) dby -= Real(by) * E(ey, ay, by - 1, u);
                     if constexpr (az > 0) daz -= Real(az) * E(ez, az - 1, bz, v);
-                    if constexpr (bz > 0) dbz -= Real(bz) * E(ez, az, bz - 1, v);
                     sA[0] += y * dax * e0y * e0z;
                     sA[1] += y * e0x * day * e0z;
                     sA[2] += y * e0x * e0y * daz;
-                    sB[0] += y * dbx * e0y * e0z;
-                    sB[1] += y * e0x * dby * e0z;
-                    sB[2] += y * e0x * e0y * dbz;
+                }
+                // A + B derivative: coefficient E_{tuv} against the block at h + e_d
+                if constexpr (t <= ax + bx && u <= ay + by && v <= az + bz) {
+                    const Real e = E(ex, ax, bx, t) * E(ey, ay, by, u) * E(ez, az, bz, v);
+                    sP[0] += e * yv(hidx(t + 1, u, v));
+                    sP[1] += e * yv(hidx(t, u + 1, v));
+                    sP[2] += e * yv(hidx(t, u, v + 1));
                 }
             });
         });
+        Real sB[3] = {sP[0] - sA[0], sP[1] - sA[1], sP[2] - sA[2]};
         for (int d = 0; d < 3; ++d) { gA[d] += double(sA[d]); gB[d] += double(sB[d]); }
     }
 }

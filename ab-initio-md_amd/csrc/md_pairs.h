@@ -35,7 +35,7 @@ MD_HD void write_record(double a, double b, double ca, double cb, const double* 
     rec[R_A] = a;
     rec[R_B] = b;
     rec[R_C] = ca * cb;
-    rec[7] = 0.0;
+    rec[R_IP] = 1.0 / p;
     double* E = rec + REC_HDR;
     for (int i = 0; i < NH(LA + LB) * NAB; ++i) E[i] = 0.0;
     for (int ia = 0; ia < NA; ++ia) {
