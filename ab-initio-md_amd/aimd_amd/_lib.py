@@ -35,7 +35,7 @@ SIGNATURES = {
     "aimdhf_jk": (_I, [_P, _I, _P, _P, _P, _P]),
     "aimdhf_jk_delta": (_I, [_P, _I, _P, _P, _P, _D, _P]),
     "aimdhf_set_jk_mode": (_I, [_P, _I, _L]),
-    "aimdhf_set_precision": (_I, [_P, _I, _D, _D]),
+    "aimdhf_set_precision": (_I, [_P, _I, _D, _D, _D]),
     "aimdhf_set_streams": (_I, [_P, _I]),
     "aimdhf_tc2_step": (_I, [_I, _P, _P, _P, _D, _P]),
     "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P, _P]),

@@ -109,13 +109,15 @@ class Engine:
 
     PRECISIONS = {"fp64": 0, "mixed": 1}
 
-    def set_precision(self, precision: str = "fp64", jk_switch: float = 0.0, grad_switch: float = 0.0) -> None:
+    def set_precision(self, precision: str = "fp64", jk_switch: float = 0.0, grad_switch: float = 0.0,
+                      delta_switch: float = 0.0) -> None:
         """"fp64", or "mixed": FP32 for quartets whose Schwarz bound (J/K store, from the next
-        set_geometry) or Schwarz-Gamma bound (derivative integrals) is below the switch."""
+        set_geometry) or Schwarz-Gamma bound (derivative integrals) is below the switch, and
+        for every stored integral in density-difference builds with max |dD| <= delta_switch."""
         if precision not in self.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(self.PRECISIONS)}, got {precision!r}")
         _lib.check(_lib.lib().aimdhf_set_precision(self._h, self.PRECISIONS[precision], float(jk_switch),
-                                                   float(grad_switch)))
+                                                   float(grad_switch), float(delta_switch)))
         self.precision = precision
 
     def set_streams(self, concurrent: bool = True) -> None:

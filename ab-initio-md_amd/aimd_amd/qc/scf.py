@@ -63,7 +63,8 @@ class SCFOptions:
     from the density difference, J(D) = J(D_prev) + J(D - D_prev), skipping in
     store mode the work units whose largest Schwarz product times max |dD| is
     below ``incremental_threshold`` (default: the Schwarz threshold) and, in
-    mixed precision, digesting the difference in FP32.
+    mixed precision once max |dD| <= ``delta_fp32_switch``, digesting the
+    difference in FP32.
     """
     conv_energy: float = 1e-10
     conv_commutator: float = 1e-7
@@ -86,7 +87,8 @@ class SCFOptions:
     density_solver: str = "purify"
     precision: str = "fp64"
     jk_fp32_switch: float = 1e-5
-    grad_fp32_switch: float = 1e-7
+    grad_fp32_switch: float = 1e-5
+    delta_fp32_switch: float = 1e-3
     incremental_fock: bool | None = None
     incremental_threshold: float | None = None
 
@@ -119,7 +121,7 @@ class SCFOptions:
             raise ValueError("max_store_bytes must be >= 0")
         if self.precision not in qengine.Engine.PRECISIONS:
             raise ValueError(f"unknown precision {self.precision!r}; use one of {sorted(qengine.Engine.PRECISIONS)}")
-        if not (self.jk_fp32_switch >= 0.0 and self.grad_fp32_switch >= 0.0):
+        if not (self.jk_fp32_switch >= 0.0 and self.grad_fp32_switch >= 0.0 and self.delta_fp32_switch >= 0.0):
             raise ValueError("fp32 switch thresholds must be >= 0")
         if self.incremental_threshold is not None and not (self.incremental_threshold >= 0.0
                                                             and np.isfinite(self.incremental_threshold)):
@@ -751,7 +753,7 @@ class SCFSolver:
         self.engine = qengine.Engine(self.basis)
         self.engine.set_jk_mode(self.options.jk_mode, self.options.max_store_bytes)
         self.engine.set_precision(self.options.precision, self.options.jk_fp32_switch,
-                                  self.options.grad_fp32_switch)
+                                  self.options.grad_fp32_switch, self.options.delta_fp32_switch)
         self._ints: SCFIntegrals | None = None
         self._ints_positions: np.ndarray | None = None
         self._ints_options: SCFOptions | None = None

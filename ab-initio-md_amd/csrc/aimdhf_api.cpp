@@ -253,6 +253,9 @@ struct aimdhf_plan {
     // and in mixed precision FP32 copies of the FP64 parts
     DevBuf d_kunit_q, d_junit_q, d_dmax, d_kdata32b, d_jdata32b;
     bool have_32b = false;
+    // largest max |dD| at which mixed-precision difference builds digest every
+    // unit in FP32 (larger differences use the FP64 parts of the store)
+    double delta_fp32_max = 1e-3;
     DevBuf d_tab;                   // store-table temporaries (sizes, unit counts, scan blocks)
     long long* h_pin = nullptr;     // pinned host words for the totals and the error flag
 
@@ -458,6 +461,7 @@ struct aimdhf_plan {
         ka.unit_q = d_kunit_q.as<double>();
         ka.dmax = d_dmax.as<double>();
         ka.dthr = 0.0;
+        ka.d32 = 0.0;
         ka.kdata32b = have_32b ? d_kdata32b.as<float>() : nullptr;
         return ka;
     }
@@ -497,6 +501,7 @@ struct aimdhf_plan {
         ja.unit_q = d_junit_q.as<double>();
         ja.dmax = d_dmax.as<double>();
         ja.dthr = 0.0;
+        ja.d32 = 0.0;
         ja.jdata32b = have_32b ? d_jdata32b.as<float>() : nullptr;
         return ja;
     }
@@ -1088,6 +1093,7 @@ static int jk_from_store(aimdhf_plan* p, int nset, const double* dens, double* J
         aimdhf::JSArgs ja = p->js_args(nset, p->d_dsym.as<double>(), J);
         ka.delta = ja.delta = delta ? 1 : 0;
         ka.dthr = ja.dthr = dthr;
+        ka.d32 = ja.d32 = p->delta_fp32_max;
         for (int t = 8; t >= 0; --t) {
             int nout = p->out_off[t + 1] - p->out_off[t];
             if (nout <= 0) continue;
@@ -1353,13 +1359,15 @@ int aimdhf_set_streams(aimdhf_plan* p, int concurrent) {
     return AIMDHF_OK;
 }
 
-int aimdhf_set_precision(aimdhf_plan* p, int mode, double jk_switch, double grad_switch) {
+int aimdhf_set_precision(aimdhf_plan* p, int mode, double jk_switch, double grad_switch, double delta_switch) {
     if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
     if (mode < 0 || mode > 1) return fail(AIMDHF_E_ARG, "precision mode must be 0 (fp64) or 1 (mixed)");
-    if (!(jk_switch >= 0.0) || !(grad_switch >= 0.0)) return fail(AIMDHF_E_ARG, "switch thresholds must be >= 0");
+    if (!(jk_switch >= 0.0) || !(grad_switch >= 0.0) || !(delta_switch >= 0.0))
+        return fail(AIMDHF_E_ARG, "switch thresholds must be >= 0");
     p->precision = mode;
     p->jk_switch = jk_switch;
     p->grad_switch = grad_switch;
+    p->delta_fp32_max = delta_switch;
     return AIMDHF_OK;
 }
 

@@ -279,11 +279,13 @@ struct KSArgs {
     double* K;
     // density-difference builds (incremental Fock): a unit whose largest
     // Schwarz product times max |dens| is below dthr contributes nothing;
-    // with kdata32b set, FP64-part units read that FP32 copy (same offsets)
+    // with kdata32b set and max |dens| <= d32, FP64-part units read that FP32
+    // copy (same offsets)
     int delta;
     const double* unit_q;       // per unit: largest q_P q_Q of its entries
     const double* dmax;         // device scalar max |dens|
     double dthr;
+    double d32;
     float* kdata32b;            // FP32 copy of the FP64 part (mixed precision), or null
 };
 
@@ -317,6 +319,7 @@ struct JSArgs {
     const double* unit_q;
     const double* dmax;
     double dthr;
+    double d32;
     float* jdata32b;            // FP32 copy of the FP64 part (mixed precision), or null
 };
 

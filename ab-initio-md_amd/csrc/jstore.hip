@@ -14,8 +14,8 @@
 // unit (lanes summed by a fixed butterfly) and k_jreduce adds each pair's
 // unit partials in unit order into J_P, so J is bitwise reproducible. For a
 // density difference (incremental Fock) units whose largest Schwarz product
-// times max |dD| is below a threshold are skipped, and in mixed precision
-// every unit is digested from FP32 copies.
+// times max |dD| is below a threshold are skipped, and in mixed precision,
+// once max |dD| is small, every unit is digested from FP32 copies.
 
 #include "aimdhf_internal.h"
 #include "md_core.h"
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(64) void k_jstore(JSArgs a) {
     with_part(cq, [&](auto pt) {
         using PT = decltype(pt);
         const size_t seg = (size_t)P * 6 + cq;
-        if (f32 || (a.delta && a.jdata32b)) {
+        if (f32 || (a.delta && a.jdata32b && a.dmax[0] <= a.d32)) {
             // FP32 sums per unit, folded into the FP64 block
             float acc[NSET * NAB];
 #pragma unroll

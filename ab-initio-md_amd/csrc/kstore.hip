@@ -16,8 +16,8 @@
 // type in its own region); k_kreduce adds each block's unit
 // partials in unit order and writes K[X][Y] and K[Y][X]. For a density
 // difference (incremental Fock) units whose largest Schwarz product times
-// max |dD| is below a threshold are skipped, and in mixed precision every
-// unit is digested from FP32 copies. No atomics, so the
+// max |dD| is below a threshold are skipped, and in mixed precision, once
+// max |dD| is small, every unit is digested from FP32 copies. No atomics, so the
 // result is bitwise reproducible run to run.
 
 #include "aimdhf_internal.h"
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(64) void k_kstore(KSArgs a) {
     with_sub(sub, [&](auto sb) {
         using SB = decltype(sb);
         const size_t seg = (size_t)o * 9 + sub;
-        if (f32 || (a.delta && a.kdata32b)) {
+        if (f32 || (a.delta && a.kdata32b && a.dmax[0] <= a.d32)) {
             // FP32 sums per unit, folded into the FP64 block
             float acc[NSET * NX * NY];
 #pragma unroll

@@ -149,6 +149,7 @@ def main() -> int:
                  "within_bounds": bool(de <= 1e-6 and dfm <= 1e-5),
                  "jk_fp32_switch": bm.solver.options.jk_fp32_switch,
                  "grad_fp32_switch": bm.solver.options.grad_fp32_switch,
+                 "delta_fp32_switch": bm.solver.options.delta_fp32_switch,
                  "incremental_fock": bm.solver.incremental,
                  "incremental_threshold": (bm.solver.options.incremental_threshold
                                            if bm.solver.options.incremental_threshold is not None

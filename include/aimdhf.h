@@ -126,11 +126,12 @@ int aimdhf_set_jk_mode(aimdhf_plan* plan, int mode, int64_t max_store_bytes);
  * derivative integrals of quartets with Q_P Q_Q max|Gamma| < grad_switch are
  * evaluated in FP32 (accumulated per quartet in FP32, summed in exact fixed
  * point), and, in store mode, integrals of quartets with Q_P Q_Q < jk_switch
- * are held and digested in FP32 (taking effect at the next set_geometry).
- * Large-bound quartets, the SCF algebra and all sums stay FP64. No
- * counterpart in the reference (FP64 only).
+ * are held and digested in FP32 (taking effect at the next set_geometry);
+ * aimdhf_jk_delta digests every stored integral in FP32 once max |dD| <=
+ * delta_switch. Large-bound quartets, the SCF algebra and all sums stay FP64.
+ * No counterpart in the reference (FP64 only).
  */
-int aimdhf_set_precision(aimdhf_plan* plan, int mode, double jk_switch, double grad_switch);
+int aimdhf_set_precision(aimdhf_plan* plan, int mode, double jk_switch, double grad_switch, double delta_switch);
 
 /*
  * One trace-correcting purification step on device matrices (n x n, row-major):

@@ -116,7 +116,7 @@ def test_density_difference_builds(oracle_lib, mode, precision):
     b = build_basis(s, x, "6-31g*")
     e = Engine(b)
     e.set_jk_mode(mode)
-    e.set_precision(precision, 1e-5, 1e-7)
+    e.set_precision(precision, 1e-5, 1e-7, 1.0)
     e.set_geometry(x)
     rng = np.random.default_rng(11)
     dD = rng.standard_normal((1, b.nao, b.nao)) * 1e-4
