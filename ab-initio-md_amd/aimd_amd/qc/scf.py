@@ -248,33 +248,42 @@ def inverse_sqrt_newton_schulz(S: torch.Tensor, max_iter: int = 60, tol: float =
 
 class TC2Graph:
     """
-    STEPS purification steps of an n x n matrix with nocc electrons captured once as
-    a HIP graph: each step is a dense FP64 product X2 = X X (BLAS) and the engine's
+    Purification steps of an n x n matrix with nocc electrons captured once as HIP
+    graphs, STEPS steps (``replay``) and TAIL_STEPS steps (``tail``) on the same
+    buffers: each step is a dense FP64 product X2 = X X (BLAS) and the engine's
     fused update (aimdhf_tc2_step), ping-ponging between two static buffers so the
     result of an even number of steps lands back in ``X``. The update kernel needs
     no reduction pass and clears no memory, so replays are exact repeats.
+    ``needed`` remembers how many replays the last purification took, so the next
+    one (the Fock matrix of the next SCF iteration) checks convergence only from
+    there on.
     """
     STEPS = 8
+    TAIL_STEPS = 2
 
     def __init__(self, n: int, nocc: int, device: torch.device) -> None:
         self.n, self.nocc = n, float(nocc)
+        self.needed = 1
         self.X = torch.zeros((n, n), dtype=torch.float64, device=device)
         self.B = torch.empty_like(self.X)
         self.X2 = torch.empty_like(self.X)
         side = torch.cuda.Stream(device=device)
         side.wait_stream(torch.cuda.current_stream(device))
         with torch.cuda.stream(side):
-            self._steps()
+            self._steps(self.STEPS)
         torch.cuda.current_stream(device).wait_stream(side)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self._steps()
+            self._steps(self.STEPS)
+        self.tail_graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.tail_graph):
+            self._steps(self.TAIL_STEPS)
 
-    def _steps(self) -> None:
+    def _steps(self, nsteps: int) -> None:
         L = _lib.lib()
         st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         src, dst = self.X, self.B
-        for _ in range(self.STEPS):
+        for _ in range(nsteps):
             torch.matmul(src, src, out=self.X2)
             _lib.check(L.aimdhf_tc2_step(self.n, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(self.X2.data_ptr()),
                                          ctypes.c_void_p(dst.data_ptr()), self.nocc, st))
@@ -284,6 +293,10 @@ class TC2Graph:
         self.graph.replay()
         return self.X
 
+    def tail(self) -> torch.Tensor:
+        self.tail_graph.replay()
+        return self.X
+
 
 _TC2_GRAPHS: dict[tuple, TC2Graph] = {}
 
@@ -291,7 +304,9 @@ _TC2_GRAPHS: dict[tuple, TC2Graph] = {}
 def purified_projector_device(F: torch.Tensor, nocc: int, tol: float = 1e-11,
                               max_replays: int = 15) -> torch.Tensor | None:
     """purified_projector on the GPU with the captured TC2Graph (same steps, same branch
-    rule, the idempotency error read back after every TC2Graph.STEPS steps)."""
+    rule): the idempotency error is read back after the number of replays the previous
+    call needed less one, then after every replay, and the TAIL_STEPS tail graph
+    squares the remaining error away."""
     n = F.shape[0]
     if nocc <= 0:
         return torch.zeros_like(F)
@@ -307,11 +322,16 @@ def purified_projector_device(F: torch.Tensor, nocc: int, tol: float = 1e-11,
     r = a.sum(dim=1) - torch.diagonal(a)
     emin, emax = (d - r).min(), (d + r).max()
     g.X.copy_((emax * eye - F) / (emax - emin))
-    for _ in range(max_replays):
+    first = max(1, min(g.needed - 1, max_replays))
+    for _ in range(first - 1):
+        g.replay()
+    for k in range(first, max_replays + 1):
         X = g.replay()
         if float((torch.trace(X) - torch.sum(X * X)).abs()) < tol:
-            X = g.replay()
+            g.needed = k
+            X = g.tail()
             return 0.5 * (X + X.T)
+    g.needed = 1
     return None
 
 
@@ -581,11 +601,13 @@ class DIIS:
         m = len(self.errors)
         if m == 1:
             return np.ones(1)
-        last = self.errors[-1].reshape(-1)
-        delta = torch.stack([e.reshape(-1) - last for e in self.errors[:-1]])
-        # Gram matrix and right-hand side as elementwise products + reductions: the
-        # (m x nao^2)(nao^2 x m) shape maps to a single-workgroup GEMM in the BLAS
-        Gr = torch.stack([(delta * d).sum(dim=1) for d in delta] + [-(delta * last).sum(dim=1)])
+        E = torch.stack([e.reshape(-1) for e in self.errors])
+        last = E[-1]
+        delta = E[:-1] - last
+        # Gram matrix and right-hand side as one broadcast product and one reduction:
+        # the (m x nao^2)(nao^2 x m) shape maps to a single-workgroup GEMM in the BLAS
+        D = torch.cat([delta, -last[None]])
+        Gr = (delta[None, :, :] * D[:, None, :]).sum(dim=2)
         Gr = Gr.cpu().numpy()
         G, r = Gr[:-1], Gr[-1]
         lam, V = np.linalg.eigh(G)
@@ -599,10 +621,9 @@ class DIIS:
         while len(self.focks) > 1:
             c = self.coefficients()
             if np.all(np.isfinite(c)) and np.abs(c).max() <= self.max_coefficient:
-                out = self.focks[0] * float(c[0])
-                for ci, Fi in zip(c[1:], self.focks[1:]):
-                    out = out + float(ci) * Fi
-                return out
+                F = torch.stack(self.focks)
+                ct = torch.tensor(c, dtype=F.dtype).to(F.device, non_blocking=False)
+                return (ct.view(-1, *([1] * (F.dim() - 1))) * F).sum(dim=0)
             self.focks.pop(0)
             self.errors.pop(0)
         return self.focks[-1].clone()
