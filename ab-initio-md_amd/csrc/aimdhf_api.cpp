@@ -212,6 +212,8 @@ struct aimdhf_plan {
     bool have_geometry = false;
     double threshold = 0.0;
     std::vector<long long> qcount;   // per class
+    std::vector<long long> q64count; // per class, quartets with q_P q_Q >= the FP32 switch (listed first)
+    std::vector<long long> g64count;
     long long qtotal = 0;
 
     DevBuf d_shell_atom, d_shell_l, d_shell_prim, d_shell_ao, d_prim_exp, d_prim_coef, d_pos;
@@ -313,26 +315,33 @@ struct aimdhf_plan {
 
     // Class-binned list of unique shell quartets with q_P q_Q >= thr into buf;
     // counts[k] receives the per-class sizes. Synchronises the stream.
-    void build_list(double thr, DevBuf& buf, std::vector<long long>& counts, hipStream_t st) {
+    // Screened quartet list binned by class, precision part (q_P q_Q >= sw
+    // first) and cost; counts per class, n64 per class the part >= sw.
+    void build_list(double thr, double sw, DevBuf& buf, std::vector<long long>& counts, std::vector<long long>& n64,
+                    hipStream_t st) {
         aimdhf::PairArgs g = pair_args();
-        const int nv = NQCLASS * aimdhf::NCOSTBIN;
+        const int nb = aimdhf::list_bins(sw), nv = NQCLASS * nb;
         d_counts.ensure(sizeof(int) * (size_t)npair * nv);
         d_offsets.ensure(sizeof(long long) * ((size_t)npair * nv + 2 * nv));
         long long* offs = d_offsets.as<long long>();
         long long* totals = offs + (size_t)npair * nv;
         long long* base = totals + nv;
-        aimdhf::launch_count_quartets(g, thr, d_counts.as<int>(), offs, totals, base, st);
+        aimdhf::launch_count_quartets(g, thr, sw, d_counts.as<int>(), offs, totals, base, st);
         HIPCHECK(hipGetLastError());
         std::vector<long long> tot(nv);
         HIPCHECK(hipMemcpyAsync(tot.data(), totals, sizeof(long long) * nv, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         counts.assign(NQCLASS, 0);
+        n64.assign(NQCLASS, 0);
         long long total = 0;
         for (int k = 0; k < NQCLASS; ++k)
-            for (int b = 0; b < aimdhf::NCOSTBIN; ++b) counts[k] += tot[k * aimdhf::NCOSTBIN + b];
+            for (int b = 0; b < nb; ++b) {
+                counts[k] += tot[k * nb + b];
+                if (b < aimdhf::NCOSTBIN) n64[k] += tot[k * nb + b];
+            }
         for (long long c : counts) total += c;
         buf.ensure(sizeof(int2) * std::max<long long>(total, 1));
-        aimdhf::launch_fill_quartets(g, thr, offs, base, buf.as<int2>(), st);
+        aimdhf::launch_fill_quartets(g, thr, sw, offs, base, buf.as<int2>(), st);
         HIPCHECK(hipGetLastError());
     }
 
@@ -532,7 +541,7 @@ struct aimdhf_plan {
                 if (!qcount[k]) continue;
                 sa.store = d_store.as<double>() + store_base[k];
                 hipStream_t s2 = fan.on(1 + i++ % Fan::NAUX);
-                prof.time(47, s2, [&] { aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], s2); });
+                prof.time(47, s2, [&] { aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], q64count[k], s2); });
             }
         }
         prof.time(48, st, [&] {
@@ -976,7 +985,8 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
         });
         HIPCHECK(hipGetLastError());
         auto t0 = Profiler::clock::now();
-        p->build_list(thr, p->d_list, p->qcount, st);
+        p->build_list(thr, p->precision == 1 && p->jk_switch > thr ? p->jk_switch : 0.0, p->d_list, p->qcount,
+                      p->q64count, st);
         p->qtotal = 0;
         for (long long c : p->qcount) p->qtotal += c;
         p->threshold = thr;
@@ -1280,7 +1290,7 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
         double gbound = 4.0 * (mc * mc + std::fabs(kfac) * nx * mx * mx);
         double lthr = gbound > 0.0 ? thr / gbound : 0.0;
         auto t0 = Profiler::clock::now();
-        p->build_list(lthr, p->d_glist, p->gcount, st);
+        p->build_list(lthr, 0.0, p->d_glist, p->gcount, p->g64count, st);
         p->prof.host(88, t0);
         p->d_gfx.ensure(sizeof(long long) * 6 * p->natm);
         aimdhf::launch_zero64(p->d_gfx.as<long long>(), 6 * p->natm, st);

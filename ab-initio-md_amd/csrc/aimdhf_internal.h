@@ -344,7 +344,7 @@ struct UnitQArgs {
     double* junit_q;
 };
 
-void launch_eri_store(int k, const StoreArgs& a, const int2* list, long long nq, hipStream_t st);
+void launch_eri_store(int k, const StoreArgs& a, const int2* list, long long nq, long long n64, hipStream_t st);
 long long store_block(int k);
 void launch_klist_count(const ListArgs& a, int nout, long long* counts, long long* band, hipStream_t st);
 void launch_klist_fill(const ListArgs& a, int nout, hipStream_t st);
@@ -376,9 +376,13 @@ void launch_build_pairs(const PairArgs& g, hipStream_t st);
 void launch_schwarz(const PairArgs& g, const int* ncls_host, hipStream_t st);
 // quartet lists: per class, NCOSTBIN bins of primitive-quartet cost (heaviest first)
 constexpr int NCOSTBIN = 5;
-void launch_count_quartets(const PairArgs& g, double thr, int* counts, long long* offs, long long* totals,
+// list bins of a class: the quartets with q_P q_Q >= sw (FP64) in cost bins,
+// then those below sw (FP32 in mixed precision) in cost bins
+constexpr int NLBIN = 2 * NCOSTBIN;
+void launch_count_quartets(const PairArgs& g, double thr, double sw, int* counts, long long* offs, long long* totals,
                            long long* base, hipStream_t st);
-void launch_fill_quartets(const PairArgs& g, double thr, const long long* offs, const long long* base, int2* list,
+int list_bins(double sw);
+void launch_fill_quartets(const PairArgs& g, double thr, double sw, const long long* offs, const long long* base, int2* list,
                           hipStream_t st);
 void launch_zero64(long long* p, size_t n, hipStream_t st);
 void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st);

@@ -96,43 +96,45 @@ __device__ __forceinline__ int qclass(int cp, int cq) {
     return hi * (hi + 1) / 2 + lo;
 }
 
-// Within a class, quartets are further binned by cost npp_P * npp_Q (kept
-// primitive pairs), heaviest bin first, so the lanes of a wavefront run
+// Within a class, quartets are further binned by precision (q_P q_Q >= sw
+// first: FP64; then below sw: FP32 in mixed precision) and cost npp_P * npp_Q
+// (kept primitive pairs), heaviest bin first, so the lanes of a wavefront run
 // loops of similar length; within a bin the order is P-major, Q ascending.
 __device__ __forceinline__ int cost_bin(int c) {
     return c > 256 ? 0 : c > 64 ? 1 : c > 16 ? 2 : c > 4 ? 3 : 4;
 }
 
+// list bin of a kept quartet: precision part, then cost bin
+__device__ __forceinline__ int list_bin(double qq, double sw, int cost) {
+    return (qq < sw ? NCOSTBIN : 0) + cost_bin(cost);
+}
+
 // One wavefront per pair P, lanes over Q <= P in chunks of 64: counts per
-// (class, bin) into counts[vc * npair + P], vc = class * NCOSTBIN + bin.
-__global__ __launch_bounds__(64) void k_count_quartets(PairArgs g, double thr, int* counts) {
+// (class, list bin) into counts[vc * npair + P], vc = class * NB + bin, with
+// NB = NLBIN (precision split at sw) or NCOSTBIN (no split). Per (partner
+// class, bin) histogram in LDS (integer adds: order-independent).
+template <int NB>
+__global__ __launch_bounds__(64) void k_count_quartets(PairArgs g, double thr, double sw, int* counts) {
+    __shared__ int hist[6 * NB];
     const int P = blockIdx.x;
     const int lane = threadIdx.x;
     const double qp = g.qscr[P];
     const int cp = g.pair_cls[P];
     const int np = g.npp[P];
-    int c[6 * NCOSTBIN];
-#pragma unroll
-    for (int i = 0; i < 6 * NCOSTBIN; ++i) c[i] = 0;
+    for (int i = lane; i < 6 * NB; i += 64) hist[i] = 0;
+    __syncthreads();
     for (int Q = lane; Q <= P; Q += 64) {
-        if (qp * g.qscr[Q] < thr) continue;
-        const int cq = g.pair_cls[Q], b = cost_bin(np * g.npp[Q]);
-#pragma unroll
-        for (int i = 0; i < 6 * NCOSTBIN; ++i) c[i] += (cq * NCOSTBIN + b == i);
+        const double qq = qp * g.qscr[Q];
+        if (qq < thr) continue;
+        atomicAdd(&hist[g.pair_cls[Q] * NB + (NB == NLBIN ? list_bin(qq, sw, np * g.npp[Q]) : cost_bin(np * g.npp[Q]))],
+                  1);
     }
-#pragma unroll
-    for (int i = 0; i < 6 * NCOSTBIN; ++i) {
-        int v = c[i];
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-        c[i] = v;
-    }
-    for (int vc = lane; vc < NQCLASS * NCOSTBIN; vc += 64) {
-        const int k = vc / NCOSTBIN, b = vc % NCOSTBIN;
+    __syncthreads();
+    for (int vc = lane; vc < NQCLASS * NB; vc += 64) {
+        const int k = vc / NB, b = vc % NB;
         int v = 0;
-#pragma unroll
         for (int cq = 0; cq < 6; ++cq)
-            if (qclass(cp, cq) == k) v += c[cq * NCOSTBIN + b];
+            if (qclass(cp, cq) == k) v += hist[cq * NB + b];
         counts[(size_t)vc * g.npair + P] = v;
     }
 }
@@ -180,34 +182,44 @@ __global__ void k_scan_totals(const long long* totals, int n, long long* base) {
     }
 }
 
-__global__ __launch_bounds__(64) void k_fill_quartets(PairArgs g, double thr, const long long* offs,
+// Ordered fill: per 64-partner chunk, one ballot per key present (the chunk's
+// lanes grouped by (partner class, bin)), running offsets in LDS.
+template <int NB>
+__global__ __launch_bounds__(64) void k_fill_quartets(PairArgs g, double thr, double sw, const long long* offs,
                                                       const long long* base, int2* list) {
+    __shared__ long long o[6 * NB];
     const int P = blockIdx.x;
     const int lane = threadIdx.x;
     const double qp = g.qscr[P];
     const int cp = g.pair_cls[P];
     const int np = g.npp[P];
-    long long o[6 * NCOSTBIN];
-#pragma unroll
-    for (int cq = 0; cq < 6; ++cq)
-#pragma unroll
-        for (int b = 0; b < NCOSTBIN; ++b) {
-            const int vc = qclass(cp, cq) * NCOSTBIN + b;
-            o[cq * NCOSTBIN + b] = base[vc] + offs[(size_t)vc * g.npair + P];
-        }
+    for (int i = lane; i < 6 * NB; i += 64) {
+        const int cq = i / NB, b = i % NB;
+        const int vc = qclass(cp, cq) * NB + b;
+        o[i] = base[vc] + offs[(size_t)vc * g.npair + P];
+    }
+    __syncthreads();
     const unsigned long long below = (lane ? (~0ull >> (64 - lane)) : 0ull);
     for (int Q0 = 0; Q0 <= P; Q0 += 64) {
         const int Q = Q0 + lane;
-        const bool in = Q <= P && !(qp * g.qscr[Q] < thr);
-        const int key = in ? g.pair_cls[Q] * NCOSTBIN + cost_bin(np * g.npp[Q]) : -1;
-#pragma unroll
-        for (int i = 0; i < 6 * NCOSTBIN; ++i) {
-            const unsigned long long m = __ballot(key == i);
-            if (key == i) {
-                const int cq = i / NCOSTBIN;
-                list[o[i] + __popcll(m & below)] = cp >= cq ? make_int2(P, Q) : make_int2(Q, P);
+        const double qq = Q <= P ? qp * g.qscr[Q] : 0.0;
+        const bool in = Q <= P && !(qq < thr);
+        const int key = in ? g.pair_cls[Q] * NB +
+                                 (NB == NLBIN ? list_bin(qq, sw, np * g.npp[Q]) : cost_bin(np * g.npp[Q]))
+                           : -1;
+        unsigned long long active = __ballot(in);
+        while (active) {
+            const int kk = __shfl(key, __ffsll((long long)active) - 1, 64);
+            const unsigned long long m = __ballot(key == kk);
+            const long long at = o[kk];
+            if (key == kk) {
+                const int cq = kk / NB;
+                list[at + __popcll(m & below)] = cp >= cq ? make_int2(P, Q) : make_int2(Q, P);
             }
-            o[i] += __popcll(m);
+            __syncthreads();
+            if (lane == 0) o[kk] = at + __popcll(m);
+            __syncthreads();
+            active &= ~m;
         }
     }
 }
@@ -283,17 +295,24 @@ void launch_schwarz(const PairArgs& g, const int* ncls_host, hipStream_t st) {
     if (g.nbpair) k_block_q<<<GRID(g.nbpair, 64), 0, st>>>(g);
 }
 
-void launch_count_quartets(const PairArgs& g, double thr, int* counts, long long* offs, long long* totals,
+// columns per class: NLBIN with a precision split (sw > 0), else NCOSTBIN
+int list_bins(double sw) { return sw > 0.0 ? NLBIN : NCOSTBIN; }
+
+void launch_count_quartets(const PairArgs& g, double thr, double sw, int* counts, long long* offs, long long* totals,
                            long long* base, hipStream_t st) {
     if (!g.npair) return;
-    k_count_quartets<<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, counts);
-    k_scan_columns<<<dim3(NQCLASS * NCOSTBIN), dim3(1024), 0, st>>>(counts, g.npair, offs, totals);
-    k_scan_totals<<<dim3(1), dim3(1), 0, st>>>(totals, NQCLASS * NCOSTBIN, base);
+    const int nb = list_bins(sw);
+    if (nb == NLBIN) k_count_quartets<NLBIN><<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, sw, counts);
+    else k_count_quartets<NCOSTBIN><<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, sw, counts);
+    k_scan_columns<<<dim3(NQCLASS * nb), dim3(1024), 0, st>>>(counts, g.npair, offs, totals);
+    k_scan_totals<<<dim3(1), dim3(1), 0, st>>>(totals, NQCLASS * nb, base);
 }
 
-void launch_fill_quartets(const PairArgs& g, double thr, const long long* offs, const long long* base, int2* list,
-                          hipStream_t st) {
-    if (g.npair) k_fill_quartets<<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, offs, base, list);
+void launch_fill_quartets(const PairArgs& g, double thr, double sw, const long long* offs, const long long* base,
+                          int2* list, hipStream_t st) {
+    if (!g.npair) return;
+    if (list_bins(sw) == NLBIN) k_fill_quartets<NLBIN><<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, sw, offs, base, list);
+    else k_fill_quartets<NCOSTBIN><<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, sw, offs, base, list);
 }
 
 void launch_zero64(long long* p, size_t n, hipStream_t st) {

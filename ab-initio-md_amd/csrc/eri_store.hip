@@ -2,7 +2,9 @@
 // (compiled once per class with -DAIMDHF_QCLASS=k) into the HBM integral
 // store. Quartet t of the class list (bra = heavier pair, ties: higher pair
 // index) keeps its contracted block at store[t * NAB * NCD + ab * NCD + cd],
-// ab / cd over the bra / ket pair records.
+// ab / cd over the bra / ket pair records. The list holds the class's FP64
+// quartets first; in mixed precision the rest (q_P q_Q below the switch, held
+// and digested in FP32 afterwards) are also evaluated in FP32.
 
 #include "aimdhf_internal.h"
 #include "md_eri.h"
@@ -26,7 +28,7 @@ constexpr bool SPLIT = split_by_cd<LA, LB, LC, LD>();
 constexpr int BLK = Qt::NAB * Qt::NCD;
 constexpr int NSPLIT = SPLIT ? Qt::NCD : 1;
 
-template <int LA_, int LB_, int LC_, int LD_>
+template <int LA_, int LB_, int LC_, int LD_, typename Real>
 __global__ __launch_bounds__(256) void k_eri_store(StoreArgs a, const int2* __restrict__ list, long long nq) {
     __shared__ double btab[BOYS_NGRID * BOYS_SROW];
     boys_stage(a.boys, LA_ + LB_ + LC_ + LD_, btab);
@@ -39,15 +41,15 @@ __global__ __launch_bounds__(256) void k_eri_store(StoreArgs a, const int2* __re
     double* out = a.store + q * BLK;
     if constexpr (SPLIT) {
         const int cd = (int)(t % NSPLIT);
-        double acc[Qt::NAB];
-        eri_quartet_cd<LA_, LB_, LC_, LD_, true>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, cd, acc);
+        Real acc[Qt::NAB];
+        eri_quartet_cd<LA_, LB_, LC_, LD_, true, Real>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, cd, acc);
 #pragma unroll
-        for (int ab = 0; ab < Qt::NAB; ++ab) out[ab * Qt::NCD + cd] = acc[ab];
+        for (int ab = 0; ab < Qt::NAB; ++ab) out[ab * Qt::NCD + cd] = double(acc[ab]);
     } else {
-        double acc[BLK];
-        eri_quartet<LA_, LB_, LC_, LD_, true>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, acc);
+        Real acc[BLK];
+        eri_quartet<LA_, LB_, LC_, LD_, true, Real>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, acc);
 #pragma unroll
-        for (int i = 0; i < BLK; ++i) out[i] = acc[i];
+        for (int i = 0; i < BLK; ++i) out[i] = double(acc[i]);
     }
 }
 }  // namespace
@@ -55,10 +57,19 @@ __global__ __launch_bounds__(256) void k_eri_store(StoreArgs a, const int2* __re
 #define CAT2(a, b) a##b
 #define CAT(a, b) CAT2(a, b)
 
-void CAT(launch_eri_store_, AIMDHF_QCLASS)(const StoreArgs& a, const int2* list, long long nq, hipStream_t st) {
-    const long long nt = nq * NSPLIT;
+// quartets [0, n64) of the class list in FP64, [n64, nq) in FP32
+void CAT(launch_eri_store_, AIMDHF_QCLASS)(const StoreArgs& a, const int2* list, long long nq, long long n64,
+                                           hipStream_t st) {
+    const long long nt = n64 * NSPLIT;
     if (nt > 0)
-        k_eri_store<LA, LB, LC, LD><<<dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st>>>(a, list, nq);
+        k_eri_store<LA, LB, LC, LD, double><<<dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st>>>(a, list, n64);
+    const long long nt32 = (nq - n64) * NSPLIT;
+    if (nt32 > 0) {
+        StoreArgs a32 = a;
+        a32.store = a.store + n64 * BLK;
+        k_eri_store<LA, LB, LC, LD, float><<<dim3((unsigned)((nt32 + 255) / 256)), dim3(256), 0, st>>>(
+            a32, list + n64, nq - n64);
+    }
 }
 
 }  // namespace aimdhf

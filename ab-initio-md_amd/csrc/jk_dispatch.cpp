@@ -66,7 +66,7 @@ void launch_kown(int type, const KOArgs& a, int nout, const int* has_l, hipStrea
     fns[type](a, nout, has_l, st);
 }
 
-#define DECLS(k) void launch_eri_store_##k(const StoreArgs&, const int2*, long long, hipStream_t);
+#define DECLS(k) void launch_eri_store_##k(const StoreArgs&, const int2*, long long, long long, hipStream_t);
 DECLS(0) DECLS(1) DECLS(2) DECLS(3) DECLS(4) DECLS(5) DECLS(6) DECLS(7) DECLS(8) DECLS(9) DECLS(10)
 DECLS(11) DECLS(12) DECLS(13) DECLS(14) DECLS(15) DECLS(16) DECLS(17) DECLS(18) DECLS(19) DECLS(20)
 #undef DECLS
@@ -79,15 +79,15 @@ DECLJS(0) DECLJS(1) DECLJS(2) DECLJS(3) DECLJS(4) DECLJS(5)
 DECLKS(0, 0) DECLKS(0, 1) DECLKS(0, 2) DECLKS(1, 0) DECLKS(1, 1) DECLKS(1, 2) DECLKS(2, 0) DECLKS(2, 1) DECLKS(2, 2)
 #undef DECLKS
 
-void launch_eri_store(int k, const StoreArgs& a, const int2* list, long long nq, hipStream_t st) {
-    typedef void (*Fn)(const StoreArgs&, const int2*, long long, hipStream_t);
+void launch_eri_store(int k, const StoreArgs& a, const int2* list, long long nq, long long n64, hipStream_t st) {
+    typedef void (*Fn)(const StoreArgs&, const int2*, long long, long long, hipStream_t);
     static const Fn fns[21] = {
         launch_eri_store_0, launch_eri_store_1, launch_eri_store_2, launch_eri_store_3, launch_eri_store_4,
         launch_eri_store_5, launch_eri_store_6, launch_eri_store_7, launch_eri_store_8, launch_eri_store_9,
         launch_eri_store_10, launch_eri_store_11, launch_eri_store_12, launch_eri_store_13, launch_eri_store_14,
         launch_eri_store_15, launch_eri_store_16, launch_eri_store_17, launch_eri_store_18, launch_eri_store_19,
         launch_eri_store_20};
-    fns[k](a, list, nq, st);
+    fns[k](a, list, nq, n64, st);
 }
 
 void launch_jcopy(int pclass, const JSArgs& a, int npairs, hipStream_t st) {
