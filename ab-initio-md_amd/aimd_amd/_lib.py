@@ -38,6 +38,8 @@ SIGNATURES = {
     "aimdhf_set_precision": (_I, [_P, _I, _D, _D, _D]),
     "aimdhf_set_streams": (_I, [_P, _I]),
     "aimdhf_tc2_step": (_I, [_I, _P, _P, _P, _D, _P]),
+    "aimdhf_tc2_init": (_I, [_I, _P, _P, _P]),
+    "aimdhf_tc2_error": (_I, [_I, _P, _P, _P]),
     "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_grad_one": (_I, [_P, _P, _P, _P, _P]),

@@ -267,6 +267,7 @@ class TC2Graph:
         self.X = torch.zeros((n, n), dtype=torch.float64, device=device)
         self.B = torch.empty_like(self.X)
         self.X2 = torch.empty_like(self.X)
+        self.err = torch.zeros(1, dtype=torch.float64, device=device)
         side = torch.cuda.Stream(device=device)
         side.wait_stream(torch.cuda.current_stream(device))
         with torch.cuda.stream(side):
@@ -297,6 +298,19 @@ class TC2Graph:
         self.tail_graph.replay()
         return self.X
 
+    def start(self, F: torch.Tensor) -> None:
+        """X = (e_max - F) / (e_max - e_min), Gershgorin bounds of F (one kernel)."""
+        F = F.contiguous()
+        _lib.check(_lib.lib().aimdhf_tc2_init(self.n, ctypes.c_void_p(F.data_ptr()), ctypes.c_void_p(self.X.data_ptr()),
+                                              ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def error(self) -> float:
+        """|tr X - sum X_ij^2| of the current iterate (one kernel and one read)."""
+        _lib.check(_lib.lib().aimdhf_tc2_error(self.n, ctypes.c_void_p(self.X.data_ptr()),
+                                               ctypes.c_void_p(self.err.data_ptr()),
+                                               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return float(self.err)
+
 
 _TC2_GRAPHS: dict[tuple, TC2Graph] = {}
 
@@ -316,18 +330,13 @@ def purified_projector_device(F: torch.Tensor, nocc: int, tol: float = 1e-11,
     g = _TC2_GRAPHS.get(key)
     if g is None:
         g = _TC2_GRAPHS[key] = TC2Graph(n, nocc, F.device)
-    eye = torch.eye(n, dtype=F.dtype, device=F.device)
-    a = F.abs()
-    d = torch.diagonal(F)
-    r = a.sum(dim=1) - torch.diagonal(a)
-    emin, emax = (d - r).min(), (d + r).max()
-    g.X.copy_((emax * eye - F) / (emax - emin))
+    g.start(F)
     first = max(1, min(g.needed - 1, max_replays))
     for _ in range(first - 1):
         g.replay()
     for k in range(first, max_replays + 1):
-        X = g.replay()
-        if float((torch.trace(X) - torch.sum(X * X)).abs()) < tol:
+        g.replay()
+        if g.error() < tol:
             g.needed = k
             X = g.tail()
             return 0.5 * (X + X.T)

@@ -347,7 +347,9 @@ struct aimdhf_plan {
 
     // Sorted ket streams (J-engine) and per-shell partner lists (exchange
     // owners) from the screening values of the current geometry. Synchronous.
-    void build_owner_lists(double thr, hipStream_t st) {
+    // direct: also the ket primitive records and Hermite-density layout the
+    // direct-mode J-engine streams (not needed when the integrals are stored)
+    void build_owner_lists(double thr, bool direct, hipStream_t st) {
         std::vector<double> q(npair);
         std::vector<int> np(npair);
         HIPCHECK(hipMemcpyAsync(q.data(), d_qscr.p, sizeof(double) * npair, hipMemcpyDeviceToHost, st));
@@ -372,7 +374,7 @@ struct aimdhf_plan {
         kpre.assign(ket_pair.size() + 1, 0);
         krec.clear();
         kpair.clear();
-        for (size_t m = 0; m < ket_pair.size(); ++m) {
+        for (size_t m = 0; direct && m < ket_pair.size(); ++m) {
             int P = ket_pair[m];
             kpre[m + 1] = kpre[m] + np[P];
             for (int k = 0; k < np[P]; ++k) {
@@ -417,14 +419,16 @@ struct aimdhf_plan {
             }
         upload(d_ket_q, ket_q);
         upload(d_ket_pair, ket_pair);
-        upload(d_kpre, kpre);
-        upload(d_krec, krec);
-        upload(d_kpair, kpair);
+        if (direct) {
+            upload(d_kpre, kpre);
+            upload(d_krec, krec);
+            upload(d_kpair, kpair);
+            d_dh.ensure(sizeof(double) * std::max<long long>(db, 1));
+        }
         upload(d_part_off, part_off);
         upload(d_part_shell, part_shell);
         upload(d_part_pair, part_pair);
         upload(d_part_q, part_q);
-        d_dh.ensure(sizeof(double) * std::max<long long>(db, 1));
     }
 
     // Bytes the integral store of the current list would take: the blocks, up
@@ -991,11 +995,9 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
         for (long long c : p->qcount) p->qtotal += c;
         p->threshold = thr;
         p->prof.host(85, t0);
-        t0 = Profiler::clock::now();
-        p->build_owner_lists(thr, st);
-        p->prof.host(86, t0);
         p->stored = false;
         p->store_bytes = p->k_entries = p->j_entries = 0;
+        bool store = false;
         if (p->jk_mode != 1 && p->nao < (1 << 14)) {
             long long limit = p->store_limit;
             if (limit <= 0) {
@@ -1003,12 +1005,16 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
                 HIPCHECK(hipMemGetInfo(&fr, &tot));
                 limit = (long long)(fr / 2);
             }
-            if (p->jk_mode == 2 || p->store_estimate() <= limit) {
-                auto t1 = Profiler::clock::now();
-                p->build_store(st);
-                p->prof.host(87, t1);
-                p->stored = true;
-            }
+            store = p->jk_mode == 2 || p->store_estimate() <= limit;
+        }
+        t0 = Profiler::clock::now();
+        p->build_owner_lists(thr, !store, st);
+        p->prof.host(86, t0);
+        if (store) {
+            auto t1 = Profiler::clock::now();
+            p->build_store(st);
+            p->prof.host(87, t1);
+            p->stored = true;
         }
         p->have_geometry = true;
     } catch (const HipError& e) {
@@ -1360,6 +1366,22 @@ int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, doub
     aimdhf::launch_tc2_step(n, X, X2, Xout, nocc, (hipStream_t)stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(AIMDHF_E_HIP, std::string("tc2 step: ") + hipGetErrorString(e));
+    return AIMDHF_OK;
+}
+
+int aimdhf_tc2_init(int n, const double* F, double* X0, void* stream) {
+    if (n <= 0 || !F || !X0) return fail(AIMDHF_E_ARG, "bad tc2 arguments");
+    aimdhf::launch_tc2_init(n, F, X0, (hipStream_t)stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(AIMDHF_E_HIP, std::string("tc2 init: ") + hipGetErrorString(e));
+    return AIMDHF_OK;
+}
+
+int aimdhf_tc2_error(int n, const double* X, double* err, void* stream) {
+    if (n <= 0 || !X || !err) return fail(AIMDHF_E_ARG, "bad tc2 arguments");
+    aimdhf::launch_tc2_error(n, X, err, (hipStream_t)stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(AIMDHF_E_HIP, std::string("tc2 error: ") + hipGetErrorString(e));
     return AIMDHF_OK;
 }
 

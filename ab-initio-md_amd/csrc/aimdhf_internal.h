@@ -365,6 +365,8 @@ void launch_tab_totals(const TabArgs& a, long long* out, hipStream_t st);
 void launch_absmax(const double* x, size_t n, double* out, hipStream_t st);
 
 void launch_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, hipStream_t st);
+void launch_tc2_init(int n, const double* F, double* X0, hipStream_t st);
+void launch_tc2_error(int n, const double* X, double* out, hipStream_t st);
 void launch_dh(const DhArgs& a, const long long* n_per_class, hipStream_t st);
 void launch_jeng(int pclass, const JEArgs& a, int nset, const int* nket_host, hipStream_t st);
 void launch_j_assemble(const JAsmArgs& a, hipStream_t st);

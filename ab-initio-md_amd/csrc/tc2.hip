@@ -26,7 +26,70 @@ __global__ __launch_bounds__(256) void k_tc2_step(int n, const double* __restric
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x)
         Xout[i] = square ? X2[i] : 2.0 * X[i] - X2[i];
 }
+// X0 = (e_max - F) / (e_max - e_min) with the Gershgorin bounds of F (rows i:
+// F_ii -/+ sum_{j != i} |F_ij|), one block: bounds reduced in LDS, then the
+// block writes X0 (min and max do not depend on the order)
+__global__ __launch_bounds__(1024) void k_tc2_init(int n, const double* __restrict__ F, double* __restrict__ X0) {
+    __shared__ double lo[16], hi[16];
+    double mn = 1e300, mx = -1e300;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double* row = F + (size_t)i * n;
+        double r = 0.0;
+        for (int j = 0; j < n; ++j) r += j == i ? 0.0 : fabs(row[j]);
+        mn = fmin(mn, row[i] - r);
+        mx = fmax(mx, row[i] + r);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, m, 64));
+        mx = fmax(mx, __shfl_xor(mx, m, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        lo[threadIdx.x >> 6] = mn;
+        hi[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    double emin = lo[0], emax = hi[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+        emin = fmin(emin, lo[w]);
+        emax = fmax(emax, hi[w]);
+    }
+    const double inv = 1.0 / (emax - emin);
+    const size_t nn = (size_t)n * n;
+    for (size_t k = threadIdx.x; k < nn; k += blockDim.x) {
+        const size_t i = k / n, j = k % n;
+        X0[k] = ((i == j ? emax : 0.0) - F[k]) * inv;
+    }
+}
+
+// out[0] = |tr X - sum X_ij^2| (idempotency error of symmetric X), one block
+__global__ __launch_bounds__(1024) void k_tc2_error(int n, const double* __restrict__ X, double* out) {
+    __shared__ double part[16];
+    double s = 0.0;
+    const size_t nn = (size_t)n * n;
+    for (size_t k = threadIdx.x; k < nn; k += blockDim.x) {
+        const double x = X[k];
+        s += (k / n == k % n ? x : 0.0) - x * x;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        out[0] = fabs(t);
+    }
+}
 }  // namespace
+
+void launch_tc2_init(int n, const double* F, double* X0, hipStream_t st) {
+    if (n > 0) k_tc2_init<<<dim3(1), dim3(1024), 0, st>>>(n, F, X0);
+}
+
+void launch_tc2_error(int n, const double* X, double* out, hipStream_t st) {
+    if (n > 0) k_tc2_error<<<dim3(1), dim3(1024), 0, st>>>(n, X, out);
+}
 
 void launch_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, hipStream_t st) {
     const size_t nn = (size_t)n * n;

@@ -143,6 +143,18 @@ int aimdhf_set_precision(aimdhf_plan* plan, int mode, double jk_switch, double g
 int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, void* stream);
 
 /*
+ * Start of the purification: X0 = (e_max - F) / (e_max - e_min) for the
+ * symmetric device matrix F (n x n), e_min / e_max its Gershgorin bounds
+ * (the start of aimd/qc/scf.py's aufbau replaced by purification, see
+ * aimdhf_tc2_step). One workgroup; no plan.
+ */
+int aimdhf_tc2_init(int n, const double* F, double* X0, void* stream);
+
+/* err[0] (device) = |tr X - sum_ij X_ij^2|, the idempotency error of a symmetric
+ * purification iterate X (n x n, device). One workgroup; no plan. */
+int aimdhf_tc2_error(int n, const double* X, double* err, void* stream);
+
+/*
  * Independent launches (the J and K kernels of a build, the store classes,
  * the derivative-ERI classes) run on the caller's stream and three extra
  * library streams of default priority, joined back into the caller's stream
