@@ -88,7 +88,7 @@ class SCFOptions:
     precision: str = "fp64"
     jk_fp32_switch: float = 1e-5
     grad_fp32_switch: float = 1e-5
-    delta_fp32_switch: float = 1e-3
+    delta_fp32_switch: float = 1e-2
     incremental_fock: bool | None = None
     incremental_threshold: float | None = None
 
