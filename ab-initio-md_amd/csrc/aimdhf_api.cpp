@@ -260,6 +260,40 @@ struct aimdhf_plan {
     double delta_fp32_max = 1e-3;
     DevBuf d_tab;                   // store-table temporaries (sizes, unit counts, scan blocks)
     long long* h_pin = nullptr;     // pinned host words for the totals and the error flag
+    // pinned host staging of the per-geometry list transfers: asynchronous copies on
+    // the caller's stream, which do not wait for the integrals on the library streams
+    char* h_stage = nullptr;
+    size_t h_stage_cap = 0, h_stage_used = 0;
+
+    // make room for `bytes` of staging (no transfer may be pending) and start over
+    void stage_reset(size_t bytes) {
+        if (bytes > h_stage_cap) {
+            if (h_stage) HIPCHECK(hipHostFree(h_stage));
+            h_stage = nullptr;
+            h_stage_cap = 0;
+            const size_t want = bytes + bytes / 8 + 4096;
+            HIPCHECK(hipHostMalloc((void**)&h_stage, want));
+            h_stage_cap = want;
+        }
+        h_stage_used = 0;
+    }
+    char* stage_take(size_t bytes) {
+        const size_t at = h_stage_used;
+        h_stage_used += (bytes + 255) & ~(size_t)255;
+        if (h_stage_used > h_stage_cap) {
+            g_err = "staging buffer overflow";
+            throw HipError{AIMDHF_E_STATE};
+        }
+        return h_stage + at;
+    }
+    template <class T>
+    void upload_async(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+        b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
+        if (v.empty()) return;
+        char* h = stage_take(sizeof(T) * v.size());
+        std::memcpy(h, v.data(), sizeof(T) * v.size());
+        HIPCHECK(hipMemcpyAsync(b.p, h, sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+    }
 
     ~aimdhf_plan() {
         for (DevBuf* b : {&d_shell_atom, &d_shell_l, &d_shell_prim, &d_shell_ao, &d_prim_exp, &d_prim_coef,
@@ -276,6 +310,7 @@ struct aimdhf_plan {
                           &d_dmax, &d_kdata32b, &d_jdata32b, &d_tab})
             b->release();
         if (h_pin) (void)hipHostFree(h_pin);
+        if (h_stage) (void)hipHostFree(h_stage);
     }
 
     aimdhf::PairArgs pair_args() const {
@@ -350,11 +385,15 @@ struct aimdhf_plan {
     // direct: also the ket primitive records and Hermite-density layout the
     // direct-mode J-engine streams (not needed when the integrals are stored)
     void build_owner_lists(double thr, bool direct, hipStream_t st) {
-        std::vector<double> q(npair);
-        std::vector<int> np(npair);
-        HIPCHECK(hipMemcpyAsync(q.data(), d_qscr.p, sizeof(double) * npair, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(np.data(), d_npp.p, sizeof(int) * npair, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
+        stage_reset(12 * (size_t)npair + 512);
+        double* hq = (double*)stage_take(sizeof(double) * npair);
+        int* hn = (int*)stage_take(sizeof(int) * npair);
+        HIPCHECK(hipMemcpyAsync(hq, d_qscr.p, sizeof(double) * npair, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(hn, d_npp.p, sizeof(int) * npair, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        std::vector<double> q(hq, hq + npair);
+        std::vector<int> np(hn, hn + npair);
         double qmax = 0.0;
         for (double v : q) qmax = std::max(qmax, v);
         // kets by class, q descending (ties: pair index)
@@ -417,18 +456,21 @@ struct aimdhf_plan {
                 }
                 part_off[X * 3 + l + 1] = (int)part_shell.size();
             }
-        upload(d_ket_q, ket_q);
-        upload(d_ket_pair, ket_pair);
+        size_t bytes = 12 * ket_q.size() + 4 * part_off.size() + 16 * part_shell.size() + 8 * 1024;
+        if (direct) bytes += 8 * kpre.size() + 12 * krec.size();
+        stage_reset(bytes);   // the downloads above have completed
+        upload_async(d_ket_q, ket_q, st);
+        upload_async(d_ket_pair, ket_pair, st);
         if (direct) {
-            upload(d_kpre, kpre);
-            upload(d_krec, krec);
-            upload(d_kpair, kpair);
+            upload_async(d_kpre, kpre, st);
+            upload_async(d_krec, krec, st);
+            upload_async(d_kpair, kpair, st);
             d_dh.ensure(sizeof(double) * std::max<long long>(db, 1));
         }
-        upload(d_part_off, part_off);
-        upload(d_part_shell, part_shell);
-        upload(d_part_pair, part_pair);
-        upload(d_part_q, part_q);
+        upload_async(d_part_off, part_off, st);
+        upload_async(d_part_shell, part_shell, st);
+        upload_async(d_part_pair, part_pair, st);
+        upload_async(d_part_q, part_q, st);
     }
 
     // Bytes the integral store of the current list would take: the blocks, up
@@ -547,6 +589,12 @@ struct aimdhf_plan {
                 hipStream_t s2 = fan.on(1 + i++ % Fan::NAUX);
                 prof.time(47, s2, [&] { aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], q64count[k], s2); });
             }
+        }
+        {
+            // partner and ket lists on the host while the integrals run
+            auto t0 = Profiler::clock::now();
+            build_owner_lists(threshold, false, st);
+            prof.host(86, t0);
         }
         prof.time(48, st, [&] {
             aimdhf::launch_fill_int(d_qmap.as<int>(), (size_t)keys, -1, st);
@@ -1007,10 +1055,11 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
             }
             store = p->jk_mode == 2 || p->store_estimate() <= limit;
         }
-        t0 = Profiler::clock::now();
-        p->build_owner_lists(thr, !store, st);
-        p->prof.host(86, t0);
-        if (store) {
+        if (!store) {
+            t0 = Profiler::clock::now();
+            p->build_owner_lists(thr, true, st);
+            p->prof.host(86, t0);
+        } else {
             auto t1 = Profiler::clock::now();
             p->build_store(st);
             p->prof.host(87, t1);
