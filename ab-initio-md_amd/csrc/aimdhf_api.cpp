@@ -259,6 +259,7 @@ struct aimdhf_plan {
     // unit in FP32 (larger differences use the FP64 parts of the store)
     double delta_fp32_max = 1e-3;
     DevBuf d_tab;                   // store-table temporaries (sizes, unit counts, scan blocks)
+    DevBuf d_gdmax;                 // per shell pair density maxima for the gradient screening
     long long* h_pin = nullptr;     // pinned host words for the totals and the error flag
     // pinned host staging of the per-geometry list transfers: asynchronous copies on
     // the caller's stream, which do not wait for the integrals on the library streams
@@ -307,7 +308,7 @@ struct aimdhf_plan {
                           &d_kdata, &d_jdata, &d_kdoff, &d_jdoff, &d_kdata32, &d_jdata32, &d_kdoff32, &d_jdoff32,
                           &d_kbig, &d_jbig, &d_kunit_os, &d_kunit_e, &d_out_unit,
                           &d_kpart2, &d_junit_pc, &d_junit_e, &d_pair_unit, &d_jpart2, &d_kunit_q, &d_junit_q,
-                          &d_dmax, &d_kdata32b, &d_jdata32b, &d_tab})
+                          &d_dmax, &d_kdata32b, &d_jdata32b, &d_tab, &d_gdmax})
             b->release();
         if (h_pin) (void)hipHostFree(h_pin);
         if (h_stage) (void)hipHostFree(h_stage);
@@ -1367,6 +1368,11 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
         a.qscr = p->d_qscr.as<double>();
         a.dc = dc;
         a.dx = dx;
+        a.npair = p->npair;
+        p->d_gdmax.ensure(sizeof(double) * (1 + nx) * std::max(p->npair, 1));
+        a.dmc = p->d_gdmax.as<double>();
+        a.dmx = a.dmc + p->npair;
+        aimdhf::launch_pair_dmax(a, p->d_gdmax.as<double>(), p->d_gdmax.as<double>() + p->npair, st);
         a.out = p->d_gfx.as<long long>();
         long long gtot = 0;
         for (int k = 0; k < NQCLASS; ++k) gtot += p->gcount[k];

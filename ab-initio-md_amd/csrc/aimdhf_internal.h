@@ -84,6 +84,9 @@ struct GradArgs {
     const double* qscr;
     const double* dc;           // (nao, nao), symmetric
     const double* dx;           // (nx, nao, nao), symmetric
+    int npair;                  // shell pairs (s1 >= s2, index s1 (s1 + 1) / 2 + s2)
+    const double* dmc;          // per shell pair: max |dc| over its block
+    const double* dmx;          // (nx, npair): max |dx| over each block
     double fp32_switch;         // quartets with Q_P Q_Q max|Gamma| below: FP32 derivative integrals
     unsigned char* keep;        // per quartet of the launched class: 0 dropped, 1 FP64, 2 FP32
     int* sel64;                 // selected quartets (indices into the class list), FP64
@@ -390,6 +393,7 @@ void launch_zero64(long long* p, size_t n, hipStream_t st);
 void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st);
 void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
 void launch_grad_keep(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
+void launch_pair_dmax(const GradArgs& a, double* dmc, double* dmx, hipStream_t st);
 void launch_oneel(const OneElArgs& g, double* S, double* T, double* M, double* V, hipStream_t st);
 void launch_grad_one(const OneElArgs& g, const double* D, const double* Wneg, long long* out, hipStream_t st);
 void launch_negate(const double* W, size_t n, double* out, hipStream_t st);

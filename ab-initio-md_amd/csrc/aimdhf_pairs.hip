@@ -315,6 +315,28 @@ void launch_fill_quartets(const PairArgs& g, double thr, double sw, const long l
     else k_fill_quartets<NCOSTBIN><<<dim3(g.npair), dim3(64), 0, st>>>(g, thr, sw, offs, base, list);
 }
 
+// per shell pair (thread per pair): largest |D| over its block, for Dc and each Dx
+__global__ void k_pair_dmax(GradArgs a, double* dmc, double* dmx) {
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= a.npair) return;
+    const int X = a.pair_a[P], Y = a.pair_b[P];
+    const int oX = a.shell_ao[X], nX = a.shell_ao[X + 1] - oX, oY = a.shell_ao[Y], nY = a.shell_ao[Y + 1] - oY;
+    const size_t n = a.nao, nn = n * n;
+    double m = 0.0, mx[2] = {0.0, 0.0};
+    for (int i = 0; i < nX; ++i)
+        for (int j = 0; j < nY; ++j) {
+            const size_t e = (size_t)(oX + i) * n + oY + j;
+            m = fmax(m, fabs(a.dc[e]));
+            for (int x = 0; x < a.nx; ++x) mx[x] = fmax(mx[x], fabs(a.dx[x * nn + e]));
+        }
+    dmc[P] = m;
+    for (int x = 0; x < a.nx; ++x) dmx[(size_t)x * a.npair + P] = mx[x];
+}
+
+void launch_pair_dmax(const GradArgs& a, double* dmc, double* dmx, hipStream_t st) {
+    if (a.npair > 0) k_pair_dmax<<<GRID(a.npair, 256), 0, st>>>(a, dmc, dmx);
+}
+
 void launch_zero64(long long* p, size_t n, hipStream_t st) {
     if (n) k_zero64<<<GRID(n, 256), 0, st>>>(p, n);
 }

@@ -54,7 +54,10 @@ __device__ __forceinline__ double gamma_el(const GradArgs& a, int m, int n, int 
 
 // Screening pre-pass, one thread per quartet (256-quartet blocks): keep[q] = 0
 // (dropped), 1 (FP64) or 2 (FP32: Q_P Q_Q max|Gamma| below fp32_switch); each
-// block's counts of 1s and 2s go to bcnt[2 * block + {0, 1}].
+// block's counts of 1s and 2s go to bcnt[2 * block + {0, 1}]. max|Gamma| is
+// bounded from the per-shell-pair density maxima (|Dc_AB| |Dc_CD| +
+// |kfac|/2 sum_x (|Dx_AC| |Dx_BD| + |Dx_AD| |Dx_BC|)), an upper bound of the
+// element-wise maximum, so no quartet above the threshold is dropped.
 template <int Q>
 __global__ __launch_bounds__(256) void k_grad_keep(GradArgs a, const int2* __restrict__ list, long long nq) {
     __shared__ int wc[2][4];
@@ -69,12 +72,14 @@ __global__ __launch_bounds__(256) void k_grad_keep(GradArgs a, const int2* __res
             if (sA != sB) deg *= 2.0;
             if (sC != sD) deg *= 2.0;
             if (pq.x != pq.y) deg *= 2.0;
-            const int oA = a.shell_ao[sA], oB = a.shell_ao[sB], oC = a.shell_ao[sC], oD = a.shell_ao[sD];
-            double gmax = 0.0;
-            for (int ab = 0; ab < NAB; ++ab)
-                for (int cd = 0; cd < NCD; ++cd)
-                    gmax = fmax(gmax, fabs(gamma_el(a, oA + ab / NB, oB + ab % NB, oC + cd / ND, oD + cd % ND, deg)));
-            const double b = a.qscr[pq.x] * a.qscr[pq.y] * gmax;
+            auto pk = [](int s, int t) { return s > t ? s * (s + 1) / 2 + t : t * (t + 1) / 2 + s; };
+            double gb = a.dmc[pk(sA, sB)] * a.dmc[pk(sC, sD)];
+            const int pAC = pk(sA, sC), pBD = pk(sB, sD), pAD = pk(sA, sD), pBC = pk(sB, sC);
+            for (int x = 0; x < a.nx; ++x) {
+                const double* m = a.dmx + (size_t)x * a.npair;
+                gb += 0.5 * fabs(a.kfac) * (m[pAC] * m[pBD] + m[pAD] * m[pBC]);
+            }
+            const double b = a.qscr[pq.x] * a.qscr[pq.y] * (gb * deg);
             keep = (b < a.thr) ? 0 : (b < a.fp32_switch ? 2 : 1);
         }
         a.keep[q] = (unsigned char)keep;
