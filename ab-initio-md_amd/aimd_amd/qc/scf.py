@@ -319,8 +319,9 @@ def purified_projector_device(F: torch.Tensor, nocc: int, tol: float = 1e-11,
                               max_replays: int = 15) -> torch.Tensor | None:
     """purified_projector on the GPU with the captured TC2Graph (same steps, same branch
     rule): the idempotency error is read back after the number of replays the previous
-    call needed less one, then after every replay, and the TAIL_STEPS tail graph
-    squares the remaining error away."""
+    call needed, then after every replay, and the TAIL_STEPS tail graph squares the
+    remaining error away. Replays past convergence leave an idempotent iterate
+    idempotent, so starting the checks late costs time, not accuracy."""
     n = F.shape[0]
     if nocc <= 0:
         return torch.zeros_like(F)
@@ -331,7 +332,7 @@ def purified_projector_device(F: torch.Tensor, nocc: int, tol: float = 1e-11,
     if g is None:
         g = _TC2_GRAPHS[key] = TC2Graph(n, nocc, F.device)
     g.start(F)
-    first = max(1, min(g.needed - 1, max_replays))
+    first = max(1, min(g.needed, max_replays))
     for _ in range(first - 1):
         g.replay()
     for k in range(first, max_replays + 1):
@@ -606,35 +607,57 @@ class DIIS:
             self.focks.pop(0)
             self.errors.pop(0)
 
-    def coefficients(self) -> np.ndarray:
+    def keep_last(self) -> None:
+        """Restart from the newest vector alone."""
+        del self.focks[:-1]
+        del self.errors[:-1]
+
+    def gram(self) -> torch.Tensor | None:
+        """Device (m, m-1) array: rows 0..m-2 the Gram matrix of the differences
+        e_i - e_m, row m-1 the right-hand side <e_i - e_m, -e_m>; None for m = 1.
+        Computed before the host needs it, so it travels with other scalars."""
         m = len(self.errors)
         if m == 1:
-            return np.ones(1)
+            return None
         E = torch.stack([e.reshape(-1) for e in self.errors])
         last = E[-1]
         delta = E[:-1] - last
         # Gram matrix and right-hand side as one broadcast product and one reduction:
         # the (m x nao^2)(nao^2 x m) shape maps to a single-workgroup GEMM in the BLAS
         D = torch.cat([delta, -last[None]])
-        Gr = (delta[None, :, :] * D[:, None, :]).sum(dim=2)
-        Gr = Gr.cpu().numpy()
+        return (delta[None, :, :] * D[:, None, :]).sum(dim=2)
+
+    def coefficients(self, Gr: np.ndarray | None = None) -> np.ndarray:
+        """Coefficients of the stored vectors (oldest first); Gr is the host copy of
+        gram() for the current vectors, computed here when not given."""
+        m = len(self.errors)
+        if m == 1:
+            return np.ones(1)
+        if Gr is None:
+            Gr = self.gram().cpu().numpy()
         G, r = Gr[:-1], Gr[-1]
         lam, V = np.linalg.eigh(G)
         sig = np.sqrt(np.maximum(lam, 0.0))
-        cut = np.finfo(float).eps * max(delta.shape) * sig.max()
+        cut = np.finfo(float).eps * max(m - 1, self.errors[-1].numel()) * sig.max()
         keep = sig > cut
         x = V[:, keep] @ ((V[:, keep].T @ r) / lam[keep]) if keep.any() else np.zeros(m - 1)
         return np.append(x, 1.0 - x.sum())
 
-    def extrapolate(self) -> torch.Tensor:
+    def extrapolate(self, Gr: np.ndarray | None = None) -> torch.Tensor:
+        """sum_i c_i F_i; dropping the oldest vector keeps the newest as the anchor,
+        so the Gram matrix of the rest is Gr[1:, 1:]."""
         while len(self.focks) > 1:
-            c = self.coefficients()
+            c = self.coefficients(Gr)
             if np.all(np.isfinite(c)) and np.abs(c).max() <= self.max_coefficient:
                 F = torch.stack(self.focks)
-                ct = torch.tensor(c, dtype=F.dtype).to(F.device, non_blocking=False)
+                ct = torch.from_numpy(c)
+                if F.is_cuda:
+                    ct = ct.pin_memory().to(F.device, non_blocking=True)
                 return (ct.view(-1, *([1] * (F.dim() - 1))) * F).sum(dim=0)
             self.focks.pop(0)
             self.errors.pop(0)
+            if Gr is not None:
+                Gr = Gr[1:, 1:]
         return self.focks[-1].clone()
 
 
@@ -915,7 +938,7 @@ class SCFSolver:
     def _fock(self, I: SCFIntegrals, Ds: torch.Tensor, jk: tuple | None = None) -> torch.Tensor:
         g = 2.0 / Ds.shape[0]
         J, K = I.engine.jk(Ds) if jk is None else jk
-        F = I.H + g * J.sum(dim=0) - K
+        F = torch.add(I.H, J[0] if J.shape[0] == 1 else J.sum(dim=0), alpha=g) - K
         return 0.5 * (F + F.transpose(1, 2))
 
     @property
@@ -934,7 +957,7 @@ class SCFSolver:
                 P = purify(Fo[s], nocc[s])
                 if P is None:
                     break
-                Dn[s] = X @ P @ X.T
+                torch.matmul(X @ P, X.T, out=Dn[s])
             else:
                 return Dn
         _, Cp = torch.linalg.eigh(Fo)
@@ -972,31 +995,41 @@ class SCFSolver:
                     jk = (jk[0] + dJ, jk[1] + dK)
                 D_built = Ds
             F = self._fock(I, Ds, jk)
-            e_dev = 0.5 * g * torch.sum(Ds * (H + F))
+            e_dev = 0.5 * g * torch.vdot(Ds.reshape(-1), (H + F).reshape(-1))
             FDS = F @ Ds @ S
             err = X.T @ (FDS - FDS.transpose(1, 2)) @ X
-            scal = torch.stack([e_dev, err.abs().max()]).cpu().numpy()
+            Fo = X.T @ F @ X
+            # the DIIS vector is pushed before the convergence test, so its Gram
+            # matrix comes back with the energy and the commutator in one read
+            parts = [e_dev.reshape(1), torch.linalg.vector_norm(err, ord=float("inf")).reshape(1)]
+            n_diis = len(diis) if diis is not None else 0
+            if diis is not None:
+                diis.push(Fo, err)
+                Gd = diis.gram()
+                if Gd is not None:
+                    parts.append(Gd.reshape(-1))
+            scal = torch.cat(parts).cpu().numpy()
             e_el, err_norm = float(scal[0]), float(scal[1])
             de = e_el - e_prev
             history.append({"energy": e_el + I.nuclear_repulsion, "energy_change": de,
-                            "commutator": err_norm, "diis": len(diis) if diis else 0})
+                            "commutator": err_norm, "diis": n_diis})
             if abs(de) < opt.conv_energy and err_norm < opt.conv_commutator:
                 converged = True
                 break
             if it == opt.max_iter:
                 break
-            Fo = X.T @ F @ X
             if diis is not None:
+                Gr = scal[2:].reshape(len(diis), len(diis) - 1) if len(diis) > 1 else None
                 if err_norm < best:
                     best, since_best = err_norm, 0
                 else:
                     since_best += 1
                 if opt.diis_restart and since_best >= opt.diis_restart:
-                    diis.reset()
+                    diis.keep_last()
+                    Gr = None
                     restarts += 1
                     best, since_best = err_norm, 0
-                diis.push(Fo, err)
-                Fo = diis.extrapolate()
+                Fo = diis.extrapolate(Gr)
             stabilize = err_norm > opt.stabilize_until
             if stabilize and opt.level_shift > 0.0:
                 Do = XtS @ Ds @ XtS.T
