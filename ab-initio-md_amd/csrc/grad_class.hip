@@ -28,7 +28,11 @@ constexpr int QC = AIMDHF_QCLASS;
 // for (pp|ss), (pp|pp) and (ds|ss), which then spill at most 32 registers and
 // run 15-32% faster on gfx950; the heavier classes slow down at two waves
 // (more than 80 spilled registers), (ss|ss) and (ps|ss) at one more wave
-constexpr int grad_waves() { return QC == 3 || QC == 5 || QC == 6 ? 2 : 1; }
+template <typename Real>
+constexpr int grad_waves() {
+    if constexpr (sizeof(Real) == 4) return QC == 10 || (QC >= 14 && QC <= 18) ? 2 : 1;
+    return QC == 3 || QC == 5 || QC == 6 ? 2 : 1;
+}
 constexpr int pc_hi() { int h = 0; while ((h + 1) * (h + 2) / 2 <= QC) ++h; return h; }
 constexpr int pc_lo() { return QC - pc_hi() * (pc_hi() + 1) / 2; }
 constexpr int la_of(int pc) { return pc == 0 ? 0 : pc <= 2 ? 1 : 2; }
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(256) void k_grad_scatter(const unsigned char* __res
 
 // Work item = (selected quartet, pass column group), derivative integrals in Real
 template <int Q, typename Real>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves()))) void k_grad_class(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<Real>()))) void k_grad_class(
     GradArgs a, const int2* __restrict__ list, const int* __restrict__ sel, const int* __restrict__ nsel) {
     extern __shared__ long long lds[];
     __shared__ double btab[BOYS_NGRID * BOYS_SROW];
