@@ -177,6 +177,15 @@ class Engine:
         _lib.check(_lib.lib().aimdhf_grad_one(self._h, _ptr(Dd), _ptr(Wd), _ptr(g), _stream()))
         return g
 
+    def grad_one_terms(self, D: torch.Tensor, W: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        """(d/dR tr(D T), d/dR tr(D V), -d/dR tr(W S)), each a (natm, 3) device tensor."""
+        Dd = D.to(device=self.device, dtype=torch.float64).contiguous()
+        Wd = W.to(device=self.device, dtype=torch.float64).contiguous()
+        g = torch.empty((3, self.natm, 3), dtype=torch.float64, device=self.device)
+        _lib.check(_lib.lib().aimdhf_grad_one_terms(self._h, _ptr(Dd), _ptr(Wd), _ptr(g[0]), _ptr(g[1]), _ptr(g[2]),
+                                                    _stream()))
+        return g[0], g[1], g[2]
+
     def grad_two(self, Dc: torch.Tensor, Dx: torch.Tensor, kfac: float,
                  threshold: float = DEFAULT_SCHWARZ_GRAD) -> torch.Tensor:
         """d/dR of the two-electron energy at fixed densities, (natm, 3) device tensor."""

@@ -46,11 +46,22 @@ def energy_weighted_density(spin_densities, fock):
 
 @dataclass(frozen=True)
 class GradientTerms:
-    """Parts of an SCF gradient, (natm, 3) each (aimd/qc/gradients.py:128-156)."""
+    """
+    Parts of an SCF gradient, (natm, 3) each, Eh/bohr (aimd/qc/gradients.py:128-156):
+    nuclear_repulsion, kinetic (sum P dT/dR), nuclear_attraction (sum P dV/dR,
+    basis-function and operator centres), overlap (-sum W dS/dR), two_electron
+    (1/2 sum Gamma d(mn|ls)/dR).
+    """
     nuclear_repulsion: np.ndarray
-    one_electron: np.ndarray
+    kinetic: np.ndarray
+    nuclear_attraction: np.ndarray
+    overlap: np.ndarray
     two_electron: np.ndarray
     timings: dict = field(default_factory=dict)
+
+    @property
+    def one_electron(self) -> np.ndarray:
+        return self.kinetic + self.nuclear_attraction + self.overlap
 
     @property
     def total(self) -> np.ndarray:
@@ -67,33 +78,56 @@ def _device_state(result: SCFResult):
     return result.integrals, spin, Fs
 
 
-def gradient_terms(result: SCFResult, schwarz_threshold: float = qengine.DEFAULT_SCHWARZ_GRAD,
-                   require_converged: bool = False) -> GradientTerms:
+def _check(result: SCFResult, require_converged: bool) -> None:
     if not isinstance(result, SCFResult):
         raise TypeError(f"expected an SCFResult, got {type(result).__name__}")
     if require_converged and not result.converged:
         raise ValueError(
             f"SCF not converged ({result.iterations} iterations, commutator "
             f"{result.commutator_norm:.2e}); its analytic gradient is not dE/dR")
+
+
+def _two_electron(I, result: SCFResult, spin: torch.Tensor, P: torch.Tensor, threshold: float) -> torch.Tensor:
+    if result.unrestricted:
+        return I.engine.grad_two(P, spin, 1.0, threshold)
+    return I.engine.grad_two(P, P[None], 0.5, threshold)
+
+
+def gradient_terms(result: SCFResult, schwarz_threshold: float = qengine.DEFAULT_SCHWARZ_GRAD,
+                   require_converged: bool = False, blas_threads=None) -> GradientTerms:
+    """The gradient of ``result`` split into its physical parts (aimd/qc/gradients.py:196-223);
+    the one-electron terms come from one engine call with three kernel passes."""
+    _check(result, require_converged)
     t0 = time.perf_counter()
     I, spin, Fs = _device_state(result)
     b = I.basis
     W = energy_weighted_density(spin, Fs)
     P = spin[0] + spin[1]
     g_nuc = nuclear_repulsion_gradient(b.nuclear_charges, b.positions)
-    g1 = I.engine.grad_one(P, W)
-    if result.unrestricted:
-        g2 = I.engine.grad_two(P, spin, 1.0, schwarz_threshold)
-    else:
-        g2 = I.engine.grad_two(P, P[None], 0.5, schwarz_threshold)
-    g1h, g2h = g1.cpu().numpy(), g2.cpu().numpy()
-    return GradientTerms(g_nuc, g1h, g2h, {"total": time.perf_counter() - t0})
+    t1 = time.perf_counter()
+    gt, gv, gs = I.engine.grad_one_terms(P, W)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g2 = _two_electron(I, result, spin, P, schwarz_threshold)
+    parts = [t.cpu().numpy() for t in (gt, gv, gs, g2)]
+    t3 = time.perf_counter()
+    timings = {"nuclear_and_w": t1 - t0, "one_electron": t2 - t1, "two_electron": t3 - t2, "total": t3 - t0}
+    return GradientTerms(g_nuc, *parts, timings=timings)
 
 
 def scf_gradient(result: SCFResult, schwarz_threshold: float = qengine.DEFAULT_SCHWARZ_GRAD,
                  require_converged: bool = False, blas_threads=None) -> np.ndarray:
-    """Analytic dE/dR (hartree/bohr, (natm, 3)) of an SCF result (aimd/qc/gradients.py:168-193)."""
-    return gradient_terms(result, schwarz_threshold, require_converged).total
+    """Analytic dE/dR (hartree/bohr, (natm, 3)) of an SCF result (aimd/qc/gradients.py:168-193):
+    one fused one-electron pass and one derivative-ERI pass on the device."""
+    _check(result, require_converged)
+    I, spin, Fs = _device_state(result)
+    b = I.basis
+    W = energy_weighted_density(spin, Fs)
+    P = spin[0] + spin[1]
+    g1 = I.engine.grad_one(P, W)
+    g2 = _two_electron(I, result, spin, P, schwarz_threshold)
+    g = (g1 + g2).cpu().numpy()
+    return nuclear_repulsion_gradient(b.nuclear_charges, b.positions) + g
 
 
 def invariance_residuals(positions: np.ndarray, gradient: np.ndarray) -> tuple[np.ndarray, np.ndarray]:

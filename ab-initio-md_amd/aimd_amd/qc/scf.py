@@ -254,9 +254,9 @@ class TC2Graph:
     fused update (aimdhf_tc2_step), ping-ponging between two static buffers so the
     result of an even number of steps lands back in ``X``. The update kernel needs
     no reduction pass and clears no memory, so replays are exact repeats.
-    ``needed`` remembers how many replays the last purification took, so the next
-    one (the Fock matrix of the next SCF iteration) checks convergence only from
-    there on.
+    ``needed`` remembers how many replays the last purification took; the next one
+    (the Fock matrix of the next SCF iteration) checks convergence from one replay
+    earlier on, so the count follows the matrices down as well as up.
     """
     STEPS = 8
     TAIL_STEPS = 2
@@ -319,9 +319,10 @@ def purified_projector_device(F: torch.Tensor, nocc: int, tol: float = 1e-11,
                               max_replays: int = 15) -> torch.Tensor | None:
     """purified_projector on the GPU with the captured TC2Graph (same steps, same branch
     rule): the idempotency error is read back after the number of replays the previous
-    call needed, then after every replay, and the TAIL_STEPS tail graph squares the
-    remaining error away. Replays past convergence leave an idempotent iterate
-    idempotent, so starting the checks late costs time, not accuracy."""
+    call needed minus one, then after every replay, and the TAIL_STEPS tail graph squares
+    the remaining error away. Replays past convergence leave an idempotent iterate
+    idempotent, so starting the checks late costs time, not accuracy; starting one
+    replay early lets the count drop again after a hard matrix."""
     n = F.shape[0]
     if nocc <= 0:
         return torch.zeros_like(F)
@@ -332,7 +333,7 @@ def purified_projector_device(F: torch.Tensor, nocc: int, tol: float = 1e-11,
     if g is None:
         g = _TC2_GRAPHS[key] = TC2Graph(n, nocc, F.device)
     g.start(F)
-    first = max(1, min(g.needed, max_replays))
+    first = max(1, min(g.needed - 1, max_replays))
     for _ in range(first - 1):
         g.replay()
     for k in range(first, max_replays + 1):

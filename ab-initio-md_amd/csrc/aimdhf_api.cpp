@@ -854,6 +854,7 @@ struct aimdhf_plan {
         g.pos = d_pos.as<double>();
         g.charges = d_charges.as<double>();
         g.boys = d_boys.as<double>();
+        g.parts = 3;
         return g;
     }
 };
@@ -1318,6 +1319,42 @@ int aimdhf_grad_one(aimdhf_plan* p, const double* D, const double* W, double* gr
         aimdhf::launch_zero64(p->d_gfx.as<long long>(), 6 * p->natm, st);
         p->prof.time(44, st, [&] { aimdhf::launch_grad_one(p->oneel_args(), Ds, Wn, p->d_gfx.as<long long>(), st); });
         aimdhf::launch_fx_to_double(p->d_gfx.as<long long>(), 3 * p->natm, grad, st);
+        HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+int aimdhf_grad_one_terms(aimdhf_plan* p, const double* D, const double* W, double* g_kin, double* g_nuc,
+                          double* g_ovl, void* stream) {
+    if (!p || !D || !W || !g_kin || !g_nuc || !g_ovl) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    hipStream_t st = (hipStream_t)stream;
+    try {
+        const size_t nn = (size_t)p->nao * p->nao;
+        p->d_gden.ensure(sizeof(double) * 3 * nn);
+        double* Ds = p->d_gden.as<double>();
+        double* Wn = Ds + nn;
+        double* Z = Wn + nn;
+        aimdhf::launch_symmetrize(D, 1, p->nao, Ds, st);
+        aimdhf::launch_symmetrize(W, 1, p->nao, Wn, st);
+        aimdhf::launch_negate(Wn, nn, Wn, st);
+        aimdhf::launch_zero_d(Z, nn, st);
+        p->d_gfx.ensure(sizeof(long long) * 18 * p->natm);
+        long long* fx = p->d_gfx.as<long long>();
+        aimdhf::launch_zero64(fx, 18 * p->natm, st);
+        aimdhf::OneElArgs g = p->oneel_args();
+        // kinetic: overlap/kinetic part with W = 0; nuclear attraction alone; overlap:
+        // overlap/kinetic part with D = 0 (the -W S term)
+        g.parts = 1;
+        aimdhf::launch_grad_one(g, Ds, Z, fx, st);
+        aimdhf::launch_grad_one(g, Z, Wn, fx + 12 * p->natm, st);
+        g.parts = 2;
+        aimdhf::launch_grad_one(g, Ds, Z, fx + 6 * p->natm, st);
+        aimdhf::launch_fx_to_double(fx, 3 * p->natm, g_kin, st);
+        aimdhf::launch_fx_to_double(fx + 6 * p->natm, 3 * p->natm, g_nuc, st);
+        aimdhf::launch_fx_to_double(fx + 12 * p->natm, 3 * p->natm, g_ovl, st);
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
         return e.code;

@@ -64,6 +64,7 @@ struct OneElArgs {
     const double* pos;
     const double* charges;
     const double* boys;
+    int parts;                  // one-electron gradient: 1 overlap/kinetic, 2 nuclear attraction, 3 both
 };
 
 struct GradArgs {
@@ -395,6 +396,9 @@ void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long
 void launch_grad_keep(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
 void launch_pair_dmax(const GradArgs& a, double* dmc, double* dmx, hipStream_t st);
 void launch_oneel(const OneElArgs& g, double* S, double* T, double* M, double* V, hipStream_t st);
+// device address of the fixed-size scratch array of the allocation-free entry
+// points on the current device (resolved once per device), and its size
+void* noalloc_scratch(size_t* bytes);
 void launch_grad_one(const OneElArgs& g, const double* D, const double* Wneg, long long* out, hipStream_t st);
 void launch_negate(const double* W, size_t n, double* out, hipStream_t st);
 void launch_fx_to_double(const long long* fx, int n, double* out, hipStream_t st);

@@ -1,9 +1,13 @@
 // Derivative-ERI gradient kernel of one quartet class (compiled once per
 // class with -DAIMDHF_QCLASS=k, like jk_class.hip).
 //
-// A pre-pass (one thread per quartet) applies the screening of
-// aimd/qc/integrals.py:1279-1329 exactly: quartets with all four centres on
-// one atom, or with Q_P Q_Q max|Gamma| below the threshold, are dropped.
+// A pre-pass (one thread per quartet) screens as aimd/qc/integrals.py:1279-1329
+// does, with one difference: quartets with all four centres on one atom are
+// dropped, and the others are kept while Q_P Q_Q times a bound of max|Gamma|
+// reaches the threshold. The bound comes from per-shell-pair density maxima
+// and is never below the element-wise max|Gamma| the reference uses, so the
+// kept list is a superset of the reference's; every extra quartet has
+// Q_P Q_Q max|Gamma| below the threshold, i.e. contributes below it.
 // Work item = (quartet, j): j < NIB is the bra pass for ket components
 // j*NCB .. j*NCB+NCB-1, NIB <= j the ket pass for NCK bra components, NCB /
 // NCK as many as fit the register budget (one R evaluation per primitive

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_grad_one_cls(OneElArgs g, const int* pa
         const int si = g.pair_a[P], sj = g.pair_b[P];
         const int ai = g.shell_atom[si], aj = g.shell_atom[sj];
         double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-        if (si != sj && ai != aj) {
+        if ((g.parts & 1) && si != sj && ai != aj) {
             // d/dR [tr(D T) - tr(W S)]: the overlap term enters with -W
             double gk[3] = {0.0, 0.0, 0.0};
             ovlkin_grad_acc<LI, LJ>(B, g.nao, si, sj, Wneg, D, lane, 64, gk);
@@ -106,8 +106,9 @@ __global__ __launch_bounds__(256) void k_grad_one_cls(OneElArgs g, const int* pa
                 gB[d] -= gk[d];
             }
         }
-        nuclear_grad_acc<LI, LJ>(B, g.nao, si, sj, lane, 64, g.natm, g.charges, D, g.boys,
-                                 [&](int atom, int d, double x) { fx_add(acc, atom, d, x); }, gA, gB);
+        if (g.parts & 2)
+            nuclear_grad_acc<LI, LJ>(B, g.nao, si, sj, lane, 64, g.natm, g.charges, D, g.boys,
+                                     [&](int atom, int d, double x) { fx_add(acc, atom, d, x); }, gA, gB);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double xa = wave_sum(gA[d]), xb = wave_sum(gB[d]);

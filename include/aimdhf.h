@@ -188,6 +188,17 @@ int aimdhf_one_electron(aimdhf_plan* plan, double* S, double* T, double* V, doub
 int aimdhf_grad_one(aimdhf_plan* plan, const double* D, const double* W, double* grad, void* stream);
 
 /*
+ * The terms of aimdhf_grad_one separately, each device (natm,3):
+ *   g_kin = d/dR sum D T,  g_nuc = d/dR sum D V (basis-function and operator
+ *   centres),  g_ovl = -d/dR sum W S  (g_kin + g_nuc + g_ovl = aimdhf_grad_one).
+ * Replaces: kinetic_gradient, nuclear_attraction_gradient and -overlap_gradient
+ * (aimd/qc/integrals.py:1452-1480) as gradient_terms uses them
+ * (aimd/qc/gradients.py:196-223).
+ */
+int aimdhf_grad_one_terms(aimdhf_plan* plan, const double* D, const double* W, double* g_kin, double* g_nuc,
+                          double* g_ovl, void* stream);
+
+/*
  * grad (device (natm,3)) = d/dR of
  *   E2 = 1/2 sum Dc_mn Dc_ls (mn|ls) - k/2 sum_x sum Dx_ml Dx_ns (mn|ls)
  * at fixed densities (RHF: Dc = P, Dx = [P], k = 0.5; UHF: Dc = Pa+Pb,

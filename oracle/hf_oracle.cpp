@@ -1369,6 +1369,56 @@ void grad_pass(const Pairs& P, const Blocks& B, int pb, int pk, const double* G,
 extern "C" {
 
 // d/dR of E2 (integrals.py:1247-1369, 1498-1520); Dc (nao,nao), Dx (nx,nao,nao), all symmetric
+// The screening quantity of _eri_grad_kernel (aimd/qc/integrals.py:1284-1329) for given
+// block quartets (p1, p2), p2 <= p1: out[i] = Q_p1 Q_p2 max|Gamma| over the block
+// quartet's elements (Gamma with its degeneracy factor), or -1 for a quartet with all
+// four blocks on one atom (always skipped). A quartet is differentiated iff
+// out[i] >= thr.
+void oracle_grad_bound(void* hp, int nao, const double* Dc, int nx, const double* Dx, double kfac,
+                       const i64* quartets, long nq, double* out) {
+    OraclePairs* h = (OraclePairs*)hp;
+    const Pairs& P = h->P;
+    const Blocks& B = h->B;
+    const double* Q = h->Q.data();
+    size_t n = nao;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long q = 0; q < nq; ++q) {
+        const int pb = (int)quartets[2 * q], pk = (int)quartets[2 * q + 1];
+        const int si = P.pair_blocks[2 * pb], sj = P.pair_blocks[2 * pb + 1];
+        const int sk = P.pair_blocks[2 * pk], sl = P.pair_blocks[2 * pk + 1];
+        if (B.atom[si] == B.atom[sj] && B.atom[sj] == B.atom[sk] && B.atom[sk] == B.atom[sl]) {
+            out[q] = -1.0;
+            continue;
+        }
+        const int nci = CART_OFF[B.l[si] + 1] - CART_OFF[B.l[si]], ncj = CART_OFF[B.l[sj] + 1] - CART_OFF[B.l[sj]];
+        const int nck = CART_OFF[B.l[sk] + 1] - CART_OFF[B.l[sk]], ncl = CART_OFF[B.l[sl] + 1] - CART_OFF[B.l[sl]];
+        double deg = 0.5;
+        if (si != sj) deg *= 2.0;
+        if (sk != sl) deg *= 2.0;
+        if (pb != pk) deg *= 2.0;
+        double gmax = 0.0;
+        for (int ia = 0; ia < B.ncon[si] * nci; ++ia) {
+            const size_t m = B.ao[si] + ia;
+            for (int jb = 0; jb < B.ncon[sj] * ncj; ++jb) {
+                const size_t nn = B.ao[sj] + jb;
+                for (int kc = 0; kc < B.ncon[sk] * nck; ++kc) {
+                    const size_t lam = B.ao[sk] + kc;
+                    for (int ld = 0; ld < B.ncon[sl] * ncl; ++ld) {
+                        const size_t sig = B.ao[sl] + ld;
+                        double v = Dc[m * n + nn] * Dc[lam * n + sig];
+                        for (int x = 0; x < nx; ++x) {
+                            const double* X = Dx + x * n * n;
+                            v -= 0.5 * kfac * (X[m * n + lam] * X[nn * n + sig] + X[m * n + sig] * X[nn * n + lam]);
+                        }
+                        gmax = std::max(gmax, std::fabs(v * deg));
+                    }
+                }
+            }
+        }
+        out[q] = Q[pb] * Q[pk] * gmax;
+    }
+}
+
 void oracle_eri_grad(void* hp, double thr, int nao, int natm, const double* Dc, int nx, const double* Dx,
                      double kfac, double* grad) {
     OraclePairs* h = (OraclePairs*)hp;

@@ -79,6 +79,7 @@ def lib():
         L.oracle_jk_from_eri.argtypes = [i, i, P, P, P, P]
         L.oracle_jk_direct.argtypes = [P, d, i, i, P, P, P]
         L.oracle_eri_grad.argtypes = [P, d, i, i, P, i, P, d, P]
+        L.oracle_grad_bound.argtypes = [P, i, P, i, P, d, P, l, P]
         _lib = L
     return _lib
 
@@ -269,6 +270,18 @@ def one_electron_gradient(b: Basis, D, W):
     return g
 
 
+def one_electron_gradient_terms(b: Basis, D, W):
+    """(d/dR sum D T, d/dR sum D V, -d/dR sum W S), the split of
+    aimd/qc/gradients.py:196-223."""
+    gk, gs, gv = np.zeros((b.natm, 3)), np.zeros((b.natm, 3)), np.zeros((b.natm, 3))
+    Ds = np.ascontiguousarray(0.5 * (D + D.T))
+    Ws = np.ascontiguousarray(0.5 * (W + W.T))
+    lib().oracle_one_electron_grad_terms(len(b.shell_l), _p(b.shell_atom), _p(b.shell_l), _p(b.shell_prim),
+                                         _p(b.shell_ao), _p(b.prim_exp), _p(b.prim_coef), _p(b.positions),
+                                         b.nao, b.natm, _p(b.charges), _p(Ds), _p(Ws), _p(gk), _p(gs), _p(gv))
+    return gk, gv, -gs
+
+
 class Pairs:
     """Block-pair data and Schwarz bounds of one geometry (C++ handle)."""
 
@@ -328,6 +341,18 @@ class Pairs:
         lib().oracle_eri_grad(self.h, float(thr), n, self.b.natm, _p(Dc), len(Dx_list), _p(Dx),
                               float(kfac), _p(g))
         return g
+
+
+    def grad_bound(self, Dc, Dx_list, kfac, quartets):
+        """Q_p1 Q_p2 max|Gamma| of each block quartet (p1, p2), the quantity
+        _eri_grad_kernel compares with its threshold; -1 for one-centre quartets."""
+        n = self.b.nao
+        Dc = np.ascontiguousarray(0.5 * (Dc + Dc.T))
+        Dx = np.ascontiguousarray(np.array([0.5 * (d + d.T) for d in Dx_list]))
+        qs = np.ascontiguousarray(np.asarray(quartets, np.int64).reshape(-1, 2))
+        out = np.zeros(len(qs))
+        lib().oracle_grad_bound(self.h, n, _p(Dc), len(Dx_list), _p(Dx), float(kfac), _p(qs), len(qs), _p(out))
+        return out
 
 
 def jk_from_eri(eri, D):

@@ -137,3 +137,20 @@ def test_nve_energy_conservation_scales_as_dt_squared():
         drifts.append(dev)
     assert drifts[0] < 1.5e-4
     assert 3.5 < drifts[0] / drifts[1] < 4.5
+
+
+def test_gradient_terms_split_and_sum():
+    """GradientTerms carries the reference's fields (kinetic, nuclear_attraction, overlap,
+    two_electron, nuclear_repulsion); one_electron and total are their sums and total
+    equals scf_gradient."""
+    from aimd_amd.qc.gradients import gradient_terms, scf_gradient
+    from aimd_amd.qc.scf import SCFSolver
+    from tests.conftest import WATER, bohr
+    s, x = bohr(WATER, 0.05, 3)
+    res = SCFSolver(s, "6-31g*", conv_energy=1e-11, conv_commutator=1e-9).run(x)
+    t = gradient_terms(res)
+    for name in ("nuclear_repulsion", "kinetic", "nuclear_attraction", "overlap", "two_electron"):
+        assert getattr(t, name).shape == (3, 3)
+    np.testing.assert_allclose(t.one_electron, t.kinetic + t.nuclear_attraction + t.overlap, atol=0)
+    np.testing.assert_allclose(t.total, scf_gradient(res), rtol=0, atol=1e-12)
+    assert set(t.timings) == {"nuclear_and_w", "one_electron", "two_electron", "total"}

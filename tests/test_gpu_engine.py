@@ -94,6 +94,9 @@ def test_one_electron_and_gradient_terms(oracle_lib, mol, basis, mult):
     W = 0.5 * (W + W.T)
     g1 = e.grad_one(torch.tensor(P, device="cuda"), torch.tensor(W, device="cuda")).cpu().numpy()
     np.testing.assert_allclose(g1, oracle_lib.one_electron_gradient(ob, P, W), rtol=0, atol=1e-11)
+    terms = e.grad_one_terms(torch.tensor(P, device="cuda"), torch.tensor(W, device="cuda"))
+    for got, want in zip(terms, oracle_lib.one_electron_gradient_terms(ob, P, W)):
+        np.testing.assert_allclose(got.cpu().numpy(), want, rtol=0, atol=1e-11)
     Dx, k = (P[None], 0.5) if mult == 1 else (spin, 1.0)
     g2 = e.grad_two(torch.tensor(P, device="cuda"), torch.tensor(Dx, device="cuda"), k).cpu().numpy()
     g2b = e.grad_two(torch.tensor(P, device="cuda"), torch.tensor(Dx, device="cuda"), k).cpu().numpy()

@@ -152,3 +152,34 @@ def test_graph_purification_matches_eigenvectors_on_every_replay():
     for o in outs:
         np.testing.assert_allclose(o, ref, atol=1e-10)
         np.testing.assert_array_equal(o, outs[0])
+
+
+def test_purification_replay_count_follows_the_gap_down():
+    # a Fock matrix with a small gap needs many replays; an easy one afterwards must be
+    # allowed to need fewer again (the count starts one below the previous one)
+    import torch
+    from aimd_amd.qc import scf as S
+    rng = np.random.default_rng(5)
+    n, nocc = 96, 30
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+
+    def fock(gap):
+        w = np.sort(rng.uniform(-10.0, 3.0, n))
+        w[nocc:] += gap + (w[nocc - 1] - w[nocc])
+        return torch.tensor((Q * w) @ Q.T, device="cuda"), Q[:, :nocc] @ Q[:, :nocc].T
+
+    S._TC2_GRAPHS.clear()
+    Fh, ref_h = fock(1e-3)
+    P = S.purified_projector_device(Fh, nocc)
+    np.testing.assert_allclose(P.cpu().numpy(), ref_h, atol=1e-9)
+    g = S._TC2_GRAPHS[(n, nocc, Fh.device)]
+    hard = g.needed
+    Fe, ref_e = fock(5.0)
+    counts = []
+    for _ in range(hard + 1):
+        P = S.purified_projector_device(Fe, nocc)
+        np.testing.assert_allclose(P.cpu().numpy(), ref_e, atol=1e-10)
+        counts.append(g.needed)
+    print("replays: hard", hard, "then", counts, flush=True)
+    assert hard > 1 and counts[-1] < hard
+    assert all(b <= a for a, b in zip(counts, counts[1:]))

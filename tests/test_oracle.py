@@ -267,3 +267,153 @@ def test_planted_mistake_is_caught_by_known_answer(oracle_lib, monkeypatch):
     b = O.make_basis(s, x, "sto-3g")
     st = O.scf(b, max_iter=60)
     assert abs(st.energy - (-74.965901192299)) > 1e-3
+
+
+# ----------------------------------------------------------------- dipole
+def test_dipole_centre_origin_shift_and_sp_closed_form(oracle_lib):
+    """<mu|r - o|nu>: a unit s function's dipole is its centre; moving the origin by d
+    subtracts d S, moving every atom by d adds d S; the same-centre s-p_x element is
+    sum_km c_k c_m (pi/p)^(3/2) / (2p), p = a_k + a_m (x^2 Gaussian moment)."""
+    O = oracle_lib
+    c = np.array([0.3, -0.2, 0.5])
+    b = O.make_basis(["H"], c[None], "sto-3g")
+    _, _, _, M = O.one_electron(b)
+    np.testing.assert_allclose(M[:, 0, 0], c, rtol=0, atol=1e-14)
+    s, x = bohr(WATER, 0.05, 4)
+    b = O.make_basis(s, x, "6-31g*")
+    S, _, _, M0 = O.one_electron(b)
+    d = np.array([0.7, -1.1, 0.4])
+    _, _, _, Mo = O.one_electron(b, origin=d)
+    np.testing.assert_allclose(Mo, M0 - d[:, None, None] * S, rtol=0, atol=1e-13)
+    _, _, _, Mt = O.one_electron(O.with_positions(b, x + d))
+    np.testing.assert_allclose(Mt, M0 + d[:, None, None] * S, rtol=0, atol=1e-13)
+    # O shells of 6-31G*: s (0), s (1), s (2), p (3: AOs 3-5), p (4), d; p_x of shell 3 is AO 3
+    e_s, c_s = b.prim_exp[b.shell_prim[1]:b.shell_prim[2]], b.prim_coef[b.shell_prim[1]:b.shell_prim[2]]
+    e_p, c_p = b.prim_exp[b.shell_prim[3]:b.shell_prim[4]], b.prim_coef[b.shell_prim[3]:b.shell_prim[4]]
+    p = e_s[:, None] + e_p[None, :]
+    moment = np.sum(c_s[:, None] * c_p[None, :] * (np.pi / p) ** 1.5 / (2.0 * p))
+    # about the O centre (origin = O position)
+    _, _, _, Mc = O.one_electron(b, origin=x[0])
+    assert b.shell_ao[3] == 3
+    assert Mc[0, 1, 3] == pytest.approx(moment, rel=1e-13)
+    assert Mc[1, 1, 4] == pytest.approx(moment, rel=1e-13)
+    assert Mc[2, 1, 5] == pytest.approx(moment, rel=1e-13)
+    assert abs(Mc[0, 1, 4]) < 1e-15 and abs(Mc[1, 1, 3]) < 1e-15
+
+
+# ------------------------------------------------- general contraction blocks
+def _segmented(O, b):
+    """The same basis with every shell its own block (the segmented path of the kernels)."""
+    nb = O.Basis(**{k: getattr(b, k) for k in b.__dataclass_fields__})
+    ns = len(b.shell_l)
+    nb.blk_shell = np.arange(ns + 1, dtype=np.int64)
+    nb.blk_atom = b.shell_atom.copy()
+    nb.blk_l = b.shell_l.copy()
+    nb.blk_ncon = np.ones(ns, np.int64)
+    nb.blk_ao = b.shell_ao[:-1].copy()
+    nb.blk_prim = b.shell_prim.copy()
+    nb.blk_exp = b.prim_exp.copy()
+    nb.blk_coef = np.ascontiguousarray(b.prim_coef[:, None])
+    return nb
+
+
+@pytest.mark.parametrize("case", ["water_ccpvdz", "hcl_ccpvdz", "custom"])
+def test_general_contraction_blocks_equal_segmented_shells(oracle_lib, case):
+    """ERIs, J/K and the ERI gradient of the block-contracted path (a general bra / ket
+    contracted once and spread over its contractions) equal those of the same basis
+    forced into one-contraction blocks, which the s-function closed forms and FD pin."""
+    from tests.conftest import HCL
+    from tests.test_basis import GENERAL_DATA, GENERAL_POS
+    O = oracle_lib
+    if case == "custom":
+        from aimd_amd.qc.basis import basis_from_table
+        pb = basis_from_table(["O", "H", "H"], GENERAL_POS, GENERAL_DATA)
+        b = O.Basis(list(pb.symbols), np.array(pb.positions), np.array(pb.shell_atom), np.array(pb.shell_l),
+                    np.array(pb.shell_prim), np.array(pb.shell_ao), np.array(pb.prim_exp), np.array(pb.prim_coef))
+        O._blocks(b)
+    else:
+        s, x = bohr(WATER, 0.05, 6) if case == "water_ccpvdz" else bohr(HCL, 0.02, 6)
+        b = O.make_basis(s, x, "cc-pvdz")
+    assert int(b.blk_ncon.max()) >= 2
+    seg = _segmented(O, b)
+    pg, ps = O.Pairs(b), O.Pairs(seg)
+    scale = max(1.0, float(np.abs(ps.eri(0.0)).max()))
+    np.testing.assert_allclose(pg.eri(0.0), ps.eri(0.0), rtol=0, atol=1e-13 * scale)
+    rng = np.random.default_rng(2)
+    D = rng.standard_normal((2, b.nao, b.nao))
+    D = D + D.transpose(0, 2, 1)
+    Jg, Kg = pg.jk_direct(D, 0.0)
+    Js, Ks = ps.jk_direct(D, 0.0)
+    np.testing.assert_allclose(Jg, Js, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(Kg, Ks, rtol=0, atol=1e-11)
+    P = D[0] @ D[0].T * 0.05
+    gg = pg.eri_gradient(P, [P], 0.5, thr=0.0)
+    gs = ps.eri_gradient(P, [P], 0.5, thr=0.0)
+    np.testing.assert_allclose(gg, gs, rtol=0, atol=1e-10 * max(1.0, np.abs(gs).max()))
+    gu = pg.eri_gradient(P, [0.6 * P, 0.4 * P], 1.0, thr=0.0)
+    gus = ps.eri_gradient(P, [0.6 * P, 0.4 * P], 1.0, thr=0.0)
+    np.testing.assert_allclose(gu, gus, rtol=0, atol=1e-10 * max(1.0, np.abs(gus).max()))
+
+
+# ------------------------------------------------ gradient screening bound
+def test_gradient_screening_bound(oracle_lib):
+    """grad_bound is the quantity eri_gradient screens on: a threshold just above its
+    largest value leaves an exactly zero gradient, just below it does not; one-centre
+    quartets are flagged; and on a tiny case it equals the maximum over the full
+    Gamma tensor built with numpy from the definition."""
+    O = oracle_lib
+    s, x = bohr(WATER, 0.05, 3)
+    b = O.make_basis(s, x, "sto-3g")
+    pr = O.Pairs(b)
+    st = O.scf(b)
+    P = 2 * st.Ds[0]
+    npair = pr.npair
+    qs = np.array([(p1, p2) for p1 in range(npair) for p2 in range(p1 + 1)], np.int64)
+    bound = pr.grad_bound(P, [P], 0.5, qs)
+    top = bound.max()
+    assert np.all(O.Pairs(b).eri_gradient(P, [P], 0.5, thr=top * (1 + 1e-12)) == 0.0)
+    assert np.any(O.Pairs(b).eri_gradient(P, [P], 0.5, thr=top * (1 - 1e-12)) != 0.0)
+    # numpy: Gamma_mnls = Pc_mn Pc_ls - k/2 (Px_ml Px_ns + Px_ms Px_nl), times deg
+    G = np.einsum("mn,ls->mnls", P, P) - 0.25 * (np.einsum("ml,ns->mnls", P, P) + np.einsum("ms,nl->mnls", P, P))
+    pp, blocks, _, _, Q = pr.arrays()
+    ao = b.blk_ao
+    nc = (b.blk_l + 1) * (b.blk_l + 2) // 2 * b.blk_ncon
+    atom = b.blk_atom
+    for i, (p1, p2) in enumerate(qs):
+        bi, bj = blocks[p1]
+        bk, bl = blocks[p2]
+        if atom[bi] == atom[bj] == atom[bk] == atom[bl]:
+            assert bound[i] == -1.0
+            continue
+        deg = 0.5 * (2 if bi != bj else 1) * (2 if bk != bl else 1) * (2 if p1 != p2 else 1)
+        g = np.abs(G[ao[bi]:ao[bi] + nc[bi], ao[bj]:ao[bj] + nc[bj], ao[bk]:ao[bk] + nc[bk],
+                     ao[bl]:ao[bl] + nc[bl]]).max() * deg
+        assert bound[i] == pytest.approx(Q[p1] * Q[p2] * g, rel=1e-13)
+
+
+def test_one_electron_gradient_terms_by_finite_differences(oracle_lib):
+    """Each term of the one-electron gradient is the derivative of its own trace at
+    fixed D and W: tr(D T), tr(D V) (nuclei move with the basis), -tr(W S)."""
+    O = oracle_lib
+    s, x = bohr(WATER, 0.05, 8)
+    b = O.make_basis(s, x, "6-31g*")
+    rng = np.random.default_rng(4)
+    D = rng.standard_normal((b.nao, b.nao))
+    D = D + D.T
+    W = rng.standard_normal((b.nao, b.nao))
+    W = W + W.T
+    gk, gv, gs = O.one_electron_gradient_terms(b, D, W)
+    np.testing.assert_allclose(gk + gv + gs, O.one_electron_gradient(b, D, W), rtol=0, atol=1e-12)
+    h = 1e-4
+    for a, d in [(0, 2), (1, 0), (2, 1), (1, 2)]:
+        vals = []
+        for sgn in (-2, -1, 1, 2):
+            y = x.copy()
+            y[a, d] += sgn * h
+            S, T, V, _ = O.one_electron(O.with_positions(b, y))
+            vals.append((np.sum(D * T), np.sum(D * V), -np.sum(W * S)))
+        vals = np.array(vals)
+        fd = (vals[0] - 8 * vals[1] + 8 * vals[2] - vals[3]) / (12 * h)
+        assert gk[a, d] == pytest.approx(fd[0], abs=1e-8)
+        assert gv[a, d] == pytest.approx(fd[1], abs=1e-8)
+        assert gs[a, d] == pytest.approx(fd[2], abs=1e-8)
