@@ -28,6 +28,8 @@ SIGNATURES = {
     "aimdhf_lmax": (_I, []),
     "aimdhf_plan_create": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "aimdhf_plan_destroy": (_I, [_P]),
+    "aimdhf_batch_create": (_I, [_I, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "aimdhf_batch_layout": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_L)]),
     "aimdhf_set_geometry": (_I, [_P, _P, _D, _P]),
     "aimdhf_schwarz_bounds": (_I, [_P, _P, _L]),
     "aimdhf_quartet_count": (_I, [_P, ctypes.POINTER(_L)]),
@@ -37,7 +39,7 @@ SIGNATURES = {
     "aimdhf_set_jk_mode": (_I, [_P, _I, _L]),
     "aimdhf_set_precision": (_I, [_P, _I, _D, _D, _D]),
     "aimdhf_set_streams": (_I, [_P, _I]),
-    "aimdhf_tc2_step": (_I, [_I, _P, _P, _P, _D, _P]),
+    "aimdhf_tc2_step": (_I, [_I, _P, _P, _P, _D, _D, _P]),
     "aimdhf_tc2_init": (_I, [_I, _P, _P, _P]),
     "aimdhf_tc2_error": (_I, [_I, _P, _P, _P]),
     "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P, _P]),

@@ -135,7 +135,16 @@ class HFBackend(ForceBackend):
     def label(self) -> str:
         return "UHF" if self.unrestricted else "RHF"
 
-    def set_density_guess(self, density: np.ndarray) -> None:
+    def set_density_guess(self, density) -> None:
+        if isinstance(density, torch.Tensor):
+            if tuple(density.shape) != self.density_shape:
+                raise ValueError(
+                    f"density guess has shape {tuple(density.shape)}, expected {self.density_shape} "
+                    f"({'unrestricted' if self.unrestricted else 'restricted'} reference)")
+            if not bool(torch.isfinite(density).all()):
+                raise ValueError("density guess contains non-finite values")
+            self._guess = density.detach().to(self.solver.device, torch.float64).clone()
+            return
         d = np.array(density, dtype=float)
         if d.shape != self.density_shape:
             raise ValueError(
@@ -144,6 +153,17 @@ class HFBackend(ForceBackend):
         if not np.all(np.isfinite(d)):
             raise ValueError("density guess contains non-finite values")
         self._guess = d
+
+    def device_density(self) -> torch.Tensor:
+        """The last SCF density on the device, in the density-guess layout (RHF: P (nao,
+        nao); UHF: (2, nao, nao)); what XL-BOMD propagates without a host round trip."""
+        res = self.last_scf
+        if res is None or res.device_spin_densities is None:
+            raise RuntimeError("no SCF has run yet")
+        Ds = res.device_spin_densities
+        if self.unrestricted:
+            return Ds.clone()
+        return Ds[0] + Ds[-1] if Ds.shape[0] == 2 else 2.0 * Ds[0]
 
     def reset_guess(self) -> None:
         self._guess = None

@@ -253,7 +253,10 @@ class TC2Graph:
     buffers: each step is a dense FP64 product X2 = X X (BLAS) and the engine's
     fused update (aimdhf_tc2_step), ping-ponging between two static buffers so the
     result of an even number of steps lands back in ``X``. The update kernel needs
-    no reduction pass and clears no memory, so replays are exact repeats.
+    no reduction pass and clears no memory, so replays are exact repeats. The
+    replayed steps hold an iterate whose idempotency error is below ``tol``
+    (steps past convergence would amplify its rounding errors until they
+    overflow); the tail steps do not, they square the remaining error away.
     ``needed`` remembers how many replays the last purification took; the next one
     (the Fock matrix of the next SCF iteration) checks convergence from one replay
     earlier on, so the count follows the matrices down as well as up.
@@ -261,8 +264,8 @@ class TC2Graph:
     STEPS = 8
     TAIL_STEPS = 2
 
-    def __init__(self, n: int, nocc: int, device: torch.device) -> None:
-        self.n, self.nocc = n, float(nocc)
+    def __init__(self, n: int, nocc: int, device: torch.device, tol: float = 1e-11) -> None:
+        self.n, self.nocc, self.tol = n, float(nocc), float(tol)
         self.needed = 1
         self.X = torch.zeros((n, n), dtype=torch.float64, device=device)
         self.B = torch.empty_like(self.X)
@@ -271,23 +274,23 @@ class TC2Graph:
         side = torch.cuda.Stream(device=device)
         side.wait_stream(torch.cuda.current_stream(device))
         with torch.cuda.stream(side):
-            self._steps(self.STEPS)
+            self._steps(self.STEPS, self.tol)
         torch.cuda.current_stream(device).wait_stream(side)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self._steps(self.STEPS)
+            self._steps(self.STEPS, self.tol)
         self.tail_graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.tail_graph):
-            self._steps(self.TAIL_STEPS)
+            self._steps(self.TAIL_STEPS, 0.0)
 
-    def _steps(self, nsteps: int) -> None:
+    def _steps(self, nsteps: int, freeze: float) -> None:
         L = _lib.lib()
         st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         src, dst = self.X, self.B
         for _ in range(nsteps):
             torch.matmul(src, src, out=self.X2)
             _lib.check(L.aimdhf_tc2_step(self.n, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(self.X2.data_ptr()),
-                                         ctypes.c_void_p(dst.data_ptr()), self.nocc, st))
+                                         ctypes.c_void_p(dst.data_ptr()), self.nocc, freeze, st))
             src, dst = dst, src
 
     def replay(self) -> torch.Tensor:
@@ -320,18 +323,18 @@ def purified_projector_device(F: torch.Tensor, nocc: int, tol: float = 1e-11,
     """purified_projector on the GPU with the captured TC2Graph (same steps, same branch
     rule): the idempotency error is read back after the number of replays the previous
     call needed minus one, then after every replay, and the TAIL_STEPS tail graph squares
-    the remaining error away. Replays past convergence leave an idempotent iterate
-    idempotent, so starting the checks late costs time, not accuracy; starting one
-    replay early lets the count drop again after a hard matrix."""
+    the remaining error away. Replayed steps hold a converged iterate unchanged, so
+    starting the checks late costs time, not accuracy; starting one replay early lets
+    the count drop again after a hard matrix."""
     n = F.shape[0]
     if nocc <= 0:
         return torch.zeros_like(F)
     if nocc >= n:
         return torch.eye(n, dtype=F.dtype, device=F.device)
-    key = (n, int(nocc), F.device)
+    key = (n, int(nocc), F.device, float(tol))
     g = _TC2_GRAPHS.get(key)
     if g is None:
-        g = _TC2_GRAPHS[key] = TC2Graph(n, nocc, F.device)
+        g = _TC2_GRAPHS[key] = TC2Graph(n, nocc, F.device, tol)
     g.start(F)
     first = max(1, min(g.needed - 1, max_replays))
     for _ in range(first - 1):
@@ -894,6 +897,20 @@ class SCFSolver:
                     F = F - torch.diag(torch.diagonal(F)) + torch.diag(h)
                 return torch.stack([self._occupied_density(F, I.X, k) for k in self._nocc()]), kind
             raise ValueError(f"unknown guess {guess!r}; use one of {_GUESSES} or a density")
+        if isinstance(guess, torch.Tensor):
+            # a device density (e.g. an XL-BOMD auxiliary density): no host round trip
+            Dt = guess.to(device=dev, dtype=torch.float64)
+            if tuple(Dt.shape) == (n, n):
+                Dt = (0.5 * Dt)[None].expand(nspin, n, n)
+            elif tuple(Dt.shape) == (2, n, n):
+                Dt = Dt if self.unrestricted else (0.5 * (Dt[0] + Dt[1]))[None]
+            else:
+                raise ValueError(
+                    f"density guess has shape {tuple(Dt.shape)}; expected {self.density_shape} "
+                    f"({self.reference.upper()}) or {(n, n) if self.unrestricted else (2, n, n)}")
+            if not bool(torch.isfinite(Dt).all()):
+                raise ValueError("density guess contains non-finite values")
+            return (0.5 * (Dt + Dt.transpose(1, 2))).contiguous(), "density"
         D = np.array(guess, dtype=float)
         if D.shape == (n, n):
             Ds = np.repeat(0.5 * D[None], nspin, axis=0)
@@ -969,6 +986,10 @@ class SCFSolver:
 
     def _iterate(self, I: SCFIntegrals, Ds: torch.Tensor) -> dict[str, Any]:
         opt = self.options
+        # every SCF starts the purification replay counts afresh, so a result depends
+        # on its own inputs only (bit-for-bit restarts), not on earlier SCFs
+        for g in _TC2_GRAPHS.values():
+            g.needed = 1
         nspin = Ds.shape[0]
         g = 2.0 / nspin
         nocc = self._nocc()

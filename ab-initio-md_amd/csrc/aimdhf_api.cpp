@@ -265,6 +265,18 @@ struct aimdhf_plan {
     // the caller's stream, which do not wait for the integrals on the library streams
     char* h_stage = nullptr;
     size_t h_stage_cap = 0, h_stage_used = 0;
+    // batched plans: independent molecules (replicas) sharing one plan; shells and
+    // atoms grouped by replica, pairs and quartets only within a replica, matrices
+    // in the padded layout (replica r's ld x ld block at r ld^2), two-electron
+    // integrals as one dense n_r^4 tensor per replica
+    bool batched = false;
+    int nrep = 1, bld = 0;
+    std::vector<int> rep_shell, rep_atom, rep_nao, shell_rep, shell_first, shell_pair0, pair_lo;
+    std::vector<long long> rep_toff, rep_item;
+    long long tens_total = 0;
+    DevBuf d_shell_rep, d_shell_first, d_shell_pair0, d_pair_lo, d_rep_atom, d_rep_nao, d_rep_toff, d_rep_item, d_tens;
+    // elements of one stack of matrices (nao^2, or nrep ld^2 for a batch)
+    size_t mat_elems() const { return batched ? (size_t)nrep * bld * bld : (size_t)nao * nao; }
 
     // make room for `bytes` of staging (no transfer may be pending) and start over
     void stage_reset(size_t bytes) {
@@ -308,7 +320,8 @@ struct aimdhf_plan {
                           &d_kdata, &d_jdata, &d_kdoff, &d_jdoff, &d_kdata32, &d_jdata32, &d_kdoff32, &d_jdoff32,
                           &d_kbig, &d_jbig, &d_kunit_os, &d_kunit_e, &d_out_unit,
                           &d_kpart2, &d_junit_pc, &d_junit_e, &d_pair_unit, &d_jpart2, &d_kunit_q, &d_junit_q,
-                          &d_dmax, &d_kdata32b, &d_jdata32b, &d_tab, &d_gdmax})
+                          &d_dmax, &d_kdata32b, &d_jdata32b, &d_tab, &d_gdmax, &d_shell_rep, &d_shell_first,
+                          &d_shell_pair0, &d_pair_lo, &d_rep_atom, &d_rep_nao, &d_rep_toff, &d_rep_item, &d_tens})
             b->release();
         if (h_pin) (void)hipHostFree(h_pin);
         if (h_stage) (void)hipHostFree(h_stage);
@@ -340,6 +353,7 @@ struct aimdhf_plan {
         g.bp_off = d_bp_off.as<int>();
         g.bp_list = d_bp_list.as<int>();
         g.boys = d_boys.as<double>();
+        g.pair_lo = batched ? d_pair_lo.as<int>() : nullptr;
         return g;
     }
 
@@ -855,7 +869,70 @@ struct aimdhf_plan {
         g.charges = d_charges.as<double>();
         g.boys = d_boys.as<double>();
         g.parts = 3;
+        g.ld = batched ? bld : nao;
+        g.mstride = (long long)mat_elems();
+        g.shell_rep = batched ? d_shell_rep.as<int>() : nullptr;
+        g.rep_atom = batched ? d_rep_atom.as<int>() : nullptr;
         return g;
+    }
+
+    aimdhf::BatchArgs batch_args() const {
+        aimdhf::BatchArgs b;
+        b.nrep = nrep;
+        b.ld = bld;
+        b.rep_nao = d_rep_nao.as<int>();
+        b.rep_toff = d_rep_toff.as<long long>();
+        b.rep_item = d_rep_item.as<long long>();
+        b.shell_rep = d_shell_rep.as<int>();
+        b.shell_ao = d_shell_ao.as<int>();
+        b.shell_l = d_shell_l.as<int>();
+        b.pair_a = d_pair_a.as<int>();
+        b.pair_b = d_pair_b.as<int>();
+        b.list = d_list.as<int2>();
+        b.qoff[0] = 0;
+        for (int k = 0; k < NQCLASS; ++k) {
+            b.qoff[k + 1] = b.qoff[k] + qcount[k];
+            b.store_base[k] = store_base[k];
+            b.blk[k] = (int)aimdhf::store_block(k);
+            int hi = 0;
+            while ((hi + 1) * (hi + 2) / 2 <= k) ++hi;
+            const int lo = k - hi * (hi + 1) / 2, lc = lo == 0 ? 0 : lo <= 2 ? 1 : 2, ld_ = lo - lc * (lc + 1) / 2;
+            b.ncd[k] = ncart(lc) * ncart(ld_);
+        }
+        b.store_base[NQCLASS] = store_base[NQCLASS];
+        b.store = d_store.as<double>();
+        b.tens = d_tens.as<double>();
+        return b;
+    }
+
+    // Batch geometry: every screened quartet of every replica into the store (one
+    // launch per class, FP64), then into the replicas' dense tensors
+    void build_batch_tensors(hipStream_t st) {
+        long long off = 0;
+        for (int k = 0; k < NQCLASS; ++k) {
+            store_base[k] = off;
+            off += qcount[k] * aimdhf::store_block(k);
+        }
+        store_base[NQCLASS] = off;
+        d_store.ensure(sizeof(double) * std::max<long long>(off, 1));
+        d_tens.ensure(sizeof(double) * std::max<long long>(tens_total, 1));
+        aimdhf::StoreArgs sa;
+        sa.recs = d_recs.as<double>();
+        sa.rec_off = d_rec_off.as<long long>();
+        sa.npp = d_npp.as<int>();
+        sa.boys = d_boys.as<double>();
+        prof.time(47, st, [&] {
+            for (int k = 0; k < NQCLASS; ++k) {
+                if (!qcount[k]) continue;
+                sa.store = d_store.as<double>() + store_base[k];
+                aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], qcount[k], st);
+            }
+        });
+        prof.time(15, st, [&] {
+            aimdhf::launch_zero_d(d_tens.as<double>(), (size_t)tens_total, st);
+            aimdhf::launch_batch_scatter(batch_args(), off, st);
+        });
+        HIPCHECK(hipGetLastError());
     }
 };
 
@@ -873,19 +950,54 @@ int aimdhf_last_error(char* buf, size_t len) {
 
 int aimdhf_lmax(void) { return 2; }
 
-int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const int64_t* shell_l,
+}  // extern "C"
+
+// Plan of one molecule (nrep = 0) or of nrep replicas (shells and atoms grouped
+// by replica: rep_shell / rep_atom hold nrep + 1 offsets)
+static int create_plan(int natm, int nshell, const int64_t* shell_atom, const int64_t* shell_l,
                        const int64_t* shell_prim, const int64_t* shell_block, const double* prim_exp,
-                       const double* prim_coef, const double* charges, aimdhf_plan** out) {
+                       const double* prim_coef, const double* charges, int nrep, const int64_t* rep_atom,
+                       const int64_t* rep_shell, aimdhf_plan** out) {
     if (!out) return fail(AIMDHF_E_ARG, "out is NULL");
     *out = nullptr;
     if (natm <= 0 || nshell <= 0) return fail(AIMDHF_E_ARG, "natm and nshell must be positive");
     if (!shell_atom || !shell_l || !shell_prim || !shell_block || !prim_exp || !prim_coef || !charges)
         return fail(AIMDHF_E_ARG, "NULL basis array");
+    if (nrep > 0) {
+        if (!rep_atom || !rep_shell) return fail(AIMDHF_E_ARG, "NULL replica offsets");
+        if (rep_atom[0] != 0 || rep_shell[0] != 0 || rep_atom[nrep] != natm || rep_shell[nrep] != nshell)
+            return fail(AIMDHF_E_ARG, "replica offsets must run from 0 to natm and nshell");
+        for (int r = 0; r < nrep; ++r)
+            if (rep_atom[r + 1] <= rep_atom[r] || rep_shell[r + 1] <= rep_shell[r])
+                return fail(AIMDHF_E_ARG, "every replica needs at least one atom and one shell");
+        for (int r = 0; r < nrep; ++r)
+            for (int64_t sh = rep_shell[r]; sh < rep_shell[r + 1]; ++sh)
+                if (shell_atom[sh] < rep_atom[r] || shell_atom[sh] >= rep_atom[r + 1])
+                    return fail(AIMDHF_E_ARG, "a shell sits on an atom of another replica");
+    }
     aimdhf_plan* p = new (std::nothrow) aimdhf_plan();
     if (!p) return fail(AIMDHF_E_NOMEM, "host allocation failed");
     try {
         p->natm = natm;
         p->nshell = nshell;
+        p->batched = nrep > 0;
+        p->nrep = nrep > 0 ? nrep : 1;
+        // replica of every shell and the first shell of its replica (one replica: 0)
+        p->shell_rep.assign(nshell, 0);
+        p->shell_first.assign(nshell, 0);
+        p->rep_shell.assign(p->nrep + 1, 0);
+        p->rep_atom.assign(p->nrep + 1, 0);
+        p->rep_shell[p->nrep] = nshell;
+        p->rep_atom[p->nrep] = natm;
+        if (p->batched)
+            for (int r = 0; r < nrep; ++r) {
+                p->rep_shell[r] = (int)rep_shell[r];
+                p->rep_atom[r] = (int)rep_atom[r];
+                for (int64_t sh = rep_shell[r]; sh < rep_shell[r + 1]; ++sh) {
+                    p->shell_rep[sh] = r;
+                    p->shell_first[sh] = (int)rep_shell[r];
+                }
+            }
         int nprim = (int)shell_prim[nshell];
         p->shell_atom.resize(nshell);
         p->shell_l.resize(nshell);
@@ -912,12 +1024,38 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
         }
         p->shell_prim[nshell] = nprim;
         p->nao = p->shell_ao[nshell];
+        if (p->batched) {
+            // AO offsets within each replica; replica sizes, padded dimension, tensors
+            p->rep_nao.assign(p->nrep, 0);
+            for (int r = 0; r < p->nrep; ++r) {
+                const int a0 = p->shell_ao[p->rep_shell[r]];
+                p->rep_nao[r] = p->shell_ao[p->rep_shell[r + 1]] - a0;
+                p->bld = std::max(p->bld, p->rep_nao[r]);
+            }
+            std::vector<int> local(nshell + 1);
+            for (int sh = 0; sh < nshell; ++sh) local[sh] = p->shell_ao[sh] - p->shell_ao[p->shell_first[sh]];
+            local[nshell] = 0;
+            p->shell_ao = local;
+            p->rep_toff.assign(p->nrep + 1, 0);
+            p->rep_item.assign(p->nrep + 1, 0);
+            for (int r = 0; r < p->nrep; ++r) {
+                const long long n = p->rep_nao[r];
+                p->rep_toff[r + 1] = p->rep_toff[r] + n * n * n * n;
+                p->rep_item[r + 1] = p->rep_item[r] + n * n;
+            }
+            p->tens_total = p->rep_toff[p->nrep];
+        }
         p->nblock = p->shell_block[nshell - 1] + 1;
         p->prim_exp.assign(prim_exp, prim_exp + nprim);
         p->prim_coef.assign(prim_coef, prim_coef + nprim);
         p->charges.assign(charges, charges + natm);
-        // shell pairs (s1 >= s2) in the reference's pair order, oriented l_A >= l_B
-        p->npair = nshell * (nshell + 1) / 2;
+        // shell pairs (s1 >= s2) in the reference's pair order, oriented l_A >= l_B; in a
+        // batch only the pairs within a replica, replica by replica
+        p->npair = 0;
+        for (int r = 0; r < p->nrep; ++r) {
+            const int ns = p->rep_shell[r + 1] - p->rep_shell[r];
+            p->npair += ns * (ns + 1) / 2;
+        }
         p->pair_a.resize(p->npair);
         p->pair_b.resize(p->npair);
         p->pair_cls.resize(p->npair);
@@ -926,10 +1064,23 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
         long long off = 0;
         int P = 0;
         std::vector<std::vector<int>> bylcls(6);
-        p->nbpair = p->nblock * (p->nblock + 1) / 2;
+        // block pairs within each replica, replica by replica (one molecule: i(i+1)/2+j)
+        std::vector<int> rep_b0(p->nrep), rep_bp0(p->nrep + 1, 0), rep_p0(p->nrep + 1, 0);
+        for (int r = 0; r < p->nrep; ++r) {
+            rep_b0[r] = p->shell_block[p->rep_shell[r]];
+            const int nb = p->shell_block[p->rep_shell[r + 1] - 1] - rep_b0[r] + 1;
+            rep_bp0[r + 1] = rep_bp0[r] + nb * (nb + 1) / 2;
+            const int ns = p->rep_shell[r + 1] - p->rep_shell[r];
+            rep_p0[r + 1] = rep_p0[r] + ns * (ns + 1) / 2;
+        }
+        p->nbpair = rep_bp0[p->nrep];
         std::vector<std::vector<int>> bpl(p->nbpair);
+        p->shell_pair0.assign(nshell, 0);
+        p->pair_lo.assign(p->npair, 0);
+        for (int s1 = 0; s1 < nshell; ++s1) p->shell_pair0[s1] = rep_p0[p->shell_rep[s1]];
         for (int s1 = 0; s1 < nshell; ++s1)
-            for (int s2 = 0; s2 <= s1; ++s2, ++P) {
+            for (int s2 = p->shell_first[s1]; s2 <= s1; ++s2, ++P) {
+                p->pair_lo[P] = p->shell_pair0[s1];
                 int a = s1, b = s2;
                 if (p->shell_l[s2] > p->shell_l[s1]) std::swap(a, b);
                 int la = p->shell_l[a], lb = p->shell_l[b];
@@ -941,8 +1092,9 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
                 p->rec_off[P] = off;
                 off += (long long)rs * (shell_prim[s1 + 1] - shell_prim[s1]) * (shell_prim[s2 + 1] - shell_prim[s2]);
                 bylcls[p->pair_cls[P]].push_back(P);
-                int b1 = p->shell_block[s1], b2 = p->shell_block[s2];
-                bpl[b1 * (b1 + 1) / 2 + b2].push_back(P);
+                const int r = p->shell_rep[s1];
+                int b1 = p->shell_block[s1] - rep_b0[r], b2 = p->shell_block[s2] - rep_b0[r];
+                bpl[rep_bp0[r] + b1 * (b1 + 1) / 2 + b2].push_back(P);
             }
         p->rec_off[p->npair] = off;
         p->rec_total = off;
@@ -958,8 +1110,9 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
             p->bp_off[i + 1] = p->bp_off[i] + (int)bpl[i].size();
             p->bp_list.insert(p->bp_list.end(), bpl[i].begin(), bpl[i].end());
         }
-        // exchange output blocks (X >= Y) grouped by (l_X, l_Y)
-        {
+        // exchange output blocks (X >= Y) grouped by (l_X, l_Y); a batch digests its dense
+        // tensors instead and needs neither these nor the AO -> shell map
+        if (!p->batched) {
             std::vector<std::vector<int>> ox(9), oy(9);
             for (int X = 0; X < nshell; ++X)
                 for (int Y = 0; Y <= X; ++Y) {
@@ -999,6 +1152,16 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
         upload(p->d_bp_list, p->bp_list);
         upload(p->d_boys, boys_table());
         upload(p->d_charges, p->charges);
+        if (p->batched) {
+            upload(p->d_shell_rep, p->shell_rep);
+            upload(p->d_shell_first, p->shell_first);
+            upload(p->d_shell_pair0, p->shell_pair0);
+            upload(p->d_pair_lo, p->pair_lo);
+            upload(p->d_rep_atom, p->rep_atom);
+            upload(p->d_rep_nao, p->rep_nao);
+            upload(p->d_rep_toff, p->rep_toff);
+            upload(p->d_rep_item, p->rep_item);
+        }
         p->d_scratch.ensure(sizeof(double) * 1024);
         p->d_pos.ensure(sizeof(double) * 3 * natm);
         p->d_recs.ensure(sizeof(double) * std::max<long long>(off, 1));
@@ -1015,6 +1178,32 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
         return fail(AIMDHF_E_NOMEM, "host allocation failed");
     }
     *out = p;
+    return AIMDHF_OK;
+}
+
+extern "C" {
+
+int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const int64_t* shell_l,
+                       const int64_t* shell_prim, const int64_t* shell_block, const double* prim_exp,
+                       const double* prim_coef, const double* charges, aimdhf_plan** out) {
+    return create_plan(natm, nshell, shell_atom, shell_l, shell_prim, shell_block, prim_exp, prim_coef, charges, 0,
+                       nullptr, nullptr, out);
+}
+
+int aimdhf_batch_create(int nrep, const int64_t* rep_atom, const int64_t* rep_shell, int natm, int nshell,
+                        const int64_t* shell_atom, const int64_t* shell_l, const int64_t* shell_prim,
+                        const int64_t* shell_block, const double* prim_exp, const double* prim_coef,
+                        const double* charges, aimdhf_plan** out) {
+    if (nrep <= 0) return fail(AIMDHF_E_ARG, "a batch needs at least one replica");
+    return create_plan(natm, nshell, shell_atom, shell_l, shell_prim, shell_block, prim_exp, prim_coef, charges, nrep,
+                       rep_atom, rep_shell, out);
+}
+
+int aimdhf_batch_layout(aimdhf_plan* p, int* nrep, int* ld, int64_t* tensor_bytes) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    if (nrep) *nrep = p->batched ? p->nrep : 0;
+    if (ld) *ld = p->batched ? p->bld : p->nao;
+    if (tensor_bytes) *tensor_bytes = p->batched ? 8 * p->tens_total : 0;
     return AIMDHF_OK;
 }
 
@@ -1047,6 +1236,13 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
         p->prof.host(85, t0);
         p->stored = false;
         p->store_bytes = p->k_entries = p->j_entries = 0;
+        if (p->batched) {
+            auto t1 = Profiler::clock::now();
+            p->build_batch_tensors(st);
+            p->prof.host(87, t1);
+            p->have_geometry = true;
+            return AIMDHF_OK;
+        }
         bool store = false;
         if (p->jk_mode != 1 && p->nao < (1 << 14)) {
             long long limit = p->store_limit;
@@ -1187,8 +1383,29 @@ int aimdhf_jk_delta(aimdhf_plan* p, int nset, const double* ddens, double* dJ, d
     if (nset < 1 || nset > 2) return fail(AIMDHF_E_ARG, "nset must be 1 or 2");
     if (!(threshold >= 0.0) || !std::isfinite(threshold)) return fail(AIMDHF_E_ARG, "threshold must be finite and >= 0");
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
-    if (!p->stored) return aimdhf_jk(p, nset, ddens, dJ, dK, stream);
+    if (p->batched || !p->stored) return aimdhf_jk(p, nset, ddens, dJ, dK, stream);
     return jk_from_store(p, nset, ddens, dJ, dK, true, threshold, (hipStream_t)stream);
+}
+
+// J and K of every replica of a batch from its dense tensor; dens, J, K in the
+// padded layout (nrep, nset, ld, ld)
+static int jk_batch(aimdhf_plan* p, int nset, const double* dens, double* J, double* K, hipStream_t st) {
+    try {
+        const size_t n = (size_t)nset * p->mat_elems();
+        p->d_dsym.ensure(sizeof(double) * n);
+        p->prof.time(45, st, [&] {
+            aimdhf::launch_symmetrize(dens, p->nrep * nset, p->bld, p->d_dsym.as<double>(), st);
+            aimdhf::launch_zero_d(J, n, st);
+            aimdhf::launch_zero_d(K, n, st);
+        });
+        p->prof.time(49, st, [&] {
+            aimdhf::launch_batch_jk(p->batch_args(), nset, p->d_dsym.as<double>(), J, K, p->rep_item[p->nrep], st);
+        });
+        HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
 }
 
 int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K, void* stream) {
@@ -1196,6 +1413,7 @@ int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K
     if (nset < 1 || nset > 2) return fail(AIMDHF_E_ARG, "nset must be 1 or 2");
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
+    if (p->batched) return jk_batch(p, nset, dens, J, K, st);
     if (p->stored) return jk_from_store(p, nset, dens, J, K, false, 0.0, st);
     try {
         const size_t nn = (size_t)p->nao * p->nao;
@@ -1295,6 +1513,14 @@ int aimdhf_one_electron(aimdhf_plan* p, double* S, double* T, double* V, double*
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
     try {
+        if (p->batched) {
+            // padded layout: only the replicas' own blocks are written
+            const size_t n = p->mat_elems();
+            aimdhf::launch_zero_d(S, n, st);
+            aimdhf::launch_zero_d(T, n, st);
+            aimdhf::launch_zero_d(V, n, st);
+            aimdhf::launch_zero_d(M, 3 * n, st);
+        }
         p->prof.time(43, st, [&] { aimdhf::launch_oneel(p->oneel_args(), S, T, M, V, st); });
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
@@ -1308,12 +1534,13 @@ int aimdhf_grad_one(aimdhf_plan* p, const double* D, const double* W, double* gr
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
     try {
-        size_t nn = (size_t)p->nao * p->nao;
+        const size_t nn = p->mat_elems();
+        const int nmat = p->batched ? p->nrep : 1, ld = p->batched ? p->bld : p->nao;
         p->d_gden.ensure(sizeof(double) * 2 * nn);
         double* Ds = p->d_gden.as<double>();
         double* Wn = Ds + nn;
-        aimdhf::launch_symmetrize(D, 1, p->nao, Ds, st);
-        aimdhf::launch_symmetrize(W, 1, p->nao, Wn, st);
+        aimdhf::launch_symmetrize(D, nmat, ld, Ds, st);
+        aimdhf::launch_symmetrize(W, nmat, ld, Wn, st);
         aimdhf::launch_negate(Wn, nn, Wn, st);
         p->d_gfx.ensure(sizeof(long long) * 6 * p->natm);
         aimdhf::launch_zero64(p->d_gfx.as<long long>(), 6 * p->natm, st);
@@ -1332,13 +1559,14 @@ int aimdhf_grad_one_terms(aimdhf_plan* p, const double* D, const double* W, doub
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
     try {
-        const size_t nn = (size_t)p->nao * p->nao;
+        const size_t nn = p->mat_elems();
+        const int nmat = p->batched ? p->nrep : 1, ld = p->batched ? p->bld : p->nao;
         p->d_gden.ensure(sizeof(double) * 3 * nn);
         double* Ds = p->d_gden.as<double>();
         double* Wn = Ds + nn;
         double* Z = Wn + nn;
-        aimdhf::launch_symmetrize(D, 1, p->nao, Ds, st);
-        aimdhf::launch_symmetrize(W, 1, p->nao, Wn, st);
+        aimdhf::launch_symmetrize(D, nmat, ld, Ds, st);
+        aimdhf::launch_symmetrize(W, nmat, ld, Wn, st);
         aimdhf::launch_negate(Wn, nn, Wn, st);
         aimdhf::launch_zero_d(Z, nn, st);
         p->d_gfx.ensure(sizeof(long long) * 18 * p->natm);
@@ -1370,12 +1598,13 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
     try {
-        size_t nn = (size_t)p->nao * p->nao;
+        const size_t nn = p->mat_elems();
+        const int nmat = p->batched ? p->nrep : 1, ld = p->batched ? p->bld : p->nao;
         p->d_gden.ensure(sizeof(double) * (1 + nx) * nn);
         double* dc = p->d_gden.as<double>();
         double* dx = dc + nn;
-        aimdhf::launch_symmetrize(Dc, 1, p->nao, dc, st);
-        aimdhf::launch_symmetrize(Dx, nx, p->nao, dx, st);
+        aimdhf::launch_symmetrize(Dc, nmat, ld, dc, st);
+        aimdhf::launch_symmetrize(Dx, nmat * nx, ld, dx, st);
         // candidate quartets: q_P q_Q >= thr / max|Gamma| over every possible element
         double mc = aimdhf::device_absmax(dc, nn, p->d_scratch.as<double>(), st);
         double mx = aimdhf::device_absmax(dx, nx * nn, p->d_scratch.as<double>(), st);
@@ -1406,10 +1635,15 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
         a.dc = dc;
         a.dx = dx;
         a.npair = p->npair;
+        a.ld = ld;
+        a.shell_rep = p->batched ? p->d_shell_rep.as<int>() : nullptr;
+        a.shell_first = p->batched ? p->d_shell_first.as<int>() : nullptr;
+        a.shell_pair0 = p->batched ? p->d_shell_pair0.as<int>() : nullptr;
         p->d_gdmax.ensure(sizeof(double) * (1 + nx) * std::max(p->npair, 1));
         a.dmc = p->d_gdmax.as<double>();
         a.dmx = a.dmc + p->npair;
-        aimdhf::launch_pair_dmax(a, p->d_gdmax.as<double>(), p->d_gdmax.as<double>() + p->npair, st);
+        aimdhf::launch_pair_dmax(a, p->d_shell_l.as<int>(), p->d_gdmax.as<double>(), p->d_gdmax.as<double>() + p->npair,
+                                 st);
         a.out = p->d_gfx.as<long long>();
         long long gtot = 0;
         for (int k = 0; k < NQCLASS; ++k) gtot += p->gcount[k];
@@ -1453,11 +1687,32 @@ int aimdhf_set_jk_mode(aimdhf_plan* p, int mode, int64_t max_store_bytes) {
     return AIMDHF_OK;
 }
 
-int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, void* stream) {
+int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, double freeze, void* stream) {
     if (n <= 0 || !X || !X2 || !Xout) return fail(AIMDHF_E_ARG, "bad tc2 arguments");
-    aimdhf::launch_tc2_step(n, X, X2, Xout, nocc, (hipStream_t)stream);
+    aimdhf::launch_tc2_step(n, X, X2, Xout, nocc, freeze, (hipStream_t)stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(AIMDHF_E_HIP, std::string("tc2 step: ") + hipGetErrorString(e));
+    return AIMDHF_OK;
+}
+
+int aimdhf_batch_purify(int nmat, int ld, const double* F, const double* nocc, double* P, int steps, double tol,
+                        double* err, void* stream) {
+    if (nmat < 0 || ld <= 0 || ld > 64 || steps < 0 || !(tol >= 0.0)) return fail(AIMDHF_E_ARG, "bad purify arguments");
+    if (nmat == 0) return AIMDHF_OK;
+    if (!F || !nocc || !P || !err) return fail(AIMDHF_E_ARG, "NULL argument");
+    aimdhf::launch_batch_purify(nmat, ld, F, nocc, P, steps, tol, err, (hipStream_t)stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(AIMDHF_E_HIP, std::string("batch purify: ") + hipGetErrorString(e));
+    return AIMDHF_OK;
+}
+
+int aimdhf_batch_diis_solve(int nrep, int m, const double* G, const double* r, double* c, void* stream) {
+    if (nrep < 0 || m <= 0 || m > 12) return fail(AIMDHF_E_ARG, "bad DIIS solve arguments (1 <= m <= 12)");
+    if (nrep == 0) return AIMDHF_OK;
+    if (!G || !r || !c) return fail(AIMDHF_E_ARG, "NULL argument");
+    aimdhf::launch_batch_diis_solve(nrep, m, G, r, c, (hipStream_t)stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(AIMDHF_E_HIP, std::string("batch DIIS solve: ") + hipGetErrorString(e));
     return AIMDHF_OK;
 }
 

@@ -30,6 +30,8 @@ struct PairArgs {
     const int* bp_off;          // nbpair + 1
     const int* bp_list;         // shell pairs of each block pair
     const double* boys;         // 402 x 43 table
+    const int* pair_lo;         // batched plans: first pair of each pair's replica (quartets pair
+                                // within a replica only); null: every pair Q <= P
 };
 
 struct JKArgs {
@@ -65,6 +67,13 @@ struct OneElArgs {
     const double* charges;
     const double* boys;
     int parts;                  // one-electron gradient: 1 overlap/kinetic, 2 nuclear attraction, 3 both
+    // matrix layout: element (m, n) of a shell pair's matrices sits at
+    // rep * ld^2 + m * ld + n (m, n: AO offsets within the replica, shell_ao);
+    // the dipole components are mstride apart. A single molecule: rep 0, ld nao.
+    int ld;
+    long long mstride;
+    const int* shell_rep;       // replica of each shell (null: 0)
+    const int* rep_atom;        // nrep + 1: atoms of each replica (null: all atoms)
 };
 
 struct GradArgs {
@@ -96,6 +105,12 @@ struct GradArgs {
     int* bcnt;                  // pre-pass block counts (2 ints per 256 quartets)
     int* boff;                  // and their exclusive prefix sums
     long long* out;             // (natm, 3) fixed point, hi/lo interleaved
+    // batched plans (see OneElArgs): dc of replica r at r ld^2, dx set x at (r nx + x) ld^2;
+    // shell pair (s, t) of a replica at shell_pair0[s] + tri(s - shell_first[s], t - shell_first[t])
+    int ld;
+    const int* shell_rep;       // null: one molecule
+    const int* shell_first;
+    const int* shell_pair0;
 };
 
 // owner-computes J/K (J-engine + exchange owners)
@@ -348,6 +363,34 @@ struct UnitQArgs {
     double* junit_q;
 };
 
+// batched plans: stored blocks -> dense per-replica tensors, and their J/K digestion
+constexpr int NQCLASS_B = 21;
+struct BatchArgs {
+    int nrep;
+    int ld;                     // padded matrix dimension (largest replica)
+    const int* rep_nao;         // AOs of each replica
+    const long long* rep_toff;  // first tensor element of each replica
+    const long long* rep_item;  // nrep + 1: prefix sums of rep_nao^2 (J/K work items)
+    const int* shell_rep;
+    const int* shell_ao;        // AO offset of each shell within its replica
+    const int* shell_l;
+    const int* pair_a;
+    const int* pair_b;
+    const int2* list;           // class-binned quartet list
+    long long qoff[NQCLASS_B + 1];
+    long long store_base[NQCLASS_B + 1];
+    int blk[NQCLASS_B];         // integrals per block of each class
+    int ncd[NQCLASS_B];         // ket components of each class
+    const double* store;
+    double* tens;
+};
+void launch_batch_scatter(const BatchArgs& b, long long total, hipStream_t st);
+void launch_batch_jk(const BatchArgs& b, int nset, const double* dens, double* J, double* K, long long nitems,
+                     hipStream_t st);
+void launch_batch_purify(int nmat, int ld, const double* F, const double* nocc, double* P, int steps, double tol,
+                         double* err, hipStream_t st);
+void launch_batch_diis_solve(int nrep, int m, const double* G, const double* r, double* c, hipStream_t st);
+
 void launch_eri_store(int k, const StoreArgs& a, const int2* list, long long nq, long long n64, hipStream_t st);
 long long store_block(int k);
 void launch_klist_count(const ListArgs& a, int nout, long long* counts, long long* band, hipStream_t st);
@@ -368,7 +411,8 @@ void launch_units(const TabArgs& a, hipStream_t st);
 void launch_tab_totals(const TabArgs& a, long long* out, hipStream_t st);
 void launch_absmax(const double* x, size_t n, double* out, hipStream_t st);
 
-void launch_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, hipStream_t st);
+void launch_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, double freeze,
+                     hipStream_t st);
 void launch_tc2_init(int n, const double* F, double* X0, hipStream_t st);
 void launch_tc2_error(int n, const double* X, double* out, hipStream_t st);
 void launch_dh(const DhArgs& a, const long long* n_per_class, hipStream_t st);
@@ -394,11 +438,8 @@ void launch_zero64(long long* p, size_t n, hipStream_t st);
 void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st);
 void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
 void launch_grad_keep(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
-void launch_pair_dmax(const GradArgs& a, double* dmc, double* dmx, hipStream_t st);
+void launch_pair_dmax(const GradArgs& a, const int* shell_l, double* dmc, double* dmx, hipStream_t st);
 void launch_oneel(const OneElArgs& g, double* S, double* T, double* M, double* V, hipStream_t st);
-// device address of the fixed-size scratch array of the allocation-free entry
-// points on the current device (resolved once per device), and its size
-void* noalloc_scratch(size_t* bytes);
 void launch_grad_one(const OneElArgs& g, const double* D, const double* Wneg, long long* out, hipStream_t st);
 void launch_negate(const double* W, size_t n, double* out, hipStream_t st);
 void launch_fx_to_double(const long long* fx, int n, double* out, hipStream_t st);

@@ -121,9 +121,10 @@ __global__ __launch_bounds__(64) void k_count_quartets(PairArgs g, double thr, d
     const double qp = g.qscr[P];
     const int cp = g.pair_cls[P];
     const int np = g.npp[P];
+    const int lo = g.pair_lo ? g.pair_lo[P] : 0;
     for (int i = lane; i < 6 * NB; i += 64) hist[i] = 0;
     __syncthreads();
-    for (int Q = lane; Q <= P; Q += 64) {
+    for (int Q = lo + lane; Q <= P; Q += 64) {
         const double qq = qp * g.qscr[Q];
         if (qq < thr) continue;
         atomicAdd(&hist[g.pair_cls[Q] * NB + (NB == NLBIN ? list_bin(qq, sw, np * g.npp[Q]) : cost_bin(np * g.npp[Q]))],
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(64) void k_fill_quartets(PairArgs g, double thr, do
     }
     __syncthreads();
     const unsigned long long below = (lane ? (~0ull >> (64 - lane)) : 0ull);
-    for (int Q0 = 0; Q0 <= P; Q0 += 64) {
+    for (int Q0 = g.pair_lo ? g.pair_lo[P] : 0; Q0 <= P; Q0 += 64) {
         const int Q = Q0 + lane;
         const double qq = Q <= P ? qp * g.qscr[Q] : 0.0;
         const bool in = Q <= P && !(qq < thr);
@@ -316,25 +317,29 @@ void launch_fill_quartets(const PairArgs& g, double thr, double sw, const long l
 }
 
 // per shell pair (thread per pair): largest |D| over its block, for Dc and each Dx
-__global__ void k_pair_dmax(GradArgs a, double* dmc, double* dmx) {
+__global__ void k_pair_dmax(GradArgs a, const int* shell_l, double* dmc, double* dmx) {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= a.npair) return;
     const int X = a.pair_a[P], Y = a.pair_b[P];
-    const int oX = a.shell_ao[X], nX = a.shell_ao[X + 1] - oX, oY = a.shell_ao[Y], nY = a.shell_ao[Y + 1] - oY;
-    const size_t n = a.nao, nn = n * n;
+    const int lX = shell_l[X], lY = shell_l[Y];
+    const int oX = a.shell_ao[X], nX = (lX + 1) * (lX + 2) / 2, oY = a.shell_ao[Y], nY = (lY + 1) * (lY + 2) / 2;
+    const size_t n = a.ld, nn = n * n;
+    const size_t r = a.shell_rep ? (size_t)a.shell_rep[X] : 0;
+    const double* dc = a.dc + r * nn;
+    const double* dx = a.dx + r * a.nx * nn;
     double m = 0.0, mx[2] = {0.0, 0.0};
     for (int i = 0; i < nX; ++i)
         for (int j = 0; j < nY; ++j) {
             const size_t e = (size_t)(oX + i) * n + oY + j;
-            m = fmax(m, fabs(a.dc[e]));
-            for (int x = 0; x < a.nx; ++x) mx[x] = fmax(mx[x], fabs(a.dx[x * nn + e]));
+            m = fmax(m, fabs(dc[e]));
+            for (int x = 0; x < a.nx; ++x) mx[x] = fmax(mx[x], fabs(dx[x * nn + e]));
         }
     dmc[P] = m;
     for (int x = 0; x < a.nx; ++x) dmx[(size_t)x * a.npair + P] = mx[x];
 }
 
-void launch_pair_dmax(const GradArgs& a, double* dmc, double* dmx, hipStream_t st) {
-    if (a.npair > 0) k_pair_dmax<<<GRID(a.npair, 256), 0, st>>>(a, dmc, dmx);
+void launch_pair_dmax(const GradArgs& a, const int* shell_l, double* dmc, double* dmx, hipStream_t st) {
+    if (a.npair > 0) k_pair_dmax<<<GRID(a.npair, 256), 0, st>>>(a, shell_l, dmc, dmx);
 }
 
 void launch_zero64(long long* p, size_t n, hipStream_t st) {

@@ -47,6 +47,9 @@ constexpr int NAB = Qt::NAB, NCD = Qt::NCD, NB = Qt::NB, ND = Qt::ND;
 constexpr int NCB = grad_ncol<LA, LB, LC, LD>(), NCK = grad_ncol<LC, LD, LA, LB>();
 constexpr int NIB = NCD / NCB, NIK = NAB / NCK;
 constexpr int NT = NIB + NIK;
+// LDS words for the per-atom sums: with the 35 KB Boys slice, a workgroup stays
+// within 64 KB of LDS
+constexpr int GRAD_LDS_WORDS = 3600;
 
 __device__ __forceinline__ void lds_add(long long* lds, int atom, int d, double x) {
     long long h, l;
@@ -55,14 +58,28 @@ __device__ __forceinline__ void lds_add(long long* lds, int atom, int d, double 
     if (l) atomicAdd((unsigned long long*)&lds[2 * (3 * atom + d) + 1], (unsigned long long)l);
 }
 
-__device__ __forceinline__ double gamma_el(const GradArgs& a, int m, int n, int l, int s, double deg) {
-    const size_t N = a.nao;
-    double v = a.dc[m * N + n] * a.dc[l * N + s];
+// Gamma element of AOs m, n, l, s (offsets within replica rep)
+__device__ __forceinline__ double gamma_el(const GradArgs& a, size_t rep, int m, int n, int l, int s, double deg) {
+    const size_t N = a.ld, NN = N * N;
+    const double* C = a.dc + rep * NN;
+    double v = C[m * N + n] * C[l * N + s];
     for (int x = 0; x < a.nx; ++x) {
-        const double* X = a.dx + x * N * N;
+        const double* X = a.dx + (rep * a.nx + x) * NN;
         v -= 0.5 * a.kfac * (X[m * N + l] * X[n * N + s] + X[m * N + s] * X[n * N + l]);
     }
     return v * deg;
+}
+
+// index of shell pair (s, t) (same replica)
+__device__ __forceinline__ int pair_index(const GradArgs& a, int s, int t) {
+    int base = 0;
+    if (a.shell_first) {
+        base = a.shell_pair0[s];
+        const int f = a.shell_first[s];
+        s -= f;
+        t -= f;
+    }
+    return base + (s > t ? s * (s + 1) / 2 + t : t * (t + 1) / 2 + s);
 }
 
 // Screening pre-pass, one thread per quartet (256-quartet blocks): keep[q] = 0
@@ -85,9 +102,9 @@ __global__ __launch_bounds__(256) void k_grad_keep(GradArgs a, const int2* __res
             if (sA != sB) deg *= 2.0;
             if (sC != sD) deg *= 2.0;
             if (pq.x != pq.y) deg *= 2.0;
-            auto pk = [](int s, int t) { return s > t ? s * (s + 1) / 2 + t : t * (t + 1) / 2 + s; };
-            double gb = a.dmc[pk(sA, sB)] * a.dmc[pk(sC, sD)];
-            const int pAC = pk(sA, sC), pBD = pk(sB, sD), pAD = pk(sA, sD), pBC = pk(sB, sC);
+            double gb = a.dmc[pair_index(a, sA, sB)] * a.dmc[pair_index(a, sC, sD)];
+            const int pAC = pair_index(a, sA, sC), pBD = pair_index(a, sB, sD), pAD = pair_index(a, sA, sD),
+                      pBC = pair_index(a, sB, sC);
             for (int x = 0; x < a.nx; ++x) {
                 const double* m = a.dmx + (size_t)x * a.npair;
                 gb += 0.5 * fabs(a.kfac) * (m[pAC] * m[pBD] + m[pAD] * m[pBC]);
@@ -184,9 +201,14 @@ __global__ __launch_bounds__(256) void k_grad_scatter(const unsigned char* __res
 template <int Q, typename Real>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<Real>()))) void k_grad_class(
     GradArgs a, const int2* __restrict__ list, const int* __restrict__ sel, const int* __restrict__ nsel) {
-    extern __shared__ long long lds[];
+    extern __shared__ long long lds_acc[];
     __shared__ double btab[BOYS_NGRID * BOYS_SROW];
-    for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
+    // per-atom sums staged in LDS when they fit next to the Boys slice, else added
+    // to the output directly (both exact integer adds)
+    const bool staged = 6 * a.natm <= GRAD_LDS_WORDS;
+    long long* lds = staged ? lds_acc : a.out;
+    if (staged)
+        for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
     boys_stage(a.boys, LA + LB + LC + LD + 1, btab);
     const long long nitems = (long long)(*nsel) * NT;
     for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
@@ -205,6 +227,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<
         if (sC != sD) deg *= 2.0;
         if (pq.x != pq.y) deg *= 2.0;
         const int oA = a.shell_ao[sA], oB = a.shell_ao[sB], oC = a.shell_ao[sC], oD = a.shell_ao[sD];
+        const size_t rep = a.shell_rep ? (size_t)a.shell_rep[sA] : 0;
         const double* rb = a.recs + a.rec_off[pq.x];
         const double* rk = a.recs + a.rec_off[pq.y];
         double g1[3], g2[3];
@@ -215,7 +238,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<
 #pragma unroll
                 for (int c = 0; c < NCB; ++c)
                     gam[ab * NCB + c] =
-                        Real(gamma_el(a, oA + ab / NB, oB + ab % NB, oC + (kc0 + c) / ND, oD + (kc0 + c) % ND, deg));
+                        Real(gamma_el(a, rep, oA + ab / NB, oB + ab % NB, oC + (kc0 + c) / ND, oD + (kc0 + c) % ND, deg));
             grad_pass<LA, LB, LC, LD, NCB, true, Real>(rb, a.npp[pq.x], rk, a.npp[pq.y], btab, a.pos + 3 * aA,
                                                        a.pos + 3 * aB, kc0, gam, g1, g2);
             for (int d = 0; d < 3; ++d) {
@@ -235,7 +258,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<
 #pragma unroll
                 for (int c = 0; c < NCK; ++c)
                     gam[cd * NCK + c] =
-                        Real(gamma_el(a, oA + (ab0 + c) / NB, oB + (ab0 + c) % NB, oC + cd / ND, oD + cd % ND, deg));
+                        Real(gamma_el(a, rep, oA + (ab0 + c) / NB, oB + (ab0 + c) % NB, oC + cd / ND, oD + cd % ND, deg));
             grad_pass<LC, LD, LA, LB, NCK, true, Real>(rk, a.npp[pq.y], rb, a.npp[pq.x], btab, a.pos + 3 * aC,
                                                        a.pos + 3 * aD, ab0, gam, g1, g2);
             for (int d = 0; d < 3; ++d) {
@@ -250,9 +273,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<
             }
         }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x)
-        if (lds[i]) atomicAdd((unsigned long long*)&a.out[i], (unsigned long long)lds[i]);
+    if (staged) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x)
+            if (lds[i]) atomicAdd((unsigned long long*)&a.out[i], (unsigned long long)lds[i]);
+    }
 }
 
 }  // namespace
@@ -276,7 +301,7 @@ void CAT(launch_grad_class_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list,
     if (items <= 0) return;
     long long blocks = (items + 255) / 256;
     if (blocks > 1024) blocks = 1024;
-    const size_t shm = sizeof(long long) * 6 * a.natm;
+    const size_t shm = 6 * a.natm <= GRAD_LDS_WORDS ? sizeof(long long) * 6 * a.natm : 0;
     k_grad_class<QC, double><<<dim3((unsigned)blocks), dim3(256), shm, st>>>(a, list, a.sel64, a.nsel);
     if (a.fp32_switch > 0.0)
         k_grad_class<QC, float><<<dim3((unsigned)blocks), dim3(256), shm, st>>>(a, list, a.sel32, a.nsel + 1);

@@ -28,6 +28,23 @@ __device__ __forceinline__ ShellView shells(const OneElArgs& g) {
     return ShellView{g.shell_atom, g.shell_l, g.shell_prim, g.shell_ao, g.prim_exp, g.prim_coef, g.pos};
 }
 
+// first element of the matrices of shell s's replica
+__device__ __forceinline__ size_t mat_base(const OneElArgs& g, int s) {
+    return g.shell_rep ? (size_t)g.shell_rep[s] * g.ld * g.ld : 0;
+}
+
+// the nuclei that act on shell s's electrons: its replica's atoms
+__device__ __forceinline__ void nuclei_of(const OneElArgs& g, int s, int& c0, int& c1) {
+    if (g.rep_atom) {
+        const int r = g.shell_rep[s];
+        c0 = g.rep_atom[r];
+        c1 = g.rep_atom[r + 1];
+    } else {
+        c0 = 0;
+        c1 = g.natm;
+    }
+}
+
 template <int LI, int LJ>
 __global__ __launch_bounds__(64) void k_oneel_st(OneElArgs g, const int* pairs, int np, double* S, double* T,
                                                  double* M) {
@@ -39,11 +56,12 @@ __global__ __launch_bounds__(64) void k_oneel_st(OneElArgs g, const int* pairs, 
     double s[NI * NJ] = {}, tk[NI * NJ] = {}, m[3 * NI * NJ] = {};
     const double O[3] = {0.0, 0.0, 0.0};
     overlap_type_acc<LI, LJ>(shells(g), si, sj, O, 0, 1, s, tk, m);
-    const size_t n = g.nao, nn = n * n;
+    const size_t n = g.ld, nn = g.mstride, base = mat_base(g, si);
     const int oi = g.shell_ao[si], oj = g.shell_ao[sj];
 #pragma unroll
     for (int q = 0; q < NI * NJ; ++q) {
-        const size_t u = (size_t)(oi + q / NJ) * n + oj + q % NJ, v = (size_t)(oj + q % NJ) * n + oi + q / NJ;
+        const size_t u = base + (size_t)(oi + q / NJ) * n + oj + q % NJ,
+                     v = base + (size_t)(oj + q % NJ) * n + oi + q / NJ;
         S[u] = S[v] = s[q];
         T[u] = T[v] = tk[q];
 #pragma unroll
@@ -57,15 +75,17 @@ __global__ __launch_bounds__(64) void k_oneel_v(OneElArgs g, const int* pairs, d
     const int si = g.pair_a[P], sj = g.pair_b[P];
     constexpr int NI = NC(LI), NJ = NC(LJ);
     double v[NI * NJ] = {};
-    nuclear_acc<LI, LJ>(shells(g), si, sj, threadIdx.x, 64, g.natm, g.charges, g.pos, g.boys, v);
-    const size_t n = g.nao;
+    int c0, c1;
+    nuclei_of(g, si, c0, c1);
+    nuclear_acc<LI, LJ>(shells(g), si, sj, c0 + (int)threadIdx.x, 64, c1, g.charges, g.pos, g.boys, v);
+    const size_t n = g.ld, base = mat_base(g, si);
     const int oi = g.shell_ao[si], oj = g.shell_ao[sj];
 #pragma unroll
     for (int q = 0; q < NI * NJ; ++q) {
         const double x = wave_sum(v[q]);
         if (threadIdx.x == 0) {
-            V[(size_t)(oi + q / NJ) * n + oj + q % NJ] = x;
-            V[(size_t)(oj + q % NJ) * n + oi + q / NJ] = x;
+            V[base + (size_t)(oi + q / NJ) * n + oj + q % NJ] = x;
+            V[base + (size_t)(oj + q % NJ) * n + oi + q / NJ] = x;
         }
     }
 }
@@ -95,20 +115,24 @@ __global__ __launch_bounds__(256) void k_grad_one_cls(OneElArgs g, const int* pa
         const int P = pairs[t];
         const int si = g.pair_a[P], sj = g.pair_b[P];
         const int ai = g.shell_atom[si], aj = g.shell_atom[sj];
+        const size_t base = mat_base(g, si);
         double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
         if ((g.parts & 1) && si != sj && ai != aj) {
             // d/dR [tr(D T) - tr(W S)]: the overlap term enters with -W
             double gk[3] = {0.0, 0.0, 0.0};
-            ovlkin_grad_acc<LI, LJ>(B, g.nao, si, sj, Wneg, D, lane, 64, gk);
+            ovlkin_grad_acc<LI, LJ>(B, g.ld, si, sj, Wneg + base, D + base, lane, 64, gk);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 gA[d] += gk[d];
                 gB[d] -= gk[d];
             }
         }
-        if (g.parts & 2)
-            nuclear_grad_acc<LI, LJ>(B, g.nao, si, sj, lane, 64, g.natm, g.charges, D, g.boys,
+        if (g.parts & 2) {
+            int c0, c1;
+            nuclei_of(g, si, c0, c1);
+            nuclear_grad_acc<LI, LJ>(B, g.ld, si, sj, c0 + lane, 64, c1, g.charges, D + base, g.boys,
                                      [&](int atom, int d, double x) { fx_add(acc, atom, d, x); }, gA, gB);
+        }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double xa = wave_sum(gA[d]), xb = wave_sum(gB[d]);

@@ -1,8 +1,15 @@
 // Trace-correcting purification step (Niklasson, Phys. Rev. B 66, 155115
 // (2002)) of the SCF density: Xout = X2 if tr X > nocc, else 2 X - X2, with
 // X2 = X X formed by the caller (a dense GEMM). Every block recomputes
-// tr X from the n diagonal elements itself, so one launch needs no
-// reduction pass and no memory clearing (it can be captured in a graph).
+// tr X and tr X2 (= sum X_ij^2 for symmetric X) from the n diagonal elements
+// itself, so one launch needs no reduction pass and no memory clearing (it can
+// be captured in a graph).
+//
+// Past convergence the step is not harmless in floating point: each branch
+// doubles the distance from {0, 1} of the eigenvalues on one side, and a run
+// of steps beyond convergence drives the iterate to overflow. With freeze > 0
+// an iterate whose idempotency error |tr X - tr X2| is below freeze is copied
+// unchanged, so any number of extra steps leaves a converged iterate as it is.
 
 #include <algorithm>
 
@@ -12,19 +19,30 @@ namespace aimdhf {
 
 namespace {
 __global__ __launch_bounds__(256) void k_tc2_step(int n, const double* __restrict__ X, const double* __restrict__ X2,
-                                                  double* __restrict__ Xout, double nocc) {
-    __shared__ double part[4];
-    double t = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) t += X[(size_t)i * n + i];
+                                                  double* __restrict__ Xout, double nocc, double freeze) {
+    __shared__ double part[2][4];
+    double t = 0.0, t2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        t += X[(size_t)i * n + i];
+        t2 += X2[(size_t)i * n + i];
+    }
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) t += __shfl_xor(t, m, 64);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+    for (int m = 32; m >= 1; m >>= 1) {
+        t += __shfl_xor(t, m, 64);
+        t2 += __shfl_xor(t2, m, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = t;
+        part[1][threadIdx.x >> 6] = t2;
+    }
     __syncthreads();
-    const double tr = part[0] + part[1] + part[2] + part[3];
+    const double tr = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+    const double tr2 = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+    const bool hold = freeze > 0.0 && fabs(tr - tr2) < freeze;
     const bool square = tr > nocc;
     const size_t nn = (size_t)n * n;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x)
-        Xout[i] = square ? X2[i] : 2.0 * X[i] - X2[i];
+        Xout[i] = hold ? X[i] : square ? X2[i] : 2.0 * X[i] - X2[i];
 }
 // X0 = (e_max - F) / (e_max - e_min) with the Gershgorin bounds of F (rows i:
 // F_ii -/+ sum_{j != i} |F_ij|), one block: bounds reduced in LDS, then the
@@ -91,10 +109,11 @@ void launch_tc2_error(int n, const double* X, double* out, hipStream_t st) {
     if (n > 0) k_tc2_error<<<dim3(1), dim3(1024), 0, st>>>(n, X, out);
 }
 
-void launch_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, hipStream_t st) {
+void launch_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, double freeze,
+                     hipStream_t st) {
     const size_t nn = (size_t)n * n;
     int blocks = (int)std::min<size_t>((nn + 255) / 256, 256);
-    if (nn) k_tc2_step<<<dim3(blocks), dim3(256), 0, st>>>(n, X, X2, Xout, nocc);
+    if (nn) k_tc2_step<<<dim3(blocks), dim3(256), 0, st>>>(n, X, X2, Xout, nocc, freeze);
 }
 
 }  // namespace aimdhf

@@ -60,6 +60,34 @@ int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const in
 int aimdhf_plan_destroy(aimdhf_plan* plan);
 
 /*
+ * A batched plan: nrep independent molecules (replicas) of any sizes in one plan,
+ * their shells and atoms concatenated replica by replica (rep_atom / rep_shell:
+ * nrep + 1 offsets; every shell on an atom of its own replica), the other arrays
+ * as in aimdhf_plan_create. Shell pairs and quartets form only within a replica
+ * and each replica's nuclei act only on its own electrons, so every replica is
+ * the molecule it would be alone. The entry points then work on all replicas at
+ * once: set_geometry takes the concatenated positions and evaluates every
+ * screened quartet of every replica (one launch per quartet class) into one dense
+ * two-electron tensor per replica; jk digests all tensors in one launch;
+ * one_electron, grad_one(_terms) and grad_two take and fill matrices in the
+ * padded layout: nset (or 1) matrices of ld x ld per replica, replica by replica
+ * ((nrep, nset, ld, ld); ld = the largest replica's AO count, see
+ * aimdhf_batch_layout), rows and columns past a replica's AO count zero; the
+ * gradient stays (natm, 3) over all atoms. Replaces the per-molecule backend
+ * calls that the reference's MD loop makes one replica at a time
+ * (aimd/integrators.py:419-491 via aimd/backends/hf.py:293-363).
+ */
+int aimdhf_batch_create(int nrep, const int64_t* rep_atom, const int64_t* rep_shell, int natm, int nshell,
+                        const int64_t* shell_atom, const int64_t* shell_l, const int64_t* shell_prim,
+                        const int64_t* shell_block, const double* prim_exp, const double* prim_coef,
+                        const double* charges, aimdhf_plan** out);
+
+/* Replicas (0 for a single-molecule plan), padded matrix dimension ld (nao for a
+ * single molecule) and bytes of the dense tensors of a batch (any pointer may be
+ * NULL). No counterpart in the reference. */
+int aimdhf_batch_layout(aimdhf_plan* plan, int* nrep, int* ld, int64_t* tensor_bytes);
+
+/*
  * Move the basis to positions (host, (natm,3) bohr) and build everything that
  * depends on the geometry: primitive-pair records (mu |A-B|^2 <= 50 kept),
  * Schwarz bounds Q = max |(ab|ab)|^(1/2), and the class-binned list of unique
@@ -136,11 +164,14 @@ int aimdhf_set_precision(aimdhf_plan* plan, int mode, double jk_switch, double g
 /*
  * One trace-correcting purification step on device matrices (n x n, row-major):
  * Xout = X2 if tr X > nocc, else 2 X - X2, where X2 = X X (computed by the
- * caller). Needs no plan; issues no memory clearing, so it may be captured in a
- * graph. Building block of the SCF density from the orthogonal-basis Fock matrix,
- * which replaces the eigenvector aufbau of aimd/qc/scf.py:913-921.
+ * caller); with freeze > 0, Xout = X when |tr X - tr X2| < freeze (a converged
+ * iterate is held: further steps would amplify its rounding errors). Needs no
+ * plan; issues no memory clearing, so it may be captured in a graph. Building
+ * block of the SCF density from the orthogonal-basis Fock matrix, which
+ * replaces the eigenvector aufbau of aimd/qc/scf.py:913-921.
  */
-int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, void* stream);
+int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, double nocc, double freeze,
+                    void* stream);
 
 /*
  * Start of the purification: X0 = (e_max - F) / (e_max - e_min) for the
@@ -149,6 +180,27 @@ int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, doub
  * aimdhf_tc2_step). One workgroup; no plan.
  */
 int aimdhf_tc2_init(int n, const double* F, double* X0, void* stream);
+
+/*
+ * Aufbau density projectors of nmat symmetric ld x ld matrices at once (ld <= 64;
+ * device F (nmat, ld, ld), nocc (nmat) doubles): trace-correcting purification
+ * from the Gershgorin start, each matrix in one workgroup's LDS for all steps;
+ * `steps` held steps (an iterate with idempotency error below tol is held), then
+ * two unheld steps. P (nmat, ld, ld) receives the symmetrised projectors, err
+ * (nmat) the idempotency error before the last two steps. No plan; no memory
+ * clearing, so a graph may capture it. The batched counterpart of the aufbau of
+ * aimd/qc/scf.py:913-921 (there: eigenvectors of one Fock matrix at a time).
+ */
+int aimdhf_batch_purify(int nmat, int ld, const double* F, const double* nocc, double* P, int steps, double tol,
+                        double* err, void* stream);
+
+/*
+ * Pulay DIIS coefficients of nrep independent problems (the solve of
+ * aimd/qc/scf.py:540-560 for many SCFs at once): device G (nrep, m, m) and r
+ * (nrep, m), c (nrep, m) = the solution of G c = r by Gaussian elimination with
+ * partial pivoting, unknowns of singular pivots 0. m <= 12. No plan; capturable.
+ */
+int aimdhf_batch_diis_solve(int nrep, int m, const double* G, const double* r, double* c, void* stream);
 
 /* err[0] (device) = |tr X - sum_ij X_ij^2|, the idempotency error of a symmetric
  * purification iterate X (n x n, device). One workgroup; no plan. */

@@ -4,11 +4,19 @@ the GPU tests compare against them without rerunning the CPU oracle:
 
     (H2O)16 RHF/6-31G*  (water16.xyz)  -> oracle_water16_631gs.json
     (H2O)8  RHF/cc-pVDZ (water8.xyz)   -> oracle_water8_ccpvdz.json
+    (H2O)32 RHF/6-31G*  (water32.xyz)  -> oracle_water32_631gs.json
 
 SCF thresholds 1e-11 Eh / 1e-9, Schwarz 1e-12, gradient screening 1e-14,
 SAD guess, integral-direct J/K.
 
-    python tests/golden/make_oracle_fixtures.py
+and the opening steps of velocity-Verlet NVE trajectories driven by the oracle
+(0.5 fs, 300 K Maxwell-Boltzmann velocities from seed 1, COM removed; SCF
+1e-10 Eh / 1e-8, each SCF from the previous step's density):
+
+    (H2O)8  RHF/cc-pVDZ, 20 steps -> oracle_nve_water8_ccpvdz.json
+    (H2O)16 RHF/6-31G*,   5 steps -> oracle_nve_water16_631gs.json
+
+    python tests/golden/make_oracle_fixtures.py [name ...]
 """
 
 import json
@@ -46,7 +54,50 @@ def run(xyz, basis, out):
     print(out, st.energy, st.iterations, rec["seconds"])
 
 
+def nve(xyz, basis, steps, out):
+    """Positions, potential and total energies of an oracle-driven NVE trajectory."""
+    sys.path.insert(0, os.path.join(ROOT, "ab-initio-md_amd"))
+    from aimd_amd.md import VelocityVerletNVE, masses_au, maxwell_boltzmann
+    sym, x = read_xyz(os.path.join(HERE, xyz))
+    m = masses_au(sym)
+    v = maxwell_boltzmann(m, 300.0, seed=1)
+
+    class OracleBackend:
+        def __init__(self):
+            self.P = None
+
+        def compute(self, y):
+            from types import SimpleNamespace
+            st, g = O.energy_and_gradient(sym, y, basis, conv_energy=1e-10, conv_commutator=1e-8,
+                                          guess=self.P if self.P is not None else "sad", stored=False)
+            self.P = 2 * st.Ds[0]
+            return SimpleNamespace(energy=st.energy, gradient=g)
+
+    t = time.time()
+    md = VelocityVerletNVE(OracleBackend(), m, x, v, 0.5)
+    md.initialize()
+    rec = {"geometry": xyz, "basis": basis, "timestep_fs": 0.5, "temperature_k": 300.0, "seed": 1,
+           "positions": [md.x.tolist()], "energy": [md.result.energy], "total_energy": [md.total_energy()]}
+    for k in range(steps):
+        md.step()
+        rec["positions"].append(md.x.tolist())
+        rec["energy"].append(md.result.energy)
+        rec["total_energy"].append(md.total_energy())
+        print(out, k + 1, md.result.energy, md.total_energy(), time.time() - t, flush=True)
+    rec["seconds"] = time.time() - t
+    with open(os.path.join(HERE, out), "w") as f:
+        json.dump(rec, f, indent=1)
+
+
+JOBS = {
+    "water8": lambda: run("water8.xyz", "cc-pvdz", "oracle_water8_ccpvdz.json"),
+    "water16": lambda: run("water16.xyz", "6-31g*", "oracle_water16_631gs.json"),
+    "water32": lambda: run("water32.xyz", "6-31g*", "oracle_water32_631gs.json"),
+    "nve_water8": lambda: nve("water8.xyz", "cc-pvdz", 20, "oracle_nve_water8_ccpvdz.json"),
+    "nve_water16": lambda: nve("water16.xyz", "6-31g*", 5, "oracle_nve_water16_631gs.json"),
+}
+
 if __name__ == "__main__":
     O.set_threads(os.cpu_count())
-    run("water8.xyz", "cc-pvdz", "oracle_water8_ccpvdz.json")
-    run("water16.xyz", "6-31g*", "oracle_water16_631gs.json")
+    for name in sys.argv[1:] or list(JOBS):
+        JOBS[name]()

@@ -172,7 +172,7 @@ def test_purification_replay_count_follows_the_gap_down():
     Fh, ref_h = fock(1e-3)
     P = S.purified_projector_device(Fh, nocc)
     np.testing.assert_allclose(P.cpu().numpy(), ref_h, atol=1e-9)
-    g = S._TC2_GRAPHS[(n, nocc, Fh.device)]
+    g = S._TC2_GRAPHS[(n, nocc, Fh.device, 1e-11)]
     hard = g.needed
     Fe, ref_e = fock(5.0)
     counts = []
