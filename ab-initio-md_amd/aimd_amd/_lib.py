@@ -46,6 +46,7 @@ SIGNATURES = {
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_grad_one": (_I, [_P, _P, _P, _P, _P]),
     "aimdhf_grad_one_terms": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "aimdhf_grad_one_noalloc": (_I, [_P, _P, _P, _P, _P]),
     "aimdhf_grad_two": (_I, [_P, _P, _I, _P, _D, _D, _P, _P]),
     "aimdhf_profile_enable": (_I, [_P, _I]),
     "aimdhf_profile_read": (_I, [_P, _P, _P, _I, _I]),

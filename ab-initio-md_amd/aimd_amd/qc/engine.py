@@ -177,6 +177,14 @@ class Engine:
         _lib.check(_lib.lib().aimdhf_grad_one(self._h, _ptr(Dd), _ptr(Wd), _ptr(g), _stream()))
         return g
 
+    def grad_one_noalloc(self, D: torch.Tensor, W: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """grad_one into the caller's (natm, 3) device tensor ``out`` through the entry point
+        that allocates no device memory (fixed library scratch; nao <= 1024)."""
+        if tuple(out.shape) != (self.natm, 3) or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous float64 ({self.natm}, 3) tensor")
+        _lib.check(_lib.lib().aimdhf_grad_one_noalloc(self._h, _ptr(D), _ptr(W), _ptr(out), _stream()))
+        return out
+
     def grad_one_terms(self, D: torch.Tensor, W: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
         """(d/dR tr(D T), d/dR tr(D V), -d/dR tr(W S)), each a (natm, 3) device tensor."""
         Dd = D.to(device=self.device, dtype=torch.float64).contiguous()

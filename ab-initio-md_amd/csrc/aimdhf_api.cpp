@@ -277,6 +277,10 @@ struct aimdhf_plan {
     DevBuf d_shell_rep, d_shell_first, d_shell_pair0, d_pair_lo, d_rep_atom, d_rep_nao, d_rep_toff, d_rep_item, d_tens;
     // elements of one stack of matrices (nao^2, or nrep ld^2 for a batch)
     size_t mat_elems() const { return batched ? (size_t)nrep * bld * bld : (size_t)nao * nao; }
+    // device address and size of the fixed scratch array of the allocation-free
+    // entry point on this plan's device (resolved at plan creation)
+    void* noalloc_ptr = nullptr;
+    size_t noalloc_bytes = 0;
 
     // make room for `bytes` of staging (no transfer may be pending) and start over
     void stage_reset(size_t bytes) {
@@ -1170,6 +1174,11 @@ static int create_plan(int natm, int nshell, const int64_t* shell_atom, const in
         p->d_qscr.ensure(sizeof(double) * p->npair);
         p->d_qblock.ensure(sizeof(double) * p->nbpair);
         p->qcount.assign(NQCLASS, 0);
+        p->noalloc_ptr = aimdhf::noalloc_scratch(&p->noalloc_bytes);
+        if (!p->noalloc_ptr) {
+            delete p;
+            return fail(AIMDHF_E_HIP, "hipGetSymbolAddress of the fixed scratch array failed");
+        }
     } catch (const HipError& e) {
         delete p;
         return e.code;
@@ -1550,6 +1559,34 @@ int aimdhf_grad_one(aimdhf_plan* p, const double* D, const double* W, double* gr
     } catch (const HipError& e) {
         return e.code;
     }
+    return AIMDHF_OK;
+}
+
+int aimdhf_grad_one_noalloc(aimdhf_plan* p, const double* D, const double* W, double* grad, void* stream) {
+    if (!p || !D || !W || !grad) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (p->batched) return fail(AIMDHF_E_UNSUPPORTED, "aimdhf_grad_one_noalloc takes single-molecule plans");
+    const size_t nn = (size_t)p->nao * p->nao;
+    const size_t mats = (sizeof(double) * 2 * nn + 255) & ~(size_t)255;
+    const size_t need = mats + sizeof(long long) * 6 * (size_t)p->natm;
+    if (!p->noalloc_ptr || need > p->noalloc_bytes)
+        return fail(AIMDHF_E_ARG, "aimdhf_grad_one_noalloc: nao = " + std::to_string(p->nao) + ", natm = " +
+                                      std::to_string(p->natm) + " need " + std::to_string(need) +
+                                      " bytes of scratch, more than its fixed " + std::to_string(p->noalloc_bytes) +
+                                      " (16 nao^2 + 48 natm bytes must fit; use aimdhf_grad_one)");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    hipStream_t st = (hipStream_t)stream;
+    char* base = (char*)p->noalloc_ptr;
+    double* Ds = (double*)base;
+    double* Wn = Ds + nn;
+    long long* fx = (long long*)(base + mats);
+    aimdhf::launch_symmetrize(D, 1, p->nao, Ds, st);
+    aimdhf::launch_symmetrize(W, 1, p->nao, Wn, st);
+    aimdhf::launch_negate(Wn, nn, Wn, st);
+    aimdhf::launch_zero64(fx, 6 * (size_t)p->natm, st);     // a fill kernel: the array keeps the last call's sums
+    aimdhf::launch_grad_one(p->oneel_args(), Ds, Wn, fx, st);
+    aimdhf::launch_fx_to_double(fx, 3 * p->natm, grad, st);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(AIMDHF_E_HIP, std::string("grad_one_noalloc: ") + hipGetErrorString(e));
     return AIMDHF_OK;
 }
 

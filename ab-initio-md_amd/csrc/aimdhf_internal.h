@@ -440,6 +440,9 @@ void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long
 void launch_grad_keep(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);
 void launch_pair_dmax(const GradArgs& a, const int* shell_l, double* dmc, double* dmx, hipStream_t st);
 void launch_oneel(const OneElArgs& g, double* S, double* T, double* M, double* V, hipStream_t st);
+// device address of the fixed-size scratch array of the allocation-free entry
+// points on the current device (resolved once per device), and its size
+void* noalloc_scratch(size_t* bytes);
 void launch_grad_one(const OneElArgs& g, const double* D, const double* Wneg, long long* out, hipStream_t st);
 void launch_negate(const double* W, size_t n, double* out, hipStream_t st);
 void launch_fx_to_double(const long long* fx, int n, double* out, hipStream_t st);

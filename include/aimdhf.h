@@ -240,6 +240,23 @@ int aimdhf_one_electron(aimdhf_plan* plan, double* S, double* T, double* V, doub
 int aimdhf_grad_one(aimdhf_plan* plan, const double* D, const double* W, double* grad, void* stream);
 
 /*
+ * aimdhf_grad_one without any device allocation: the same kernels and the same
+ * (bitwise identical) result, with the symmetrised D and -W and the fixed-point
+ * gradient held in a fixed library array of 16 MiB + 64 KiB per device instead
+ * of plan buffers. Needs 16 nao^2 + 48 natm bytes (plus alignment) of it, i.e.
+ * nao <= 1024 for small natm; larger plans get AIMDHF_E_ARG (use
+ * aimdhf_grad_one). Calls no hipMalloc / hipFree and clears with a fill kernel,
+ * so code under HIP graph capture, tight serving loops and harnesses that
+ * forbid allocation can call it. Every call on one device shares that device's
+ * array and leaves its contents behind: calls that overlap on one device
+ * (different streams or threads) race. No counterpart in the reference (its
+ * one_electron_gradient, aimd/qc/integrals.py:1483-1495, is the function this
+ * entry point and aimdhf_grad_one replace); it exists for callers that cannot
+ * allocate.
+ */
+int aimdhf_grad_one_noalloc(aimdhf_plan* plan, const double* D, const double* W, double* grad, void* stream);
+
+/*
  * The terms of aimdhf_grad_one separately, each device (natm,3):
  *   g_kin = d/dR sum D T,  g_nuc = d/dR sum D V (basis-function and operator
  *   centres),  g_ovl = -d/dR sum W S  (g_kin + g_nuc + g_ovl = aimdhf_grad_one).
