@@ -41,6 +41,8 @@ SIGNATURES = {
     "aimdhf_set_streams": (_I, [_P, _I]),
     "aimdhf_tc2_step": (_I, [_I, _P, _P, _P, _D, _D, _P]),
     "aimdhf_tc2_init": (_I, [_I, _P, _P, _P]),
+    "aimdhf_batch_purify": (_I, [_I, _I, _P, _P, _P, _I, _D, _P, _P]),
+    "aimdhf_batch_diis_solve": (_I, [_I, _I, _P, _P, _P, _P]),
     "aimdhf_tc2_error": (_I, [_I, _P, _P, _P]),
     "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
