@@ -74,7 +74,7 @@ class ReplicaBatch:
                  schwarz_threshold: float = qengine.DEFAULT_SCHWARZ,
                  gradient_screening: float = qengine.DEFAULT_SCHWARZ_GRAD, precision: str = "fp64",
                  jk_fp32_switch: float = 1e-5, grad_fp32_switch: float = 1e-5, purify_steps: int = 60,
-                 lindep_threshold: float = 1e-7) -> None:
+                 lindep_threshold: float = 1e-7, capture: bool = True) -> None:
         if not molecules:
             raise ValueError("a batch needs at least one molecule")
         if not 2 <= diis_space <= 12:
@@ -84,6 +84,7 @@ class ReplicaBatch:
         self.max_cycles, self.diis_space, self.purify_steps = int(max_cycles), int(diis_space), int(purify_steps)
         self.schwarz, self.gradient_screening = float(schwarz_threshold), float(gradient_screening)
         self.lindep = float(lindep_threshold)
+        self.capture = bool(capture)            # replay the SCF iteration as one captured graph
         self.molecules = [(list(s), int(c), int(m)) for s, c, m in molecules]
         self.bases: list[BasisSet] = []
         cache: dict[tuple, BasisSet] = {}
@@ -212,7 +213,7 @@ class ReplicaBatch:
         self.e_nuc = np.array([nuclear_repulsion_energy(self.charges[a:b], xc[a:b])
                                for a, b in zip(self.rep_atom[:-1], self.rep_atom[1:])])
 
-    def _inv_sqrt(self, S: torch.Tensor, iters: int = 60) -> torch.Tensor:
+    def _inv_sqrt(self, S: torch.Tensor, iters: int = 30) -> torch.Tensor:
         """S^(-1/2) of every replica by the coupled Newton-Schulz iteration (batched products);
         the residual |X S X - 1| is checked once per geometry on the host."""
         eye = torch.eye(self.ld, dtype=S.dtype, device=S.device).expand_as(S)
@@ -271,9 +272,10 @@ class ReplicaBatch:
         self.rhs.copy_(rs)
         _lib.check(L.aimdhf_batch_diis_solve(R, M, _ptr(self.G), _ptr(self.rhs), _ptr(self.coef), _stream()))
         c = torch.where(valid[None, :], self.coef, torch.zeros_like(self.coef))
-        c = c + torch.nn.functional.one_hot(self.slot, M).to(c.dtype) * (1.0 - c.sum(dim=1, keepdim=True))
+        anchor = (self.slots[None, :] == self.slot[:, None]).to(c.dtype)                # (1, M)
+        c = c + anchor * (1.0 - c.sum(dim=1, keepdim=True))
         bad = ~(torch.isfinite(c).all(dim=1) & (c.abs().amax(dim=1) <= 1e4))
-        c = torch.where(bad[:, None], torch.nn.functional.one_hot(self.slot, M).to(c.dtype).expand(R, M), c)
+        c = torch.where(bad[:, None], anchor.expand(R, M), c)
         Fx = (c[:, :, None, None, None] * self.Fbuf).sum(dim=1)                         # (R, 2, n, n)
         Fx = Fx + PAD_LEVEL * self.pad_diag[:, None]
         _lib.check(L.aimdhf_batch_purify(R * 2, n, _ptr(Fx.contiguous()), _ptr(self.nocc_t), _ptr(self.Pbuf),
@@ -287,6 +289,9 @@ class ReplicaBatch:
         self.count.copy_(torch.clamp(self.count + 1, max=M))
 
     def _replay(self) -> None:
+        if not self.capture:
+            self._iteration()
+            return
         if self._graph is None:
             side = torch.cuda.Stream(device=self.device)
             side.wait_stream(torch.cuda.current_stream(self.device))

@@ -239,7 +239,7 @@ class AuxiliaryDensity:
         if tuple(density.shape) != tuple(h.shape[1:]):
             raise ValueError(f"SCF density shape {tuple(density.shape)} != auxiliary density shape {tuple(h.shape[1:])}")
         if torch is not None and isinstance(h, torch.Tensor):
-            d = density.detach().to(h.device, torch.float64)
+            d = torch.as_tensor(density, dtype=torch.float64, device=h.device)
             w = torch.as_tensor(self.c, dtype=torch.float64, device=h.device)
             damp = torch.tensordot(w, h, dims=([0], [0]))
             nxt = 2.0 * h[0] - h[1] + self.kappa * (d - h[0]) + self.alpha * damp
@@ -287,7 +287,7 @@ class XLBOMD(VelocityVerlet):
 
     def _density(self, result):
         dev = getattr(self.backend, "device_density", None)
-        if dev is not None:
+        if dev is not None and getattr(self.backend, "last_scf", None) is not None:
             return dev()
         if result.density is None:
             raise RuntimeError("the backend returned no SCF density, which XL-BOMD needs")

@@ -929,7 +929,7 @@ struct aimdhf_plan {
             for (int k = 0; k < NQCLASS; ++k) {
                 if (!qcount[k]) continue;
                 sa.store = d_store.as<double>() + store_base[k];
-                aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], qcount[k], st);
+                aimdhf::launch_eri_store(k, sa, list_of_class(k), qcount[k], q64count[k], st);
             }
         });
         prof.time(15, st, [&] {

@@ -86,6 +86,9 @@ MD_HD inline double md_rsqrt(double x) {
     double y = __builtin_amdgcn_rsq(x);
 #else
     double y = 1.0 / sqrt(x);
+#ifdef MD_HOST_RSQ_ESTIMATE
+    y = MD_HOST_RSQ_ESTIMATE(x, y);     // host checks: model of the hardware estimate's error
+#endif
 #endif
     const double hx = 0.5 * x;
     y = fma(y, fma(-hx * y, y, 0.5), y);
@@ -195,10 +198,19 @@ MD_HD void boys_all_slice(Real T, const double* __restrict__ tab, Real* F) {
         // square root in both this branch and quartet_factors, gfx950 builds
         // of the derivative kernels gave force errors ~5e-5 that neither use
         // alone shows (and the host build of the same source does not)
+#ifdef MD_HOST_LARGE_T_RSQ
+        const Real rt = md_rsqrt(T);                // host checks of that variant's arithmetic
+        Real f = Real(0.5 * 1.7724538509055159) * rt;
+#else
         Real f = Real(0.5) * md_sqrt(Real(MD_PI) / T);
+#endif
         if constexpr (L > 0) {
             ex = md_exp(-T);
+#ifdef MD_HOST_LARGE_T_RSQ
+            const Real inv2x = Real(0.5) * rt * rt;
+#else
             const Real inv2x = Real(0.5) / T;
+#endif
             for (int m = 0; m < L; ++m) f = (Real(2 * m + 1) * f - ex) * inv2x;
         }
         F[L] = f;

@@ -25,6 +25,15 @@ oracle's integral-direct J/K build and ERI gradient for the same geometry on
 the host cores, scaled to a step with the measured mean SCF iteration count).
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--precision both|fp64|mixed] [--no-cpu-baseline]
+
+Two further workloads print their own JSON line (not the default bench line):
+
+    --workload water32-xlbomd   (H2O)32 RHF/6-31G* (608 AOs) XL-BOMD (K=5) NVE at 0.5 fs: the FP64
+                                trajectory, then one mixed-precision trajectory per FP32 Schwarz
+                                switch (1e-5, 1e-7, 1e-9), each step's energy and forces compared
+                                with an FP64 SCF at the same geometry; drift of every trajectory
+    --workload replicas         128 replicas of water / methanol / H4 / methyl radical (UHF) in
+                                6-31G**, Langevin BAOAB in one batched plan, FP64 and mixed
 """
 
 from __future__ import annotations
@@ -57,11 +66,19 @@ def main() -> int:
                     help="keep every launch on one stream in the timed region too (for kernel traces)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--workload", choices=["water16", "water32-xlbomd", "replicas"], default="water16")
+    ap.add_argument("--md-steps", type=int, default=0,
+                    help="trajectory length of the water32-xlbomd (default 200) and replicas (default 100) workloads")
+    ap.add_argument("--replicas", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the engine has no multi-GPU path "
               "in this build", file=sys.stderr)
         return 2
+    if args.workload == "water32-xlbomd":
+        return bench_water32_xlbomd(args)
+    if args.workload == "replicas":
+        return bench_replicas(args)
 
     import numpy as np
     import torch
@@ -241,6 +258,167 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sym, md.x, np.mean(iters), args.cpu_threads, be.last_scf.density)
+    print(json.dumps(out))
+    return 0
+
+
+def _progress(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _drift(e, dt_fs):
+    """(straight-line drift in Eh/ps, max |E - E0|) of a total-energy series."""
+    import numpy as np
+    e = np.asarray(e, dtype=float)
+    t_ps = np.arange(len(e)) * dt_fs * 1e-3
+    return float(np.polyfit(t_ps, e, 1)[0]), float(np.abs(e - e[0]).max())
+
+
+def bench_water32_xlbomd(args):
+    """(H2O)32 RHF/6-31G* XL-BOMD: FP64 trajectory and the FP32-switch sweep (module docstring)."""
+    import numpy as np
+    import torch
+    from aimd_amd import md
+    from aimd_amd.backends import get_backend
+    from aimd_amd.units import ANG_TO_BOHR
+
+    steps = args.md_steps or 200
+    sym, xyz = md.read_xyz(os.path.join(HERE, "tests", "golden", "water32.xyz"))
+    x0 = xyz * ANG_TO_BOHR
+    v0 = md.maxwell_boltzmann(md.masses_au(sym), 300.0, seed=1)
+
+    def backend(precision, switch=1e-5):
+        opts = {"precision": precision, "jk_fp32_switch": switch, "grad_fp32_switch": switch}
+        return get_backend("hf")(sym, basis="6-31g*", conv_tol=1e-10, conv_tol_grad=1e-7, scf_options=opts)
+
+    def trajectory(be, label, check=None):
+        sysm = md.System.create(sym, x0, v0)
+        xl = md.XLBOMD(be, 0.5, k=5)
+        xl.initialize(sysm)
+        e_tot, de, dfm, its = [xl.conserved_energy(sysm)], [], [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t_check = 0.0
+        for k in range(steps):
+            r = xl.step(sysm)
+            e_tot.append(xl.conserved_energy(sysm))
+            its.append(r.info["scf_iterations"])
+            if check is not None:
+                tc = time.perf_counter()
+                check.set_density_guess(r.density)
+                ref = check.compute(sysm.positions)
+                de.append(abs(r.energy - ref.energy))
+                dfm.append(float(np.abs(r.gradient - ref.gradient).max()))
+                t_check += time.perf_counter() - tc
+            if (k + 1) % 20 == 0:
+                _progress(f"{label}: step {k + 1}/{steps}, E_tot {e_tot[-1]:.8f}")
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0 - t_check) / steps
+        drift, span = _drift(e_tot, 0.5)
+        out = {"ms_per_step": round(ms, 2), "drift_Eh_per_ps": drift, "max_abs_dEtot_Eh": span,
+               "scf_iterations_mean": float(np.mean(its))}
+        if check is not None:
+            out.update({"max_abs_dE_Eh": max(de), "max_abs_dF_Eh_bohr": max(dfm),
+                        "within_bounds": bool(max(de) <= 1e-6 and max(dfm) <= 1e-5)})
+        return out
+
+    fp64 = trajectory(backend("fp64"), "fp64")
+    check = backend("fp64")
+    sweep = {}
+    for sw in (1e-5, 1e-7, 1e-9):
+        sweep[f"{sw:g}"] = trajectory(backend("mixed", sw), f"mixed {sw:g}", check)
+    best = min((v["ms_per_step"], k) for k, v in sweep.items() if v["within_bounds"]) if any(
+        v["within_bounds"] for v in sweep.values()) else None
+    out = {"metric": "ms per XL-BOMD step (SCF+forces), (H2O)32/6-31G*: FP64 vs mixed FP32-switch sweep",
+           "value": best[0] if best else fp64["ms_per_step"], "unit": "ms", "n_gpus": 1, "steps": steps,
+           "higher_is_better": False, "dtype": "f64+f32 (mixed)" if best else "f64", "data": "synthetic",
+           "config": {"workload": "(H2O)32 RHF/6-31G* (608 AOs), XL-BOMD K=5 NVE at 0.5 fs from 300 K "
+                                  "Maxwell-Boltzmann velocities (seed 1), SCF 1e-10 Eh / 1e-7; per mixed step an "
+                                  "FP64 SCF at the same geometry gives the energy and force error",
+                      "geometry": "tests/golden/water32.xyz", "best_switch": best[1] if best else None},
+           "fp64": fp64, "mixed_sweep": sweep}
+    print(json.dumps(out))
+    return 0
+
+
+def bench_replicas(args):
+    """Batched replicas under Langevin BAOAB (module docstring)."""
+    import numpy as np
+    import torch
+    from aimd_amd.batch import BatchLangevinBAOAB, ReplicaBatch
+    from aimd_amd.md import maxwell_boltzmann, read_xyz
+    from aimd_amd.units import ANG_TO_BOHR
+
+    steps = args.md_steps or 100
+    kinds = [("water.xyz", 0, 1), ("methanol.xyz", 0, 1), ("h4_cluster.xyz", 0, 1), ("methyl_radical.xyz", 0, 2)]
+    rng = np.random.default_rng(1)
+    pick = rng.integers(0, len(kinds), args.replicas)
+    mols, xs = [], []
+    for k in pick:
+        f, c, mult = kinds[k]
+        s, x = read_xyz(os.path.join(HERE, "tests", "golden", "replicas", f))
+        mols.append((s, c, mult))
+        xs.append((x + rng.normal(0.0, 0.05, x.shape)) * ANG_TO_BOHR)
+    x0 = np.concatenate(xs)
+    results = {}
+    geoms = None
+    for precision in ("fp64", "mixed"):
+        batch = ReplicaBatch(mols, basis="6-31g**", precision=precision)
+        dev = batch.device
+        v0 = np.concatenate([maxwell_boltzmann(batch.masses[a:b], 300.0, seed=int(r))
+                             for r, (a, b) in enumerate(zip(batch.rep_atom[:-1], batch.rep_atom[1:]))])
+        if precision == "fp64":
+            integ = BatchLangevinBAOAB(batch, 0.5, 300.0, friction_per_fs=0.001, seed=1)
+            integ.initialize(torch.tensor(x0, device=dev), torch.tensor(v0, device=dev))
+            for _ in range(max(args.warmup, 1)):
+                integ.step()
+            torch.cuda.synchronize()
+            geoms, energies, grads, iters = [], [], [], []
+            t0 = time.perf_counter()
+            for k in range(steps):
+                r = integ.step()
+                geoms.append(integ.x.cpu().numpy())
+                energies.append(r["energy"])
+                grads.append(r["gradient"])
+                iters.append(r["iterations"])
+                if (k + 1) % 20 == 0:
+                    _progress(f"replicas fp64: step {k + 1}/{steps}")
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            results["fp64"] = {"ms_per_step": round(1e3 * wall / steps, 3),
+                               "replica_steps_per_s": round(batch.nrep * steps / wall, 1),
+                               "scf_iterations_mean": float(np.mean(iters))}
+        else:
+            batch.compute(x0)
+            torch.cuda.synchronize()
+            de = np.zeros(batch.nrep)
+            dfm = np.zeros(batch.nrep)
+            t0 = time.perf_counter()
+            for k, g in enumerate(geoms):
+                r = batch.compute(g)
+                de = np.maximum(de, np.abs(r["energy"] - energies[k]))
+                d = np.abs(r["gradient"] - grads[k]).max(axis=1)
+                dfm = np.maximum(dfm, [d[a:b].max() for a, b in zip(batch.rep_atom[:-1], batch.rep_atom[1:])])
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            results["mixed"] = {"ms_per_step": round(1e3 * wall / steps, 3),
+                                "replica_steps_per_s": round(batch.nrep * steps / wall, 1),
+                                "max_abs_dE_Eh": float(de.max()), "max_abs_dF_Eh_bohr": float(dfm.max()),
+                                "per_replica_max_abs_dE_Eh": [float(v) for v in de],
+                                "within_bounds": bool(de.max() <= 1e-6 and dfm.max() <= 1e-5)}
+        batch.close()
+    use = results["mixed"] if results["mixed"]["within_bounds"] else results["fp64"]
+    counts = {f: int((pick == i).sum()) for i, (f, _, _) in enumerate(kinds)}
+    out = {"metric": f"replica-steps per second, {args.replicas} replicas, 6-31G**, Langevin BAOAB",
+           "value": use["replica_steps_per_s"], "unit": "replica-steps/s", "n_gpus": 1, "steps": steps,
+           "higher_is_better": True, "dtype": "f64+f32 (mixed)" if use is results["mixed"] else "f64",
+           "data": "synthetic",
+           "config": {"workload": f"{args.replicas} replicas drawn by seed 1 from water / methanol / H4 / methyl "
+                                  "radical (UHF doublet, others RHF), 6-31G**, iid 0.05 A displacements, 300 K "
+                                  "Maxwell-Boltzmann velocities, BAOAB gamma 1/ps, 0.5 fs; one batched plan; "
+                                  "mixed computed on the FP64 trajectory's geometries",
+                      "replicas_per_kind": counts},
+           "fp64": results["fp64"], "mixed": results["mixed"]}
     print(json.dumps(out))
     return 0
 

@@ -34,7 +34,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(HERE, "liboracle.so")
+# AIMD_ORACLE_LIB selects another build of the same source (the sanitizer build)
+_LIB_PATH = os.environ.get("AIMD_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 _DATA = os.path.join(os.path.dirname(HERE), "ab-initio-md_amd", "aimd_amd", "qc", "basis_data.json")
 
 _lib = None

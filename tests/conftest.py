@@ -64,10 +64,19 @@ def oracle_lib():
     return O
 
 
-@pytest.fixture(scope="session")
-def harness():
-    lib = os.path.join(ROOT, "tests", "harness", "libhostcheck.so")
+def _harness_lib(name):
+    lib = os.path.join(ROOT, "tests", "harness", name)
     if not _have(lib):
-        subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "harness"), "-j8"], check=True)
+        subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "harness"), "-j8", name], check=True)
     from tests.harness import hostcheck
     return hostcheck.Harness(lib)
+
+
+@pytest.fixture(scope="session")
+def harness():
+    return _harness_lib("libhostcheck.so")
+
+
+@pytest.fixture(scope="session")
+def harness_rsq():
+    return _harness_lib("libhostcheck_rsq.so")

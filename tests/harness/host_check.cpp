@@ -1,128 +1,15 @@
 // Test harness (CPU only): runs the engine's own kernel arithmetic -- the
 // __host__ __device__ templates of md_core.h / md_pairs.h / md_eri.h /
-// md_digest.h -- serially on the host, with the same pair orientation,
-// record layout, screening and fixed-point J/K accumulation as the gfx950
-// path. The CPU test suite compares it with the oracle, which checks the
-// kernels' mathematics without a GPU. Not part of the product.
+// md_digest.h / md_grad.h / md_oneel.h -- serially on the host, with the same
+// pair orientation, record layout, screening and fixed-point accumulation as
+// the gfx950 path. The CPU test suite compares it with the oracle, which
+// checks the kernels' mathematics without a GPU. Not part of the product.
+// The device kernel's own derivative work items (column groups, staged Boys
+// slice, FP64 and FP32) live in host_check_dev.cpp.
 
-#include "md_digest.h"
-#include "md_grad.h"
-#include "md_oneel.h"
-#include "md_pairs.h"
-
-#include <algorithm>
-#include <cmath>
-#include <cstring>
-#include <vector>
-
-using namespace md;
+#include "host_check_common.h"
 
 namespace {
-
-struct Basis {
-    int nshell, natm;
-    const long long *atom, *l, *prim, *block;
-    const double *ex, *coef, *pos;
-};
-
-struct State {
-    int npair, nao;
-    std::vector<int> pa, pb, cls, ao, npp;
-    std::vector<long long> off;
-    std::vector<int> rs;
-    std::vector<double> recs, qshell;
-};
-
-std::vector<double> boys_tab() {
-    const int NG = 402, NR = 43, top = NR - 1;
-    std::vector<double> tab((size_t)NG * NR), col(NR);
-    for (int k = 0; k < NG; ++k) {
-        double x = k * 0.1, term = 1.0 / (2 * top + 1), total = term;
-        for (int m = 1; m < 400; ++m) {
-            term = term * (2.0 * x) / (double)(2 * top + 2 * m + 1);
-            total += term;
-        }
-        double e = std::exp(-x);
-        col[top] = e * total;
-        for (int n = top - 1; n >= 0; --n) col[n] = (2.0 * x * col[n + 1] + e) / (double)(2 * n + 1);
-        for (int n = 0; n < NR; ++n) tab[(size_t)k * NR + n] = col[n];
-    }
-    return tab;
-}
-
-const std::vector<double>& TAB() {
-    static std::vector<double> t = boys_tab();
-    return t;
-}
-
-void build(const Basis& B, State& S) {
-    S.ao.assign(B.nshell + 1, 0);
-    for (int s = 0; s < B.nshell; ++s) S.ao[s + 1] = S.ao[s] + NC((int)B.l[s]);
-    S.nao = S.ao[B.nshell];
-    S.npair = B.nshell * (B.nshell + 1) / 2;
-    S.pa.resize(S.npair); S.pb.resize(S.npair); S.cls.resize(S.npair); S.npp.resize(S.npair);
-    S.off.resize(S.npair); S.rs.resize(S.npair); S.qshell.resize(S.npair);
-    long long off = 0;
-    int P = 0;
-    for (int s1 = 0; s1 < B.nshell; ++s1)
-        for (int s2 = 0; s2 <= s1; ++s2, ++P) {
-            int a = s1, b = s2;
-            if (B.l[s2] > B.l[s1]) std::swap(a, b);
-            S.pa[P] = a; S.pb[P] = b;
-            int la = (int)B.l[a], lb = (int)B.l[b];
-            S.cls[P] = pair_class(la, lb);
-            S.rs[P] = REC_HDR + NH(la + lb) * NC(la) * NC(lb);
-            S.off[P] = off;
-            off += (long long)S.rs[P] * (B.prim[a + 1] - B.prim[a]) * (B.prim[b + 1] - B.prim[b]);
-        }
-    S.recs.assign(off, 0.0);
-    for (P = 0; P < S.npair; ++P) {
-        int a = S.pa[P], b = S.pb[P];
-        const double* A = B.pos + 3 * B.atom[a];
-        const double* C = B.pos + 3 * B.atom[b];
-        double r2 = 0.0;
-        for (int d = 0; d < 3; ++d) { double x = A[d] - C[d]; r2 += x * x; }
-        int n = 0;
-        for (long long ka = B.prim[a]; ka < B.prim[a + 1]; ++ka)
-            for (long long kb = B.prim[b]; kb < B.prim[b + 1]; ++kb) {
-                double ea = B.ex[ka], eb = B.ex[kb];
-                if (!(ea * eb / (ea + eb) * r2 <= 50.0)) continue;
-                write_record_any((int)B.l[a], (int)B.l[b], ea, eb, B.coef[ka], B.coef[kb], A, C,
-                                 S.recs.data() + S.off[P] + (long long)n * S.rs[P]);
-                ++n;
-            }
-        S.npp[P] = n;
-    }
-}
-
-template <int LA, int LB>
-double schwarz(const State& S, int P) {
-    using Qt = Quartet<LA, LB, LA, LB>;
-    const double* r = S.recs.data() + S.off[P];
-    double m = 0.0;
-    if constexpr (split_by_cd<LA, LB, LA, LB>()) {
-        for (int cd = 0; cd < Qt::NCD; ++cd) {
-            double acc[Qt::NAB];
-            eri_quartet_cd<LA, LB, LA, LB>(r, S.npp[P], r, S.npp[P], TAB().data(), cd, acc);
-            m = std::max(m, std::fabs(acc[cd]));
-        }
-    } else {
-        double acc[Qt::NAB * Qt::NCD];
-        eri_quartet<LA, LB, LA, LB>(r, S.npp[P], r, S.npp[P], TAB().data(), acc);
-        for (int ab = 0; ab < Qt::NAB; ++ab) m = std::max(m, std::fabs(acc[ab * Qt::NCD + ab]));
-    }
-    return std::sqrt(m);
-}
-
-constexpr int la_of(int pc) { return pc == 0 ? 0 : pc <= 2 ? 1 : 2; }
-constexpr int lb_of(int pc) { return pc - la_of(pc) * (la_of(pc) + 1) / 2; }
-
-struct HostSink {
-    long long *Jh, *Jl, *Kh, *Kl;
-    void j(size_t i, double x) const { long long h, l; fx_split(x, h, l); Jh[i] += h; Jl[i] += l; }
-    void k(size_t i, double x) const { long long h, l; fx_split(x, h, l); Kh[i] += h; Kl[i] += l; }
-};
-
 template <int QC>
 void jk_quartet(const State& S, int P, int Q, const double* D, int nset, std::vector<long long>& fx) {
     constexpr int hi = [] { int h = 0; while ((h + 1) * (h + 2) / 2 <= QC) ++h; return h; }();
@@ -154,6 +41,63 @@ void jk_quartet(const State& S, int P, int Q, const double* D, int nset, std::ve
     }
 }
 
+
+
+template <int QC>
+void grad_quartet(const State& S, const Basis& Bs, int P, int Q, const double* dc, const double* dx, int nx, double kfac,
+                  double thr, const std::vector<double>& qscr, std::vector<long long>& fx) {
+    constexpr int hi = [] { int h = 0; while ((h + 1) * (h + 2) / 2 <= QC) ++h; return h; }();
+    constexpr int lo = QC - hi * (hi + 1) / 2;
+    constexpr int LA = la_of(hi), LB = lb_of(hi), LC = la_of(lo), LD = lb_of(lo);
+    using Qt = Quartet<LA, LB, LC, LD>;
+    constexpr int NAB = Qt::NAB, NCD = Qt::NCD, NB = Qt::NB, ND = Qt::ND;
+    int sA = S.pa[P], sB = S.pb[P], sC = S.pa[Q], sD = S.pb[Q];
+    int aA = (int)Bs.atom[sA], aB = (int)Bs.atom[sB], aC = (int)Bs.atom[sC], aD = (int)Bs.atom[sD];
+    if (aA == aB && aB == aC && aC == aD) return;
+    double deg = 0.5;
+    if (sA != sB) deg *= 2.0;
+    if (sC != sD) deg *= 2.0;
+    if (P != Q) deg *= 2.0;
+    const size_t n = S.nao;
+    int oA = S.ao[sA], oB = S.ao[sB], oC = S.ao[sC], oD = S.ao[sD];
+    auto gam = [&](int ab, int cd) {
+        size_t m = oA + ab / NB, nn = oB + ab % NB, l = oC + cd / ND, s = oD + cd % ND;
+        double v = dc[m * n + nn] * dc[l * n + s];
+        for (int x = 0; x < nx; ++x) {
+            const double* X = dx + x * n * n;
+            v -= 0.5 * kfac * (X[m * n + l] * X[nn * n + s] + X[m * n + s] * X[nn * n + l]);
+        }
+        return v * deg;
+    };
+    double gmax = 0.0;
+    for (int ab = 0; ab < NAB; ++ab)
+        for (int cd = 0; cd < NCD; ++cd) gmax = std::max(gmax, std::fabs(gam(ab, cd)));
+    if (qscr[P] * qscr[Q] * gmax < thr) return;
+    bool need_bra = aA != aB, need_ket = aC != aD;
+    auto add = [&](int a, int d, double x) { long long h, l2; fx_split(x, h, l2); fx[2 * (3 * a + d)] += h; fx[2 * (3 * a + d) + 1] += l2; };
+    const double* rb = S.recs.data() + S.off[P];
+    const double* rk = S.recs.data() + S.off[Q];
+    if (need_bra || (!need_ket && NAB >= NCD))
+        for (int cd = 0; cd < NCD; ++cd) {
+            double g[NAB], g1[3], g2[3];
+            for (int ab = 0; ab < NAB; ++ab) g[ab] = gam(ab, cd);
+            grad_pass_kc<LA, LB, LC, LD>(rb, S.npp[P], rk, S.npp[Q], TAB().data(), Bs.pos + 3 * aA, Bs.pos + 3 * aB, cd, g, g1, g2);
+            for (int d = 0; d < 3; ++d) {
+                if (need_bra) { add(aA, d, g1[d]); add(aB, d, g2[d]); if (!need_ket) add(aC, d, -(g1[d] + g2[d])); }
+                else { add(aA, d, g1[d] + g2[d]); add(aC, d, -(g1[d] + g2[d])); }
+            }
+        }
+    if (need_ket || (!need_bra && NAB < NCD))
+        for (int ab = 0; ab < NAB; ++ab) {
+            double g[NCD], g1[3], g2[3];
+            for (int cd = 0; cd < NCD; ++cd) g[cd] = gam(ab, cd);
+            grad_pass_kc<LC, LD, LA, LB>(rk, S.npp[Q], rb, S.npp[P], TAB().data(), Bs.pos + 3 * aC, Bs.pos + 3 * aD, ab, g, g1, g2);
+            for (int d = 0; d < 3; ++d) {
+                if (need_ket) { add(aC, d, g1[d]); add(aD, d, g2[d]); if (!need_bra) add(aA, d, -(g1[d] + g2[d])); }
+                else { add(aC, d, g1[d] + g2[d]); add(aA, d, -(g1[d] + g2[d])); }
+            }
+        }
+}
 }  // namespace
 
 extern "C" {
@@ -254,104 +198,42 @@ void hc_oneel(int nshell, int natm, const long long* atom, const long long* l, c
 
 }  // extern "C"
 
-namespace {
-template <int QC>
-void grad_quartet(const State& S, const Basis& Bs, int P, int Q, const double* dc, const double* dx, int nx, double kfac,
-                  double thr, const std::vector<double>& qscr, std::vector<long long>& fx) {
-    constexpr int hi = [] { int h = 0; while ((h + 1) * (h + 2) / 2 <= QC) ++h; return h; }();
-    constexpr int lo = QC - hi * (hi + 1) / 2;
-    constexpr int LA = la_of(hi), LB = lb_of(hi), LC = la_of(lo), LD = lb_of(lo);
-    using Qt = Quartet<LA, LB, LC, LD>;
-    constexpr int NAB = Qt::NAB, NCD = Qt::NCD, NB = Qt::NB, ND = Qt::ND;
-    int sA = S.pa[P], sB = S.pb[P], sC = S.pa[Q], sD = S.pb[Q];
-    int aA = (int)Bs.atom[sA], aB = (int)Bs.atom[sB], aC = (int)Bs.atom[sC], aD = (int)Bs.atom[sD];
-    if (aA == aB && aB == aC && aC == aD) return;
-    double deg = 0.5;
-    if (sA != sB) deg *= 2.0;
-    if (sC != sD) deg *= 2.0;
-    if (P != Q) deg *= 2.0;
-    const size_t n = S.nao;
-    int oA = S.ao[sA], oB = S.ao[sB], oC = S.ao[sC], oD = S.ao[sD];
-    auto gam = [&](int ab, int cd) {
-        size_t m = oA + ab / NB, nn = oB + ab % NB, l = oC + cd / ND, s = oD + cd % ND;
-        double v = dc[m * n + nn] * dc[l * n + s];
-        for (int x = 0; x < nx; ++x) {
-            const double* X = dx + x * n * n;
-            v -= 0.5 * kfac * (X[m * n + l] * X[nn * n + s] + X[m * n + s] * X[nn * n + l]);
-        }
-        return v * deg;
-    };
-    double gmax = 0.0;
-    for (int ab = 0; ab < NAB; ++ab)
-        for (int cd = 0; cd < NCD; ++cd) gmax = std::max(gmax, std::fabs(gam(ab, cd)));
-    if (qscr[P] * qscr[Q] * gmax < thr) return;
-    bool need_bra = aA != aB, need_ket = aC != aD;
-    auto add = [&](int a, int d, double x) { long long h, l2; fx_split(x, h, l2); fx[2 * (3 * a + d)] += h; fx[2 * (3 * a + d) + 1] += l2; };
-    const double* rb = S.recs.data() + S.off[P];
-    const double* rk = S.recs.data() + S.off[Q];
-    if (need_bra || (!need_ket && NAB >= NCD))
-        for (int cd = 0; cd < NCD; ++cd) {
-            double g[NAB], g1[3], g2[3];
-            for (int ab = 0; ab < NAB; ++ab) g[ab] = gam(ab, cd);
-            grad_pass_kc<LA, LB, LC, LD>(rb, S.npp[P], rk, S.npp[Q], TAB().data(), Bs.pos + 3 * aA, Bs.pos + 3 * aB, cd, g, g1, g2);
-            for (int d = 0; d < 3; ++d) {
-                if (need_bra) { add(aA, d, g1[d]); add(aB, d, g2[d]); if (!need_ket) add(aC, d, -(g1[d] + g2[d])); }
-                else { add(aA, d, g1[d] + g2[d]); add(aC, d, -(g1[d] + g2[d])); }
-            }
-        }
-    if (need_ket || (!need_bra && NAB < NCD))
-        for (int ab = 0; ab < NAB; ++ab) {
-            double g[NCD], g1[3], g2[3];
-            for (int cd = 0; cd < NCD; ++cd) g[cd] = gam(ab, cd);
-            grad_pass_kc<LC, LD, LA, LB>(rk, S.npp[Q], rb, S.npp[P], TAB().data(), Bs.pos + 3 * aC, Bs.pos + 3 * aD, ab, g, g1, g2);
-            for (int d = 0; d < 3; ++d) {
-                if (need_ket) { add(aC, d, g1[d]); add(aD, d, g2[d]); if (!need_bra) add(aA, d, -(g1[d] + g2[d])); }
-                else { add(aC, d, g1[d] + g2[d]); add(aA, d, -(g1[d] + g2[d])); }
-            }
-        }
-}
-}  // namespace
-
 extern "C" {
-// two-electron gradient through md_grad.h over all unique shell quartets
+void hc_grad_two_dev_double(int, int, const long long*, const long long*, const long long*, const long long*,
+                            const double*, const double*, const double*, const double*, int, const double*, double,
+                            double, double*);
+void hc_grad_two_dev_float(int, int, const long long*, const long long*, const long long*, const long long*,
+                           const double*, const double*, const double*, const double*, int, const double*, double,
+                           double, double*);
+
+// two-electron gradient through md_grad.h over all unique shell quartets, one ket
+// column per pass with the full Boys table in FP64 (the reference's structure)
 void hc_grad_two(int nshell, int natm, const long long* atom, const long long* l, const long long* prim,
                  const long long* block, const double* ex, const double* coef, const double* pos, const double* dc,
                  int nx, const double* dx, double kfac, double thr, double* g) {
     Basis B{nshell, natm, atom, l, prim, block, ex, coef, pos};
     State S;
     build(B, S);
-    for (int P = 0; P < S.npair; ++P) {
-        switch (S.cls[P]) {
-            case 0: S.qshell[P] = schwarz<0, 0>(S, P); break;
-            case 1: S.qshell[P] = schwarz<1, 0>(S, P); break;
-            case 2: S.qshell[P] = schwarz<1, 1>(S, P); break;
-            case 3: S.qshell[P] = schwarz<2, 0>(S, P); break;
-            case 4: S.qshell[P] = schwarz<2, 1>(S, P); break;
-            default: S.qshell[P] = schwarz<2, 2>(S, P); break;
-        }
-    }
-    int nblock = (int)block[nshell - 1] + 1;
-    std::vector<double> qb(nblock * (nblock + 1) / 2, 0.0), qscr(S.npair);
-    std::vector<int> bpof(S.npair);
-    int P = 0;
-    for (int s1 = 0; s1 < nshell; ++s1)
-        for (int s2 = 0; s2 <= s1; ++s2, ++P) {
-            int b1 = (int)block[s1], b2 = (int)block[s2];
-            bpof[P] = b1 * (b1 + 1) / 2 + b2;
-            qb[bpof[P]] = std::max(qb[bpof[P]], S.qshell[P]);
-        }
-    for (P = 0; P < S.npair; ++P) qscr[P] = qb[bpof[P]];
+    std::vector<double> qscr;
     std::vector<long long> fx(6 * natm, 0);
-    for (P = 0; P < S.npair; ++P)
-        for (int Q = 0; Q <= P; ++Q) {
-            int cp = S.cls[P], cq = S.cls[Q];
-            int hi = std::max(cp, cq), lo = std::min(cp, cq);
-            int bra = cp >= cq ? P : Q, ket = cp >= cq ? Q : P;
-            int qc = hi * (hi + 1) / 2 + lo;
-            static_for<0, NQCLASS>([&](auto k) {
-                if (decltype(k)::value == qc) grad_quartet<decltype(k)::value>(S, B, bra, ket, dc, dx, nx, kfac, thr, qscr, fx);
-            });
-        }
+    for_each_quartet(S, B, qscr, [&](int bra, int ket, int qc) {
+        static_for<0, NQCLASS>([&](auto k) {
+            if (decltype(k)::value == qc) grad_quartet<decltype(k)::value>(S, B, bra, ket, dc, dx, nx, kfac, thr, qscr, fx);
+        });
+    });
     for (int i = 0; i < 3 * natm; ++i) g[i] = fx_value(fx[2 * i], fx[2 * i + 1]);
+}
+
+// variant 0: hc_grad_two; 1: the device kernel's work items in FP64; 2: in FP32
+void hc_grad_two_variant(int variant, int nshell, int natm, const long long* atom, const long long* l,
+                         const long long* prim, const long long* block, const double* ex, const double* coef,
+                         const double* pos, const double* dc, int nx, const double* dx, double kfac, double thr,
+                         double* g) {
+    if (variant == 1)
+        hc_grad_two_dev_double(nshell, natm, atom, l, prim, block, ex, coef, pos, dc, nx, dx, kfac, thr, g);
+    else if (variant == 2)
+        hc_grad_two_dev_float(nshell, natm, atom, l, prim, block, ex, coef, pos, dc, nx, dx, kfac, thr, g);
+    else
+        hc_grad_two(nshell, natm, atom, l, prim, block, ex, coef, pos, dc, nx, dx, kfac, thr, g);
 }
 }  // extern "C"

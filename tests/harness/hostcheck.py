@@ -11,13 +11,21 @@ def _p(a):
     return a.ctypes.data_as(_P)
 
 
+_GRAD = [ctypes.c_int, ctypes.c_int] + [_P] * 8 + [ctypes.c_int, _P, ctypes.c_double, ctypes.c_double, _P]
+
+
 class Harness:
     def __init__(self, path):
         L = ctypes.CDLL(path)
-        L.hc_run.argtypes = [ctypes.c_int, ctypes.c_int] + [_P] * 7 + [ctypes.c_double, ctypes.c_int] + [_P] * 6 + [ctypes.c_long]
-        L.hc_run.restype = ctypes.c_long
-        L.hc_oneel.argtypes = [ctypes.c_int, ctypes.c_int] + [_P] * 14
-        L.hc_grad_two.argtypes = [ctypes.c_int, ctypes.c_int] + [_P] * 8 + [ctypes.c_int, _P, ctypes.c_double, ctypes.c_double, _P]
+        self.full = hasattr(L, "hc_run")          # the rsq-model library exports the device variant only
+        if self.full:
+            L.hc_run.argtypes = [ctypes.c_int, ctypes.c_int] + [_P] * 7 + [ctypes.c_double, ctypes.c_int] + [_P] * 6 + [ctypes.c_long]
+            L.hc_run.restype = ctypes.c_long
+            L.hc_oneel.argtypes = [ctypes.c_int, ctypes.c_int] + [_P] * 14
+            L.hc_grad_two.argtypes = _GRAD
+            L.hc_grad_two_variant.argtypes = [ctypes.c_int] + _GRAD
+            L.hc_grad_two_dev_float.argtypes = _GRAD
+        L.hc_grad_two_dev_double.argtypes = _GRAD
         self.L = L
 
     @staticmethod
@@ -57,4 +65,14 @@ class Harness:
         g = np.zeros((b.natm, 3))
         self.L.hc_grad_two(b.nshell, b.natm, _p(at), _p(l), _p(pr), _p(blk), _p(ex), _p(co), _p(pos),
                            _p(np.ascontiguousarray(Dc, float)), Dx.shape[0], _p(Dx), float(kfac), float(thr), _p(g))
+        return g
+
+    def grad_two_variant(self, variant, b, Dc, Dx, kfac, thr=1e-14):
+        """variant 1: the device kernel's work items in FP64; 2: the same in FP32."""
+        at, l, pr, blk, ex, co, pos = self._arrays(b)
+        Dx = np.ascontiguousarray(np.asarray(Dx, float).reshape(-1, b.nao, b.nao))
+        g = np.zeros((b.natm, 3))
+        fn = {1: self.L.hc_grad_two_dev_double, 2: getattr(self.L, "hc_grad_two_dev_float", None)}[int(variant)]
+        fn(b.nshell, b.natm, _p(at), _p(l), _p(pr), _p(blk), _p(ex), _p(co), _p(pos),
+           _p(np.ascontiguousarray(Dc, float)), Dx.shape[0], _p(Dx), float(kfac), float(thr), _p(g))
         return g

@@ -65,3 +65,30 @@ def test_one_electron_and_gradients_match_oracle(harness, oracle_lib, mol, basis
     g2 = harness.grad_two(b, P, Dx, k)
     g2o = oracle_lib.Pairs(ob).eri_gradient(P, list(Dx), k)
     np.testing.assert_allclose(g2, g2o, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("mol,basis,mult", [("water", "6-31g*", 1), ("water", "cc-pvdz", 1),
+                                            ("methyl", "6-31g*", 2), ("dimer", "6-31g**", 1)])
+def test_device_derivative_work_items_match_oracle(harness, harness_rsq, oracle_lib, mol, basis, mult):
+    """The gfx950 derivative kernel's own work items -- column groups per pass, the
+    staged Boys slice with its exp(-T_k) column -- run on the host in FP64 agree with
+    the oracle at 1e-11; in FP32 within FP32 rounding; and with a model of the
+    hardware reciprocal-square-root estimate (relative error 5e-8 before the Newton
+    steps), used in both the prefactor and the large-T Boys branch, still at 1e-11:
+    the device-only force error of that variant is not in this arithmetic."""
+    from aimd_amd.qc.basis import build_basis
+    s, x = bohr(METHYL, 0.03, 1) if mol == "methyl" else _mol(mol)
+    b = build_basis(s, x, basis)
+    ob = oracle_lib.make_basis(s, x, basis)
+    st = oracle_lib.scf(ob, mult=mult, conv_commutator=1e-9)
+    spin = np.array([st.Ds[0], st.Ds[-1]])
+    P = spin[0] + spin[1]
+    Dx, k = (P[None], 0.5) if mult == 1 else (spin, 1.0)
+    g2o = oracle_lib.Pairs(ob).eri_gradient(P, list(Dx), k)
+    g64 = harness.grad_two_variant(1, b, P, Dx, k)
+    np.testing.assert_allclose(g64, g2o, rtol=0, atol=1e-11)
+    g32 = harness.grad_two_variant(2, b, P, Dx, k)
+    err32 = np.abs(g32 - g2o).max()
+    assert 0.0 < err32 < 1e-5, err32
+    grsq = harness_rsq.grad_two_variant(1, b, P, Dx, k)
+    np.testing.assert_allclose(grsq, g2o, rtol=0, atol=1e-11)
