@@ -71,23 +71,59 @@ def test_batch_against_oracle(oracle_lib):
     batch.close()
 
 
-def test_batch_baoab_steps():
+def test_batch_baoab_without_friction_is_velocity_verlet_per_replica():
+    """BAOAB at zero friction and temperature reduces to velocity Verlet: every replica of
+    the batch follows the single-molecule VelocityVerlet trajectory of the hf backend."""
+    import torch
+    from aimd_amd import md
+    from aimd_amd.backends import get_backend
+    from aimd_amd.batch import BatchLangevinBAOAB, ReplicaBatch
+    names = ["water", "methanol", "methyl", "water"]
+    mols, xs = _replicas(names, seed=5)
+    kw = dict(conv_tol=1e-11, conv_tol_grad=1e-9)
+    batch = ReplicaBatch(mols, basis="6-31g**", **kw)
+    v = [md.maxwell_boltzmann(md.masses_au(s), 300.0, seed=r) for r, (s, _, _) in enumerate(mols)]
+    integ = BatchLangevinBAOAB(batch, 0.5, 0.0, friction_per_fs=0.0, seed=1)
+    dev = batch.device
+    integ.initialize(torch.tensor(np.concatenate(xs), device=dev), torch.tensor(np.concatenate(v), device=dev))
+    singles = []
+    for (s, c, m), x, vr in zip(mols, xs, v):
+        be = get_backend("hf")(s, charge=c, multiplicity=m, basis="6-31g**", **kw)
+        sysm = md.System.create(s, x, vr)
+        vv = md.VelocityVerlet(be, 0.5)
+        vv.initialize(sysm)
+        singles.append((sysm, vv))
+    for step in range(3):
+        r = integ.step()
+        assert r["converged"].all()
+        xb = integ.x.cpu().numpy()
+        for k, ((sysm, vv), a, b) in enumerate(zip(singles, batch.rep_atom[:-1], batch.rep_atom[1:])):
+            vv.step(sysm)
+            dx = np.abs(xb[a:b] - sysm.positions).max()
+            de = abs(r["energy"][k] - vv.result.energy)
+            assert dx < 1e-8 and de < 1e-9, (step, k, dx, de)
+    batch.close()
+
+
+def test_batch_baoab_thermostat_bookkeeping():
+    """At 300 K with friction every replica stays converged, and E_pot + E_kin - heat moves
+    only by the integration error (the O step's kinetic-energy change is booked as heat)."""
     import torch
     from aimd_amd.batch import BatchLangevinBAOAB, ReplicaBatch
     from aimd_amd.md import maxwell_boltzmann
-    names = ["water", "methanol", "h4", "methyl"] * 4
-    mols, xs = _replicas(names, seed=5)
+    names = ["water", "methanol", "methyl"] * 4
+    mols, xs = _replicas(names, seed=5, sigma_ang=0.01)
     batch = ReplicaBatch(mols, basis="6-31g**")
     v = np.concatenate([maxwell_boltzmann(batch.masses[a:b], 300.0, seed=r)
                         for r, (a, b) in enumerate(zip(batch.rep_atom[:-1], batch.rep_atom[1:]))])
-    md = BatchLangevinBAOAB(batch, 0.5, 300.0, friction_per_fs=0.001, seed=1)
+    integ = BatchLangevinBAOAB(batch, 0.5, 300.0, friction_per_fs=0.01, seed=1)
     dev = batch.device
-    md.initialize(torch.tensor(np.concatenate(xs), device=dev), torch.tensor(v, device=dev))
-    e0 = md.conserved_energy()
+    integ.initialize(torch.tensor(np.concatenate(xs), device=dev), torch.tensor(v, device=dev))
+    e0 = integ.conserved_energy()
     for _ in range(5):
-        r = md.step()
-        assert r["converged"].all()
-    drift = np.abs(md.conserved_energy() - e0).max()
+        assert integ.step()["converged"].all()
+    assert float(integ.heat.abs().max()) > 0.0
+    drift = np.abs(integ.conserved_energy() - e0).max()
     print("batch BAOAB: 5 steps, max |effective energy change|", drift, flush=True)
-    assert drift < 1e-4
+    assert drift < 1e-3
     batch.close()
