@@ -45,6 +45,8 @@ SIGNATURES = {
     "aimdhf_batch_diis_solve": (_I, [_I, _I, _P, _P, _P, _P]),
     "aimdhf_tc2_error": (_I, [_I, _P, _P, _P]),
     "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P, _P]),
+    "aimdhf_jk_type_info": (_I, [_P, _P, _P, _P, _P, _P]),
+    "aimdhf_profile_replay": (_I, [_P, _I, _I, _P, _P]),
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_grad_one": (_I, [_P, _P, _P, _P, _P]),
     "aimdhf_grad_one_terms": (_I, [_P, _P, _P, _P, _P, _P, _P]),

@@ -45,7 +45,7 @@ import numpy as np
 from aimd_amd.elements import ATOMIC_NUMBERS, normalize_symbol
 from aimd_amd.qc.basis_data import get_basis_shells, normalize_basis_name
 
-LMAX_DEVICE = 2     # highest l the gfx950 kernels are instantiated for (all built-in bases)
+LMAX_DEVICE = 4     # highest shell l the engine takes (l <= 2 class kernels, l = 3, 4 the runtime-l kernels)
 _SHELL_LETTERS = "spdfghi"
 
 

@@ -135,6 +135,15 @@ class Engine:
         return {"stored": bool(st.value), "store_bytes": int(b.value), "k_entries": int(ke.value),
                 "j_entries": int(je.value), "gather_bytes": int(gd.value)}
 
+    def jk_type_info(self) -> dict:
+        """Store traffic per exchange output type (9), Coulomb pair class (6) and the unique
+        integral bytes per quartet class (21) of the current geometry (zeros in direct mode)."""
+        kb, ke = np.zeros(9, np.int64), np.zeros(9, np.int64)
+        jb, je = np.zeros(6, np.int64), np.zeros(6, np.int64)
+        ub = np.zeros(21, np.int64)
+        _lib.check(_lib.lib().aimdhf_jk_type_info(self._h, _ptr(kb), _ptr(ke), _ptr(jb), _ptr(je), _ptr(ub)))
+        return {"k_bytes": kb, "k_entries": ke, "j_bytes": jb, "j_entries": je, "unique_bytes": ub}
+
     def jk(self, dens: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
         """J, K of (nset, nao, nao) (or (nao, nao)) densities; symmetric part of each D is used."""
         single = dens.dim() == 2
@@ -217,11 +226,11 @@ class Engine:
                   + [f"grad_screen_{k}" for k in range(21)]
                   + ["host_energy_list", "host_owner_lists", "host_store_build", "host_gradient_list"]
                   + ["host_store_counts", "host_store_bands", "host_store_fill", "host_store_units"]
-                  + [f"unused_{k}" for k in range(93, 96)])
-    NKIND = 96
-    FOCK_KINDS = list(range(0, 15)) + [45, 46] + list(range(49, 64))    # per-iteration J/K work, either mode
+                  + ["kstore_reduce", "jstore_reduce", "generic_jk", "generic_grad_two", "unused_97"])
+    NKIND = 98
+    FOCK_KINDS = list(range(0, 15)) + [45, 46] + list(range(49, 64)) + [93, 94, 95]   # per-iteration J/K work
     STORE_KINDS = [15, 47, 48]                                          # per-geometry store build
-    GRAD_KINDS = list(range(21, 42)) + list(range(64, 85))              # derivative ERIs
+    GRAD_KINDS = list(range(21, 42)) + list(range(64, 85)) + [96]       # derivative ERIs
 
     def profile(self, on: bool = True) -> None:
         """Record HIP events around every engine launch (on torch's current stream)."""
@@ -233,6 +242,13 @@ class Engine:
         n = np.zeros(self.NKIND, np.int64)
         _lib.check(_lib.lib().aimdhf_profile_read(self._h, _ptr(ms), _ptr(n), self.NKIND, int(bool(reset))))
         return ms, n
+
+    def profile_replay(self, kind: int, reps: int = 20) -> float:
+        """Average milliseconds of `reps` back-to-back launches of a store-mode digestion
+        kernel (kind 49..63) with the arguments of the last J/K build."""
+        out = ctypes.c_double()
+        _lib.check(_lib.lib().aimdhf_profile_replay(self._h, int(kind), int(reps), _stream(), ctypes.byref(out)))
+        return float(out.value)
 
     def class_counts(self) -> tuple[np.ndarray, np.ndarray]:
         """Per-class sizes of the energy and (last) gradient quartet lists."""

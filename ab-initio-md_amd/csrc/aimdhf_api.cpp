@@ -94,7 +94,7 @@ std::vector<double> boys_table() {
 // Optional per-launch timing with HIP events on the caller's stream.
 // Kinds as listed at aimdhf_profile_enable in aimdhf.h (0..63).
 struct Profiler {
-    static constexpr int NKIND = 96;
+    static constexpr int NKIND = 98;
     bool on = false;
     struct Rec {
         int kind;
@@ -281,6 +281,64 @@ struct aimdhf_plan {
     // entry point on this plan's device (resolved at plan creation)
     void* noalloc_ptr = nullptr;
     size_t noalloc_bytes = 0;
+    // runtime-l path (a shell with l = 3 or 4): every shell pair i >= j in index
+    // order; the energy quartets in d_list / qtotal, the last gradient list in
+    // d_glist / gen_gq; bounds per shell pair in d_qshell, per block pair in d_qblock
+    // arguments of the last store-mode J/K build (kernel replays for timing)
+    aimdhf::KSArgs last_ka{};
+    aimdhf::JSArgs last_ja{};
+    bool have_last_store = false;
+    bool generic = false;
+    std::vector<int> gpair_i, gpair_j, gpair_bp, gen_npp;
+    std::vector<double> gen_qb;
+    long long gen_gq = 0;
+    DevBuf d_gpair_i, d_gpair_j;
+
+    aimdhf::GenArgs gen_args() const {
+        aimdhf::GenArgs g{};
+        g.nao = nao;
+        g.natm = natm;
+        g.nset = 1;
+        g.shell_atom = d_shell_atom.as<int>();
+        g.shell_l = d_shell_l.as<int>();
+        g.shell_prim = d_shell_prim.as<int>();
+        g.shell_ao = d_shell_ao.as<int>();
+        g.prim_exp = d_prim_exp.as<double>();
+        g.prim_coef = d_prim_coef.as<double>();
+        g.pos = d_pos.as<double>();
+        g.charges = d_charges.as<double>();
+        g.boys = d_boys.as<double>();
+        g.pair_i = d_gpair_i.as<int>();
+        g.pair_j = d_gpair_j.as<int>();
+        g.q = d_qshell.as<double>();
+        return g;
+    }
+
+    // unique shell quartets (P >= Q) whose block-pair bounds give q_P q_Q >= thr, the
+    // heaviest first, into dst; returns their number
+    long long gen_build_list(double thr, DevBuf& dst) {
+        std::vector<int> ord(npair);
+        for (int i = 0; i < npair; ++i) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int a, int b) { return gen_qb[gpair_bp[a]] > gen_qb[gpair_bp[b]]; });
+        std::vector<int2> out;
+        for (int x = 0; x < npair; ++x) {
+            const double qa = gen_qb[gpair_bp[ord[x]]];
+            for (int y = 0; y < npair; ++y) {
+                if (!(qa * gen_qb[gpair_bp[ord[y]]] >= thr)) break;
+                const int P = ord[x], Q = ord[y];
+                if (P >= Q) out.push_back(make_int2(P, Q));
+            }
+        }
+        auto cost = [&](const int2& q) {
+            const int a = gpair_i[q.x], b = gpair_j[q.x], c = gpair_i[q.y], d = gpair_j[q.y];
+            const long long np = (long long)gen_npp[q.x] * gen_npp[q.y];
+            return np * ncart(shell_l[a]) * ncart(shell_l[b]) * ncart(shell_l[c]) * ncart(shell_l[d]);
+        };
+        std::stable_sort(out.begin(), out.end(), [&](const int2& a, const int2& b) { return cost(a) > cost(b); });
+        upload(dst, out);
+        return (long long)out.size();
+    }
 
     // make room for `bytes` of staging (no transfer may be pending) and start over
     void stage_reset(size_t bytes) {
@@ -325,7 +383,8 @@ struct aimdhf_plan {
                           &d_kbig, &d_jbig, &d_kunit_os, &d_kunit_e, &d_out_unit,
                           &d_kpart2, &d_junit_pc, &d_junit_e, &d_pair_unit, &d_jpart2, &d_kunit_q, &d_junit_q,
                           &d_dmax, &d_kdata32b, &d_jdata32b, &d_tab, &d_gdmax, &d_shell_rep, &d_shell_first,
-                          &d_shell_pair0, &d_pair_lo, &d_rep_atom, &d_rep_nao, &d_rep_toff, &d_rep_item, &d_tens})
+                          &d_shell_pair0, &d_pair_lo, &d_rep_atom, &d_rep_nao, &d_rep_toff, &d_rep_item, &d_tens,
+                          &d_gpair_i, &d_gpair_j})
             b->release();
         if (h_pin) (void)hipHostFree(h_pin);
         if (h_stage) (void)hipHostFree(h_stage);
@@ -952,7 +1011,7 @@ int aimdhf_last_error(char* buf, size_t len) {
     return AIMDHF_OK;
 }
 
-int aimdhf_lmax(void) { return 2; }
+int aimdhf_lmax(void) { return aimdhf::GEN_LMAX; }
 
 }  // extern "C"
 
@@ -1010,10 +1069,12 @@ static int create_plan(int natm, int nshell, const int64_t* shell_atom, const in
         p->shell_ao.resize(nshell + 1);
         p->shell_ao[0] = 0;
         for (int s = 0; s < nshell; ++s) {
-            if (shell_l[s] < 0 || shell_l[s] > 2) {
+            if (shell_l[s] < 0 || shell_l[s] > aimdhf::GEN_LMAX || (nrep > 0 && shell_l[s] > 2)) {
                 delete p;
-                return fail(AIMDHF_E_UNSUPPORTED, "angular momentum l = " + std::to_string(shell_l[s]) +
-                                                      " > 2 is not supported by the gfx950 kernels of this build");
+                return fail(AIMDHF_E_UNSUPPORTED,
+                            "angular momentum l = " + std::to_string(shell_l[s]) + " is not supported" +
+                                (nrep > 0 ? std::string(" in batched plans (l <= 2)")
+                                          : " (l <= " + std::to_string(aimdhf::GEN_LMAX) + ")"));
             }
             if (shell_atom[s] < 0 || shell_atom[s] >= natm || shell_prim[s + 1] <= shell_prim[s]) {
                 delete p;
@@ -1028,6 +1089,41 @@ static int create_plan(int natm, int nshell, const int64_t* shell_atom, const in
         }
         p->shell_prim[nshell] = nprim;
         p->nao = p->shell_ao[nshell];
+        if (p->lmax > 2) {
+            // runtime-l kernels for every integral: shell pairs, bounds, no pair records
+            p->generic = true;
+            p->nblock = p->shell_block[nshell - 1] + 1;
+            p->nbpair = p->nblock * (p->nblock + 1) / 2;
+            p->prim_exp.assign(prim_exp, prim_exp + nprim);
+            p->prim_coef.assign(prim_coef, prim_coef + nprim);
+            p->charges.assign(charges, charges + natm);
+            p->npair = nshell * (nshell + 1) / 2;
+            for (int i = 0; i < nshell; ++i)
+                for (int j = 0; j <= i; ++j) {
+                    p->gpair_i.push_back(i);
+                    p->gpair_j.push_back(j);
+                    const int b1 = p->shell_block[i], b2 = p->shell_block[j];
+                    p->gpair_bp.push_back(b1 * (b1 + 1) / 2 + b2);
+                }
+            p->qcount.assign(NQCLASS, 0);
+            p->gcount.assign(NQCLASS, 0);
+            upload(p->d_shell_atom, p->shell_atom);
+            upload(p->d_shell_l, p->shell_l);
+            upload(p->d_shell_prim, p->shell_prim);
+            upload(p->d_shell_ao, p->shell_ao);
+            upload(p->d_prim_exp, p->prim_exp);
+            upload(p->d_prim_coef, p->prim_coef);
+            upload(p->d_charges, p->charges);
+            upload(p->d_boys, boys_table());
+            upload(p->d_gpair_i, p->gpair_i);
+            upload(p->d_gpair_j, p->gpair_j);
+            p->d_scratch.ensure(sizeof(double) * 1024);
+            p->d_pos.ensure(sizeof(double) * 3 * natm);
+            p->d_qshell.ensure(sizeof(double) * p->npair);
+            p->d_qblock.ensure(sizeof(double) * p->nbpair);
+            *out = p;
+            return AIMDHF_OK;
+        }
         if (p->batched) {
             // AO offsets within each replica; replica sizes, padded dimension, tensors
             p->rep_nao.assign(p->nrep, 0);
@@ -1190,6 +1286,130 @@ static int create_plan(int natm, int nshell, const int64_t* shell_atom, const in
     return AIMDHF_OK;
 }
 
+// ------------------------------------------------------------ runtime-l path
+static int set_geometry_generic(aimdhf_plan* p, const double* positions, double thr, hipStream_t st) {
+    try {
+        p->pos.assign(positions, positions + 3 * p->natm);
+        HIPCHECK(hipMemcpyAsync(p->d_pos.p, positions, sizeof(double) * 3 * p->natm, hipMemcpyHostToDevice, st));
+        aimdhf::GenArgs g = p->gen_args();
+        p->prof.time(42, st, [&] { aimdhf::launch_gen_schwarz(g, p->npair, st); });
+        HIPCHECK(hipGetLastError());
+        std::vector<double> qs(p->npair);
+        HIPCHECK(hipMemcpyAsync(qs.data(), p->d_qshell.p, sizeof(double) * p->npair, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        auto t0 = Profiler::clock::now();
+        // block-pair bounds: the largest shell-pair bound of the block pair
+        p->gen_qb.assign(p->nbpair, 0.0);
+        for (int P = 0; P < p->npair; ++P) p->gen_qb[p->gpair_bp[P]] = std::max(p->gen_qb[p->gpair_bp[P]], qs[P]);
+        upload(p->d_qblock, p->gen_qb);
+        // kept primitive pairs (mu |AB|^2 <= 50) per shell pair
+        p->gen_npp.assign(p->npair, 0);
+        for (int P = 0; P < p->npair; ++P) {
+            const int a = p->gpair_i[P], b = p->gpair_j[P];
+            double r2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const double x = p->pos[3 * p->shell_atom[a] + d] - p->pos[3 * p->shell_atom[b] + d];
+                r2 += x * x;
+            }
+            int n = 0;
+            for (int i = p->shell_prim[a]; i < p->shell_prim[a + 1]; ++i)
+                for (int j = p->shell_prim[b]; j < p->shell_prim[b + 1]; ++j) {
+                    const double x = p->prim_exp[i], y = p->prim_exp[j];
+                    if (x * y / (x + y) * r2 <= 50.0) ++n;
+                }
+            p->gen_npp[P] = n;
+        }
+        p->qtotal = p->gen_build_list(thr, p->d_list);
+        p->threshold = thr;
+        p->stored = false;
+        p->store_bytes = p->k_entries = p->j_entries = 0;
+        p->prof.host(85, t0);
+        p->have_geometry = true;
+    } catch (const HipError& e) {
+        p->have_geometry = false;
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+static int jk_generic(aimdhf_plan* p, int nset, const double* dens, double* J, double* K, hipStream_t st) {
+    try {
+        const size_t nn = (size_t)p->nao * p->nao;
+        p->d_dsym.ensure(sizeof(double) * nset * nn);
+        p->d_fx.ensure(sizeof(long long) * 4 * nset * nn);
+        aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st);
+        aimdhf::launch_zero64(p->d_fx.as<long long>(), 4 * nset * nn, st);
+        aimdhf::GenArgs g = p->gen_args();
+        g.nset = nset;
+        g.dens = p->d_dsym.as<double>();
+        g.fx = p->d_fx.as<long long>();
+        g.quartets = p->d_list.as<int2>();
+        g.nq = p->qtotal;
+        p->prof.time(95, st, [&] { aimdhf::launch_gen_jk(g, st); });
+        aimdhf::launch_jk_finalize(p->d_fx.as<long long>(), nset, p->nao, J, K, st);
+        HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
+// one-electron gradient terms `parts` (1 kinetic, 2 nuclear attraction, 4 overlap) into grad
+static void grad_one_generic(aimdhf_plan* p, const double* D, const double* W, int parts, double* grad, hipStream_t st) {
+    const size_t nn = (size_t)p->nao * p->nao;
+    p->d_gden.ensure(sizeof(double) * 2 * nn);
+    double* Ds = p->d_gden.as<double>();
+    double* Ws = Ds + nn;
+    aimdhf::launch_symmetrize(D, 1, p->nao, Ds, st);
+    aimdhf::launch_symmetrize(W, 1, p->nao, Ws, st);
+    p->d_gfx.ensure(sizeof(long long) * 6 * p->natm);
+    aimdhf::launch_zero64(p->d_gfx.as<long long>(), 6 * p->natm, st);
+    aimdhf::GenArgs g = p->gen_args();
+    g.D = Ds;
+    g.W = Ws;
+    g.parts = parts;
+    g.fx = p->d_gfx.as<long long>();
+    p->prof.time(44, st, [&] { aimdhf::launch_gen_grad1(g, p->npair, st); });
+    aimdhf::launch_fx_to_double(p->d_gfx.as<long long>(), 3 * p->natm, grad, st);
+    HIPCHECK(hipGetLastError());
+}
+
+static int grad_two_generic(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, double kfac, double thr,
+                            double* grad, hipStream_t st) {
+    try {
+        const size_t nn = (size_t)p->nao * p->nao;
+        p->d_gden.ensure(sizeof(double) * (1 + nx) * nn);
+        double* dc = p->d_gden.as<double>();
+        double* dx = dc + nn;
+        aimdhf::launch_symmetrize(Dc, 1, p->nao, dc, st);
+        aimdhf::launch_symmetrize(Dx, nx, p->nao, dx, st);
+        // quartets with q_P q_Q max|Gamma| >= thr for the largest possible |Gamma| element
+        const double mc = aimdhf::device_absmax(dc, nn, p->d_scratch.as<double>(), st);
+        const double mx = aimdhf::device_absmax(dx, nx * nn, p->d_scratch.as<double>(), st);
+        if (mc < 0.0 || mx < 0.0) return fail(AIMDHF_E_HIP, "max reduction failed");
+        const double gbound = 4.0 * (mc * mc + std::fabs(kfac) * nx * mx * mx);
+        auto t0 = Profiler::clock::now();
+        p->gen_gq = p->gen_build_list(gbound > 0.0 ? thr / gbound : 0.0, p->d_glist);
+        p->prof.host(88, t0);
+        p->d_gfx.ensure(sizeof(long long) * 6 * p->natm);
+        aimdhf::launch_zero64(p->d_gfx.as<long long>(), 6 * p->natm, st);
+        aimdhf::GenArgs g = p->gen_args();
+        g.dc = dc;
+        g.dx = dx;
+        g.nx = nx;
+        g.kfac = kfac;
+        g.fx = p->d_gfx.as<long long>();
+        g.quartets = p->d_glist.as<int2>();
+        g.nq = p->gen_gq;
+        p->prof.time(96, st, [&] { aimdhf::launch_gen_grad2(g, st); });
+        aimdhf::launch_fx_to_double(p->d_gfx.as<long long>(), 3 * p->natm, grad, st);
+        HIPCHECK(hipGetLastError());
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
 extern "C" {
 
 int aimdhf_plan_create(int natm, int nshell, const int64_t* shell_atom, const int64_t* shell_l,
@@ -1227,6 +1447,8 @@ int aimdhf_set_geometry(aimdhf_plan* p, const double* positions, double thr, voi
         if (!std::isfinite(positions[i])) return fail(AIMDHF_E_ARG, "atomic positions must be finite (got NaN or inf)");
     if (!(thr >= 0.0)) return fail(AIMDHF_E_ARG, "schwarz threshold must be >= 0");
     hipStream_t st = (hipStream_t)stream;
+    p->have_last_store = false;     // the store's buffers may move
+    if (p->generic) return set_geometry_generic(p, positions, thr, st);
     try {
         p->pos.assign(positions, positions + 3 * p->natm);
         HIPCHECK(hipMemcpyAsync(p->d_pos.p, positions, sizeof(double) * 3 * p->natm, hipMemcpyHostToDevice, st));
@@ -1323,6 +1545,17 @@ int aimdhf_grad_quartets(aimdhf_plan* p, int64_t* out, int64_t cap, int64_t* cou
     if (!p || !count) return fail(AIMDHF_E_ARG, "NULL argument");
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     try {
+        if (p->generic) {
+            std::vector<int2> lst(p->gen_gq);
+            if (p->gen_gq)
+                HIPCHECK(hipMemcpy(lst.data(), p->d_glist.p, sizeof(int2) * p->gen_gq, hipMemcpyDeviceToHost));
+            for (long long i = 0; out && i < std::min<long long>(cap, p->gen_gq); ++i) {
+                out[2 * i] = lst[i].x;
+                out[2 * i + 1] = lst[i].y;
+            }
+            *count = p->gen_gq;
+            return AIMDHF_OK;
+        }
         long long tot = 0;
         for (long long c : p->gcount) tot += c;
         std::vector<int2> lst(tot);
@@ -1366,17 +1599,22 @@ static int jk_from_store(aimdhf_plan* p, int nset, const double* dens, double* J
         ka.delta = ja.delta = delta ? 1 : 0;
         ka.dthr = ja.dthr = dthr;
         ka.d32 = ja.d32 = p->delta_fp32_max;
+        p->last_ka = ka;
+        p->last_ja = ja;
+        p->have_last_store = true;
         for (int t = 8; t >= 0; --t) {
             int nout = p->out_off[t + 1] - p->out_off[t];
             if (nout <= 0) continue;
             hipStream_t s2 = p->fan.on(i++);
             p->prof.time(55 + t, s2, [&] { aimdhf::launch_kstore(t, ka, nout, s2); });
+            p->prof.time(93, s2, [&] { aimdhf::launch_kreduce(t, ka, nout, s2); });
         }
         for (int c = 5; c >= 0; --c) {
             if (!p->ncls[c]) continue;
             ja.cls_off = p->cls_pair_off[c];
             hipStream_t s2 = p->fan.on(i++);
             p->prof.time(49 + c, s2, [&] { aimdhf::launch_jstore(c, ja, p->ncls[c], s2); });
+            p->prof.time(94, s2, [&] { aimdhf::launch_jreduce(c, ja, p->ncls[c], s2); });
         }
         p->fan.join();
         HIPCHECK(hipGetLastError());
@@ -1423,6 +1661,7 @@ int aimdhf_jk(aimdhf_plan* p, int nset, const double* dens, double* J, double* K
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
     if (p->batched) return jk_batch(p, nset, dens, J, K, st);
+    if (p->generic) return jk_generic(p, nset, dens, J, K, st);
     if (p->stored) return jk_from_store(p, nset, dens, J, K, false, 0.0, st);
     try {
         const size_t nn = (size_t)p->nao * p->nao;
@@ -1530,7 +1769,16 @@ int aimdhf_one_electron(aimdhf_plan* p, double* S, double* T, double* V, double*
             aimdhf::launch_zero_d(V, n, st);
             aimdhf::launch_zero_d(M, 3 * n, st);
         }
-        p->prof.time(43, st, [&] { aimdhf::launch_oneel(p->oneel_args(), S, T, M, V, st); });
+        if (p->generic) {
+            aimdhf::GenArgs g = p->gen_args();
+            g.S = S;
+            g.T = T;
+            g.V = V;
+            g.M = M;
+            p->prof.time(43, st, [&] { aimdhf::launch_gen_oneel(g, p->npair, st); });
+        } else {
+            p->prof.time(43, st, [&] { aimdhf::launch_oneel(p->oneel_args(), S, T, M, V, st); });
+        }
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
         return e.code;
@@ -1543,6 +1791,10 @@ int aimdhf_grad_one(aimdhf_plan* p, const double* D, const double* W, double* gr
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
     try {
+        if (p->generic) {
+            grad_one_generic(p, D, W, 7, grad, st);
+            return AIMDHF_OK;
+        }
         const size_t nn = p->mat_elems();
         const int nmat = p->batched ? p->nrep : 1, ld = p->batched ? p->bld : p->nao;
         p->d_gden.ensure(sizeof(double) * 2 * nn);
@@ -1565,6 +1817,7 @@ int aimdhf_grad_one(aimdhf_plan* p, const double* D, const double* W, double* gr
 int aimdhf_grad_one_noalloc(aimdhf_plan* p, const double* D, const double* W, double* grad, void* stream) {
     if (!p || !D || !W || !grad) return fail(AIMDHF_E_ARG, "NULL argument");
     if (p->batched) return fail(AIMDHF_E_UNSUPPORTED, "aimdhf_grad_one_noalloc takes single-molecule plans");
+    if (p->generic) return fail(AIMDHF_E_UNSUPPORTED, "aimdhf_grad_one_noalloc takes bases with l <= 2");
     const size_t nn = (size_t)p->nao * p->nao;
     const size_t mats = (sizeof(double) * 2 * nn + 255) & ~(size_t)255;
     const size_t need = mats + sizeof(long long) * 6 * (size_t)p->natm;
@@ -1596,6 +1849,12 @@ int aimdhf_grad_one_terms(aimdhf_plan* p, const double* D, const double* W, doub
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
     try {
+        if (p->generic) {
+            grad_one_generic(p, D, W, 1, g_kin, st);
+            grad_one_generic(p, D, W, 2, g_nuc, st);
+            grad_one_generic(p, D, W, 4, g_ovl, st);
+            return AIMDHF_OK;
+        }
         const size_t nn = p->mat_elems();
         const int nmat = p->batched ? p->nrep : 1, ld = p->batched ? p->bld : p->nao;
         p->d_gden.ensure(sizeof(double) * 3 * nn);
@@ -1634,6 +1893,7 @@ int aimdhf_grad_two(aimdhf_plan* p, const double* Dc, int nx, const double* Dx, 
     if (!(thr >= 0.0)) return fail(AIMDHF_E_ARG, "gradient screening threshold must be >= 0");
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     hipStream_t st = (hipStream_t)stream;
+    if (p->generic) return grad_two_generic(p, Dc, nx, Dx, kfac, thr, grad, st);
     try {
         const size_t nn = p->mat_elems();
         const int nmat = p->batched ? p->nrep : 1, ld = p->batched ? p->bld : p->nao;
@@ -1800,6 +2060,53 @@ int aimdhf_jk_info(aimdhf_plan* p, int* stored, int64_t* store_bytes, int64_t* k
     return AIMDHF_OK;
 }
 
+int aimdhf_jk_type_info(aimdhf_plan* p, int64_t* k_bytes, int64_t* k_entries, int64_t* j_bytes, int64_t* j_entries,
+                        int64_t* unique_bytes) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
+    try {
+        if (unique_bytes)
+            for (int k = 0; k < NQCLASS; ++k)
+                unique_bytes[k] = p->stored ? 8 * (p->store_base[k + 1] - p->store_base[k]) : 0;
+        for (int t = 0; t < 9; ++t) {
+            if (k_bytes) k_bytes[t] = 0;
+            if (k_entries) k_entries[t] = 0;
+        }
+        for (int c = 0; c < 6; ++c) {
+            if (j_bytes) j_bytes[c] = 0;
+            if (j_entries) j_entries[c] = 0;
+        }
+        if (!p->stored || p->batched) return AIMDHF_OK;
+        // K segments are (output block, subclass) with the output blocks grouped by type
+        auto at = [&](const DevBuf& b, long long i) {
+            long long v = 0;
+            HIPCHECK(hipMemcpy(&v, b.as<long long>() + i, sizeof(long long), hipMemcpyDeviceToHost));
+            return v;
+        };
+        for (int t = 0; t < 9; ++t) {
+            const long long a = 9LL * p->out_off[t], b = 9LL * p->out_off[t + 1];
+            if (k_bytes)
+                k_bytes[t] = 8 * (at(p->d_kdoff, b) - at(p->d_kdoff, a)) + 4 * (at(p->d_kdoff32, b) - at(p->d_kdoff32, a));
+            if (k_entries) k_entries[t] = at(p->d_kseg, b) - at(p->d_kseg, a);
+        }
+        // J segments are indexed by pair; each pair's six segments go to its class
+        const size_t njs = (size_t)p->npair * 6 + 1;
+        std::vector<long long> seg(njs), d64(njs), d32(njs);
+        HIPCHECK(hipMemcpy(seg.data(), p->d_jseg.p, sizeof(long long) * njs, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(d64.data(), p->d_jdoff.p, sizeof(long long) * njs, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(d32.data(), p->d_jdoff32.p, sizeof(long long) * njs, hipMemcpyDeviceToHost));
+        for (int P = 0; P < p->npair; ++P) {
+            const int c = p->pair_cls[P];
+            const size_t a = (size_t)P * 6, b = a + 6;
+            if (j_bytes) j_bytes[c] += 8 * (d64[b] - d64[a]) + 4 * (d32[b] - d32[a]);
+            if (j_entries) j_entries[c] += seg[b] - seg[a];
+        }
+    } catch (const HipError& e) {
+        return e.code;
+    }
+    return AIMDHF_OK;
+}
+
 int aimdhf_profile_enable(aimdhf_plan* p, int on) {
     if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
     try {
@@ -1808,6 +2115,45 @@ int aimdhf_profile_enable(aimdhf_plan* p, int on) {
         return e.code;
     }
     p->prof.on = on != 0;
+    return AIMDHF_OK;
+}
+
+int aimdhf_profile_replay(aimdhf_plan* p, int kind, int reps, void* stream, double* avg_ms) {
+    if (!p || !avg_ms) return fail(AIMDHF_E_ARG, "NULL argument");
+    if (reps < 1) return fail(AIMDHF_E_ARG, "reps must be >= 1");
+    const bool ks = kind >= 55 && kind < 64, js = kind >= 49 && kind < 55;
+    if (!ks && !js) return fail(AIMDHF_E_UNSUPPORTED, "replay covers the store-mode digestion kinds 49..63");
+    if (!p->have_geometry || !p->stored || !p->have_last_store)
+        return fail(AIMDHF_E_STATE, "replay needs a store-mode J/K build at the current geometry");
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t a = nullptr, b = nullptr;
+    try {
+        HIPCHECK(hipEventCreate(&a));
+        HIPCHECK(hipEventCreate(&b));
+        aimdhf::JSArgs ja = p->last_ja;
+        const int t = kind - 55, c = kind - 49;
+        if (js) ja.cls_off = p->cls_pair_off[c];
+        const int nout = ks ? p->out_off[t + 1] - p->out_off[t] : p->ncls[c];
+        // the digestion writes only its unit partials: replays leave J and K alone
+        HIPCHECK(hipEventRecord(a, st));
+        for (int r = 0; r < reps; ++r) {
+            if (ks)
+                aimdhf::launch_kstore(t, p->last_ka, nout, st);
+            else
+                aimdhf::launch_jstore(c, ja, nout, st);
+        }
+        HIPCHECK(hipEventRecord(b, st));
+        HIPCHECK(hipEventSynchronize(b));
+        float ms = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&ms, a, b));
+        *avg_ms = ms / reps;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    } catch (const HipError& e) {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        return e.code;
+    }
     return AIMDHF_OK;
 }
 
@@ -1839,6 +2185,10 @@ int aimdhf_pair_npp(aimdhf_plan* p, int32_t* npp_host, int64_t cap) {
     if (!p || !npp_host) return fail(AIMDHF_E_ARG, "NULL argument");
     if (!p->have_geometry) return fail(AIMDHF_E_STATE, "no geometry set");
     if (cap < p->npair) return fail(AIMDHF_E_ARG, "output too small");
+    if (p->generic) {
+        for (int P = 0; P < p->npair; ++P) npp_host[P] = p->gen_npp[P];
+        return AIMDHF_OK;
+    }
     try {
         HIPCHECK(hipMemcpy(npp_host, p->d_npp.p, sizeof(int) * p->npair, hipMemcpyDeviceToHost));
     } catch (const HipError& e) {

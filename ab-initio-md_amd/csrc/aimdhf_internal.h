@@ -399,6 +399,8 @@ void launch_jlist_count(const ListArgs& a, long long* counts, long long* band, h
 void launch_jlist_fill(const ListArgs& a, hipStream_t st);
 void launch_kstore(int type, const KSArgs& a, int nout, hipStream_t st);
 void launch_jstore(int pclass, const JSArgs& a, int npairs, hipStream_t st);
+void launch_kreduce(int type, const KSArgs& a, int nout, hipStream_t st);
+void launch_jreduce(int pclass, const JSArgs& a, int npairs, hipStream_t st);
 void launch_kcopy(int type, const KSArgs& a, int nout, hipStream_t st);
 void launch_jcopy(int pclass, const JSArgs& a, int npairs, hipStream_t st);
 void launch_fill_int(int* p, size_t n, int v, hipStream_t st);
@@ -434,6 +436,41 @@ void launch_count_quartets(const PairArgs& g, double thr, double sw, int* counts
 int list_bins(double sw);
 void launch_fill_quartets(const PairArgs& g, double thr, double sw, const long long* offs, const long long* base, int2* list,
                           hipStream_t st);
+// ---------------------------------------------------------------- runtime-l path (l <= GEN_LMAX)
+constexpr int GEN_LMAX = 4;
+struct GenArgs {
+    int nao, natm, nset;
+    const int* shell_atom;
+    const int* shell_l;
+    const int* shell_prim;
+    const int* shell_ao;
+    const double* prim_exp;
+    const double* prim_coef;
+    const double* pos;
+    const double* charges;
+    const double* boys;
+    const int* pair_i;          // shell pairs P: pair_i[P] >= pair_j[P]
+    const int* pair_j;
+    const int2* quartets;       // unique (P, Q), P >= Q
+    long long nq;
+    const double* dens;         // J/K: nset symmetric densities
+    long long* fx;              // fixed-point accumulators (J/K planes, or 3 natm gradient words)
+    const double* dc;           // derivative ERIs: Coulomb density, nx exchange densities, factor
+    const double* dx;
+    int nx;
+    double kfac;
+    double *S, *T, *V, *M;      // one-electron outputs
+    const double* D;            // one-electron gradient densities
+    const double* W;
+    int parts;                  // one-electron gradient terms: 1 kinetic, 2 nuclear attraction, 4 overlap
+    double* q;                  // Schwarz bounds per shell pair
+};
+void launch_gen_schwarz(const GenArgs& g, int npair, hipStream_t st);
+void launch_gen_jk(const GenArgs& g, hipStream_t st);
+void launch_gen_grad2(const GenArgs& g, hipStream_t st);
+void launch_gen_oneel(const GenArgs& g, int npair, hipStream_t st);
+void launch_gen_grad1(const GenArgs& g, int npair, hipStream_t st);
+
 void launch_zero64(long long* p, size_t n, hipStream_t st);
 void launch_symmetrize(const double* D, int nset, int n, double* out, hipStream_t st);
 void launch_grad_one_class(int k, const GradArgs& a, const int2* list, long long nq, hipStream_t st);

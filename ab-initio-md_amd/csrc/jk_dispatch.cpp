@@ -71,10 +71,12 @@ DECLS(0) DECLS(1) DECLS(2) DECLS(3) DECLS(4) DECLS(5) DECLS(6) DECLS(7) DECLS(8)
 DECLS(11) DECLS(12) DECLS(13) DECLS(14) DECLS(15) DECLS(16) DECLS(17) DECLS(18) DECLS(19) DECLS(20)
 #undef DECLS
 #define DECLJS(c) void launch_jstore_##c(const JSArgs&, int, hipStream_t); \
+    void launch_jreduce_##c(const JSArgs&, int, hipStream_t); \
     void launch_jcopy_##c(const JSArgs&, int, hipStream_t);
 DECLJS(0) DECLJS(1) DECLJS(2) DECLJS(3) DECLJS(4) DECLJS(5)
 #undef DECLJS
 #define DECLKS(x, y) void launch_kstore_##x##_##y(const KSArgs&, int, hipStream_t); \
+    void launch_kreduce_##x##_##y(const KSArgs&, int, hipStream_t); \
     void launch_kcopy_##x##_##y(const KSArgs&, int, hipStream_t);
 DECLKS(0, 0) DECLKS(0, 1) DECLKS(0, 2) DECLKS(1, 0) DECLKS(1, 1) DECLKS(1, 2) DECLKS(2, 0) DECLKS(2, 1) DECLKS(2, 2)
 #undef DECLKS
@@ -126,6 +128,21 @@ void launch_kstore(int type, const KSArgs& a, int nout, hipStream_t st) {
     static const Fn fns[9] = {launch_kstore_0_0, launch_kstore_0_1, launch_kstore_0_2,
                               launch_kstore_1_0, launch_kstore_1_1, launch_kstore_1_2,
                               launch_kstore_2_0, launch_kstore_2_1, launch_kstore_2_2};
+    fns[type](a, nout, st);
+}
+
+void launch_jreduce(int pclass, const JSArgs& a, int npairs, hipStream_t st) {
+    typedef void (*Fn)(const JSArgs&, int, hipStream_t);
+    static const Fn fns[6] = {launch_jreduce_0, launch_jreduce_1, launch_jreduce_2,
+                              launch_jreduce_3, launch_jreduce_4, launch_jreduce_5};
+    fns[pclass](a, npairs, st);
+}
+
+void launch_kreduce(int type, const KSArgs& a, int nout, hipStream_t st) {
+    typedef void (*Fn)(const KSArgs&, int, hipStream_t);
+    static const Fn fns[9] = {launch_kreduce_0_0, launch_kreduce_0_1, launch_kreduce_0_2,
+                              launch_kreduce_1_0, launch_kreduce_1_1, launch_kreduce_1_2,
+                              launch_kreduce_2_0, launch_kreduce_2_1, launch_kreduce_2_2};
     fns[type](a, nout, st);
 }
 

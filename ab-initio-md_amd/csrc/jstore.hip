@@ -240,14 +240,21 @@ __global__ void k_jreduce(JSArgs a, int npairs) {
 void CAT(launch_jstore_, AIMDHF_PCLASS)(const JSArgs& a, int npairs, hipStream_t st) {
     if (npairs <= 0) return;
     const int nu = a.unit_base[CP + 1] - a.unit_base[CP];
+    if (nu <= 0) return;
+    if (a.nset == 1)
+        k_jstore<CP, 1><<<dim3(nu), dim3(64), 0, st>>>(a);
+    else
+        k_jstore<CP, 2><<<dim3(nu), dim3(64), 0, st>>>(a);
+}
+
+// the unit partials of launch_jstore into the J blocks of this pair class
+void CAT(launch_jreduce_, AIMDHF_PCLASS)(const JSArgs& a, int npairs, hipStream_t st) {
+    if (npairs <= 0) return;
     const int nr = npairs * a.nset * NAB;
-    if (a.nset == 1) {
-        if (nu > 0) k_jstore<CP, 1><<<dim3(nu), dim3(64), 0, st>>>(a);
+    if (a.nset == 1)
         k_jreduce<CP, 1><<<dim3((nr + 127) / 128), dim3(128), 0, st>>>(a, npairs);
-    } else {
-        if (nu > 0) k_jstore<CP, 2><<<dim3(nu), dim3(64), 0, st>>>(a);
+    else
         k_jreduce<CP, 2><<<dim3((nr + 127) / 128), dim3(128), 0, st>>>(a, npairs);
-    }
 }
 
 void CAT(launch_jcopy_, AIMDHF_PCLASS)(const JSArgs& a, int npairs, hipStream_t st) {

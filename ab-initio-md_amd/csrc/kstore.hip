@@ -261,14 +261,21 @@ __global__ void k_kreduce(KSArgs a, int nout) {
 void CAT3X(launch_kstore_, AIMDHF_LX, AIMDHF_LY)(const KSArgs& a, int nout, hipStream_t st) {
     if (nout <= 0) return;
     const int nu = a.unit_base[OT + 1] - a.unit_base[OT];
+    if (nu <= 0) return;
+    if (a.nset == 1)
+        k_kstore<LX, LY, 1><<<dim3(nu), dim3(64), 0, st>>>(a);
+    else
+        k_kstore<LX, LY, 2><<<dim3(nu), dim3(64), 0, st>>>(a);
+}
+
+// the unit partials of launch_kstore into K (every output block is written)
+void CAT3X(launch_kreduce_, AIMDHF_LX, AIMDHF_LY)(const KSArgs& a, int nout, hipStream_t st) {
+    if (nout <= 0) return;
     const int nr = nout * a.nset * NX * NY;
-    if (a.nset == 1) {
-        if (nu > 0) k_kstore<LX, LY, 1><<<dim3(nu), dim3(64), 0, st>>>(a);
+    if (a.nset == 1)
         k_kreduce<LX, LY, 1><<<dim3((nr + 127) / 128), dim3(128), 0, st>>>(a, nout);
-    } else {
-        if (nu > 0) k_kstore<LX, LY, 2><<<dim3(nu), dim3(64), 0, st>>>(a);
+    else
         k_kreduce<LX, LY, 2><<<dim3((nr + 127) / 128), dim3(128), 0, st>>>(a, nout);
-    }
 }
 
 void CAT3X(launch_kcopy_, AIMDHF_LX, AIMDHF_LY)(const KSArgs& a, int nout, hipStream_t st) {

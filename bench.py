@@ -18,11 +18,14 @@ its own density reuse, and its energies and forces are compared with the
 FP64 ones at every step. ``value`` is the mixed step time when it stays
 within |dE| <= 1e-6 Eh and |dF| <= 1e-5 Eh/bohr at every step, else FP64.
 
-Printed: one JSON line with the step times, the dominant kernel's roofline
-(algorithmic flops per launch from aimd_amd.qc.flopmodel over the kernel's
-average HIP-event duration on the engine stream) and a CPU baseline (the
-oracle's integral-direct J/K build and ERI gradient for the same geometry on
-the host cores, scaled to a step with the measured mean SCF iteration count).
+Printed: one JSON line with the step times, the roofline of the kernel with
+the largest time per step (HIP events on the engine stream in one extra step
+with every launch serialised: a derivative-ERI class priced in FP64 flops from
+aimd_amd.qc.flopmodel against the FP64 vector peak, or a store-mode digestion
+kernel priced in the bytes it streams against HBM), the store's streamed and
+unique integral bytes, and a CPU baseline: one whole BOMD step of the oracle
+(SCF from the previous step's density plus the gradient) timed on the host
+cores, with the CPU model.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--precision both|fp64|mixed] [--no-cpu-baseline]
 
@@ -70,6 +73,8 @@ def main() -> int:
     ap.add_argument("--md-steps", type=int, default=0,
                     help="trajectory length of the water32-xlbomd (default 200) and replicas (default 100) workloads")
     ap.add_argument("--replicas", type=int, default=128)
+    ap.add_argument("--check-every", type=int, default=1,
+                    help="water32-xlbomd: compare every n-th mixed step with an FP64 SCF (the last step always)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the engine has no multi-GPU path "
@@ -142,6 +147,7 @@ def main() -> int:
     rec, ms_per_step, ms_step_kind, n_step_kind = timed(eng, run_md)
     # per-kernel figures (roofline): one more step with every launch serialised on the engine
     # stream, so each kernel's HIP-event time is its own and not shared with overlapping ones
+    P_prev = be.last_scf.density.copy()
     eng.set_streams(False)
     steps_timed = args.steps
     args.steps = 1
@@ -175,28 +181,24 @@ def main() -> int:
                  "phase_ms": {k: round(1e3 * float(np.mean([o.info["timings"][k] for o in outs])), 3)
                               for k in outs[0].info["timings"]}}
 
-    # Fock build (J and K kernels + their prologue/epilogue) priced with the algorithmic
-    # work of the unique screened quartets of the last timed geometry
+    # Fock build time per build (J and K kernels, reductions, prologue/epilogue)
     quartets = eng.quartets()
     npp = eng.pair_npp()
-    work = flopmodel.jk_work(be.basis, quartets, npp, nset=1)
     info = eng.jk_info()
+    tinfo = eng.jk_type_info()
     per_build_45 = 1 if info["stored"] else 3        # prologue/epilogue launches per build
     n_builds = max(int(n_kind[45]) // per_build_45, 1)
-    fock_ms = float(ms_kind[eng.FOCK_KINDS].sum()) / n_builds
+    jk_ms = float(ms_kind[eng.FOCK_KINDS].sum()) / n_builds
     store_ms = float(ms_kind[eng.STORE_KINDS].sum())
-    flops = work["total"]
-    jk_ms = fock_ms
     grad_ms = float(ms_kind[eng.GRAD_KINDS].sum())
-    # derivative ERIs of the last timed step priced with the gradient work model; the
-    # dominant single kernel of the step is the heaviest derivative-ERI class kernel
     gw = flopmodel.grad_work(be.basis, eng.grad_quartets(), npp)
-    cls_ms = np.array([ms_kind[21 + k] / max(int(n_kind[21 + k]), 1) for k in range(21)])
-    kd = int(np.argmax(cls_ms))
-    achieved = gw["per_class"][kd] / (cls_ms[kd] * 1e-3) / 1e12
     grad_main_ms = float(ms_kind[21:42].sum())
     per_kernel = {eng.KIND_NAMES[k]: round(float(ms_kind[k]) / max(int(n_kind[k]), 1), 4)
                   for k in range(eng.NKIND) if n_kind[k] > 0}
+    roofline = dominant_kernel_roofline(eng, ms_kind, n_kind, tinfo, gw)
+    unique_b = int(tinfo["unique_bytes"].sum())
+    streamed_b = int(tinfo["k_bytes"].sum() + tinfo["j_bytes"].sum()
+                     + 8 * (tinfo["k_entries"].sum() + tinfo["j_entries"].sum()))
     use_mixed = mixed is not None and mixed["within_bounds"]
     value = mixed["ms_per_step"] if use_mixed else ms_per_step
     out = {
@@ -233,31 +235,22 @@ def main() -> int:
         "kernel_figures": "jk_*, store_*, gradient_*, roofline and avg_ms_per_launch come from one extra step "
                           "with every launch serialised on the engine stream after the timed region",
         "jk_build_ms": round(float(jk_ms), 3),
-        "jk_build_tflops": round(flops / (jk_ms * 1e-3) / 1e12, 3) if jk_ms > 0 else None,
-        "jk_stream_gbs": round(info["gather_bytes"] / (jk_ms * 1e-3) / 1e9, 1) if jk_ms > 0 else None,
+        "jk_traffic": {"unique_integral_bytes": unique_b, "streamed_bytes_per_build": streamed_b,
+                       "streamed_over_unique": round(streamed_b / unique_b, 3) if unique_b else None,
+                       "stream_gbs": round(streamed_b / (jk_ms * 1e-3) / 1e9, 1) if jk_ms > 0 and unique_b else None,
+                       "note": "streamed = gather-ordered integral copies plus 8-byte list entries that one "
+                               "J/K build reads; unique = the screened integrals stored once"},
         "store_build_ms_per_step": round(store_ms, 3),
         "store_bytes": info["store_bytes"],
         "gradient_kernels_ms_per_step": round(float(grad_ms), 3),
         "gradient_tflops": round(gw["total"] / (grad_main_ms * 1e-3) / 1e12, 3) if grad_main_ms > 0 else None,
-        "eri_quartets_per_s": float(len(quartets) / (jk_ms * 1e-3)) if jk_ms > 0 else None,
+        "digested_quartets_per_s": float(len(quartets) / (jk_ms * 1e-3)) if jk_ms > 0 else None,
         "nve_total_energy_span_Eh": float(np.ptp(e_tot)) if e_tot else None,
-        "roofline": {
-            "bound": "mfma",
-            "kernel": f"k_grad_class<{kd}> (derivative ERIs, quartet class {kd}: "
-                      f"({'/'.join(str(v) for v in flopmodel.class_ls(kd))}), fp64 VALU)",
-            "achieved": round(achieved, 4),
-            "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_TFLOPS, 5),
-            "traffic": measured_traffic(kd),
-            "flops_per_launch": float(gw["per_class"][kd]),
-            "avg_launch_ms": round(float(cls_ms[kd]), 4),
-            "note": "FP64 vector kernel priced against MI355X's FP64 peak (78.6 TF/s, vector and matrix)",
-        },
+        "roofline": roofline,
         "avg_ms_per_launch": per_kernel,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sym, md.x, np.mean(iters), args.cpu_threads, be.last_scf.density)
+        out["cpu_baseline"] = cpu_baseline(sym, md.x, P_prev, args.cpu_threads)
     print(json.dumps(out))
     return 0
 
@@ -291,42 +284,51 @@ def bench_water32_xlbomd(args):
         opts = {"precision": precision, "jk_fp32_switch": switch, "grad_fp32_switch": switch}
         return get_backend("hf")(sym, basis="6-31g*", conv_tol=1e-10, conv_tol_grad=1e-7, scf_options=opts)
 
-    def trajectory(be, label, check=None):
+    def trajectory(be, label, record=False):
+        """Timed XL-BOMD run; with record, every step's geometry, density and result are
+        kept on the host for the FP64 checks made after this backend is closed."""
         sysm = md.System.create(sym, x0, v0)
         xl = md.XLBOMD(be, 0.5, k=5)
         xl.initialize(sysm)
-        e_tot, de, dfm, its = [xl.conserved_energy(sysm)], [], [], []
+        e_tot, its, rec = [xl.conserved_energy(sysm)], [], []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        t_check = 0.0
         for k in range(steps):
             r = xl.step(sysm)
             e_tot.append(xl.conserved_energy(sysm))
             its.append(r.info["scf_iterations"])
-            if check is not None:
-                tc = time.perf_counter()
-                check.set_density_guess(r.density)
-                ref = check.compute(sysm.positions)
-                de.append(abs(r.energy - ref.energy))
-                dfm.append(float(np.abs(r.gradient - ref.gradient).max()))
-                t_check += time.perf_counter() - tc
+            if record:
+                rec.append((sysm.positions.copy(), np.asarray(r.density), r.energy, np.asarray(r.gradient)))
             if (k + 1) % 20 == 0:
                 _progress(f"{label}: step {k + 1}/{steps}, E_tot {e_tot[-1]:.8f}")
         torch.cuda.synchronize()
-        ms = 1e3 * (time.perf_counter() - t0 - t_check) / steps
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        be.close()
         drift, span = _drift(e_tot, 0.5)
-        out = {"ms_per_step": round(ms, 2), "drift_Eh_per_ps": drift, "max_abs_dEtot_Eh": span,
-               "scf_iterations_mean": float(np.mean(its))}
-        if check is not None:
-            out.update({"max_abs_dE_Eh": max(de), "max_abs_dF_Eh_bohr": max(dfm),
-                        "within_bounds": bool(max(de) <= 1e-6 and max(dfm) <= 1e-5)})
-        return out
+        return {"ms_per_step": round(ms, 2), "drift_Eh_per_ps": drift, "max_abs_dEtot_Eh": span,
+                "scf_iterations_mean": float(np.mean(its))}, rec
 
-    fp64 = trajectory(backend("fp64"), "fp64")
-    check = backend("fp64")
-    sweep = {}
+    fp64, _ = trajectory(backend("fp64"), "fp64")
+    sweep, recs = {}, {}
     for sw in (1e-5, 1e-7, 1e-9):
-        sweep[f"{sw:g}"] = trajectory(backend("mixed", sw), f"mixed {sw:g}", check)
+        sweep[f"{sw:g}"], recs[f"{sw:g}"] = trajectory(backend("mixed", sw), f"mixed {sw:g}", record=True)
+    # per-step errors of every mixed trajectory: an FP64 SCF at the same geometry, started
+    # from the mixed density (the timed mixed runs above shared the GPU with nothing else)
+    check = backend("fp64")
+    every = max(int(args.check_every), 1)
+    for key, rec in recs.items():
+        de, dfm = [], []
+        for k, (x, dens, e, g) in enumerate(rec):
+            if k % every and k != len(rec) - 1:
+                continue
+            check.set_density_guess(dens)
+            ref = check.compute(x)
+            de.append(abs(e - ref.energy))
+            dfm.append(float(np.abs(g - ref.gradient).max()))
+        _progress(f"checked mixed {key}: {len(de)} steps, max |dE| {max(de):.2e}")
+        sweep[key].update({"max_abs_dE_Eh": max(de), "max_abs_dF_Eh_bohr": max(dfm), "steps_checked": len(de),
+                           "within_bounds": bool(max(de) <= 1e-6 and max(dfm) <= 1e-5)})
+    check.close()
     best = min((v["ms_per_step"], k) for k, v in sweep.items() if v["within_bounds"]) if any(
         v["within_bounds"] for v in sweep.values()) else None
     out = {"metric": "ms per XL-BOMD step (SCF+forces), (H2O)32/6-31G*: FP64 vs mixed FP32-switch sweep",
@@ -423,10 +425,60 @@ def bench_replicas(args):
     return 0
 
 
-def measured_traffic(kernel_class):
-    """HBM bytes per launch of k_grad_class<k> from its committed rocprofv3 --pmc summary
+def dominant_kernel_roofline(eng, ms_kind, n_kind, tinfo, gw):
+    """Roofline of the kernel with the largest time per step in the serialised profiling
+    step, among the kinds that time one kernel per launch: the derivative-ERI classes
+    (FP64 VALU, flops from flopmodel.grad_work), and the store-mode exchange and Coulomb
+    digestion (HBM, the gather-ordered integral copies and list entries one launch
+    streams)."""
+    from aimd_amd.qc import flopmodel
+    kstore = {55 + t: t for t in range(9)}
+    jstore = {49 + c: c for c in range(6)}
+    cands = list(range(21, 42)) + list(kstore) + list(jstore)
+    k = max(cands, key=lambda i: float(ms_kind[i]))
+    avg_ms = float(ms_kind[k]) / max(int(n_kind[k]), 1)
+    name = eng.KIND_NAMES[k]
+    out = {"kind": name, "ms_per_step": round(float(ms_kind[k]), 4), "launches_per_step": int(n_kind[k]),
+           "event_avg_launch_ms": round(avg_ms, 4)}
+    if k >= 49:
+        # store digestion: 20 back-to-back launches with the last build's arguments between
+        # one event pair (the per-launch event pairs of the step add dispatch gaps)
+        avg_ms = eng.profile_replay(k, 20)
+        out["replay_launches"] = 20
+    out["avg_launch_ms"] = round(avg_ms, 4)
+    if k < 42:
+        c = k - 21
+        achieved = gw["per_class"][c] / (avg_ms * 1e-3) / 1e12
+        out.update({"bound": "valu", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS,
+                    "kernel": f"k_grad_class<{c}, double> (derivative ERIs, quartet class "
+                              f"({'/'.join(str(v) for v in flopmodel.class_ls(c))}))",
+                    "flops_per_launch": float(gw["per_class"][c]),
+                    "note": "FP64 vector-ALU kernel (no MFMA instructions) against MI355X's FP64 vector peak"})
+    else:
+        if k in kstore:
+            t = kstore[k]
+            nbytes = int(tinfo["k_bytes"][t] + 8 * tinfo["k_entries"][t])
+            kern = f"k_kstore<{t // 3}, {t % 3}, 1> (exchange from the integral store, output blocks " \
+                   f"{'spd'[t // 3]}{'spd'[t % 3]})"
+        else:
+            c = jstore[k]
+            nbytes = int(tinfo["j_bytes"][c] + 8 * tinfo["j_entries"][c])
+            kern = f"k_jstore<{c}, 1> (Coulomb from the integral store, pair class {eng.PAIR_CLASS_NAMES[c]})"
+        achieved = nbytes / (avg_ms * 1e-3) / 1e9
+        out.update({"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "kernel": kern,
+                    "bytes_per_launch": nbytes,
+                    "note": "bytes = gather-ordered integral copies (8 B FP64, 4 B FP32) and 8-byte list "
+                            "entries one launch streams; density reads hit L2 and are not counted"})
+    out["achieved"] = round(achieved, 4)
+    out["frac"] = round(achieved / out["peak"], 5)
+    out["traffic"] = measured_traffic(out["kind"])
+    return out
+
+
+def measured_traffic(kind):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE scaled by a calibration copy), or None if there is none."""
-    path = os.path.join(HERE, "profiles", f"k_grad_class_{kernel_class}_water16_631gs_pmc.txt")
+    path = os.path.join(HERE, "profiles", f"{kind}_water16_631gs_pmc.txt")
     if not os.path.exists(path):
         return None
     with open(path, encoding="utf-8") as f:
@@ -436,31 +488,37 @@ def measured_traffic(kernel_class):
     return None
 
 
-def cpu_baseline(sym, x, mean_iters, threads, P):
-    """The oracle (C++/OpenMP restatement of the reference algorithm) on the host cores:
-    one integral-direct J/K build and one ERI gradient of the same geometry, scaled to a
-    step as mean_iters * t_jk + t_grad (the one-electron and dense-algebra parts, < 1% of
-    the oracle's step, are not included)."""
-    import numpy as np
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo", encoding="utf-8") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sym, x, P_prev, threads):
+    """One BOMD step of the oracle (C++/OpenMP restatement of the reference algorithm) on
+    the host cores, timed whole: basis and pairs, the integral-direct SCF from the previous
+    step's density to 1e-10 Eh / 1e-7, and the analytic gradient."""
     sys.path.insert(0, HERE)
     from oracle import oracle as O
     n = threads or min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "64") or 64))
     used = O.set_threads(n)
+    t0 = time.perf_counter()
     b = O.make_basis(sym, x, "6-31g*")
-    t0 = time.perf_counter()
     pr = O.Pairs(b)
-    t_pairs = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    pr.jk_direct(0.5 * P)
-    t_jk = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    pr.eri_gradient(P, [P], 0.5)
-    t_grad = time.perf_counter() - t0
-    ms = 1e3 * (mean_iters * t_jk + t_grad + t_pairs)
-    return {"value": round(ms, 1), "unit": "ms", "cores": int(used), "kind": "port",
-            "sample": f"oracle on (H2O)16/6-31G*: pair build + Schwarz {t_pairs:.2f} s, one integral-direct J/K "
-                      f"build {t_jk:.2f} s, one ERI gradient {t_grad:.2f} s; step = {mean_iters:.2f} x J/K + "
-                      f"gradient + pairs"}
+    st = O.scf(b, guess=P_prev, conv_energy=1e-10, conv_commutator=1e-7, pairs=pr)
+    t1 = time.perf_counter()
+    O.scf_gradient(b, st, pr)
+    t2 = time.perf_counter()
+    return {"value": round(1e3 * (t2 - t0), 1), "unit": "ms", "cores": int(used), "kind": "port",
+            "cpu": _cpu_model(),
+            "sample": f"one oracle BOMD step of (H2O)16/6-31G* at the last trajectory geometry, timed whole: SCF "
+                      f"from the previous step's density ({st.iterations} iterations, integral-direct J/K) "
+                      f"{t1 - t0:.2f} s, gradient {t2 - t1:.2f} s"}
 
 
 if __name__ == "__main__":

@@ -224,6 +224,18 @@ int aimdhf_jk_info(aimdhf_plan* plan, int* stored, int64_t* store_bytes, int64_t
                    int64_t* gather_bytes);
 
 /*
+ * The integral store of the current geometry broken down for traffic
+ * accounting: per exchange output type t = 3 l_X + l_Y (9) the gather-ordered
+ * bytes one build streams and the gather-list entries, the same per Coulomb
+ * pair class (6: ss ps pp ds dp dd), and the bytes of the unique screened
+ * integrals per quartet class (21). All zero in direct mode (any pointer may
+ * be NULL). No counterpart in the reference (its integrals are not stored);
+ * for profilers and benchmarks.
+ */
+int aimdhf_jk_type_info(aimdhf_plan* plan, int64_t* k_bytes, int64_t* k_entries, int64_t* j_bytes,
+                        int64_t* j_entries, int64_t* unique_bytes);
+
+/*
  * One-electron matrices of the current geometry into device buffers:
  * S, T, V (nao,nao) and the dipole integrals M (3,nao,nao) about the origin,
  * V over all nuclei as point charges Z_A.
@@ -297,6 +309,17 @@ int aimdhf_grad_two(aimdhf_plan* plan, const double* Dc, int nx, const double* D
  */
 int aimdhf_profile_enable(aimdhf_plan* plan, int on);
 int aimdhf_profile_read(aimdhf_plan* plan, double* ms, int64_t* launches, int nkind, int reset);
+
+/*
+ * Average duration in milliseconds of `reps` back-to-back launches, on
+ * `stream`, of one store-mode digestion kernel with the arguments of the last
+ * J/K build at the current geometry: kind 55 + t (exchange, output type t) or
+ * 49 + c (Coulomb, pair class c). The kernels write only their unit partials,
+ * so J and K are left as they were. No counterpart in the reference; for
+ * benchmarks (one event pair around many launches times the kernel without
+ * the per-launch event and dispatch gaps).
+ */
+int aimdhf_profile_replay(aimdhf_plan* plan, int kind, int reps, void* stream, double* avg_ms);
 
 /* Per-class sizes (21 each) of the energy quartet list and of the last
  * gradient quartet list (either pointer may be NULL). */

@@ -158,9 +158,14 @@ class Basis:
 
 
 def make_basis(symbols, positions, name) -> Basis:
-    key = "".join(str(name).split()).lower()
-    key = _ALIAS.get(key, key)
-    table = _BASIS[key]
+    """Basis of a named set, or of a table {symbol: [[l, [exp, c_1, ...], ...], ...]}
+    in the layout of basis_data.json (extra coefficient columns: general contractions)."""
+    if isinstance(name, dict):
+        table = {k.strip()[:1].upper() + k.strip()[1:].lower(): v for k, v in name.items()}
+    else:
+        key = "".join(str(name).split()).lower()
+        key = _ALIAS.get(key, key)
+        table = _BASIS[key]
     sa, sl, sp, pe, pc = [], [], [0], [], []
     for a, s in enumerate(symbols):
         sym = s.strip()[:1].upper() + s.strip()[1:].lower()

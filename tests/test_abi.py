@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     for n in _declared():
         assert hasattr(L, n), n
     assert set(_declared()) == set(_lib.SIGNATURES)
-    assert L.aimdhf_lmax() == 2
+    assert L.aimdhf_lmax() == 4
 
 
 def test_last_error_is_callable_without_a_device():

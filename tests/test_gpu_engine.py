@@ -151,6 +151,6 @@ def test_density_difference_builds(oracle_lib, mode, precision):
 def test_unsupported_angular_momentum_is_a_value_error():
     from aimd_amd.qc.basis import BasisSet
     from aimd_amd.qc.engine import Engine
-    b = BasisSet("custom", ["H"], np.zeros((1, 3)), [0], [3], [0, 1], [1.0], [1.0])
-    with pytest.raises(ValueError, match="l = 3"):
+    b = BasisSet("custom", ["H"], np.zeros((1, 3)), [0], [5], [0, 1], [1.0], [1.0])
+    with pytest.raises(ValueError, match="l = 5"):
         Engine(b)

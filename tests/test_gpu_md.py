@@ -3,6 +3,7 @@ MD with the GPU backend: XL-BOMD (device-resident auxiliary density) against
 BOMD at a loose SCF tolerance, bit-for-bit continuation through a checkpoint
 file, NVE trajectories and energy drift against the oracle-driven fixtures
 ((H2O)8/cc-pVDZ 20 steps, (H2O)16/6-31G* 5 steps), 100 steps of (H2O)8/cc-pVDZ,
+50 steps of (H2O)16/6-31G*,
 and the (H2O)32/6-31G* single point (FP64 and mixed) against the oracle.
 """
 
@@ -112,7 +113,7 @@ def test_nve_trajectory_against_the_oracle(name, xyz, basis, steps):
 
 
 def test_water8_ccpvdz_100_nve_steps():
-    """Config 3's trajectory length: energy conservation over 100 steps at 0.5 fs."""
+    """Energy conservation of (H2O)8/cc-pVDZ over 100 steps at 0.5 fs."""
     from aimd_amd.md import VelocityVerletNVE, masses_au, maxwell_boltzmann
     s, x = cluster("water8.xyz")
     m = masses_au(s)
@@ -127,6 +128,29 @@ def test_water8_ccpvdz_100_nve_steps():
     slope = np.polyfit(np.arange(len(e)), e, 1)[0]
     print(f"(H2O)8/cc-pVDZ 100 steps: span {np.ptp(e):.3e} Eh, drift {slope * 2000:.3e} Eh/ps", flush=True)
     assert np.ptp(e) < 1e-3
+
+
+def test_water16_631gs_50_nve_steps():
+    """The (H2O)16/6-31G* trajectory of the benchmark: 50 steps at 0.5 fs from 300 K, the
+    first 5 on the oracle-driven trajectory, total energy bounded with no systematic drift."""
+    from aimd_amd.md import VelocityVerletNVE, masses_au, maxwell_boltzmann
+    ref = _fixture("oracle_nve_water16_631gs.json")
+    s, x = cluster("water16.xyz")
+    m = masses_au(s)
+    traj = VelocityVerletNVE(_hf(s, "6-31g*", conv_tol=1e-10, conv_tol_grad=1e-8), m, x,
+                             maxwell_boltzmann(m, 300.0, seed=1), 0.5)
+    traj.initialize()
+    e = [traj.total_energy()]
+    for k in range(50):
+        traj.step()
+        e.append(traj.total_energy())
+        if k < len(ref["energy"]) - 1:
+            assert np.abs(traj.x - np.array(ref["positions"][k + 1])).max() < 1e-6
+            assert abs(traj.result.energy - ref["energy"][k + 1]) < 1e-7
+    e = np.array(e)
+    slope = np.polyfit(np.arange(len(e)), e, 1)[0]
+    print(f"(H2O)16/6-31G* 50 steps: span {np.ptp(e):.3e} Eh, straight-line change {slope * 50:.3e} Eh", flush=True)
+    assert np.ptp(e) < 3e-3 and abs(slope * 50) < 1e-3
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

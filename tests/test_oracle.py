@@ -417,3 +417,143 @@ def test_one_electron_gradient_terms_by_finite_differences(oracle_lib):
         assert gk[a, d] == pytest.approx(fd[0], abs=1e-8)
         assert gv[a, d] == pytest.approx(fd[1], abs=1e-8)
         assert gs[a, d] == pytest.approx(fd[2], abs=1e-8)
+
+
+# ------------------------------------------------------- f and g shells
+# Shells past d only come from custom tables (no shipped basis has them): He
+# with f and H with g, contracted, on different atoms.
+FG_DATA = {"He": [[0, [2.0, 1.0]], [1, [3.0, 0.4], [0.7, 0.7]], [3, [2.5, 0.4], [0.9, 0.6]]],
+           "H": [[0, [1.5, 1.0]], [2, [1.1, 1.0]], [4, [1.3, 1.0]]]}
+FG_POS = np.array([[0.1, -0.2, 0.0], [0.3, 0.9, 1.6]])
+
+
+def _cart_factor(a):
+    def df(n):
+        return math.prod(range(n, 0, -2)) if n > 0 else 1
+    l = sum(a)
+    return math.sqrt(df(2 * l - 1) / (df(2 * a[0] - 1) * df(2 * a[1] - 1) * df(2 * a[2] - 1)))
+
+
+def _comps(l):
+    return [(x, y, l - x - y) for x in range(l, -1, -1) for y in range(l - x, -1, -1)]
+
+
+def test_fg_shells_are_unit_normalised_and_symmetric(oracle_lib):
+    O = oracle_lib
+    b = O.make_basis(["He", "H"], FG_POS, FG_DATA)
+    assert (int(b.shell_l.max()), b.nao) == (4, 36)
+    S, T, V, M = O.one_electron(b)
+    np.testing.assert_allclose(np.diag(S), 1.0, rtol=0, atol=1e-13)
+    for X in (S, T, V, M[0], M[1], M[2]):
+        np.testing.assert_allclose(X, X.T, rtol=0, atol=1e-14)
+    eri = O.Pairs(b).eri(0.0)
+    for perm in ((1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1)):
+        np.testing.assert_allclose(eri, eri.transpose(perm), rtol=0, atol=1e-14)
+    n = b.nao
+    diag = eri.reshape(n * n, n * n)
+    assert np.all(np.linalg.eigvalsh(diag) > -1e-12)      # (pq|rs) is a positive semidefinite Gram matrix
+
+
+def _ghost_basis(O, lo, alpha):
+    """Ghost atom (no charge) at the origin with uncontracted shells l = lo-1, lo, lo+1 of
+    one exponent, plus charged atoms carrying s, p and d shells."""
+    table = {"He": [[lo - 1, [alpha, 1.0]], [lo, [alpha, 1.0]], [lo + 1, [alpha, 1.0]]],
+             "Li": [[0, [1.1, 0.6], [0.4, 0.5]], [1, [0.9, 1.0]], [2, [0.7, 1.0]]],
+             "H": [[0, [1.3, 1.0]], [1, [0.6, 1.0]]]}
+    pos = np.array([[0.0, 0.0, 0.0], [0.4, -0.9, 1.2], [-1.1, 0.5, -0.6]])
+
+    class Ghost(O.Basis):
+        @property
+        def charges(self):
+            return np.array([0.0, 3.0, 1.0])
+
+    def make(x):
+        b = O.make_basis(["He", "Li", "H"], x, table)
+        return Ghost(**{k: getattr(b, k) for k in b.__dataclass_fields__})
+    return make, pos
+
+
+@pytest.mark.parametrize("lo", [2, 3])
+def test_fg_integrals_follow_the_centre_derivative_of_lower_l(oracle_lib, lo):
+    """For an unnormalised primitive g_a = x^ax y^ay z^az exp(-alpha r^2) on centre G,
+    d/dG_k g_a = 2 alpha g_{a+1k} - a_k g_{a-1k}. With l = lo pinned (d by the l <= 2
+    closed forms and finite differences, f by this test at lo = 2), central differences
+    of the l = lo integrals in the ghost centre give the l = lo + 1 integrals: S, T, V,
+    the dipole integrals and the ERIs, independently of the Hermite recursions."""
+    O = oracle_lib
+    alpha, h = 0.8, 1e-4
+    make, pos = _ghost_basis(O, lo, alpha)
+    b = make(pos)
+    sh = {int(l): (int(b.shell_ao[i]), float(b.prim_coef[b.shell_prim[i]])) for i, l in enumerate(b.shell_l)
+          if b.shell_atom[i] == 0}
+    other = [int(i) for i in range(b.nao) if b.ao_atom[i] != 0]
+
+    def ints(x):
+        bb = make(x)
+        S, T, V, M = O.one_electron(bb)
+        return [S, T, V, M[0], M[1], M[2]], O.Pairs(bb).eri(0.0)
+
+    base1, base2 = ints(pos)
+    for k in range(3):
+        xp, xm = pos.copy(), pos.copy()
+        xp[0, k] += h
+        xm[0, k] -= h
+        p1, p2 = ints(xp)
+        m1, m2 = ints(xm)
+        for ci, a in enumerate(_comps(lo)):
+            up = list(a)
+            up[k] += 1
+            dn = list(a)
+            dn[k] -= 1
+            # AO index and normalisation (contraction coefficient x Cartesian factor) of each function
+            ia = sh[lo][0] + ci
+            na = sh[lo][1] * _cart_factor(a)
+            iu = sh[lo + 1][0] + _comps(lo + 1).index(tuple(up))
+            nu = sh[lo + 1][1] * _cart_factor(up)
+            if a[k] > 0:
+                idn = sh[lo - 1][0] + _comps(lo - 1).index(tuple(dn))
+                nd = sh[lo - 1][1] * _cart_factor(dn)
+            for X, XP, XM in zip(base1, p1, m1):
+                d = (XP[ia, other] - XM[ia, other]) / (2 * h) / na
+                if a[k] > 0:
+                    d = d + a[k] * X[idn, other] / nd
+                np.testing.assert_allclose(X[iu, other] / nu, d / (2 * alpha), rtol=0,
+                                           atol=2e-8 * max(1.0, np.abs(X[iu, other] / nu).max()))
+            ix = np.ix_(other, other, other)
+            d = (p2[ia][ix] - m2[ia][ix]) / (2 * h) / na
+            if a[k] > 0:
+                d = d + a[k] * base2[idn][ix] / nd
+            np.testing.assert_allclose(base2[iu][ix] / nu, d / (2 * alpha), rtol=0,
+                                       atol=2e-8 * max(1.0, np.abs(base2[iu][ix] / nu).max()))
+
+
+def test_fg_derivative_integrals_by_finite_differences(oracle_lib):
+    """The one-electron gradient terms and the ERI gradient of the f/g basis at fixed
+    densities are the central differences of tr(D T), tr(D V), -tr(W S) and
+    1/2 tr(D J(D)) - 1/4 tr(D K(D))."""
+    O = oracle_lib
+    rng = np.random.default_rng(17)
+    b = O.make_basis(["He", "H"], FG_POS, FG_DATA)
+    n = b.nao
+    A = rng.standard_normal((n, n)) * 0.1
+    D = A @ A.T
+    W = rng.standard_normal((n, n)) * 0.1
+    W = W + W.T
+    gk, gv, gs = O.one_electron_gradient_terms(b, D, W)
+    g2 = O.Pairs(b).eri_gradient(D, [D], 0.5, thr=0.0)
+    h = 1e-4
+
+    def energies(x):
+        bb = O.with_positions(b, x)
+        S, T, V, _ = O.one_electron(bb)
+        J, K = O.Pairs(bb).jk_direct(D, 0.0)
+        return np.array([np.vdot(D, T), np.vdot(D, V), -np.vdot(W, S), 0.5 * np.vdot(D, J) - 0.25 * np.vdot(D, K)])
+
+    for atom in range(2):
+        for k in range(3):
+            xp, xm = FG_POS.copy(), FG_POS.copy()
+            xp[atom, k] += h
+            xm[atom, k] -= h
+            fd = (energies(xp) - energies(xm)) / (2 * h)
+            got = np.array([gk[atom, k], gv[atom, k], gs[atom, k], g2[atom, k]])
+            np.testing.assert_allclose(got, fd, rtol=0, atol=2e-7 * max(1.0, np.abs(fd).max()))
