@@ -43,9 +43,11 @@ SIGNATURES = {
     "aimdhf_tc2_init": (_I, [_I, _P, _P, _P]),
     "aimdhf_batch_purify": (_I, [_I, _I, _P, _P, _P, _I, _D, _P, _P]),
     "aimdhf_batch_diis_solve": (_I, [_I, _I, _P, _P, _P, _P]),
+    "aimdhf_batch_set_jk_lanes": (_I, [_P, _I]),
     "aimdhf_tc2_error": (_I, [_I, _P, _P, _P]),
     "aimdhf_jk_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_jk_type_info": (_I, [_P, _P, _P, _P, _P, _P]),
+    "aimdhf_set_store_units": (_I, [_P, _I]),
     "aimdhf_profile_replay": (_I, [_P, _I, _I, _P, _P]),
     "aimdhf_one_electron": (_I, [_P, _P, _P, _P, _P, _P]),
     "aimdhf_grad_one": (_I, [_P, _P, _P, _P, _P]),

@@ -66,7 +66,10 @@ class ReplicaBatch:
     change and the largest commutator element, Schwarz threshold, gradient
     screening; ``precision`` "fp64" or "mixed" (FP32 for quartets below the
     switch thresholds in the integral evaluation and the derivative
-    integrals, as the single-molecule engine).
+    integrals, as the single-molecule engine). ``jk_lanes`` (16, 32 or 64)
+    fixes how many lanes digest one J/K element; None lets :meth:`autotune`
+    time the choices on the first geometry. The choice sets the summation
+    order, so runs that must agree bit for bit across processes fix it.
     """
 
     def __init__(self, molecules: Sequence[tuple], basis: str = "6-31g**", conv_tol: float = 1e-10,
@@ -74,7 +77,7 @@ class ReplicaBatch:
                  schwarz_threshold: float = qengine.DEFAULT_SCHWARZ,
                  gradient_screening: float = qengine.DEFAULT_SCHWARZ_GRAD, precision: str = "fp64",
                  jk_fp32_switch: float = 1e-5, grad_fp32_switch: float = 1e-5, purify_steps: int = 60,
-                 lindep_threshold: float = 1e-7, capture: bool = True) -> None:
+                 lindep_threshold: float = 1e-7, capture: bool = True, jk_lanes: int | None = None) -> None:
         if not molecules:
             raise ValueError("a batch needs at least one molecule")
         if not 2 <= diis_space <= 12:
@@ -106,6 +109,11 @@ class ReplicaBatch:
         self.nocc = np.array(occ, dtype=float)                      # (nrep, 2): alpha, beta
         self.unrestricted = np.array([m != 1 for _, _, m in self.molecules])
         self._create_plan(precision, jk_fp32_switch, grad_fp32_switch)
+        self.jk_lanes = None
+        self.jk_times: dict[int, float] = {}
+        if jk_lanes is not None:
+            _lib.check(_lib.lib().aimdhf_batch_set_jk_lanes(self._h, int(jk_lanes)))
+            self.jk_lanes = int(jk_lanes)
         self._alloc()
         self._graph = None
         self.Ds = None                                              # (nrep, 2, ld, ld) last SCF densities
@@ -238,6 +246,30 @@ class ReplicaBatch:
     def _jk(self, D: torch.Tensor) -> None:
         _lib.check(_lib.lib().aimdhf_jk(self._h, 2, _ptr(D), _ptr(self.J), _ptr(self.K), _stream()))
 
+    def autotune(self, reps: int = 5) -> dict[int, float]:
+        """Time the batched J/K digestion at 64, 32 and 16 lanes per output element on the
+        current geometry's integral tensors (reps builds each, HIP events on the current
+        stream), keep the fastest, and drop the captured SCF graph so the next iteration
+        records the new launch. Returns {lanes: ms per build}."""
+        L = _lib.lib()
+        D = self.Ds if self.Ds is not None else self._guess()
+        times = {}
+        for lanes in (64, 32, 16):
+            _lib.check(L.aimdhf_batch_set_jk_lanes(self._h, lanes))
+            self._jk(D)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                self._jk(D)
+            b.record()
+            b.synchronize()
+            times[lanes] = a.elapsed_time(b) / reps
+        best = min(times, key=times.get)
+        _lib.check(L.aimdhf_batch_set_jk_lanes(self._h, best))
+        self.jk_lanes, self.jk_times = best, times
+        self._graph = None
+        return times
+
     def _iteration(self) -> None:
         """One SCF iteration of every replica on static buffers (captured as a graph)."""
         L = _lib.lib()
@@ -328,6 +360,8 @@ class ReplicaBatch:
         """SCF of every replica at ``positions``; the guess is the previous densities
         (or SAD on the first call) unless given."""
         self._set_geometry(positions)
+        if self.jk_lanes is None:
+            self.autotune()
         D0 = guess if guess is not None else (self.Ds if self.Ds is not None else self._guess())
         self.D.copy_(D0)
         self.frozen.zero_()

@@ -107,6 +107,11 @@ class Engine:
         _lib.check(_lib.lib().aimdhf_set_jk_mode(self._h, self.JK_MODES[mode], int(max_store_bytes)))
         self.jk_mode = mode
 
+    def set_store_units(self, items: int) -> None:
+        """Lane items per store-mode digestion work unit (from the next set_geometry)."""
+        _lib.check(_lib.lib().aimdhf_set_store_units(self._h, int(items)))
+        self.store_units = int(items)
+
     PRECISIONS = {"fp64": 0, "mixed": 1}
 
     def set_precision(self, precision: str = "fp64", jk_switch: float = 0.0, grad_switch: float = 0.0,

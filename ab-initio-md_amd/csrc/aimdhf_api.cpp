@@ -247,8 +247,8 @@ struct aimdhf_plan {
     DevBuf d_store, d_qmap, d_kcnt, d_jcnt, d_kseg, d_jseg, d_klist, d_jlist, d_lerr;
     DevBuf d_kdata, d_jdata, d_kdoff, d_jdoff, d_kdata32, d_jdata32, d_kdoff32, d_jdoff32, d_kbig, d_jbig;
     long long kdata_total = 0, jdata_total = 0, kdata32_total = 0, jdata32_total = 0;
-    // work units of the store kernels (lane items per unit: UNIT_ITEMS)
-    static constexpr int UNIT_ITEMS = 1024;
+    // work units of the store kernels: at most unit_items lane items each
+    int unit_items = 1024;
     int kunit_base[10] = {0}, junit_base[7] = {0};
     DevBuf d_kunit_os, d_kunit_e, d_out_unit, d_kpart2, d_junit_pc, d_junit_e, d_pair_unit, d_jpart2;
     // density-difference builds: per-unit largest Schwarz products, max |dD|,
@@ -271,6 +271,7 @@ struct aimdhf_plan {
     // integrals as one dense n_r^4 tensor per replica
     bool batched = false;
     int nrep = 1, bld = 0;
+    int batch_jk_lanes = 64;        // lanes per output element of the batched J/K digestion
     std::vector<int> rep_shell, rep_atom, rep_nao, shell_rep, shell_first, shell_pair0, pair_lo;
     std::vector<long long> rep_toff, rep_item;
     long long tens_total = 0;
@@ -741,7 +742,7 @@ struct aimdhf_plan {
         ta.nout = nout;
         ta.npair = npair;
         ta.nbig = nbig;
-        ta.unit_items = UNIT_ITEMS;
+        ta.unit_items = unit_items;
         for (int t = 0; t < 10; ++t) ta.out_off[t] = out_off[t];
         for (int c = 0; c < 7; ++c) ta.cls_pair_off[c] = cls_pair_off[c];
         ta.out_x = d_out_x.as<int>();
@@ -1646,7 +1647,8 @@ static int jk_batch(aimdhf_plan* p, int nset, const double* dens, double* J, dou
             aimdhf::launch_zero_d(K, n, st);
         });
         p->prof.time(49, st, [&] {
-            aimdhf::launch_batch_jk(p->batch_args(), nset, p->d_dsym.as<double>(), J, K, p->rep_item[p->nrep], st);
+            aimdhf::launch_batch_jk(p->batch_args(), nset, p->d_dsym.as<double>(), J, K, p->rep_item[p->nrep],
+                                    p->batch_jk_lanes, st);
         });
         HIPCHECK(hipGetLastError());
     } catch (const HipError& e) {
@@ -1992,6 +1994,14 @@ int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, doub
     return AIMDHF_OK;
 }
 
+int aimdhf_batch_set_jk_lanes(aimdhf_plan* p, int lanes) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    if (!p->batched) return fail(AIMDHF_E_STATE, "not a batched plan");
+    if (lanes != 16 && lanes != 32 && lanes != 64) return fail(AIMDHF_E_ARG, "lanes must be 16, 32 or 64");
+    p->batch_jk_lanes = lanes;
+    return AIMDHF_OK;
+}
+
 int aimdhf_batch_purify(int nmat, int ld, const double* F, const double* nocc, double* P, int steps, double tol,
                         double* err, void* stream) {
     if (nmat < 0 || ld <= 0 || ld > 64 || steps < 0 || !(tol >= 0.0)) return fail(AIMDHF_E_ARG, "bad purify arguments");
@@ -2044,6 +2054,13 @@ int aimdhf_set_precision(aimdhf_plan* p, int mode, double jk_switch, double grad
     p->jk_switch = jk_switch;
     p->grad_switch = grad_switch;
     p->delta_fp32_max = delta_switch;
+    return AIMDHF_OK;
+}
+
+int aimdhf_set_store_units(aimdhf_plan* p, int items) {
+    if (!p) return fail(AIMDHF_E_ARG, "NULL plan");
+    if (items < 64 || items > (1 << 16)) return fail(AIMDHF_E_ARG, "store work units take 64 .. 65536 lane items");
+    p->unit_items = items;
     return AIMDHF_OK;
 }
 

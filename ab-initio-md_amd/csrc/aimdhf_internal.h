@@ -386,7 +386,7 @@ struct BatchArgs {
 };
 void launch_batch_scatter(const BatchArgs& b, long long total, hipStream_t st);
 void launch_batch_jk(const BatchArgs& b, int nset, const double* dens, double* J, double* K, long long nitems,
-                     hipStream_t st);
+                     int lanes, hipStream_t st);
 void launch_batch_purify(int nmat, int ld, const double* F, const double* nocc, double* P, int steps, double tol,
                          double* err, hipStream_t st);
 void launch_batch_diis_solve(int nrep, int m, const double* G, const double* r, double* c, hipStream_t st);

@@ -55,15 +55,17 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// one wavefront per output element (r, m, c):
+// G lanes (64, 32 or 16) per output element (r, m, c), lanes over (l, s):
 //   J_x[m][c] = sum_{l,s} (m c|l s) D_x[l][s]     (the contiguous slab T[m][c])
-//   K_x[m][c] = sum_{l,s} (m l|c s) D_x[l][s]     (rows T[m][l][c][.], lanes over s)
-// fixed per-lane order and a butterfly sum: bitwise reproducible
+//   K_x[m][c] = sum_{l,s} (m l|c s) D_x[l][s]     (rows T[m][l][c][.])
+// both against the same density element; fixed per-lane order and a butterfly
+// sum within the G lanes: bitwise reproducible for a given G
+template <int G>
 __global__ __launch_bounds__(256) void k_batch_jk(BatchArgs b, int nset, const double* __restrict__ dens,
                                                   double* __restrict__ J, double* __restrict__ K, long long nitems) {
-    const long long item = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (item >= nitems) return;
+    const long long item = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    if (item >= nitems) return;                 // whole groups of G lanes leave together
     int r = 0;
     for (int step = 1 << 20; step > 0; step >>= 1)
         if (r + step < b.nrep && b.rep_item[r + step] <= item) r += step;
@@ -75,21 +77,27 @@ __global__ __launch_bounds__(256) void k_batch_jk(BatchArgs b, int nset, const d
     const double* D0 = dens + (size_t)r * nset * ld2;
     double ja[2] = {0.0, 0.0}, ka[2] = {0.0, 0.0};
     const double* slab = T + ((size_t)m * N + c) * N * N;
-    for (int e = lane; e < N * N; e += 64) {
-        const double t = slab[e];
-        const size_t de = (size_t)(e / N) * ld + e % N;
-        ja[0] += t * D0[de];
-        if (nset == 2) ja[1] += t * D0[ld2 + de];
-    }
-    for (int s = lane; s < N; s += 64)
-        for (int l = 0; l < N; ++l) {
-            const double t = T[(((size_t)m * N + l) * N + c) * N + s];
-            const size_t de = (size_t)l * ld + s;
-            ka[0] += t * D0[de];
-            if (nset == 2) ka[1] += t * D0[ld2 + de];
+    const double* krow = T + (size_t)m * N * N * N + (size_t)c * N;   // + l N^2 + s
+    for (int e = lane; e < N * N; e += G) {
+        const int l = e / N, s = e % N;
+        const size_t de = (size_t)l * ld + s;
+        const double tj = slab[e], tk = krow[(size_t)l * N * N + s];
+        const double d0 = D0[de];
+        ja[0] += tj * d0;
+        ka[0] += tk * d0;
+        if (nset == 2) {
+            const double d1 = D0[ld2 + de];
+            ja[1] += tj * d1;
+            ka[1] += tk * d1;
         }
+    }
     for (int x = 0; x < nset; ++x) {
-        const double jv = wave_sum(ja[x]), kv = wave_sum(ka[x]);
+        double jv = ja[x], kv = ka[x];
+#pragma unroll
+        for (int o = G / 2; o >= 1; o >>= 1) {
+            jv += __shfl_xor(jv, o, 64);
+            kv += __shfl_xor(kv, o, 64);
+        }
         if (lane == 0) {
             const size_t o = ((size_t)r * nset + x) * ld2 + (size_t)m * ld + c;
             J[o] = jv;
@@ -104,9 +112,19 @@ __global__ __launch_bounds__(256) void k_batch_jk(BatchArgs b, int nset, const d
 // an iterate with |tr X - tr X^2| < tol held (more steps would amplify its
 // rounding errors), and two final unheld steps that square the remaining error.
 // err[i] receives the idempotency error |tr X - tr X^2| before those two steps.
+//
+// X^2 on the matrix cores: X sits zero-padded to 64 x 64 in LDS (row stride 65
+// doubles, so the 16 rows one operand load touches fall in distinct banks);
+// wavefront w forms the 16 x 64 row block w of X^2 as four 16 x 16 tiles, each
+// 16 steps of v_mfma_f64_16x16x4f64 over k. Operand and result lanes (measured on
+// gfx950): A(i, k) from lane i + 16 k, B(k, j) from lane j + 16 k, D(i, j) in
+// register i / 4 of lane j + 16 (i % 4). Each result element has one owner lane,
+// which applies the update.
 constexpr int PUR_THREADS = 256;
 constexpr int PUR_MAXLD = 64;
-constexpr int PUR_PER_THREAD = PUR_MAXLD * PUR_MAXLD / PUR_THREADS;
+constexpr int PUR_LS = PUR_MAXLD + 1;       // LDS row stride (doubles)
+
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
 #pragma unroll
@@ -120,7 +138,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 __global__ __launch_bounds__(PUR_THREADS) void k_batch_purify(int ld, const double* __restrict__ F,
                                                               const double* __restrict__ nocc, double* __restrict__ P,
                                                               int steps, double tol, double* __restrict__ err) {
-    __shared__ double X[PUR_MAXLD * PUR_MAXLD];
+    __shared__ double X[PUR_MAXLD * PUR_LS];
     __shared__ double red[4], lo[4], hi[4];
     const int mat = blockIdx.x, t = threadIdx.x, n2 = ld * ld;
     const double* Fm = F + (size_t)mat * n2;
@@ -145,27 +163,34 @@ __global__ __launch_bounds__(PUR_THREADS) void k_batch_purify(int ld, const doub
     const double emin = fmin(fmin(lo[0], lo[1]), fmin(lo[2], lo[3]));
     const double emax = fmax(fmax(hi[0], hi[1]), fmax(hi[2], hi[3]));
     const double inv = 1.0 / (emax - emin);
-    for (int e = t; e < n2; e += PUR_THREADS) X[e] = ((e / ld == e % ld ? emax : 0.0) - Fm[e]) * inv;
+    for (int e = t; e < PUR_MAXLD * PUR_LS; e += PUR_THREADS) {
+        const int i = e / PUR_LS, j = e % PUR_LS;
+        X[e] = (i < ld && j < ld) ? ((i == j ? emax : 0.0) - Fm[i * ld + j]) * inv : 0.0;
+    }
     __syncthreads();
     const double ne = nocc[mat];
-    double x2[PUR_PER_THREAD];
+    const int w = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
+    const int row0 = 16 * w + lk;                   // rows row0 + 4 v (v < 4) of the lane's results
     double last = 0.0;
     for (int it = 0; it < steps + 2; ++it) {
-        double dx = 0.0, dx2 = 0.0;
+        v4d acc[4];
 #pragma unroll
-        for (int k = 0; k < PUR_PER_THREAD; ++k) {
-            const int e = t + k * PUR_THREADS;
-            double v = 0.0;
-            if (e < n2) {
-                const int i = e / ld, j = e % ld;
-                for (int q = 0; q < ld; ++q) v += X[i * ld + q] * X[q * ld + j];
-                if (i == j) {
-                    dx += X[e];
-                    dx2 += v;
-                }
+        for (int tj = 0; tj < 4; ++tj) acc[tj] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int k0 = 0; k0 < PUR_MAXLD; k0 += 4) {
+            const double a = X[(16 * w + li) * PUR_LS + k0 + lk];
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                const double b = X[(k0 + lk) * PUR_LS + 16 * tj + li];
+                acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[tj], 0, 0, 0);
             }
-            x2[k] = v;
         }
+        // traces: the diagonal of X, and of X^2 in the wavefront's diagonal tile
+        double dx = 0.0, dx2 = 0.0;
+        if (t < PUR_MAXLD) dx = X[t * PUR_LS + t];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (row0 + 4 * v == 16 * w + li) dx2 += acc[w][v];
         const double tr = block_sum(dx, red), tr2 = block_sum(dx2, red);
         if (it == steps) last = fabs(tr - tr2);
         const bool hold = it < steps && fabs(tr - tr2) < tol;
@@ -173,14 +198,19 @@ __global__ __launch_bounds__(PUR_THREADS) void k_batch_purify(int ld, const doub
         __syncthreads();
         if (!hold)
 #pragma unroll
-            for (int k = 0; k < PUR_PER_THREAD; ++k) {
-                const int e = t + k * PUR_THREADS;
-                if (e < n2) X[e] = square ? x2[k] : 2.0 * X[e] - x2[k];
-            }
+            for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double& x = X[(row0 + 4 * v) * PUR_LS + 16 * tj + li];
+                    x = square ? acc[tj][v] : 2.0 * x - acc[tj][v];
+                }
         __syncthreads();
     }
     double* Pm = P + (size_t)mat * n2;
-    for (int e = t; e < n2; e += PUR_THREADS) Pm[e] = 0.5 * (X[e] + X[(e % ld) * ld + e / ld]);
+    for (int e = t; e < n2; e += PUR_THREADS) {
+        const int i = e / ld, j = e % ld;
+        Pm[e] = 0.5 * (X[i * PUR_LS + j] + X[j * PUR_LS + i]);
+    }
     if (t == 0) err[mat] = last;
 }
 
@@ -249,9 +279,16 @@ void launch_batch_scatter(const BatchArgs& b, long long total, hipStream_t st) {
 }
 
 void launch_batch_jk(const BatchArgs& b, int nset, const double* dens, double* J, double* K, long long nitems,
-                     hipStream_t st) {
-    if (nitems > 0)
-        k_batch_jk<<<dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, st>>>(b, nset, dens, J, K, nitems);
+                     int lanes, hipStream_t st) {
+    if (nitems <= 0) return;
+    const long long per = 256 / lanes;          // output elements per workgroup
+    const dim3 grid((unsigned)((nitems + per - 1) / per));
+    if (lanes == 16)
+        k_batch_jk<16><<<grid, dim3(256), 0, st>>>(b, nset, dens, J, K, nitems);
+    else if (lanes == 32)
+        k_batch_jk<32><<<grid, dim3(256), 0, st>>>(b, nset, dens, J, K, nitems);
+    else
+        k_batch_jk<64><<<grid, dim3(256), 0, st>>>(b, nset, dens, J, K, nitems);
 }
 
 }  // namespace aimdhf

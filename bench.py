@@ -389,7 +389,9 @@ def bench_replicas(args):
             wall = time.perf_counter() - t0
             results["fp64"] = {"ms_per_step": round(1e3 * wall / steps, 3),
                                "replica_steps_per_s": round(batch.nrep * steps / wall, 1),
-                               "scf_iterations_mean": float(np.mean(iters))}
+                               "scf_iterations_mean": float(np.mean(iters)),
+                               "jk_lanes": batch.jk_lanes,
+                               "jk_ms_by_lanes": {str(k): round(v, 4) for k, v in batch.jk_times.items()}}
         else:
             batch.compute(x0)
             torch.cuda.synchronize()

@@ -186,13 +186,22 @@ int aimdhf_tc2_init(int n, const double* F, double* X0, void* stream);
  * device F (nmat, ld, ld), nocc (nmat) doubles): trace-correcting purification
  * from the Gershgorin start, each matrix in one workgroup's LDS for all steps;
  * `steps` held steps (an iterate with idempotency error below tol is held), then
- * two unheld steps. P (nmat, ld, ld) receives the symmetrised projectors, err
- * (nmat) the idempotency error before the last two steps. No plan; no memory
+ * two unheld steps; X^2 on the matrix cores (FP64 MFMA). P (nmat, ld, ld)
+ * receives the symmetrised projectors, err (nmat) the idempotency error before
+ * the last two steps. No plan; no memory
  * clearing, so a graph may capture it. The batched counterpart of the aufbau of
  * aimd/qc/scf.py:913-921 (there: eigenvectors of one Fock matrix at a time).
  */
 int aimdhf_batch_purify(int nmat, int ld, const double* F, const double* nocc, double* P, int steps, double tol,
                         double* err, void* stream);
+
+/*
+ * Lanes (16, 32 or 64; default 64) that digest one output element of a
+ * batched plan's J/K, for batches of small replicas where a full wavefront per
+ * element leaves lanes idle. A tuning knob with no counterpart in the
+ * reference; aimd_amd.batch.ReplicaBatch.autotune times the choices.
+ */
+int aimdhf_batch_set_jk_lanes(aimdhf_plan* plan, int lanes);
 
 /*
  * Pulay DIIS coefficients of nrep independent problems (the solve of
@@ -220,6 +229,14 @@ int aimdhf_set_streams(aimdhf_plan* plan, int concurrent);
  * gather-list lengths of the K and J kernels and the bytes of their
  * gather-ordered integral data (FP64 and, in mixed precision, FP32), i.e. what
  * one J/K build streams (any pointer may be NULL). */
+/*
+ * Lane items per work unit of the store-mode J/K digestion (64 .. 65536,
+ * default 1024; one wavefront digests one unit), from the next
+ * aimdhf_set_geometry on. A tuning knob with no counterpart in the
+ * reference; results do not depend on it beyond the order of the unit sums.
+ */
+int aimdhf_set_store_units(aimdhf_plan* plan, int items);
+
 int aimdhf_jk_info(aimdhf_plan* plan, int* stored, int64_t* store_bytes, int64_t* k_entries, int64_t* j_entries,
                    int64_t* gather_bytes);
 

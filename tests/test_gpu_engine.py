@@ -55,9 +55,7 @@ def test_schwarz_quartets_and_jk(oracle_lib, mol, basis, mode):
         assert len(ql) == len(oq)
         key = lambda a: a[:, 0].astype(np.int64) * (1 << 32) + a[:, 1]
         np.testing.assert_array_equal(np.sort(key(ql)), np.sort(key(oq)))
-        # margin: no surviving pair product within 1e-12 relative of the threshold
-        prod = Qo[oq[:, 0]] * Qo[oq[:, 1]]
-        assert np.all(prod >= 1e-12)
+        # (general contractions and the threshold's edge: test_gpu_parity.py)
     rng = np.random.default_rng(3)
     D = rng.standard_normal((2, b.nao, b.nao))
     D = D + D.transpose(0, 2, 1)
