@@ -128,6 +128,14 @@ __device__ __forceinline__ void k_unit(const KSArgs& a, const Real* __restrict__
     constexpr int U = batch_items<W>();
     const size_t n = a.nao, nn = n * n;
     const long long nt = (u1 - u0) * NS;
+    // the list entries run one pass ahead: a pass's density gathers (addressed by
+    // its entries) issue with its integral loads instead of after an extra round trip
+    unsigned long long ent_next[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const long long tk = lane + 64 * k;
+        ent_next[k] = lane < nt ? (unsigned long long)a.klist[u0 + (tk < nt ? tk : lane) / NS] : 0ull;
+    }
     for (long long t = lane; t < nt; t += 64 * U) {
         // loads of all U items first; an item past the end repeats item t
         // with its density weight zeroed
@@ -138,13 +146,19 @@ __device__ __forceinline__ void k_unit(const KSArgs& a, const Real* __restrict__
         for (int k = 0; k < U; ++k) {
             const long long tk = t + 64 * k;
             const long long tc = tk < nt ? tk : t;
+            ent[k] = ent_next[k];
             wk[k] = tk < nt ? Real(1) : Real(0);
-            ent[k] = (unsigned long long)a.klist[u0 + tc / NS];
             sk[k] = (int)(tc % NS);
             const long long T = t0 + tc;
             const Real* p = tile0 + (T >> 6) * (64 * W) + (T & 63);
 #pragma unroll
             for (int i = 0; i < W; ++i) v[k][i] = p[i * 64];
+        }
+        const long long tn = t + 64 * U;
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const long long tk = tn + 64 * k;
+            ent_next[k] = tn < nt ? (unsigned long long)a.klist[u0 + (tk < nt ? tk : tn) / NS] : 0ull;
         }
 #pragma unroll
         for (int k = 0; k < U; ++k) {
