@@ -589,52 +589,81 @@ class DIIS:
     from the eigen-decomposition of the small Gram matrix of the differences
     (singular values below eps max(M, N) of the largest are dropped, numpy's
     lstsq rule), and the oldest vectors are dropped while |c| > 1e4.
+
+    The vectors live in two ring buffers on the device of the first push; each
+    push copies its pair in and adds one row of the raw Gram matrix (one GEMV),
+    from which the Gram matrix of the differences follows without touching the
+    vectors again; the extrapolation is one GEMV over the ring.
     """
 
     def __init__(self, space: int = 8, max_coefficient: float = 1e4) -> None:
         self.space = int(space)
         self.max_coefficient = float(max_coefficient)
-        self.focks: list[torch.Tensor] = []
-        self.errors: list[torch.Tensor] = []
+        self._F: torch.Tensor | None = None     # (space, *fock.shape)
+        self._E: torch.Tensor | None = None     # (space, *error.shape)
+        self._G: torch.Tensor | None = None     # (space, space) raw Gram <e_i, e_j> of the ring slots
+        self._head = 0                          # slot of the oldest vector
+        self._m = 0                             # vectors held (slots head .. head+m-1 mod space)
 
     def __len__(self) -> int:
-        return len(self.focks)
+        return self._m
+
+    def _slot(self, k: int) -> int:
+        return (self._head + k) % self.space
+
+    @property
+    def focks(self) -> list[torch.Tensor]:
+        return [self._F[self._slot(k)] for k in range(self._m)]
+
+    @property
+    def errors(self) -> list[torch.Tensor]:
+        return [self._E[self._slot(k)] for k in range(self._m)]
 
     def reset(self) -> None:
-        self.focks.clear()
-        self.errors.clear()
+        self._m = 0
+        if self._F is not None:
+            self._F.zero_()
+            self._E.zero_()
 
     def push(self, fock: torch.Tensor, error: torch.Tensor) -> None:
-        self.focks.append(fock.clone())
-        self.errors.append(error.clone())
-        while len(self.focks) > self.space:
-            self.focks.pop(0)
-            self.errors.pop(0)
+        if self._F is None or self._F.shape[1:] != fock.shape or self._F.device != fock.device:
+            self._F = torch.zeros((self.space,) + tuple(fock.shape), dtype=fock.dtype, device=fock.device)
+            self._E = torch.zeros((self.space,) + tuple(error.shape), dtype=error.dtype, device=error.device)
+            self._G = torch.zeros((self.space, self.space), dtype=error.dtype, device=error.device)
+            self._head, self._m = 0, 0
+        if self._m == self.space:
+            self._head = (self._head + 1) % self.space
+            self._m -= 1
+        slot = self._slot(self._m)
+        self._F[slot].copy_(fock)
+        self._E[slot].copy_(error)
+        g = self._E.view(self.space, -1) @ error.reshape(-1)
+        self._G[slot] = g
+        self._G[:, slot] = g
+        self._m += 1
 
     def keep_last(self) -> None:
         """Restart from the newest vector alone."""
-        del self.focks[:-1]
-        del self.errors[:-1]
+        self._head = self._slot(self._m - 1)
+        self._m = 1
 
     def gram(self) -> torch.Tensor | None:
         """Device (m, m-1) array: rows 0..m-2 the Gram matrix of the differences
         e_i - e_m, row m-1 the right-hand side <e_i - e_m, -e_m>; None for m = 1.
         Computed before the host needs it, so it travels with other scalars."""
-        m = len(self.errors)
+        m = self._m
         if m == 1:
             return None
-        E = torch.stack([e.reshape(-1) for e in self.errors])
-        last = E[-1]
-        delta = E[:-1] - last
-        # Gram matrix and right-hand side as one broadcast product and one reduction:
-        # the (m x nao^2)(nao^2 x m) shape maps to a single-workgroup GEMM in the BLAS
-        D = torch.cat([delta, -last[None]])
-        return (delta[None, :, :] * D[:, None, :]).sum(dim=2)
+        Gs = torch.roll(self._G, shifts=(-self._head, -self._head), dims=(0, 1))[:m, :m]
+        a = m - 1
+        Gd = Gs[:a, :a] - Gs[:a, a:] - Gs[a:, :a] + Gs[a, a]
+        r = Gs[a, a] - Gs[a, :a]
+        return torch.cat([Gd, r[None]])
 
     def coefficients(self, Gr: np.ndarray | None = None) -> np.ndarray:
         """Coefficients of the stored vectors (oldest first); Gr is the host copy of
         gram() for the current vectors, computed here when not given."""
-        m = len(self.errors)
+        m = self._m
         if m == 1:
             return np.ones(1)
         if Gr is None:
@@ -642,7 +671,7 @@ class DIIS:
         G, r = Gr[:-1], Gr[-1]
         lam, V = np.linalg.eigh(G)
         sig = np.sqrt(np.maximum(lam, 0.0))
-        cut = np.finfo(float).eps * max(m - 1, self.errors[-1].numel()) * sig.max()
+        cut = np.finfo(float).eps * max(m - 1, self._E[0].numel()) * sig.max()
         keep = sig > cut
         x = V[:, keep] @ ((V[:, keep].T @ r) / lam[keep]) if keep.any() else np.zeros(m - 1)
         return np.append(x, 1.0 - x.sum())
@@ -650,19 +679,21 @@ class DIIS:
     def extrapolate(self, Gr: np.ndarray | None = None) -> torch.Tensor:
         """sum_i c_i F_i; dropping the oldest vector keeps the newest as the anchor,
         so the Gram matrix of the rest is Gr[1:, 1:]."""
-        while len(self.focks) > 1:
+        while self._m > 1:
             c = self.coefficients(Gr)
             if np.all(np.isfinite(c)) and np.abs(c).max() <= self.max_coefficient:
-                F = torch.stack(self.focks)
-                ct = torch.from_numpy(c)
-                if F.is_cuda:
-                    ct = ct.pin_memory().to(F.device, non_blocking=True)
-                return (ct.view(-1, *([1] * (F.dim() - 1))) * F).sum(dim=0)
-            self.focks.pop(0)
-            self.errors.pop(0)
+                full = np.zeros(self.space)
+                for k in range(self._m):
+                    full[self._slot(k)] = c[k]
+                ct = torch.from_numpy(full)
+                if self._F.is_cuda:
+                    ct = ct.pin_memory().to(self._F.device, non_blocking=True)
+                return (ct @ self._F.view(self.space, -1)).view(self._F.shape[1:])
+            self._head = (self._head + 1) % self.space
+            self._m -= 1
             if Gr is not None:
                 Gr = Gr[1:, 1:]
-        return self.focks[-1].clone()
+        return self._F[self._slot(self._m - 1)].clone()
 
 
 # ================================================================ results

@@ -49,53 +49,54 @@ __global__ __launch_bounds__(256) void k_tc2_step(int n, const double* __restric
 // block writes X0 (min and max do not depend on the order)
 __global__ __launch_bounds__(1024) void k_tc2_init(int n, const double* __restrict__ F, double* __restrict__ X0) {
     __shared__ double lo[16], hi[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // Gershgorin discs: one wavefront per row, lanes over the columns
     double mn = 1e300, mx = -1e300;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = w; i < n; i += nw) {
         const double* row = F + (size_t)i * n;
         double r = 0.0;
-        for (int j = 0; j < n; ++j) r += j == i ? 0.0 : fabs(row[j]);
+        for (int j = lane; j < n; j += 64) r += j == i ? 0.0 : fabs(row[j]);
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) r += __shfl_xor(r, m, 64);
         mn = fmin(mn, row[i] - r);
         mx = fmax(mx, row[i] + r);
     }
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        mn = fmin(mn, __shfl_xor(mn, m, 64));
-        mx = fmax(mx, __shfl_xor(mx, m, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        lo[threadIdx.x >> 6] = mn;
-        hi[threadIdx.x >> 6] = mx;
+    if (lane == 0) {
+        lo[w] = mn;
+        hi[w] = mx;
     }
     __syncthreads();
     double emin = lo[0], emax = hi[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
-        emin = fmin(emin, lo[w]);
-        emax = fmax(emax, hi[w]);
+    for (int k = 1; k < nw; ++k) {
+        emin = fmin(emin, lo[k]);
+        emax = fmax(emax, hi[k]);
     }
     const double inv = 1.0 / (emax - emin);
-    const size_t nn = (size_t)n * n;
-    for (size_t k = threadIdx.x; k < nn; k += blockDim.x) {
-        const size_t i = k / n, j = k % n;
-        X0[k] = ((i == j ? emax : 0.0) - F[k]) * inv;
+    for (int i = w; i < n; i += nw) {
+        const size_t r0 = (size_t)i * n;
+        for (int j = lane; j < n; j += 64) X0[r0 + j] = ((i == j ? emax : 0.0) - F[r0 + j]) * inv;
     }
 }
 
-// out[0] = |tr X - sum X_ij^2| (idempotency error of symmetric X), one block
+// |tr X - sum_ij X_ij^2| (the idempotency error of a symmetric X), one workgroup
 __global__ __launch_bounds__(1024) void k_tc2_error(int n, const double* __restrict__ X, double* out) {
     __shared__ double part[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     double s = 0.0;
-    const size_t nn = (size_t)n * n;
-    for (size_t k = threadIdx.x; k < nn; k += blockDim.x) {
-        const double x = X[k];
-        s += (k / n == k % n ? x : 0.0) - x * x;
+    for (int i = w; i < n; i += nw) {
+        const double* row = X + (size_t)i * n;
+        for (int j = lane; j < n; j += 64) {
+            const double x = row[j];
+            s += (i == j ? x : 0.0) - x * x;
+        }
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    if (lane == 0) part[w] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        for (int k = 0; k < nw; ++k) t += part[k];
         out[0] = fabs(t);
     }
 }
