@@ -138,6 +138,137 @@ MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __rest
     }
 }
 
+// The bra pass of a quartet whose ket pair is two s shells, with the derivative
+// on the ket's first centre C folded in: the same R serves both. For s functions
+// d/dC_x phi_c = 2 gamma phi_{c+1x}, whose Hermite expansion about Q from the
+// stored E^{cd}_0 is 2 gamma (X_QC E_0 at degree 0, E_0 / (2q) at degree 1), so
+//   (ab| d_C(cd)) = sum_t E^{ab}_t 2 gamma E_0 (X_QC R_t - R_{t+1_x} / (2q)),
+// accumulated over the ket primitive pairs as Y[t][x] and contracted with the
+// stored bra expansion and Gamma. gA, gB as grad_pass (NCOL = 1, kc0 = 0), gC the
+// C derivative; D follows from translational invariance.
+template <int LA, int LB, bool SLICE = false, typename Real = double>
+MD_HD void grad_pass_c(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
+                       const double* __restrict__ tab, const double* Apos, const double* Bpos, const double* Cpos,
+                       const Real* gam, double* gA, double* gB, double* gC) {
+    using Q = Quartet<LA, LB, 0, 0>;
+    constexpr int NAB = Q::NAB, NHB = Q::NHB;
+    constexpr int NHB1 = NH(Q::LAB + 1), L1 = Q::L + 1;
+    constexpr int NA = Q::NA, NB = Q::NB;
+    constexpr int JS0 = LB + 1, TS0 = LA + LB + 2;
+    Real xab[3];
+    for (int d = 0; d < 3; ++d) {
+        gA[d] = 0.0;
+        gB[d] = 0.0;
+        gC[d] = 0.0;
+        xab[d] = Real(Apos[d] - Bpos[d]);
+    }
+    Real fcart[NAB];
+    static_for<0, NAB>([&](auto aa) {
+        constexpr int ab = decltype(aa)::value;
+        constexpr int ia = ab / NB, ib = ab % NB;
+        fcart[ab] = Real(cart_factor(cart_x(LA, ia), cart_y(LA, ia), cart_z(LA, ia)) *
+                         cart_factor(cart_x(LB, ib), cart_y(LB, ib), cart_z(LB, ib)));
+    });
+    for (int ibp = 0; ibp < nb; ++ibp) {
+        const double* Bp = rb + ibp * Q::RSB;
+        const Real p = Real(Bp[R_P]), a = Real(Bp[R_A]), b = Real(Bp[R_B]), cab = Real(Bp[R_C]);
+        const Real px = Real(Bp[R_PX]), py = Real(Bp[R_PY]), pz = Real(Bp[R_PZ]);
+        Real X[NHB1], Y[NHB * 3];
+#pragma unroll
+        for (int i = 0; i < NHB1; ++i) X[i] = Real(0);
+#pragma unroll
+        for (int i = 0; i < NHB * 3; ++i) Y[i] = Real(0);
+        for (int ik = 0; ik < nk; ++ik) {
+            const double* K = rk + ik * Q::RSK;
+            const Real q = Real(K[R_P]);
+            Real alpha, pref;
+            if constexpr (SLICE) {
+                quartet_factors(p, q, Real(Bp[R_IP]), Real(K[R_IP]), alpha, pref);
+            } else {
+                const Real s = p + q;
+                alpha = p * q / s;
+                pref = Real(TWO_PI_2_5) / (p * q * md_sqrt(s));
+            }
+            Real R[NH(L1)];
+            hermite_r<L1, SLICE, Real>(alpha, px - Real(K[R_PX]), py - Real(K[R_PY]), pz - Real(K[R_PZ]), pref, tab, R);
+            const Real e0 = Real(K[REC_HDR]);
+            static_for<0, NHB1>([&](auto hh) {
+                constexpr int h = decltype(hh)::value;
+                X[h] += R[h] * e0;
+            });
+            // C derivative: 2 gamma E_0 (X_QC R_t - R_{t+1} / (2q)) per direction
+            const Real g2 = Real(2.0) * Real(K[R_A]) * e0, h2q = Real(0.5) * Real(K[R_IP]);
+            const Real qc[3] = {Real(K[R_PX]) - Real(Cpos[0]), Real(K[R_PY]) - Real(Cpos[1]),
+                                Real(K[R_PZ]) - Real(Cpos[2])};
+            static_for<0, NHB>([&](auto hh) {
+                constexpr int h = decltype(hh)::value;
+                constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
+                Y[h * 3 + 0] += g2 * (qc[0] * R[h] - h2q * R[hidx(t + 1, u, v)]);
+                Y[h * 3 + 1] += g2 * (qc[1] * R[h] - h2q * R[hidx(t, u + 1, v)]);
+                Y[h * 3 + 2] += g2 * (qc[2] * R[h] - h2q * R[hidx(t, u, v + 1)]);
+            });
+        }
+        // C: the stored bra expansion (coefficients and Cartesian factors folded in)
+        static_for<0, NAB>([&](auto aa) {
+            constexpr int ab = decltype(aa)::value;
+            Real s3[3] = {0, 0, 0};
+            static_for<0, NHB>([&](auto hh) {
+                constexpr int h = decltype(hh)::value;
+                constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
+                if constexpr (in_box(LA, LB, ab, t, u, v)) {
+                    const Real e = Real(Bp[REC_HDR + h * NAB + ab]);
+                    s3[0] += e * Y[h * 3 + 0];
+                    s3[1] += e * Y[h * 3 + 1];
+                    s3[2] += e * Y[h * 3 + 2];
+                }
+            });
+            for (int d = 0; d < 3; ++d) gC[d] += double(gam[ab] * s3[d]);
+        });
+        // A and B: as grad_pass with the single ket column
+        const Real mu = a * b / p;
+        Real ex[(LA + 2) * JS0 * TS0], ey[(LA + 2) * JS0 * TS0], ez[(LA + 2) * JS0 * TS0];
+        hermite_e<LA + 1, LB, Real>(a, b, xab[0], md_exp(-mu * xab[0] * xab[0]), ex);
+        hermite_e<LA + 1, LB, Real>(a, b, xab[1], md_exp(-mu * xab[1] * xab[1]), ey);
+        hermite_e<LA + 1, LB, Real>(a, b, xab[2], md_exp(-mu * xab[2] * xab[2]), ez);
+        const Real ta = Real(2.0) * a;
+        Real sA[3] = {0, 0, 0}, sP[3] = {0, 0, 0};
+        static_for<0, NAB>([&](auto aa) {
+            constexpr int ab = decltype(aa)::value;
+            constexpr int ia = ab / NB, ib = ab % NB;
+            constexpr int ax = cart_x(LA, ia), ay = cart_y(LA, ia), az = cart_z(LA, ia);
+            constexpr int bx = cart_x(LB, ib), by = cart_y(LB, ib), bz = cart_z(LB, ib);
+            const Real f = fcart[ab] * cab * gam[ab];
+            auto E = [](const Real* e, int i, int j, int tt) { return e[(i * JS0 + j) * TS0 + tt]; };
+            static_for<0, NHB1>([&](auto hh) {
+                constexpr int h = decltype(hh)::value;
+                constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
+                constexpr int ox = t > ax + bx, oy = u > ay + by, oz = v > az + bz;
+                if constexpr (ox + oy + oz <= 1 && t <= ax + bx + 1 && u <= ay + by + 1 && v <= az + bz + 1) {
+                    const Real y = f * X[h];
+                    const Real e0x = E(ex, ax, bx, t), e0y = E(ey, ay, by, u), e0z = E(ez, az, bz, v);
+                    Real dax = ta * E(ex, ax + 1, bx, t), day = ta * E(ey, ay + 1, by, u), daz = ta * E(ez, az + 1, bz, v);
+                    if constexpr (ax > 0) dax -= Real(ax) * E(ex, ax - 1, bx, t);
+                    if constexpr (ay > 0) day -= Real(ay) * E(ey, ay - 1, by, u);
+                    if constexpr (az > 0) daz -= Real(az) * E(ez, az - 1, bz, v);
+                    sA[0] += y * dax * e0y * e0z;
+                    sA[1] += y * e0x * day * e0z;
+                    sA[2] += y * e0x * e0y * daz;
+                }
+                if constexpr (t <= ax + bx && u <= ay + by && v <= az + bz) {
+                    const Real e = E(ex, ax, bx, t) * E(ey, ay, by, u) * E(ez, az, bz, v);
+                    sP[0] += e * f * X[hidx(t + 1, u, v)];
+                    sP[1] += e * f * X[hidx(t, u + 1, v)];
+                    sP[2] += e * f * X[hidx(t, u, v + 1)];
+                }
+            });
+        });
+        for (int d = 0; d < 3; ++d) {
+            gA[d] += double(sA[d]);
+            gB[d] += double(sP[d] - sA[d]);
+        }
+    }
+}
+
 // single ket component kc, gam[ab] = Gamma[ab][kc]
 template <int LA, int LB, int LC, int LD>
 MD_HD void grad_pass_kc(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
