@@ -50,7 +50,7 @@ constexpr int NCB = grad_ncol<LA, LB, LC, LD>(), NCK = grad_ncol<LC, LD, LA, LB>
 constexpr int NIB = NCD / NCB, NIK = NAB / NCK;
 // quartets with an s-s ket and a light bra take all four centres from one item:
 // the bra pass with the ket's first-centre derivative folded in (grad_pass_c)
-constexpr bool FUSED = LC == 0 && LD == 0 && NH(LA + LB) * 3 <= 60;
+constexpr bool FUSED = LC == 0 && LD == 0 && NH(LA + LB) * 3 <= 105;
 constexpr int NT = FUSED ? 1 : NIB + NIK;
 // LDS words for the per-atom sums: with the 35 KB Boys slice, a workgroup stays
 // within 64 KB of LDS
