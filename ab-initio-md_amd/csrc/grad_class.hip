@@ -12,9 +12,10 @@
 // j*NCB .. j*NCB+NCB-1, NIB <= j the ket pass for NCK bra components, NCB /
 // NCK as many as fit the register budget (one R evaluation per primitive
 // quartet and item); a pass is replaced by translational invariance when its
-// pair sits on one atom. Classes with an s-s ket and a light bra ((ss|ss),
-// (ps|ss), (pp|ss), (ds|ss)) take one item per quartet that also folds in the
-// ket's first-centre derivative, so R is evaluated once per primitive quartet. Contributions are accumulated per workgroup in LDS
+// pair sits on one atom. Classes with an (s|s) or (p|s) ket instead take items
+// of the bra pass that also fold in the ket's first-centre derivative (one item
+// per quartet, or per ket column for the heavier (p|s) classes), so the ket
+// pass and its R evaluations disappear. Contributions are accumulated per workgroup in LDS
 // and flushed once, as exact fixed-point integers, so the gradient is
 // bitwise reproducible.
 
@@ -48,10 +49,12 @@ using Qt = Quartet<LA, LB, LC, LD>;
 constexpr int NAB = Qt::NAB, NCD = Qt::NCD, NB = Qt::NB, ND = Qt::ND;
 constexpr int NCB = grad_ncol<LA, LB, LC, LD>(), NCK = grad_ncol<LC, LD, LA, LB>();
 constexpr int NIB = NCD / NCB, NIK = NAB / NCK;
-// quartets with an s-s ket and a light bra take all four centres from one item:
-// the bra pass with the ket's first-centre derivative folded in (grad_pass_c)
-constexpr bool FUSED = LC == 0 && LD == 0 && NH(LA + LB) * 3 <= 105;
-constexpr int NT = FUSED ? 1 : NIB + NIK;
+// quartets with an (s|s) or (p|s) ket take all four centres from the bra pass
+// with the ket's first-centre derivative folded in (grad_pass_c), FOLD ket
+// columns per item
+constexpr int FOLD = fold_ncol<LA, LB, LC, LD>();
+constexpr bool FUSED = FOLD > 0;
+constexpr int NT = FUSED ? NCD / FOLD : NIB + NIK;
 // LDS words for the per-atom sums: with the 35 KB Boys slice, a workgroup stays
 // within 64 KB of LDS
 constexpr int GRAD_LDS_WORDS = 3600;
@@ -202,6 +205,35 @@ __global__ __launch_bounds__(256) void k_grad_scatter(const unsigned char* __res
     if (k == 2) sel32[o2 + __popcll(m2 & below)] = (int)q;
 }
 
+// The folded item for ket columns KC0 .. KC0+FOLD-1 of quartet pq: A, B and C
+// derivatives from one pass, D by translational invariance
+template <int KC0, typename Real>
+__device__ __forceinline__ void fold_item(const GradArgs& a, int2 pq, const double* btab, long long* lds) {
+    const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
+    const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
+    double deg = 0.5;
+    if (sA != sB) deg *= 2.0;
+    if (sC != sD) deg *= 2.0;
+    if (pq.x != pq.y) deg *= 2.0;
+    const int oA = a.shell_ao[sA], oB = a.shell_ao[sB], oC = a.shell_ao[sC], oD = a.shell_ao[sD];
+    const size_t rep = a.shell_rep ? (size_t)a.shell_rep[sA] : 0;
+    Real gam[NAB * (FUSED ? FOLD : 1)];
+    for (int ab = 0; ab < NAB; ++ab)
+#pragma unroll
+        for (int c = 0; c < FOLD; ++c)
+            gam[ab * FOLD + c] = Real(gamma_el(a, rep, oA + ab / NB, oB + ab % NB, oC + KC0 + c, oD, deg));
+    double gA[3], gB[3], gC[3];
+    grad_pass_c<LA, LB, LC, FOLD, KC0, true, Real>(a.recs + a.rec_off[pq.x], a.npp[pq.x], a.recs + a.rec_off[pq.y],
+                                                   a.npp[pq.y], btab, a.pos + 3 * aA, a.pos + 3 * aB, a.pos + 3 * aC,
+                                                   gam, gA, gB, gC);
+    for (int d = 0; d < 3; ++d) {
+        lds_add(lds, aA, d, gA[d]);
+        lds_add(lds, aB, d, gB[d]);
+        lds_add(lds, aC, d, gC[d]);
+        lds_add(lds, aD, d, -(gA[d] + gB[d] + gC[d]));
+    }
+}
+
 // Work item = (selected quartet, pass column group), derivative integrals in Real
 template <int Q, typename Real>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<Real>()))) void k_grad_class(
@@ -224,24 +256,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<
         const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
         const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
         if constexpr (FUSED) {
-            double deg = 0.5;
-            if (sA != sB) deg *= 2.0;
-            if (sC != sD) deg *= 2.0;
-            if (pq.x != pq.y) deg *= 2.0;
-            const int oA = a.shell_ao[sA], oB = a.shell_ao[sB], oC = a.shell_ao[sC], oD = a.shell_ao[sD];
-            const size_t rep = a.shell_rep ? (size_t)a.shell_rep[sA] : 0;
-            Real gam[NAB];
-            for (int ab = 0; ab < NAB; ++ab) gam[ab] = Real(gamma_el(a, rep, oA + ab / NB, oB + ab % NB, oC, oD, deg));
-            double gA[3], gB[3], gC[3];
-            grad_pass_c<LA, LB, true, Real>(a.recs + a.rec_off[pq.x], a.npp[pq.x], a.recs + a.rec_off[pq.y],
-                                            a.npp[pq.y], btab, a.pos + 3 * aA, a.pos + 3 * aB, a.pos + 3 * aC, gam,
-                                            gA, gB, gC);
-            for (int d = 0; d < 3; ++d) {
-                lds_add(lds, aA, d, gA[d]);
-                lds_add(lds, aB, d, gB[d]);
-                lds_add(lds, aC, d, gC[d]);
-                lds_add(lds, aD, d, -(gA[d] + gB[d] + gC[d]));
-            }
+            static_for<0, NT>([&](auto jj) {
+                constexpr int J = decltype(jj)::value;
+                if (j == J) fold_item<J * FOLD, Real>(a, pq, btab, lds);
+            });
             continue;
         }
         const bool need_bra = aA != aB, need_ket = aC != aD;

@@ -32,6 +32,17 @@ MD_HD constexpr int grad_ncol() {
     return best;
 }
 
+// Ket columns per folded item (grad_pass_c) of class (LA LB | LC LD), 0 where the
+// class is not folded: kets (s|s) with a bra up to (dd|, one item per quartet;
+// kets (p|s) with all three columns in one item while the bra's two Hermite
+// blocks stay small ((ps|, (pp|, (ds|), else one item per column
+template <int LA, int LB, int LC, int LD>
+MD_HD constexpr int fold_ncol() {
+    if (LD != 0 || LC > 1) return 0;
+    if (LC == 0) return NH(LA + LB) * 3 <= 105 ? 1 : 0;
+    return 3 * (NH(LA + LB) + NH(LA + LB + 1)) <= 90 ? 3 : 1;
+}
+
 template <int LA, int LB, int LC, int LD, int NCOL, bool SLICE = false, typename Real = double>
 MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
                      const double* __restrict__ tab, const double* Apos, const double* Bpos, int kc0,
@@ -138,23 +149,29 @@ MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __rest
     }
 }
 
-// The bra pass of a quartet whose ket pair is two s shells, with the derivative
-// on the ket's first centre C folded in: the same R serves both. For s functions
-// d/dC_x phi_c = 2 gamma phi_{c+1x}, whose Hermite expansion about Q from the
-// stored E^{cd}_0 is 2 gamma (X_QC E_0 at degree 0, E_0 / (2q) at degree 1), so
-//   (ab| d_C(cd)) = sum_t E^{ab}_t 2 gamma E_0 (X_QC R_t - R_{t+1_x} / (2q)),
-// accumulated over the ket primitive pairs as Y[t][x] and contracted with the
-// stored bra expansion and Gamma. gA, gB as grad_pass (NCOL = 1, kc0 = 0), gC the
-// C derivative; D follows from translational invariance.
-template <int LA, int LB, bool SLICE = false, typename Real = double>
+// The bra pass of a quartet whose ket pair is (l_C <= 1 | s), with the derivative
+// on the ket's first centre C folded in: one R per primitive quartet serves the
+// A, B and C derivatives (D follows from translational invariance). For ket
+// component c, d/dC_x phi_c = 2 gamma phi_{c+1x} - c_x phi_{c-1x}; both
+// expansions about Q follow from the stored table E^{c}_k of the ket pair:
+//   raised:  U_k = E_{k-1x} / (2q) + X_QC E_k + (k_x + 1) E_{k+1x}
+//   lowered (c_x = 1, the s function): 2q E_{1x} at degree 0,
+// and (ab| d_C(cd)) = sum_t E^{ab}_t sum_k (-1)^|k| (2 gamma U_k - c_x L_k) R_{t+k}.
+// Gamma is contracted with the stored bra expansion first (Z[t][j] per bra
+// primitive pair), so the C part costs one pass over t per ket term. The item
+// covers ket columns KC0 .. KC0+NCOL-1 (their share of the C derivative); gA, gB
+// as grad_pass for the same columns.
+template <int LA, int LB, int LC, int NCOL, int KC0, bool SLICE = false, typename Real = double>
 MD_HD void grad_pass_c(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
                        const double* __restrict__ tab, const double* Apos, const double* Bpos, const double* Cpos,
                        const Real* gam, double* gA, double* gB, double* gC) {
-    using Q = Quartet<LA, LB, 0, 0>;
-    constexpr int NAB = Q::NAB, NHB = Q::NHB;
+    static_assert(LC <= 1, "the folded C derivative takes s and p ket shells");
+    using Q = Quartet<LA, LB, LC, 0>;
+    constexpr int NAB = Q::NAB, NHB = Q::NHB, NKC = Q::NCD, NHK = Q::NHK;
     constexpr int NHB1 = NH(Q::LAB + 1), L1 = Q::L + 1;
     constexpr int NA = Q::NA, NB = Q::NB;
     constexpr int JS0 = LB + 1, TS0 = LA + LB + 2;
+    constexpr int NHU = NH(LC + 1);      // degrees of the raised ket expansion
     Real xab[3];
     for (int d = 0; d < 3; ++d) {
         gA[d] = 0.0;
@@ -173,11 +190,25 @@ MD_HD void grad_pass_c(const double* __restrict__ rb, int nb, const double* __re
         const double* Bp = rb + ibp * Q::RSB;
         const Real p = Real(Bp[R_P]), a = Real(Bp[R_A]), b = Real(Bp[R_B]), cab = Real(Bp[R_C]);
         const Real px = Real(Bp[R_PX]), py = Real(Bp[R_PY]), pz = Real(Bp[R_PZ]);
-        Real X[NHB1], Y[NHB * 3];
+        // Z[t][j] = sum_ab Gamma[ab][j] E^{ab}_t (stored bra expansion)
+        Real Z[NHB * NCOL];
+        static_for<0, NHB>([&](auto hh) {
+            constexpr int h = decltype(hh)::value;
 #pragma unroll
-        for (int i = 0; i < NHB1; ++i) X[i] = Real(0);
+            for (int j = 0; j < NCOL; ++j) Z[h * NCOL + j] = Real(0);
+            static_for<0, NAB>([&](auto aa) {
+                constexpr int ab = decltype(aa)::value;
+                if constexpr (in_box(LA, LB, ab, HT.t[h], HT.u[h], HT.v[h])) {
+                    const Real e = Real(Bp[REC_HDR + h * NAB + ab]);
 #pragma unroll
-        for (int i = 0; i < NHB * 3; ++i) Y[i] = Real(0);
+                    for (int j = 0; j < NCOL; ++j) Z[h * NCOL + j] += gam[ab * NCOL + j] * e;
+                }
+            });
+        });
+        Real X[NHB1 * NCOL];
+#pragma unroll
+        for (int i = 0; i < NHB1 * NCOL; ++i) X[i] = Real(0);
+        Real sC[3] = {0, 0, 0};
         for (int ik = 0; ik < nk; ++ik) {
             const double* K = rk + ik * Q::RSK;
             const Real q = Real(K[R_P]);
@@ -191,40 +222,68 @@ MD_HD void grad_pass_c(const double* __restrict__ rb, int nb, const double* __re
             }
             Real R[NH(L1)];
             hermite_r<L1, SLICE, Real>(alpha, px - Real(K[R_PX]), py - Real(K[R_PY]), pz - Real(K[R_PZ]), pref, tab, R);
-            const Real e0 = Real(K[REC_HDR]);
-            static_for<0, NHB1>([&](auto hh) {
-                constexpr int h = decltype(hh)::value;
-                X[h] += R[h] * e0;
-            });
-            // C derivative: 2 gamma E_0 (X_QC R_t - R_{t+1} / (2q)) per direction
-            const Real g2 = Real(2.0) * Real(K[R_A]) * e0, h2q = Real(0.5) * Real(K[R_IP]);
-            const Real qc[3] = {Real(K[R_PX]) - Real(Cpos[0]), Real(K[R_PY]) - Real(Cpos[1]),
-                                Real(K[R_PZ]) - Real(Cpos[2])};
-            static_for<0, NHB>([&](auto hh) {
-                constexpr int h = decltype(hh)::value;
-                constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
-                Y[h * 3 + 0] += g2 * (qc[0] * R[h] - h2q * R[hidx(t + 1, u, v)]);
-                Y[h * 3 + 1] += g2 * (qc[1] * R[h] - h2q * R[hidx(t, u + 1, v)]);
-                Y[h * 3 + 2] += g2 * (qc[2] * R[h] - h2q * R[hidx(t, u, v + 1)]);
-            });
-        }
-        // C: the stored bra expansion (coefficients and Cartesian factors folded in)
-        static_for<0, NAB>([&](auto aa) {
-            constexpr int ab = decltype(aa)::value;
-            Real s3[3] = {0, 0, 0};
-            static_for<0, NHB>([&](auto hh) {
-                constexpr int h = decltype(hh)::value;
-                constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
-                if constexpr (in_box(LA, LB, ab, t, u, v)) {
-                    const Real e = Real(Bp[REC_HDR + h * NAB + ab]);
-                    s3[0] += e * Y[h * 3 + 0];
-                    s3[1] += e * Y[h * 3 + 1];
-                    s3[2] += e * Y[h * 3 + 2];
+            // A, B: the ket-summed block X of grad_pass
+            static_for<0, NHK>([&](auto kk) {
+                constexpr int k = decltype(kk)::value;
+                constexpr int tk = HT.t[k], uk = HT.u[k], vk = HT.v[k];
+                constexpr bool odd = (tk + uk + vk) & 1;
+#pragma unroll
+                for (int j = 0; j < NCOL; ++j) {
+                    const Real e = odd ? -Real(K[REC_HDR + k * NKC + KC0 + j]) : Real(K[REC_HDR + k * NKC + KC0 + j]);
+                    static_for<0, NHB1>([&](auto hh) {
+                        constexpr int h = decltype(hh)::value;
+                        X[h * NCOL + j] += R[hidx(HT.t[h] + tk, HT.u[h] + uk, HT.v[h] + vk)] * e;
+                    });
                 }
             });
-            for (int d = 0; d < 3; ++d) gC[d] += double(gam[ab] * s3[d]);
-        });
-        // A and B: as grad_pass with the single ket column
+            // C: raised and lowered ket expansions of each column, against Z and R
+            const Real g2 = Real(2.0) * Real(K[R_A]), iq = Real(K[R_IP]), h2q = Real(0.5) * iq, q2 = Real(2.0) * q;
+            const Real qc[3] = {Real(K[R_PX]) - Real(Cpos[0]), Real(K[R_PY]) - Real(Cpos[1]),
+                                Real(K[R_PZ]) - Real(Cpos[2])};
+            static_for<0, NCOL>([&](auto jj) {
+                constexpr int j = decltype(jj)::value, cj = KC0 + j;
+                constexpr int c3[3] = {cart_x(LC, cj), cart_y(LC, cj), cart_z(LC, cj)};
+                // stored E^{cj}_{(t,u,v)}, zero outside the component's box
+                auto Ek = [&](int t, int u, int v) -> Real {
+                    return (t >= 0 && u >= 0 && v >= 0 && t <= c3[0] && u <= c3[1] && v <= c3[2])
+                               ? Real(K[REC_HDR + hidx(t, u, v) * NKC + cj]) : Real(0);
+                };
+                static_for<0, 3>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    Real acc = 0;
+                    static_for<0, NHU>([&](auto kk) {
+                        constexpr int k = decltype(kk)::value;
+                        constexpr int kt = HT.t[k], ku = HT.u[k], kv = HT.v[k];
+                        constexpr int kd = d == 0 ? kt : d == 1 ? ku : kv;
+                        constexpr int e0 = d == 0, e1 = d == 1, e2 = d == 2;
+                        // structurally nonzero terms only
+                        constexpr bool lower_in = kt - e0 >= 0 && ku - e1 >= 0 && kv - e2 >= 0 && kt - e0 <= c3[0] &&
+                                                  ku - e1 <= c3[1] && kv - e2 <= c3[2];
+                        constexpr bool here_in = kt <= c3[0] && ku <= c3[1] && kv <= c3[2];
+                        constexpr bool upper_in = kt + e0 <= c3[0] && ku + e1 <= c3[1] && kv + e2 <= c3[2];
+                        constexpr bool low_term = c3[d] == 1 && k == 0;
+                        if constexpr (lower_in || here_in || upper_in || low_term) {
+                            Real u = 0;
+                            if constexpr (lower_in) u += h2q * Ek(kt - e0, ku - e1, kv - e2);
+                            if constexpr (here_in) u += qc[d] * Ek(kt, ku, kv);
+                            if constexpr (upper_in) u += Real(kd + 1) * Ek(kt + e0, ku + e1, kv + e2);
+                            Real w = g2 * u;
+                            if constexpr (low_term) w -= q2 * Ek(e0, e1, e2);
+                            if constexpr ((kt + ku + kv) & 1) w = -w;
+                            Real zr = 0;
+                            static_for<0, NHB>([&](auto hh) {
+                                constexpr int h = decltype(hh)::value;
+                                zr += Z[h * NCOL + j] * R[hidx(HT.t[h] + kt, HT.u[h] + ku, HT.v[h] + kv)];
+                            });
+                            acc += w * zr;
+                        }
+                    });
+                    sC[d] += acc;
+                });
+            });
+        }
+        for (int d = 0; d < 3; ++d) gC[d] += double(sC[d]);
+        // A and B: as grad_pass
         const Real mu = a * b / p;
         Real ex[(LA + 2) * JS0 * TS0], ey[(LA + 2) * JS0 * TS0], ez[(LA + 2) * JS0 * TS0];
         hermite_e<LA + 1, LB, Real>(a, b, xab[0], md_exp(-mu * xab[0] * xab[0]), ex);
@@ -237,14 +296,20 @@ MD_HD void grad_pass_c(const double* __restrict__ rb, int nb, const double* __re
             constexpr int ia = ab / NB, ib = ab % NB;
             constexpr int ax = cart_x(LA, ia), ay = cart_y(LA, ia), az = cart_z(LA, ia);
             constexpr int bx = cart_x(LB, ib), by = cart_y(LB, ib), bz = cart_z(LB, ib);
-            const Real f = fcart[ab] * cab * gam[ab];
+            const Real f = fcart[ab] * cab;
             auto E = [](const Real* e, int i, int j, int tt) { return e[(i * JS0 + j) * TS0 + tt]; };
+            auto yv = [&](int h) {
+                Real xg = 0;
+#pragma unroll
+                for (int j = 0; j < NCOL; ++j) xg += gam[ab * NCOL + j] * X[h * NCOL + j];
+                return f * xg;
+            };
             static_for<0, NHB1>([&](auto hh) {
                 constexpr int h = decltype(hh)::value;
                 constexpr int t = HT.t[h], u = HT.u[h], v = HT.v[h];
                 constexpr int ox = t > ax + bx, oy = u > ay + by, oz = v > az + bz;
                 if constexpr (ox + oy + oz <= 1 && t <= ax + bx + 1 && u <= ay + by + 1 && v <= az + bz + 1) {
-                    const Real y = f * X[h];
+                    const Real y = yv(h);
                     const Real e0x = E(ex, ax, bx, t), e0y = E(ey, ay, by, u), e0z = E(ez, az, bz, v);
                     Real dax = ta * E(ex, ax + 1, bx, t), day = ta * E(ey, ay + 1, by, u), daz = ta * E(ez, az + 1, bz, v);
                     if constexpr (ax > 0) dax -= Real(ax) * E(ex, ax - 1, bx, t);
@@ -256,9 +321,9 @@ MD_HD void grad_pass_c(const double* __restrict__ rb, int nb, const double* __re
                 }
                 if constexpr (t <= ax + bx && u <= ay + by && v <= az + bz) {
                     const Real e = E(ex, ax, bx, t) * E(ey, ay, by, u) * E(ez, az, bz, v);
-                    sP[0] += e * f * X[hidx(t + 1, u, v)];
-                    sP[1] += e * f * X[hidx(t, u + 1, v)];
-                    sP[2] += e * f * X[hidx(t, u, v + 1)];
+                    sP[0] += e * yv(hidx(t + 1, u, v));
+                    sP[1] += e * yv(hidx(t, u + 1, v));
+                    sP[2] += e * yv(hidx(t, u, v + 1));
                 }
             });
         });

@@ -69,19 +69,24 @@ void grad_quartet_device(const State& S, const Basis& Bs, int P, int Q, const do
     auto add = [&](int a, int d, double x) { long long h, l2; fx_split(x, h, l2); fx[2 * (3 * a + d)] += h; fx[2 * (3 * a + d) + 1] += l2; };
     const double* rb = S.recs.data() + S.off[P];
     const double* rk = S.recs.data() + S.off[Q];
-    // the kernel's folded item for an s-s ket and a light bra (grad_class.hip, FUSED)
-    if constexpr (LC == 0 && LD == 0 && NH(LA + LB) * 3 <= 30) {
-        Real gm[NAB];
-        for (int ab = 0; ab < NAB; ++ab) gm[ab] = Real(gam(ab, 0));
-        double gA[3], gB[3], gC[3];
-        grad_pass_c<LA, LB, true, Real>(rb, S.npp[P], rk, S.npp[Q], btab.data(), Bs.pos + 3 * aA, Bs.pos + 3 * aB,
-                                        Bs.pos + 3 * aC, gm, gA, gB, gC);
-        for (int d = 0; d < 3; ++d) {
-            add(aA, d, gA[d]);
-            add(aB, d, gB[d]);
-            add(aC, d, gC[d]);
-            add(aD, d, -(gA[d] + gB[d] + gC[d]));
-        }
+    // the kernel's folded items for (s|s) and (p|s) kets (grad_class.hip, fold_item)
+    constexpr int FOLD = fold_ncol<LA, LB, LC, LD>();
+    if constexpr (FOLD > 0) {
+        static_for<0, NCD / FOLD>([&](auto jj) {
+            constexpr int KC0 = decltype(jj)::value * FOLD;
+            Real gm[NAB * FOLD];
+            for (int ab = 0; ab < NAB; ++ab)
+                for (int c = 0; c < FOLD; ++c) gm[ab * FOLD + c] = Real(gam(ab, KC0 + c));
+            double gA[3], gB[3], gC[3];
+            grad_pass_c<LA, LB, LC, FOLD, KC0, true, Real>(rb, S.npp[P], rk, S.npp[Q], btab.data(), Bs.pos + 3 * aA,
+                                                           Bs.pos + 3 * aB, Bs.pos + 3 * aC, gm, gA, gB, gC);
+            for (int d = 0; d < 3; ++d) {
+                add(aA, d, gA[d]);
+                add(aB, d, gB[d]);
+                add(aC, d, gC[d]);
+                add(aD, d, -(gA[d] + gB[d] + gC[d]));
+            }
+        });
         return;
     }
     for (int j = 0; j < NIB + NIK; ++j) {
