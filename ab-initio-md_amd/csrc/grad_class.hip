@@ -12,10 +12,10 @@
 // j*NCB .. j*NCB+NCB-1, NIB <= j the ket pass for NCK bra components, NCB /
 // NCK as many as fit the register budget (one R evaluation per primitive
 // quartet and item); a pass is replaced by translational invariance when its
-// pair sits on one atom. Classes with an (s|s) or (p|s) ket instead take items
-// of the bra pass that also fold in the ket's first-centre derivative (one item
-// per quartet, or per ket column for the heavier (p|s) classes), so the ket
-// pass and its R evaluations disappear. Contributions are accumulated per workgroup in LDS
+// pair sits on one atom. Folded classes (fold_ncol) instead take items of the
+// bra pass that also fold in the ket's first-centre derivative (one item per
+// ket column, or all columns of a light class in one), so the ket pass and its
+// R evaluations disappear. Contributions are accumulated per workgroup in LDS
 // and flushed once, as exact fixed-point integers, so the gradient is
 // bitwise reproducible.
 
@@ -32,9 +32,9 @@ using namespace md;
 namespace {
 constexpr int QC = AIMDHF_QCLASS;
 // minimum waves per SIMD asked of the register allocator: two instead of one
-// for (pp|ss), (pp|pp) and (ds|ss), which then spill at most 32 registers and
-// run 15-32% faster on gfx950; the heavier classes slow down at two waves
-// (more than 80 spilled registers), (ss|ss) and (ps|ss) at one more wave
+// for (pp|ss), (pp|pp) and (ds|ss) in FP64, which then spill little and run
+// faster on gfx950; the heavier classes slow down at two waves (more than 80
+// spilled registers), (ss|ss) and (ps|ss) at one more wave
 template <typename Real>
 constexpr int grad_waves() {
     if constexpr (sizeof(Real) == 4) return QC == 10 || (QC >= 14 && QC <= 18) ? 2 : 1;
@@ -49,12 +49,14 @@ using Qt = Quartet<LA, LB, LC, LD>;
 constexpr int NAB = Qt::NAB, NCD = Qt::NCD, NB = Qt::NB, ND = Qt::ND;
 constexpr int NCB = grad_ncol<LA, LB, LC, LD>(), NCK = grad_ncol<LC, LD, LA, LB>();
 constexpr int NIB = NCD / NCB, NIK = NAB / NCK;
-// quartets with an (s|s) or (p|s) ket take all four centres from the bra pass
-// with the ket's first-centre derivative folded in (grad_pass_c), FOLD ket
-// columns per item
-constexpr int FOLD = fold_ncol<LA, LB, LC, LD>();
-constexpr bool FUSED = FOLD > 0;
-constexpr int NT = FUSED ? NCD / FOLD : NIB + NIK;
+// classes with FOLD<Real> > 0 take all four centres from the bra pass with the
+// ket's first-centre derivative folded in (grad_pass_c), FOLD ket columns per
+// item; NT<Real> items per quartet
+template <typename Real>
+constexpr int FOLD = fold_ncol<LA, LB, LC, LD, sizeof(Real) == 8>();
+template <typename Real>
+constexpr int NT = FOLD<Real> > 0 ? NCD / FOLD<Real> : NIB + NIK;
+constexpr int NT_MAX = NT<double> > NT<float> ? NT<double> : NT<float>;
 // LDS words for the per-atom sums: with the 35 KB Boys slice, a workgroup stays
 // within 64 KB of LDS
 constexpr int GRAD_LDS_WORDS = 3600;
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(256) void k_grad_scatter(const unsigned char* __res
     if (k == 2) sel32[o2 + __popcll(m2 & below)] = (int)q;
 }
 
-// The folded item for ket columns KC0 .. KC0+FOLD-1 of quartet pq: A, B and C
+// The folded item for ket columns KC0 .. KC0+FOLD<Real>-1 of quartet pq: A, B and C
 // derivatives from one pass, D by translational invariance
 template <int KC0, typename Real>
 __device__ __forceinline__ void fold_item(const GradArgs& a, int2 pq, const double* btab, long long* lds) {
@@ -217,15 +219,17 @@ __device__ __forceinline__ void fold_item(const GradArgs& a, int2 pq, const doub
     if (pq.x != pq.y) deg *= 2.0;
     const int oA = a.shell_ao[sA], oB = a.shell_ao[sB], oC = a.shell_ao[sC], oD = a.shell_ao[sD];
     const size_t rep = a.shell_rep ? (size_t)a.shell_rep[sA] : 0;
-    Real gam[NAB * (FUSED ? FOLD : 1)];
+    constexpr int F = FOLD<Real>;
+    Real gam[NAB * F];
     for (int ab = 0; ab < NAB; ++ab)
 #pragma unroll
-        for (int c = 0; c < FOLD; ++c)
-            gam[ab * FOLD + c] = Real(gamma_el(a, rep, oA + ab / NB, oB + ab % NB, oC + KC0 + c, oD, deg));
+        for (int c = 0; c < F; ++c)
+            gam[ab * F + c] =
+                Real(gamma_el(a, rep, oA + ab / NB, oB + ab % NB, oC + (KC0 + c) / ND, oD + (KC0 + c) % ND, deg));
     double gA[3], gB[3], gC[3];
-    grad_pass_c<LA, LB, LC, FOLD, KC0, true, Real>(a.recs + a.rec_off[pq.x], a.npp[pq.x], a.recs + a.rec_off[pq.y],
-                                                   a.npp[pq.y], btab, a.pos + 3 * aA, a.pos + 3 * aB, a.pos + 3 * aC,
-                                                   gam, gA, gB, gC);
+    grad_pass_c<LA, LB, LC, LD, F, KC0, true, Real>(a.recs + a.rec_off[pq.x], a.npp[pq.x],
+                                                       a.recs + a.rec_off[pq.y], a.npp[pq.y], btab, a.pos + 3 * aA,
+                                                       a.pos + 3 * aB, a.pos + 3 * aC, aC != aD, gam, gA, gB, gC);
     for (int d = 0; d < 3; ++d) {
         lds_add(lds, aA, d, gA[d]);
         lds_add(lds, aB, d, gB[d]);
@@ -247,21 +251,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<
     if (staged)
         for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
     boys_stage(a.boys, LA + LB + LC + LD + 1, btab);
-    const long long nitems = (long long)(*nsel) * NT;
+    constexpr int NTR = NT<Real>;
+    if constexpr (FOLD<Real> > 0) {
+        // folded items wave by wave: the 64 lanes of a wave take 64 consecutive
+        // selected quartets and one column group, so the column dispatch is uniform
+        const long long ns = *nsel, nitems = (ns + 63) / 64 * 64 * NTR;
+        for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
+             it += (long long)gridDim.x * blockDim.x) {
+            const long long g = it >> 6;
+            const int j = (int)(g % NTR);
+            const long long qi = g / NTR * 64 + (it & 63);
+            if (qi >= ns) continue;
+            const int2 pq = list[sel[qi]];
+            static_for<0, NTR>([&](auto jj) {
+                constexpr int J = decltype(jj)::value;
+                if (j == J) fold_item<J * FOLD<Real>, Real>(a, pq, btab, lds);
+            });
+        }
+    }
+    const long long nitems = FOLD<Real> > 0 ? 0 : (long long)(*nsel) * NTR;
     for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
          it += (long long)gridDim.x * blockDim.x) {
-        const int q = sel[it / NT];
-        const int j = (int)(it % NT);
+        const int q = sel[it / NTR];
+        const int j = (int)(it % NTR);
         const int2 pq = list[q];
         const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
         const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
-        if constexpr (FUSED) {
-            static_for<0, NT>([&](auto jj) {
-                constexpr int J = decltype(jj)::value;
-                if (j == J) fold_item<J * FOLD, Real>(a, pq, btab, lds);
-            });
-            continue;
-        }
         const bool need_bra = aA != aB, need_ket = aC != aD;
         const bool bra = j < NIB;
         if (bra && !(need_bra || (!need_ket && NAB >= NCD))) continue;
@@ -341,7 +356,7 @@ void CAT(launch_grad_keep_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list, 
 // FP64 items, then FP32 items (the grid is sized for the whole list; each kernel
 // reads its selection count on the device)
 void CAT(launch_grad_class_, AIMDHF_QCLASS)(const GradArgs& a, const int2* list, long long nq, hipStream_t st) {
-    long long items = nq * NT;
+    long long items = nq * NT_MAX;
     if (items <= 0) return;
     long long blocks = (items + 255) / 256;
     if (blocks > 1024) blocks = 1024;

@@ -32,15 +32,21 @@ MD_HD constexpr int grad_ncol() {
     return best;
 }
 
-// Ket columns per folded item (grad_pass_c) of class (LA LB | LC LD), 0 where the
-// class is not folded: kets (s|s) with a bra up to (dd|, one item per quartet;
-// kets (p|s) with all three columns in one item while the bra's two Hermite
-// blocks stay small ((ps|, (pp|, (ds|), else one item per column
-template <int LA, int LB, int LC, int LD>
+// Ket columns per folded item (grad_pass_c) of class (LA LB | LC LD) in FP64 or
+// FP32, 0 where the class is not folded: kets (p|s) with a light bra ((ps|,
+// (pp|, (ds|) take all three columns in one item, other (s|s) and (p|s) kets one
+// item per column; (d|s) kets, and in FP32 also (p|p) kets, under a bra up to
+// (dp| one item per column. Heavier classes keep the separate bra and ket
+// passes: their folded items need far more registers than one wave has and
+// multiply the compiled code by the ket's column count. FP64 items of (p|p) kets
+// disagree with the oracle on gfx950 (1.6e-2 Eh/bohr on (H2O)8) while the same
+// code agrees in FP32 there and in FP64 on the host; they stay unfolded.
+template <int LA, int LB, int LC, int LD, bool FP64>
 MD_HD constexpr int fold_ncol() {
-    if (LD != 0 || LC > 1) return 0;
-    if (LC == 0) return NH(LA + LB) * 3 <= 105 ? 1 : 0;
-    return 3 * (NH(LA + LB) + NH(LA + LB + 1)) <= 90 ? 3 : 1;
+    if (LC == 0 && LD == 0) return 1;
+    if (LC == 1 && LD == 0) return 3 * (NH(LA + LB) + NH(LA + LB + 1)) <= 90 ? 3 : 1;
+    if (LC + LD == 2 && LA + LB <= 3) return FP64 && LC == 1 ? 0 : 1;
+    return 0;
 }
 
 template <int LA, int LB, int LC, int LD, int NCOL, bool SLICE = false, typename Real = double>
@@ -149,29 +155,29 @@ MD_HD void grad_pass(const double* __restrict__ rb, int nb, const double* __rest
     }
 }
 
-// The bra pass of a quartet whose ket pair is (l_C <= 1 | s), with the derivative
-// on the ket's first centre C folded in: one R per primitive quartet serves the
-// A, B and C derivatives (D follows from translational invariance). For ket
-// component c, d/dC_x phi_c = 2 gamma phi_{c+1x} - c_x phi_{c-1x}; both
-// expansions about Q follow from the stored table E^{c}_k of the ket pair:
+// The bra pass of a quartet with the derivative on the ket's first centre C
+// folded in: one R per primitive quartet serves the A, B and C derivatives (D
+// follows from translational invariance). For ket component (c, d),
+// d/dC_x phi_c = 2 gamma (x - C_x) phi_c - c_x phi_{c-1x}, and both functions'
+// products with phi_d expand about Q from the stored table E_k of (c, d):
 //   raised:  U_k = E_{k-1x} / (2q) + X_QC E_k + (k_x + 1) E_{k+1x}
-//   lowered (c_x = 1, the s function): 2q E_{1x} at degree 0,
+//   lowered: L_k = E^{(c-1x) d}_k, from the same recursion run downwards along x,
+//            L_{top} = 2q E_{top+1},  L_{s-1} = 2q (E_s - X_QC L_s - (s+1) L_{s+1}),
 // and (ab| d_C(cd)) = sum_t E^{ab}_t sum_k (-1)^|k| (2 gamma U_k - c_x L_k) R_{t+k}.
 // Gamma is contracted with the stored bra expansion first (Z[t][j] per bra
 // primitive pair), so the C part costs one pass over t per ket term. The item
 // covers ket columns KC0 .. KC0+NCOL-1 (their share of the C derivative); gA, gB
-// as grad_pass for the same columns.
-template <int LA, int LB, int LC, int NCOL, int KC0, bool SLICE = false, typename Real = double>
+// as grad_pass for the same columns; with need_c false (C and D on one atom) the
+// C part is skipped and gC is zero.
+template <int LA, int LB, int LC, int LD, int NCOL, int KC0, bool SLICE = false, typename Real = double>
 MD_HD void grad_pass_c(const double* __restrict__ rb, int nb, const double* __restrict__ rk, int nk,
                        const double* __restrict__ tab, const double* Apos, const double* Bpos, const double* Cpos,
-                       const Real* gam, double* gA, double* gB, double* gC) {
-    static_assert(LC <= 1, "the folded C derivative takes s and p ket shells");
-    using Q = Quartet<LA, LB, LC, 0>;
-    constexpr int NAB = Q::NAB, NHB = Q::NHB, NKC = Q::NCD, NHK = Q::NHK;
+                       bool need_c, const Real* gam, double* gA, double* gB, double* gC) {
+    using Q = Quartet<LA, LB, LC, LD>;
+    constexpr int NAB = Q::NAB, NHB = Q::NHB, NKC = Q::NCD, NHK = Q::NHK, ND = Q::ND;
     constexpr int NHB1 = NH(Q::LAB + 1), L1 = Q::L + 1;
     constexpr int NA = Q::NA, NB = Q::NB;
     constexpr int JS0 = LB + 1, TS0 = LA + LB + 2;
-    constexpr int NHU = NH(LC + 1);      // degrees of the raised ket expansion
     Real xab[3];
     for (int d = 0; d < 3; ++d) {
         gA[d] = 0.0;
@@ -192,7 +198,7 @@ MD_HD void grad_pass_c(const double* __restrict__ rb, int nb, const double* __re
         const Real px = Real(Bp[R_PX]), py = Real(Bp[R_PY]), pz = Real(Bp[R_PZ]);
         // Z[t][j] = sum_ab Gamma[ab][j] E^{ab}_t (stored bra expansion)
         Real Z[NHB * NCOL];
-        static_for<0, NHB>([&](auto hh) {
+        if (need_c) static_for<0, NHB>([&](auto hh) {
             constexpr int h = decltype(hh)::value;
 #pragma unroll
             for (int j = 0; j < NCOL; ++j) Z[h * NCOL + j] = Real(0);
@@ -236,47 +242,60 @@ MD_HD void grad_pass_c(const double* __restrict__ rb, int nb, const double* __re
                     });
                 }
             });
-            // C: raised and lowered ket expansions of each column, against Z and R
-            const Real g2 = Real(2.0) * Real(K[R_A]), iq = Real(K[R_IP]), h2q = Real(0.5) * iq, q2 = Real(2.0) * q;
+            // C: raised and lowered ket expansions of each column, line by line along
+            // the derivative's direction, against Z and R
+            if (!need_c) continue;
+            const Real g2 = Real(2.0) * Real(K[R_A]), h2q = Real(0.5) * Real(K[R_IP]), q2 = Real(2.0) * q;
             const Real qc[3] = {Real(K[R_PX]) - Real(Cpos[0]), Real(K[R_PY]) - Real(Cpos[1]),
                                 Real(K[R_PZ]) - Real(Cpos[2])};
             static_for<0, NCOL>([&](auto jj) {
-                constexpr int j = decltype(jj)::value, cj = KC0 + j;
-                constexpr int c3[3] = {cart_x(LC, cj), cart_y(LC, cj), cart_z(LC, cj)};
-                // stored E^{cj}_{(t,u,v)}, zero outside the component's box
-                auto Ek = [&](int t, int u, int v) -> Real {
-                    return (t >= 0 && u >= 0 && v >= 0 && t <= c3[0] && u <= c3[1] && v <= c3[2])
-                               ? Real(K[REC_HDR + hidx(t, u, v) * NKC + cj]) : Real(0);
-                };
+                constexpr int j = decltype(jj)::value, cj = KC0 + j, ci = cj / ND, di = cj % ND;
+                constexpr int c3[3] = {cart_x(LC, ci), cart_y(LC, ci), cart_z(LC, ci)};
+                constexpr int b3[3] = {c3[0] + cart_x(LD, di), c3[1] + cart_y(LD, di), c3[2] + cart_z(LD, di)};
                 static_for<0, 3>([&](auto dd) {
                     constexpr int d = decltype(dd)::value;
+                    constexpr int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+                    constexpr int bd = b3[d], cd = c3[d];
                     Real acc = 0;
-                    static_for<0, NHU>([&](auto kk) {
-                        constexpr int k = decltype(kk)::value;
-                        constexpr int kt = HT.t[k], ku = HT.u[k], kv = HT.v[k];
-                        constexpr int kd = d == 0 ? kt : d == 1 ? ku : kv;
-                        constexpr int e0 = d == 0, e1 = d == 1, e2 = d == 2;
-                        // structurally nonzero terms only
-                        constexpr bool lower_in = kt - e0 >= 0 && ku - e1 >= 0 && kv - e2 >= 0 && kt - e0 <= c3[0] &&
-                                                  ku - e1 <= c3[1] && kv - e2 <= c3[2];
-                        constexpr bool here_in = kt <= c3[0] && ku <= c3[1] && kv <= c3[2];
-                        constexpr bool upper_in = kt + e0 <= c3[0] && ku + e1 <= c3[1] && kv + e2 <= c3[2];
-                        constexpr bool low_term = c3[d] == 1 && k == 0;
-                        if constexpr (lower_in || here_in || upper_in || low_term) {
-                            Real u = 0;
-                            if constexpr (lower_in) u += h2q * Ek(kt - e0, ku - e1, kv - e2);
-                            if constexpr (here_in) u += qc[d] * Ek(kt, ku, kv);
-                            if constexpr (upper_in) u += Real(kd + 1) * Ek(kt + e0, ku + e1, kv + e2);
-                            Real w = g2 * u;
-                            if constexpr (low_term) w -= q2 * Ek(e0, e1, e2);
-                            if constexpr ((kt + ku + kv) & 1) w = -w;
-                            Real zr = 0;
-                            static_for<0, NHB>([&](auto hh) {
-                                constexpr int h = decltype(hh)::value;
-                                zr += Z[h * NCOL + j] * R[hidx(HT.t[h] + kt, HT.u[h] + ku, HT.v[h] + kv)];
+                    static_for<0, b3[o1] + 1>([&](auto ii1) {
+                        static_for<0, b3[o2] + 1>([&](auto ii2) {
+                            constexpr int i1 = decltype(ii1)::value, i2 = decltype(ii2)::value;
+                            // the line's stored coefficients E_s, s = 0 .. bd
+                            Real E[bd + 1];
+                            static_for<0, bd + 1>([&](auto ss) {
+                                constexpr int s = decltype(ss)::value;
+                                constexpr int t = d == 0 ? s : i1, u = d == 1 ? s : (d == 0 ? i1 : i2),
+                                              v = d == 2 ? s : i2;
+                                E[s] = Real(K[REC_HDR + hidx(t, u, v) * NKC + cj]);
                             });
-                            acc += w * zr;
-                        }
+                            Real Lw[bd + 2];
+#pragma unroll
+                            for (int s = 0; s < bd + 2; ++s) Lw[s] = Real(0);
+                            if constexpr (cd > 0) {
+                                Lw[bd - 1] = q2 * E[bd];
+#pragma unroll
+                                for (int s = bd - 1; s >= 1; --s)
+                                    Lw[s - 1] = q2 * (E[s] - qc[d] * Lw[s] - Real(s + 1) * Lw[s + 1]);
+                            }
+                            static_for<0, bd + 2>([&](auto ss) {
+                                constexpr int s = decltype(ss)::value;
+                                constexpr int t = d == 0 ? s : i1, u = d == 1 ? s : (d == 0 ? i1 : i2),
+                                              v = d == 2 ? s : i2;
+                                Real w = 0;
+                                if constexpr (s <= bd) w += qc[d] * E[s];
+                                if constexpr (s + 1 <= bd) w += Real(s + 1) * E[s + 1];
+                                if constexpr (s >= 1) w += h2q * E[s - 1];
+                                w *= g2;
+                                if constexpr (cd > 0 && s < bd) w -= Real(cd) * Lw[s];
+                                if constexpr ((t + u + v) & 1) w = -w;
+                                Real zr = 0;
+                                static_for<0, NHB>([&](auto hh) {
+                                    constexpr int h = decltype(hh)::value;
+                                    zr += Z[h * NCOL + j] * R[hidx(HT.t[h] + t, HT.u[h] + u, HT.v[h] + v)];
+                                });
+                                acc += w * zr;
+                            });
+                        });
                     });
                     sC[d] += acc;
                 });

@@ -69,8 +69,8 @@ void grad_quartet_device(const State& S, const Basis& Bs, int P, int Q, const do
     auto add = [&](int a, int d, double x) { long long h, l2; fx_split(x, h, l2); fx[2 * (3 * a + d)] += h; fx[2 * (3 * a + d) + 1] += l2; };
     const double* rb = S.recs.data() + S.off[P];
     const double* rk = S.recs.data() + S.off[Q];
-    // the kernel's folded items for (s|s) and (p|s) kets (grad_class.hip, fold_item)
-    constexpr int FOLD = fold_ncol<LA, LB, LC, LD>();
+    // the kernel's folded items (grad_class.hip, fold_item)
+    constexpr int FOLD = fold_ncol<LA, LB, LC, LD, sizeof(Real) == 8>();
     if constexpr (FOLD > 0) {
         static_for<0, NCD / FOLD>([&](auto jj) {
             constexpr int KC0 = decltype(jj)::value * FOLD;
@@ -78,8 +78,9 @@ void grad_quartet_device(const State& S, const Basis& Bs, int P, int Q, const do
             for (int ab = 0; ab < NAB; ++ab)
                 for (int c = 0; c < FOLD; ++c) gm[ab * FOLD + c] = Real(gam(ab, KC0 + c));
             double gA[3], gB[3], gC[3];
-            grad_pass_c<LA, LB, LC, FOLD, KC0, true, Real>(rb, S.npp[P], rk, S.npp[Q], btab.data(), Bs.pos + 3 * aA,
-                                                           Bs.pos + 3 * aB, Bs.pos + 3 * aC, gm, gA, gB, gC);
+            grad_pass_c<LA, LB, LC, LD, FOLD, KC0, true, Real>(rb, S.npp[P], rk, S.npp[Q], btab.data(),
+                                                               Bs.pos + 3 * aA, Bs.pos + 3 * aB, Bs.pos + 3 * aC,
+                                                               aC != aD, gm, gA, gB, gC);
             for (int d = 0; d < 3; ++d) {
                 add(aA, d, gA[d]);
                 add(aB, d, gB[d]);
