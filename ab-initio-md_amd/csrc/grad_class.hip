@@ -15,7 +15,8 @@
 // pair sits on one atom. Folded classes (fold_ncol) instead take items of the
 // bra pass that also fold in the ket's first-centre derivative (one item per
 // ket column, or all columns of a light class in one), so the ket pass and its
-// R evaluations disappear. Contributions are accumulated per workgroup in LDS
+// R evaluations disappear. Items are issued wave by wave (64 quartets, one pass
+// or column group per wavefront), so the lanes of a wavefront run the same code. Contributions are accumulated per workgroup in LDS
 // and flushed once, as exact fixed-point integers, so the gradient is
 // bitwise reproducible.
 
@@ -251,30 +252,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_waves<
     if (staged)
         for (int i = threadIdx.x; i < 6 * a.natm; i += blockDim.x) lds[i] = 0;
     boys_stage(a.boys, LA + LB + LC + LD + 1, btab);
+    // items wave by wave: the 64 lanes of a wave take 64 consecutive selected
+    // quartets and one pass / column group, so the pass dispatch is uniform
     constexpr int NTR = NT<Real>;
-    if constexpr (FOLD<Real> > 0) {
-        // folded items wave by wave: the 64 lanes of a wave take 64 consecutive
-        // selected quartets and one column group, so the column dispatch is uniform
-        const long long ns = *nsel, nitems = (ns + 63) / 64 * 64 * NTR;
-        for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
-             it += (long long)gridDim.x * blockDim.x) {
-            const long long g = it >> 6;
-            const int j = (int)(g % NTR);
-            const long long qi = g / NTR * 64 + (it & 63);
-            if (qi >= ns) continue;
-            const int2 pq = list[sel[qi]];
+    const long long ns = *nsel, nitems = (ns + 63) / 64 * 64 * NTR;
+    for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
+         it += (long long)gridDim.x * blockDim.x) {
+        const long long g = it >> 6;
+        const int j = (int)(g % NTR);
+        const long long qi = g / NTR * 64 + (it & 63);
+        if (qi >= ns) continue;
+        const int2 pq = list[sel[qi]];
+        if constexpr (FOLD<Real> > 0) {
             static_for<0, NTR>([&](auto jj) {
                 constexpr int J = decltype(jj)::value;
                 if (j == J) fold_item<J * FOLD<Real>, Real>(a, pq, btab, lds);
             });
+            continue;
         }
-    }
-    const long long nitems = FOLD<Real> > 0 ? 0 : (long long)(*nsel) * NTR;
-    for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
-         it += (long long)gridDim.x * blockDim.x) {
-        const int q = sel[it / NTR];
-        const int j = (int)(it % NTR);
-        const int2 pq = list[q];
         const int sA = a.pair_a[pq.x], sB = a.pair_b[pq.x], sC = a.pair_a[pq.y], sD = a.pair_b[pq.y];
         const int aA = a.shell_atom[sA], aB = a.shell_atom[sB], aC = a.shell_atom[sC], aD = a.shell_atom[sD];
         const bool need_bra = aA != aB, need_ket = aC != aD;
