@@ -1054,7 +1054,10 @@ class SCFSolver:
             Fo = X.T @ F @ X
             # the DIIS vector is pushed before the convergence test, so its Gram
             # matrix comes back with the energy and the commutator in one read
-            parts = [e_dev.reshape(1), torch.linalg.vector_norm(err, ord=float("inf")).reshape(1)]
+            # max|err| from its extremes (one reduction kernel; torch's inf-norm takes
+            # several times longer on this size)
+            mn, mx = torch.aminmax(err)
+            parts = [e_dev.reshape(1), mx.reshape(1), mn.reshape(1)]
             n_diis = len(diis) if diis is not None else 0
             if diis is not None:
                 diis.push(Fo, err)
@@ -1062,7 +1065,7 @@ class SCFSolver:
                 if Gd is not None:
                     parts.append(Gd.reshape(-1))
             scal = torch.cat(parts).cpu().numpy()
-            e_el, err_norm = float(scal[0]), float(scal[1])
+            e_el, err_norm = float(scal[0]), max(float(scal[1]), -float(scal[2]))
             de = e_el - e_prev
             history.append({"energy": e_el + I.nuclear_repulsion, "energy_change": de,
                             "commutator": err_norm, "diis": n_diis})
@@ -1072,7 +1075,7 @@ class SCFSolver:
             if it == opt.max_iter:
                 break
             if diis is not None:
-                Gr = scal[2:].reshape(len(diis), len(diis) - 1) if len(diis) > 1 else None
+                Gr = scal[3:].reshape(len(diis), len(diis) - 1) if len(diis) > 1 else None
                 if err_norm < best:
                     best, since_best = err_norm, 0
                 else:

@@ -177,7 +177,9 @@ int aimdhf_tc2_step(int n, const double* X, const double* X2, double* Xout, doub
  * Start of the purification: X0 = (e_max - F) / (e_max - e_min) for the
  * symmetric device matrix F (n x n), e_min / e_max its Gershgorin bounds
  * (the start of aimd/qc/scf.py's aufbau replaced by purification, see
- * aimdhf_tc2_step). One workgroup; no plan.
+ * aimdhf_tc2_step). Two launches (row bounds, then the fill); the row bounds
+ * go through a file-scope device array, so calls on one device must not
+ * overlap. No plan.
  */
 int aimdhf_tc2_init(int n, const double* F, double* X0, void* stream);
 
@@ -212,7 +214,9 @@ int aimdhf_batch_set_jk_lanes(aimdhf_plan* plan, int lanes);
 int aimdhf_batch_diis_solve(int nrep, int m, const double* G, const double* r, double* c, void* stream);
 
 /* err[0] (device) = |tr X - sum_ij X_ij^2|, the idempotency error of a symmetric
- * purification iterate X (n x n, device). One workgroup; no plan. */
+ * purification iterate X (n x n, device). Row sums, then their sum in a fixed
+ * order (bitwise reproducible); the row sums go through the same device array
+ * as aimdhf_tc2_init's bounds, so calls on one device must not overlap. No plan. */
 int aimdhf_tc2_error(int n, const double* X, double* err, void* stream);
 
 /*
