@@ -876,39 +876,46 @@ struct aimdhf_plan {
             fan.join();
         });
         HIPCHECK(hipGetLastError());
-        // largest Schwarz product of every work unit, for density-difference screening
-        {
-            const int nku = kunit_base[9], nju = junit_base[6];
-            d_kunit_q.ensure(sizeof(double) * std::max(nku, 1));
-            d_junit_q.ensure(sizeof(double) * std::max(nju, 1));
-            d_dmax.ensure(sizeof(double));
-            aimdhf::UnitQArgs ua;
-            ua.list = d_list.as<int2>();
-            long long qo = 0;
-            for (int k = 0; k < NQCLASS; ++k) {
-                ua.qoff[k] = qo;
-                qo += qcount[k];
-            }
-            ua.qscr = d_qscr.as<double>();
-            ua.shell_l = d_shell_l.as<int>();
-            ua.out_x = d_out_x.as<int>();
-            ua.out_y = d_out_y.as<int>();
-            ua.pair_cls = d_pair_cls.as<int>();
-            ua.klist = d_klist.as<long long>();
-            ua.kunit_os = d_kunit_os.as<int>();
-            ua.kunit_e = d_kunit_e.as<long long>();
-            ua.nkunit = nku;
-            ua.kunit_q = d_kunit_q.as<double>();
-            ua.jlist = d_jlist.as<long long>();
-            ua.junit_pc = d_junit_pc.as<int>();
-            ua.junit_e = d_junit_e.as<long long>();
-            ua.njunit = nju;
-            ua.junit_q = d_junit_q.as<double>();
-            prof.time(48, st, [&] { aimdhf::launch_unit_q(ua, st); });
-            HIPCHECK(hipGetLastError());
-        }
+        have_unit_q = false;   // unit bounds: on the first density-difference build
         store_bytes = 8 * (off + ko + jo) + 4 * (ko32 + jo32) + (have_32b ? 4 * (ko + jo) : 0) + 4 * keys +
                       8 * (k_entries + j_entries + 4 * nks + 4 * njs);
+    }
+
+    // Largest Schwarz product of every work unit of the integral store, which
+    // density-difference builds screen with; computed once per store, on the first
+    // such build (stream-ordered, no synchronisation).
+    bool have_unit_q = false;
+    void ensure_unit_q(hipStream_t st) {
+        if (have_unit_q) return;
+        const int nku = kunit_base[9], nju = junit_base[6];
+        d_kunit_q.ensure(sizeof(double) * std::max(nku, 1));
+        d_junit_q.ensure(sizeof(double) * std::max(nju, 1));
+        d_dmax.ensure(sizeof(double));
+        aimdhf::UnitQArgs ua;
+        ua.list = d_list.as<int2>();
+        long long qo = 0;
+        for (int k = 0; k < NQCLASS; ++k) {
+            ua.qoff[k] = qo;
+            qo += qcount[k];
+        }
+        ua.qscr = d_qscr.as<double>();
+        ua.shell_l = d_shell_l.as<int>();
+        ua.out_x = d_out_x.as<int>();
+        ua.out_y = d_out_y.as<int>();
+        ua.pair_cls = d_pair_cls.as<int>();
+        ua.klist = d_klist.as<long long>();
+        ua.kunit_os = d_kunit_os.as<int>();
+        ua.kunit_e = d_kunit_e.as<long long>();
+        ua.nkunit = nku;
+        ua.kunit_q = d_kunit_q.as<double>();
+        ua.jlist = d_jlist.as<long long>();
+        ua.junit_pc = d_junit_pc.as<int>();
+        ua.junit_e = d_junit_e.as<long long>();
+        ua.njunit = nju;
+        ua.junit_q = d_junit_q.as<double>();
+        prof.time(48, st, [&] { aimdhf::launch_unit_q(ua, st); });
+        HIPCHECK(hipGetLastError());
+        have_unit_q = true;
     }
 
     aimdhf::OneElArgs oneel_args() const {
@@ -1588,6 +1595,7 @@ static int jk_from_store(aimdhf_plan* p, int nset, const double* dens, double* J
     try {
         const size_t nn = (size_t)p->nao * p->nao;
         p->d_dsym.ensure(sizeof(double) * nset * nn);
+        if (delta) p->ensure_unit_q(st);
         p->prof.time(45, st, [&] {
             aimdhf::launch_symmetrize(dens, nset, p->nao, p->d_dsym.as<double>(), st);
             if (delta) aimdhf::launch_absmax(p->d_dsym.as<double>(), nset * nn, p->d_dmax.as<double>(), st);
