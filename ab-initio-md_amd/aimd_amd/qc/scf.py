@@ -62,9 +62,11 @@ class SCFOptions:
     precision, off in FP64) builds J and K of every iteration after the first
     from the density difference, J(D) = J(D_prev) + J(D - D_prev), skipping in
     store mode the work units whose largest Schwarz product times max |dD| is
-    below ``incremental_threshold`` (default: the Schwarz threshold) and, in
-    mixed precision once max |dD| <= ``delta_fp32_switch``, digesting the
-    difference in FP32.
+    below ``incremental_threshold`` (default: the Schwarz threshold in FP64; in
+    mixed precision a tenth of ``conv_energy`` when that is larger, so the
+    skipped contributions stay an order below the SCF's own energy tolerance)
+    and, in mixed precision once max |dD| <= ``delta_fp32_switch``, digesting
+    the difference in FP32.
     """
     conv_energy: float = 1e-10
     conv_commutator: float = 1e-7
@@ -991,6 +993,16 @@ class SCFSolver:
         return 0.5 * (F + F.transpose(1, 2))
 
     @property
+    def incremental_threshold(self) -> float:
+        """Screening threshold of the density-difference builds (SCFOptions docstring)."""
+        opt = self.options
+        if opt.incremental_threshold is not None:
+            return opt.incremental_threshold
+        if opt.precision == "mixed":
+            return max(opt.schwarz_threshold, 0.1 * opt.conv_energy)
+        return opt.schwarz_threshold
+
+    @property
     def incremental(self) -> bool:
         """Whether SCF iterations after the first build J/K from density differences."""
         inc = self.options.incremental_fock
@@ -1035,7 +1047,7 @@ class SCFSolver:
         converged = False
         eye = torch.eye(I.nmo, dtype=torch.float64, device=S.device)
         incremental = self.incremental
-        dthr = opt.schwarz_threshold if opt.incremental_threshold is None else opt.incremental_threshold
+        dthr = self.incremental_threshold
         jk = None
         D_built = None
         for it in range(1, opt.max_iter + 1):

@@ -174,9 +174,7 @@ def main() -> int:
                  "grad_fp32_switch": bm.solver.options.grad_fp32_switch,
                  "delta_fp32_switch": bm.solver.options.delta_fp32_switch,
                  "incremental_fock": bm.solver.incremental,
-                 "incremental_threshold": (bm.solver.options.incremental_threshold
-                                           if bm.solver.options.incremental_threshold is not None
-                                           else bm.solver.options.schwarz_threshold),
+                 "incremental_threshold": bm.solver.incremental_threshold,
                  "scf_iterations_mean": float(np.mean([o.info["scf_iterations"] for o in outs])),
                  "phase_ms": {k: round(1e3 * float(np.mean([o.info["timings"][k] for o in outs])), 3)
                               for k in outs[0].info["timings"]}}
