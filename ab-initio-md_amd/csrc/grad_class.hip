@@ -16,9 +16,9 @@
 // bra pass that also fold in the ket's first-centre derivative (one item per
 // ket column, or all columns of a light class in one), so the ket pass and its
 // R evaluations disappear. Items are issued wave by wave (64 quartets, one pass
-// or column group per wavefront), so the lanes of a wavefront run the same code. Contributions are accumulated per workgroup in LDS
-// and flushed once, as exact fixed-point integers, so the gradient is
-// bitwise reproducible.
+// or column group per wavefront), so the lanes of a wavefront run the same
+// code. Contributions are accumulated per workgroup in LDS and flushed once, as
+// exact fixed-point integers, so the gradient is bitwise reproducible.
 
 #include "aimdhf_internal.h"
 #include "md_grad.h"
