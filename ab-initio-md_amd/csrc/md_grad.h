@@ -35,17 +35,18 @@ MD_HD constexpr int grad_ncol() {
 // Ket columns per folded item (grad_pass_c) of class (LA LB | LC LD) in FP64 or
 // FP32, 0 where the class is not folded: kets (p|s) with a light bra ((ps|,
 // (pp|, (ds|) take all three columns in one item, other (s|s) and (p|s) kets one
-// item per column; (d|s) kets, and in FP32 also (p|p) kets, under a bra up to
-// (dp| one item per column. Heavier classes keep the separate bra and ket
-// passes: their folded items need far more registers than one wave has and
-// multiply the compiled code by the ket's column count. FP64 items of (p|p) kets
-// disagree with the oracle on gfx950 (1.6e-2 Eh/bohr on (H2O)8) while the same
-// code agrees in FP32 there and in FP64 on the host; they stay unfolded.
+// item per column; (p|p) and (d|s) kets under a bra up to (dp| one item per
+// column. Heavier classes keep the separate bra and ket passes: their folded
+// items need far more registers than one wave has and multiply the compiled code
+// by the ket's column count. The FP64 items of (pp|pp) disagree with the oracle
+// on gfx950 (1.3e-3 Eh/bohr on (H2O)8) while the same code agrees in FP32 there,
+// in FP64 for (ds|pp) and (dp|pp), and in FP64 on the host; (pp|pp) stays
+// unfolded in FP64.
 template <int LA, int LB, int LC, int LD, bool FP64>
 MD_HD constexpr int fold_ncol() {
     if (LC == 0 && LD == 0) return 1;
     if (LC == 1 && LD == 0) return 3 * (NH(LA + LB) + NH(LA + LB + 1)) <= 90 ? 3 : 1;
-    if (LC + LD == 2 && LA + LB <= 3) return FP64 && LC == 1 ? 0 : 1;
+    if (LC + LD == 2 && LA + LB <= 3) return FP64 && LA == 1 && LC == 1 ? 0 : 1;
     return 0;
 }
 
