@@ -217,10 +217,10 @@ def inverse_sqrt_newton_schulz(S: torch.Tensor, max_iter: int = 60, tol: float =
     S^{-1/2} of a symmetric positive definite S by the coupled Newton-Schulz
     iteration (Higham, Functions of Matrices, eq. 6.35): Y0 = S/c, Z0 = 1,
     T = (3 - Z Y)/2, Y <- Y T, Z <- T Z, with c a Gershgorin bound on the largest
-    eigenvalue; Z -> (S/c)^{-1/2}. Returns (S^{-1/2}, estimate of the smallest
-    eigenvalue of S from 30 power steps on S^{-1}: an upper bound, the Rayleigh
-    quotient never exceeds the largest eigenvalue of S^{-1}), or None if |X S X - 1| does not
-    fall below 1e-10 within max_iter steps.
+    eigenvalue; Z -> (S/c)^{-1/2}. Returns (S^{-1/2}, a lower bound of the smallest
+    eigenvalue of S: 1 / max_i sum_j |(S^{-1})_ij|, since the largest eigenvalue of
+    S^{-1} = X X is at most its largest absolute row sum), or None if |X S X - 1|
+    does not fall below 1e-10 within max_iter steps.
     """
     n = S.shape[0]
     eye = torch.eye(n, dtype=S.dtype, device=S.device)
@@ -237,14 +237,7 @@ def inverse_sqrt_newton_schulz(S: torch.Tensor, max_iter: int = 60, tol: float =
     X = 0.5 * (X + X.T)
     if not float((X @ S @ X - eye).abs().max()) < 1e-10:
         return None
-    X2 = X @ X
-    # fixed pseudo-random start (no symmetry can make it orthogonal to the smallest mode)
-    v = torch.cos(torch.arange(n, dtype=S.dtype, device=S.device) * 0.7548776662466927 + 0.3)
-    v = v / torch.linalg.vector_norm(v)
-    for _ in range(30):
-        v = X2 @ v
-        v = v / torch.linalg.vector_norm(v)
-    lam = float(v @ (X2 @ v))
+    lam = float((X @ X).abs().sum(dim=1).max())
     return X, 1.0 / lam
 
 
@@ -356,9 +349,10 @@ def orthogonalizer(S: torch.Tensor, method: str = "auto",
     """
     X with X^T S X = 1, removed combinations, smallest eigenvalue (aimd/qc/scf.py:284-303).
     The symmetric S^{-1/2} is formed by Newton-Schulz iteration (dense FP64 products)
-    when the smallest eigenvalue is clearly above the threshold (estimate > 1e3 x
-    threshold); otherwise, and for canonical orthogonalisation, from eigh as the
-    reference does, with its linear-dependence handling.
+    when the smallest eigenvalue is clearly above the threshold (a lower bound of it
+    > 1e3 x threshold; that bound is then the third value returned); otherwise, and
+    for canonical orthogonalisation, from eigh as the reference does, with its
+    linear-dependence handling.
     """
     if method not in _ORTHO:
         raise ValueError(f"unknown orthogonalization {method!r}; use one of {_ORTHO}")
