@@ -73,7 +73,8 @@ __global__ __launch_bounds__(64) void k_jcopy(JSArgs a) {
         const long long t0 = (u0 - a.jseg[seg] - (f32 ? a.jbig[seg] : 0)) * NS;
         const long long nt = (u1 - u0) * NS;
         // lanes over consecutive items, so each value row is written by one
-        // coalesced store; the W source loads of an item are independent
+        // coalesced (non-temporal: the copies are read only by later builds) store;
+        // the W source loads of an item are independent
         for (long long i = lane; i < nt; i += 64) {
             const unsigned long long ent = (unsigned long long)a.jlist[u0 + i / NS];
             const int s = (int)(i % NS);
@@ -91,15 +92,15 @@ __global__ __launch_bounds__(64) void k_jcopy(JSArgs a) {
             if (f32) {
                 float* out = a.jdata32 + a.jdoff32[seg] + dst;
 #pragma unroll
-                for (int v = 0; v < W; ++v) out[v * 64] = (float)val[v];
+                for (int v = 0; v < W; ++v) __builtin_nontemporal_store((float)val[v], out + v * 64);
             } else {
                 double* out = a.jdata + a.jdoff[seg] + dst;
 #pragma unroll
-                for (int v = 0; v < W; ++v) out[v * 64] = val[v];
+                for (int v = 0; v < W; ++v) __builtin_nontemporal_store(val[v], out + v * 64);
                 if (a.jdata32b) {
                     float* out32 = a.jdata32b + a.jdoff[seg] + dst;
 #pragma unroll
-                    for (int v = 0; v < W; ++v) out32[v * 64] = (float)val[v];
+                    for (int v = 0; v < W; ++v) __builtin_nontemporal_store((float)val[v], out32 + v * 64);
                 }
             }
         }
@@ -122,12 +123,14 @@ __device__ __forceinline__ void j_unit(const JSArgs& a, const Real* __restrict__
     const size_t n = a.nao, nn = n * n;
     const long long nt = (u1 - u0) * NS;
     // the list entries run one pass ahead: a pass's density gathers (addressed by
-    // its entries) issue with its integral loads instead of after an extra round trip
+    // its entries) issue with its integral loads instead of after an extra round trip.
+    // Entries and integral copies are read once per build and loaded non-temporally,
+    // so they do not push the density (read by every unit) out of the caches
     unsigned long long ent_next[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
         const long long tk = lane + 64 * k;
-        ent_next[k] = lane < nt ? (unsigned long long)a.jlist[u0 + (tk < nt ? tk : lane) / NS] : 0ull;
+        ent_next[k] = lane < nt ? (unsigned long long)__builtin_nontemporal_load(a.jlist + u0 + (tk < nt ? tk : lane) / NS) : 0ull;
     }
     for (long long t = lane; t < nt; t += 64 * U) {
         // loads of all U items first; an item past the end repeats item t
@@ -145,13 +148,13 @@ __device__ __forceinline__ void j_unit(const JSArgs& a, const Real* __restrict__
             const long long T = t0 + tc;
             const Real* p = tile0 + (T >> 6) * (64 * W) + (T & 63);
 #pragma unroll
-            for (int i = 0; i < W; ++i) v[k][i] = p[i * 64];
+            for (int i = 0; i < W; ++i) v[k][i] = __builtin_nontemporal_load(p + i * 64);
         }
         const long long tn = t + 64 * U;
 #pragma unroll
         for (int k = 0; k < U; ++k) {
             const long long tk = tn + 64 * k;
-            ent_next[k] = tn < nt ? (unsigned long long)a.jlist[u0 + (tk < nt ? tk : tn) / NS] : 0ull;
+            ent_next[k] = tn < nt ? (unsigned long long)__builtin_nontemporal_load(a.jlist + u0 + (tk < nt ? tk : tn) / NS) : 0ull;
         }
 #pragma unroll
         for (int k = 0; k < U; ++k) {
