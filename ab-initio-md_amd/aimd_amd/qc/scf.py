@@ -656,6 +656,24 @@ class DIIS:
         r = Gs[a, a] - Gs[a, :a]
         return torch.cat([Gd, r[None]])
 
+    def gram_from(self, G_raw: np.ndarray) -> np.ndarray | None:
+        """gram() from a host copy of the raw (space, space) slot Gram matrix, which
+        travels with the other per-iteration scalars: the same differences, formed
+        on the host."""
+        m = self._m
+        if m == 1:
+            return None
+        Gs = np.roll(G_raw, shift=(-self._head, -self._head), axis=(0, 1))[:m, :m]
+        a = m - 1
+        Gd = Gs[:a, :a] - Gs[:a, a:] - Gs[a:, :a] + Gs[a, a]
+        r = Gs[a, a] - Gs[a, :a]
+        return np.concatenate([Gd, r[None]])
+
+    @property
+    def raw_gram(self) -> torch.Tensor:
+        """The device (space, space) Gram matrix of the ring slots."""
+        return self._G
+
     def coefficients(self, Gr: np.ndarray | None = None) -> np.ndarray:
         """Coefficients of the stored vectors (oldest first); Gr is the host copy of
         gram() for the current vectors, computed here when not given."""
@@ -684,7 +702,7 @@ class DIIS:
                 ct = torch.from_numpy(full)
                 if self._F.is_cuda:
                     ct = ct.pin_memory().to(self._F.device, non_blocking=True)
-                return (ct @ self._F.view(self.space, -1)).view(self._F.shape[1:])
+                return torch.mv(self._F.view(self.space, -1).t(), ct).view(self._F.shape[1:])
             self._head = (self._head + 1) % self.space
             self._m -= 1
             if Gr is not None:
@@ -1067,9 +1085,7 @@ class SCFSolver:
             n_diis = len(diis) if diis is not None else 0
             if diis is not None:
                 diis.push(Fo, err)
-                Gd = diis.gram()
-                if Gd is not None:
-                    parts.append(Gd.reshape(-1))
+                parts.append(diis.raw_gram.reshape(-1))
             scal = torch.cat(parts).cpu().numpy()
             e_el, err_norm = float(scal[0]), max(float(scal[1]), -float(scal[2]))
             de = e_el - e_prev
@@ -1081,7 +1097,7 @@ class SCFSolver:
             if it == opt.max_iter:
                 break
             if diis is not None:
-                Gr = scal[3:].reshape(len(diis), len(diis) - 1) if len(diis) > 1 else None
+                Gr = diis.gram_from(scal[3:].reshape(diis.space, diis.space))
                 if err_norm < best:
                     best, since_best = err_norm, 0
                 else:

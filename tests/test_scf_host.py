@@ -49,6 +49,25 @@ def test_diis_matches_least_squares_on_random_vectors():
     assert abs(c.sum() - 1.0) < 1e-12
 
 
+def test_diis_host_gram_from_the_raw_slot_matrix_equals_the_device_gram():
+    # the SCF loop reads the raw (space, space) slot Gram with its scalars and forms the
+    # differences on the host; with the ring wrapped (more pushes than slots) and after a
+    # restart from the newest vector both must equal gram() bit for bit
+    rng = np.random.default_rng(12)
+    d = S.DIIS(4)
+    for k in range(7):
+        d.push(_sym(6, rng), _sym(6, rng))
+        host = d.gram_from(d.raw_gram.numpy().copy())
+        dev = d.gram()
+        assert (host is None) == (dev is None)
+        if dev is not None:
+            np.testing.assert_array_equal(host, dev.numpy())
+    d.keep_last()
+    assert d.gram_from(d.raw_gram.numpy().copy()) is None
+    d.push(_sym(6, rng), _sym(6, rng))
+    np.testing.assert_array_equal(d.gram_from(d.raw_gram.numpy().copy()), d.gram().numpy())
+
+
 def test_diis_drops_the_oldest_vectors_when_the_extrapolation_blows_up():
     rng = np.random.default_rng(7)
     u, w = _sym(4, rng), _sym(4, rng)
